@@ -1,0 +1,141 @@
+"""``Simulation``: the public call surface of the HIP time-stepping backend.
+
+Mirrors the parts of ``beamz/simulation/api.py`` that drive the hot path:
+``compile`` (``api.py:1071``), ``initial_state``, ``advance`` (``api.py:1339-1458``)
+and ``run`` (``api.py:1460-1537``), with backend resolution following
+``beamz/simulation/backend.py:223-258``.  Geometry is given as a permittivity
+raster (the lowering of ``Design`` objects is outside this backend).
+"""
+
+from __future__ import annotations
+
+import time as _time
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _abi as abi
+from . import native, runtime
+from .constants import LIGHT_SPEED
+from .monitors import flux as _flux, phasors as _phasors
+from .program import compile_program, initial_state
+
+BACKENDS = ("auto", "hip")
+
+
+def resolve_backend(name="auto"):
+    """``backend.py:223-258``: ``auto`` and ``hip`` both need the HIP library;
+    there is no CPU fallback, so an unavailable library raises."""
+    name = str(name or "auto").lower()
+    if name not in BACKENDS:
+        raise ValueError(f"unknown backend {name!r}; expected one of {BACKENDS}")
+    ok, why = native.status()
+    if not ok:
+        raise native.HipBackendUnavailable(why)
+    return "hip"
+
+
+def courant_dt(resolution, ndim, courant=0.99):
+    """``dt = S dx / (c sqrt(ndim))`` (``beamz/_helpers.py:76-164``)."""
+    return courant * resolution / (LIGHT_SPEED * np.sqrt(ndim))
+
+
+@dataclass
+class SimulationPerformance:
+    """``results.py:59-93``: GCUPS = material cells x steps / runtime / 1e9."""
+
+    cells: int
+    steps: int
+    runtime_s: float
+
+    @property
+    def gcups(self):
+        return self.cells * self.steps / max(self.runtime_s, 1e-30) / 1e9
+
+
+@dataclass
+class SimulationResults:
+    state: object
+    program: object
+    performance: SimulationPerformance
+
+    def monitor_flux(self, index=0):
+        m = self.program.monitors[index]
+        s = self.state
+        return _flux(m, s.dft_re.cpu().numpy(), s.dft_im.cpu().numpy(), s.dft_w.cpu().numpy(),
+                     self.program.dt)
+
+    def monitor_phasors(self, index=0):
+        m = self.program.monitors[index]
+        s = self.state
+        return _phasors(m, s.dft_re.cpu().numpy(), s.dft_im.cpu().numpy(), s.dft_w.cpu().numpy(),
+                        self.program.dt)
+
+    def field(self, name):
+        return getattr(self.state, name.lower()).float().cpu().numpy()
+
+
+class Simulation:
+    """A time-domain run on a uniform Yee grid.
+
+    Parameters
+    ----------
+    permittivity : array (z, y, x) or (y, x)
+        Relative permittivity per material cell.
+    resolution : float
+        Cell size in metres.
+    dt : float, optional
+        Time step; defaults to ``courant / sqrt(ndim)`` of the Courant limit.
+    boundaries, sources, monitors : sequences of ``PEC``/``PML``,
+        ``SoftSource``/``CustomSource`` and ``FluxMonitor``.
+    steps : int
+        Number of time steps of ``run``.
+    """
+
+    def __init__(self, permittivity, resolution, *, dt=None, courant=0.99, boundaries=(),
+                 sources=(), monitors=(), steps=1, t0=0.0, conductivity=None,
+                 backend="auto", field_dtype=torch.float32, device=None,
+                 flags=abi.DEFAULT_FLAGS):
+        self.permittivity = np.asarray(permittivity, dtype=np.float64)
+        self.resolution = float(resolution)
+        self.dt = float(dt) if dt is not None else courant_dt(resolution, self.permittivity.ndim, courant)
+        self.boundaries = tuple(boundaries)
+        self.sources = tuple(sources)
+        self.monitors = tuple(monitors)
+        self.steps = int(steps)
+        self.t0 = float(t0)
+        self.conductivity = conductivity
+        self.backend = backend
+        self.field_dtype = field_dtype
+        self.device = device
+        self.flags = flags
+        self._program = None
+
+    def compile(self):
+        resolve_backend(self.backend)
+        if self._program is None:
+            self._program = compile_program(
+                self.permittivity, self.resolution, self.dt, boundaries=self.boundaries,
+                sources=self.sources, monitors=self.monitors, num_steps=self.steps,
+                t0=self.t0, conductivity=self.conductivity, device=self.device,
+                field_dtype=self.field_dtype, flags=self.flags)
+        return self._program
+
+    def initial_state(self, fields=None):
+        return initial_state(self.compile(), fields)
+
+    def advance(self, state=None, steps=None, *, chunk=runtime.GRAPH_MAX_STEPS):
+        prog = self.compile()
+        state = self.initial_state() if state is None else state
+        return runtime.advance(state, prog, self.steps if steps is None else int(steps), chunk=chunk)
+
+    def run(self, state=None, *, chunk=runtime.GRAPH_MAX_STEPS):
+        prog = self.compile()
+        state = self.initial_state() if state is None else state
+        torch.cuda.synchronize()
+        t0 = _time.perf_counter()
+        runtime.advance(state, prog, self.steps, chunk=chunk)
+        torch.cuda.synchronize()
+        perf = SimulationPerformance(prog.cells, self.steps, _time.perf_counter() - t0)
+        return SimulationResults(state, prog, perf)

@@ -1,0 +1,970 @@
+// C-ABI layer of libbeamz_hip.so: descriptor validation, positional decoding
+// of the reference's FFI layouts, the per-step schedule and the hipGraph
+// cache.
+//
+// Reference counterparts:
+//   ffi_handler.cc:46-189, 290-334, 336-708  -> decode_* below
+//   program.cu:110-170 (ValidateProgram)     -> validate_* below
+//   program.cu:172-335 (schedules)           -> build_schedule
+//   graph.cu:84-154 (graph key, cache)       -> GraphCache
+// The per-step order is execute.py:467-541: pre_e sources, H, h sources,
+// (H masks, folded into the kernels), E, e sources, (E masks), monitors.
+
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "common.h"
+
+namespace bzh {
+
+namespace {
+thread_local std::string g_error;
+}
+
+void set_error(const std::string& message) { g_error = message; }
+void clear_error() { g_error.clear(); }
+
+namespace {
+
+// ---- buffer helpers -----------------------------------------------------------
+int64_t numel(const bzh_buffer& b) {
+  if (b.rank < 0 || b.rank > 4) return -1;
+  int64_t n = 1;
+  for (int d = 0; d < b.rank; ++d) n *= b.dims[d];
+  return n;
+}
+
+bool is_empty(const bzh_buffer& b) { return b.data == nullptr || numel(b) == 0; }
+
+bool dims_equal(const bzh_buffer& b, int rank, const int64_t* dims) {
+  if (b.rank != rank) return false;
+  for (int d = 0; d < rank; ++d)
+    if (b.dims[d] != dims[d]) return false;
+  return true;
+}
+
+std::string shape_str(const bzh_buffer& b) {
+  std::string s = "(";
+  for (int d = 0; d < b.rank; ++d) s += (d ? ", " : "") + std::to_string(b.dims[d]);
+  return s + ")";
+}
+
+bool is_float_storage(int t) { return t == BZH_F32 || t == BZH_BF16 || t == BZH_F16; }
+
+// Index of a component in the (Hx, Hy, Hz, Ex, Ey, Ez) bank order.
+enum { kHx = 0, kHy, kHz, kEx, kEy, kEz };
+
+// Complete Yee support of each component (lattice.py:32-45), 3D.
+void yee_shape3d(int c, int nz, int ny, int nx, int64_t out[3]) {
+  static const int off[6][3] = {
+      // (z, y, x) "+1" where the component's offset is 0
+      {0, 0, 1},  // Hx (nz, ny, nx+1)
+      {0, 1, 0},  // Hy (nz, ny+1, nx)
+      {1, 0, 0},  // Hz (nz+1, ny, nx)
+      {1, 1, 0},  // Ex (nz+1, ny+1, nx)
+      {1, 0, 1},  // Ey (nz+1, ny, nx+1)
+      {0, 1, 1},  // Ez (nz, ny+1, nx+1)
+  };
+  out[0] = nz + off[c][0];
+  out[1] = ny + off[c][1];
+  out[2] = nx + off[c][2];
+}
+
+struct Grid {
+  bool is3d = true;
+  int nz = 1, ny = 0, nx = 0;
+  int dtype = BZH_F32;
+};
+
+// Infer and validate the material grid from the six fields (bank order).
+Grid infer_grid(const bzh_buffer* f[6]) {
+  Grid g;
+  const bzh_buffer& ez = *f[kEz];
+  if (ez.rank == 2) {
+    // TMxy: Ez (ny+1, nx+1), Hx (ny, nx+1), Hy (ny+1, nx)
+    g.is3d = false;
+    g.ny = static_cast<int>(ez.dims[0]) - 1;
+    g.nx = static_cast<int>(ez.dims[1]) - 1;
+    BZH_REQUIRE(g.ny >= 1 && g.nx >= 1, "TMxy grid must have at least one cell");
+    const int64_t hx[2] = {g.ny, g.nx + 1}, hy[2] = {g.ny + 1, g.nx};
+    BZH_REQUIRE(dims_equal(*f[kHx], 2, hx), "Hx shape " + shape_str(*f[kHx]) + " does not match Ez");
+    BZH_REQUIRE(dims_equal(*f[kHy], 2, hy), "Hy shape " + shape_str(*f[kHy]) + " does not match Ez");
+    g.dtype = ez.element_type;
+    for (int c : {kHx, kHy, kEz}) {
+      BZH_REQUIRE(f[c]->element_type == g.dtype, "TMxy fields must share one dtype");
+      BZH_REQUIRE(f[c]->data != nullptr, "TMxy field pointer is null");
+    }
+  } else {
+    BZH_REQUIRE(f[kHx]->rank == 3, "3D fields must be rank 3");
+    g.nz = static_cast<int>(f[kHx]->dims[0]);
+    g.ny = static_cast<int>(f[kHx]->dims[1]);
+    g.nx = static_cast<int>(f[kHx]->dims[2]) - 1;
+    BZH_REQUIRE(g.nz >= 1 && g.ny >= 1 && g.nx >= 1, "3D grid must have at least one cell");
+    g.dtype = f[kHx]->element_type;
+    for (int c = 0; c < 6; ++c) {
+      int64_t s[3];
+      yee_shape3d(c, g.nz, g.ny, g.nx, s);
+      BZH_REQUIRE(dims_equal(*f[c], 3, s),
+                  "field " + std::to_string(c) + " shape " + shape_str(*f[c]) +
+                      " is not a complete Yee support of the grid");
+      BZH_REQUIRE(f[c]->element_type == g.dtype, "fields must share one dtype");
+      BZH_REQUIRE(f[c]->data != nullptr, "field pointer is null");
+    }
+  }
+  BZH_REQUIRE(is_float_storage(g.dtype), "fields must be f32, bf16 or f16");
+  const int64_t cells = static_cast<int64_t>(g.nz + 1) * (g.ny + 1) * (g.nx + 1);
+  BZH_REQUIRE(cells < (int64_t{1} << 31), "grid exceeds 32-bit indexing");
+  return g;
+}
+
+// Which ends of each (z, y, x) axis mask a component on PEC faces
+// (_boundary_compile.py:118-160).  2D uses axes 1 (y), 2 (x).
+void mask_ends(int comp, int metallic, bool is3d, int lo[3], int hi[3]) {
+  for (int a = 0; a < 3; ++a) lo[a] = hi[a] = 0;
+  auto face = [&](int axis) {
+    static const int bits[3][2] = {{kFront, kBack}, {kBottom, kTop}, {kLeft, kRight}};
+    lo[axis] = (metallic & bits[axis][0]) != 0;
+    hi[axis] = (metallic & bits[axis][1]) != 0;
+  };
+  if (is3d) {
+    switch (comp) {
+      case kEx: face(0); face(1); break;
+      case kEy: face(0); face(2); break;
+      case kEz: face(1); face(2); break;
+      case kHx: face(2); break;
+      case kHy: face(1); break;
+      case kHz: face(0); break;
+    }
+  } else {
+    switch (comp) {
+      case kEz: face(1); face(2); break;
+      case kHx: face(2); break;
+      case kHy: face(1); break;
+      default: break;
+    }
+  }
+}
+
+// ---- phase decoding ---------------------------------------------------------
+struct PhaseIn {
+  const bzh_buffer* tin[3];
+  const bzh_buffer* tout[3];
+  const bzh_buffer* src[3];
+  const bzh_buffer* mat[6];
+  const bzh_buffer* meta;
+  const bzh_buffer* terms[18];
+  const bzh_buffer* psi_in[6];
+  const bzh_buffer* psi_out[6];
+  const bzh_buffer* metric[3];
+  int nterms;
+};
+
+// Derivative axis of each term in storage order (z, y, x) = (0, 1, 2).
+// 3D: update.cu order Hx_y, Hx_z, Hy_z, Hy_x, Hz_x, Hz_y (and E alike).
+const int kAxes3d[6] = {1, 0, 0, 2, 2, 1};
+// TMxy H terms (Hx_y, Hy_x) and E terms (Ez_x, Ez_y), in (z, y, x) axes.
+const int kAxesTmH[2] = {1, 2};
+const int kAxesTmE[2] = {2, 1};
+
+void fill_coef(const bzh_buffer* b, int64_t expect, const float** ptr, int32_t* st,
+               float* s, const char* what) {
+  *ptr = nullptr;
+  *st = 0;
+  *s = 0.0f;
+  if (b == nullptr) return;
+  BZH_REQUIRE(b->element_type == BZH_F32, std::string(what) + " must be f32");
+  const int64_t n = numel(*b);
+  if (b->rank == 0 || n == 1) {
+    BZH_REQUIRE(b->data != nullptr, std::string(what) + " scalar pointer is null");
+    *ptr = static_cast<const float*>(b->data);
+    *st = 0;
+    return;
+  }
+  BZH_REQUIRE(n == expect, std::string(what) + " has " + std::to_string(n) +
+                               " elements, expected " + std::to_string(expect));
+  *ptr = static_cast<const float*>(b->data);
+  *st = 1;
+}
+
+// Build kernel parameters for one phase.  ``fields`` is the bank-ordered field
+// set used for geometry (targets and sources may come from different banks).
+PhaseParams make_phase(const PhaseIn& in, int phase, const Grid& g, float dt,
+                       float resolution, int metallic, int uniform_l) {
+  PhaseParams p;
+  std::memset(&p, 0, sizeof(p));
+  p.nz = g.nz;
+  p.ny = g.ny;
+  p.nx = g.nx;
+  p.metallic = metallic;
+  p.inv_dx = 1.0f / resolution;
+  p.ratio = 1.0f;
+  const int base = phase == BZH_PHASE_H ? kHx : kEx;
+  const int other = phase == BZH_PHASE_H ? kEx : kHx;
+  (void)dt;
+  for (int c = 0; c < 3; ++c) {
+    const bool active = g.is3d || (phase == BZH_PHASE_H ? c < 2 : c == 2);
+    const bool src_active = g.is3d || (phase == BZH_PHASE_H ? c == 2 : c < 2);
+    int64_t tshape[3] = {1, 1, 1};
+    if (g.is3d) yee_shape3d(base + c, g.nz, g.ny, g.nx, tshape);
+    if (active) {
+      BZH_REQUIRE(in.tin[c]->data != nullptr, "target field pointer is null");
+      p.tin[c] = in.tin[c]->data;
+      p.tout[c] = (in.tout[c] && in.tout[c]->data) ? in.tout[c]->data : in.tin[c]->data;
+      if (in.tout[c] && in.tout[c]->data)
+        BZH_REQUIRE(numel(*in.tout[c]) == numel(*in.tin[c]) &&
+                        in.tout[c]->element_type == in.tin[c]->element_type,
+                    "output field does not match its input");
+      const int64_t n = numel(*in.tin[c]);
+      fill_coef(in.mat[c], n, &p.dec[c], &p.dec_st[c], &p.dec_s[c], "decay coefficient");
+      fill_coef(in.mat[3 + c], n, &p.scf[c], &p.scf_st[c], &p.scf_s[c], "source coefficient");
+      BZH_REQUIRE(p.dec[c] != nullptr && p.scf[c] != nullptr,
+                  "every active component needs decay and source coefficients");
+    }
+    if (src_active) {
+      BZH_REQUIRE(in.src[c]->data != nullptr, "source field pointer is null");
+      p.src[c] = in.src[c]->data;
+    }
+    (void)other;
+  }
+  // metrics (z, y, x): empty -> inv_dx, rank 0 -> device scalar, rank 1 -> array
+  const int ext[3] = {g.nz, g.ny, g.nx};
+  for (int a = 0; a < 3; ++a) {
+    const bzh_buffer* m = in.metric[a];
+    p.metric[a] = nullptr;
+    p.met_st[a] = 0;
+    if (m == nullptr || m->data == nullptr || numel(*m) == 0) continue;
+    BZH_REQUIRE(m->element_type == BZH_F32, "metrics must be f32");
+    if (numel(*m) == 1) {
+      p.metric[a] = static_cast<const float*>(m->data);
+      continue;
+    }
+    const int64_t need = phase == BZH_PHASE_H ? ext[a] : ext[a] + 1;
+    BZH_REQUIRE(numel(*m) >= need, "metric array too short for its axis");
+    p.metric[a] = static_cast<const float*>(m->data);
+    p.met_st[a] = 1;
+  }
+  // CPML terms
+  p.nterms = in.nterms;
+  p.uniform_l = uniform_l;
+  if (in.nterms > 0) {
+    const int want = g.is3d ? 6 : 2;
+    BZH_REQUIRE(in.nterms == want, g.is3d ? "3D CPML needs six terms per phase"
+                                          : "TMxy CPML needs two terms per phase");
+    BZH_REQUIRE(in.meta && in.meta->element_type == BZH_S32 && in.meta->rank == 2 &&
+                    in.meta->dims[0] == want && in.meta->dims[1] == 5,
+                "term metadata must be s32[nterms, 5]");
+    p.meta = static_cast<const int32_t*>(in.meta->data);
+    for (int t = 0; t < in.nterms; ++t) {
+      const int comp = g.is3d ? t / 2 : (phase == BZH_PHASE_H ? t : 2);
+      const int axis = g.is3d ? kAxes3d[t] : (phase == BZH_PHASE_H ? kAxesTmH[t] : kAxesTmE[t]);
+      const bzh_buffer& tgt = *in.tin[comp];
+      const bzh_buffer& psi = *in.psi_in[t];
+      const int raxis = g.is3d ? axis : axis - 1;  // axis within the target's rank
+      BZH_REQUIRE(psi.rank == tgt.rank, "psi rank must match its target");
+      int64_t ext_axis = 0;
+      for (int d = 0; d < tgt.rank; ++d) {
+        if (d == raxis) {
+          ext_axis = psi.dims[d];
+        } else {
+          BZH_REQUIRE(psi.dims[d] == tgt.dims[d], "psi " + std::to_string(t) + " shape " +
+                                                      shape_str(psi) + " does not match its target " +
+                                                      shape_str(tgt));
+        }
+      }
+      BZH_REQUIRE(psi.element_type == BZH_F32 || psi.element_type == BZH_BF16,
+                  "psi must be f32 or bf16");
+      if (uniform_l > 0)
+        BZH_REQUIRE(ext_axis == 2 * uniform_l || ext_axis == 0,
+                    "psi extent does not match the uniform CPML thickness");
+      DevTerm& d = p.term[t];
+      d.psi = psi.data;
+      if (in.psi_out[t] && in.psi_out[t]->data)
+        BZH_REQUIRE(in.psi_out[t]->data == psi.data,
+                    "per-phase launches update psi in place: outputs must alias inputs");
+      d.low = static_cast<int32_t>(ext_axis);
+      d.high = 0;
+      d.n = static_cast<int32_t>(tgt.dims[raxis]);
+      d.psi_bf16 = psi.element_type == BZH_BF16;
+      for (int q = 0; q < 3; ++q) {
+        const bzh_buffer* tb = in.terms[t * 3 + q];
+        BZH_REQUIRE(tb && tb->element_type == BZH_F32, "CPML term arrays must be f32");
+        BZH_REQUIRE(numel(*tb) >= ext_axis, "CPML term array shorter than its slab");
+        BZH_REQUIRE(ext_axis == 0 || tb->data != nullptr, "CPML term array pointer is null");
+      }
+      d.a = static_cast<const float*>(in.terms[t * 3 + 0]->data);
+      d.b = static_cast<const float*>(in.terms[t * 3 + 1]->data);
+      d.ik = static_cast<const float*>(in.terms[t * 3 + 2]->data);
+      if (ext_axis == 0) d.psi = nullptr;
+    }
+  }
+  return p;
+}
+
+void run_phase(hipStream_t s, const PhaseParams& p, int phase, const Grid& g) {
+  if (g.is3d) launch_phase3d(s, p, phase, g.dtype);
+  else launch_phase_tm(s, p, phase, g.dtype);
+}
+
+// ---- schedule ------------------------------------------------------------------
+enum class Kind : int32_t { kPhase3d, kPhaseTm, kSource, kDft, kFused };
+
+struct Launch {
+  Kind kind;
+  int32_t phase;
+  int32_t dtype;
+  std::vector<uint8_t> bytes;
+};
+
+template <typename P>
+Launch make_launch(Kind kind, int phase, int dtype, const P& p) {
+  Launch l;
+  l.kind = kind;
+  l.phase = phase;
+  l.dtype = dtype;
+  l.bytes.resize(sizeof(P));
+  std::memcpy(l.bytes.data(), &p, sizeof(P));
+  return l;
+}
+
+void enqueue(hipStream_t s, const Launch& l) {
+  switch (l.kind) {
+    case Kind::kPhase3d: {
+      PhaseParams p;
+      std::memcpy(&p, l.bytes.data(), sizeof(p));
+      launch_phase3d(s, p, l.phase, l.dtype);
+      break;
+    }
+    case Kind::kPhaseTm: {
+      PhaseParams p;
+      std::memcpy(&p, l.bytes.data(), sizeof(p));
+      launch_phase_tm(s, p, l.phase, l.dtype);
+      break;
+    }
+    case Kind::kSource: {
+      SourceParams p;
+      std::memcpy(&p, l.bytes.data(), sizeof(p));
+      launch_source(s, p);
+      break;
+    }
+    case Kind::kDft: {
+      DftParams p;
+      std::memcpy(&p, l.bytes.data(), sizeof(p));
+      launch_dft(s, p);
+      break;
+    }
+    case Kind::kFused:
+      throw Failure{"fused schedule not available"};
+  }
+}
+
+// ---- graph cache (graph.cu:84-154) -------------------------------------------
+class GraphCache {
+ public:
+  ~GraphCache() {
+    // Executables are owned by the process; the runtime may already be torn down
+    // at static destruction, so they are intentionally not destroyed here.
+  }
+
+  // Launch ``launches`` on ``s``; replay a cached graph when the kernel
+  // arguments match a previous call exactly.
+  void run(hipStream_t s, const std::vector<Launch>& launches, bool use_graph) {
+    if (!use_graph || s == nullptr) {
+      for (const auto& l : launches) enqueue(s, l);
+      return;
+    }
+    std::string key;
+    int device = 0;
+    BZH_HIP(hipGetDevice(&device));
+    key.append(reinterpret_cast<const char*>(&device), sizeof(device));
+    for (const auto& l : launches) {
+      key.append(reinterpret_cast<const char*>(&l.kind), sizeof(l.kind));
+      key.append(reinterpret_cast<const char*>(&l.phase), sizeof(l.phase));
+      key.append(reinterpret_cast<const char*>(&l.dtype), sizeof(l.dtype));
+      key.append(reinterpret_cast<const char*>(l.bytes.data()), l.bytes.size());
+    }
+    hipGraphExec_t exec = nullptr;
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      auto it = cache_.find(key);
+      if (it != cache_.end()) exec = it->second;
+    }
+    if (exec == nullptr) {
+      hipGraph_t graph = nullptr;
+      if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipGetLastError();
+        for (const auto& l : launches) enqueue(s, l);
+        return;
+      }
+      bool ok = true;
+      std::string why;
+      try {
+        for (const auto& l : launches) enqueue(s, l);
+      } catch (const Failure& f) {
+        ok = false;
+        why = f.message;
+      }
+      hipError_t end = hipStreamEndCapture(s, &graph);
+      if (!ok) {
+        if (graph) (void)hipGraphDestroy(graph);
+        throw Failure{why};
+      }
+      if (end != hipSuccess || graph == nullptr ||
+          hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        if (graph) (void)hipGraphDestroy(graph);
+        for (const auto& l : launches) enqueue(s, l);
+        return;
+      }
+      (void)hipGraphDestroy(graph);
+      std::lock_guard<std::mutex> lock(mu_);
+      if (cache_.size() >= kCapacity) {
+        for (auto& kv : cache_) (void)hipGraphExecDestroy(kv.second);
+        cache_.clear();
+      }
+      cache_[key] = exec;
+    }
+    BZH_HIP(hipGraphLaunch(exec, s));
+  }
+
+  size_t size() {
+    std::lock_guard<std::mutex> lock(mu_);
+    return cache_.size();
+  }
+
+ private:
+  static constexpr size_t kCapacity = 32;
+  std::mutex mu_;
+  std::map<std::string, hipGraphExec_t> cache_;
+};
+
+GraphCache& graph_cache() {
+  static GraphCache* cache = new GraphCache();
+  return *cache;
+}
+
+// ---- source groups ---------------------------------------------------------------
+// (timing, component) order of the nine groups: execute.py:48-52
+//   pre_e: Ex Ey Ez, h: Hx Hy Hz, e: Ex Ey Ez
+struct SourceGroupIn {
+  const bzh_buffer* coeffs;
+  const bzh_buffer* waves;
+  const bzh_buffer* starts;
+  int coincident;
+};
+
+void add_source_launches(std::vector<Launch>& out, const SourceGroupIn& grp, int timing,
+                         int comp_in_phase, const bzh_buffer* const* bank, const Grid& g,
+                         int metallic, const int32_t* step0, int k) {
+  const bzh_buffer& c = *grp.coeffs;
+  if (c.data == nullptr || c.rank < 1 || c.dims[0] == 0) return;
+  const int n = static_cast<int>(c.dims[0]);
+  const int comp = (timing == 1 ? kHx : kEx) + comp_in_phase;
+  const bzh_buffer& field = *bank[comp];
+  BZH_REQUIRE(field.data != nullptr, "source targets an absent field component");
+  BZH_REQUIRE(c.element_type == BZH_F32 && grp.waves->element_type == BZH_F32 &&
+                  grp.starts->element_type == BZH_S32,
+              "source groups are f32 coefficients, f32 waveforms and s32 starts");
+  BZH_REQUIRE(grp.waves->rank == 2 && grp.waves->dims[0] == n && grp.waves->dims[1] >= 1,
+              "source waveforms must be [n, T]");
+  const int fr = field.rank;
+  BZH_REQUIRE(grp.starts->rank == 2 && grp.starts->dims[0] == n && grp.starts->dims[1] == 3,
+              "source starts must be s32[n, 3]");
+  BZH_REQUIRE(c.rank == 4, "source coefficients must be [n, sz, sy, sx]");
+  SourceParams p;
+  std::memset(&p, 0, sizeof(p));
+  p.field = field.data;
+  p.dtype = field.element_type;
+  p.scale = 1.0f;
+  if (fr == 3) {
+    for (int d = 0; d < 3; ++d) p.fdims[d] = static_cast<int>(field.dims[d]);
+  } else {
+    BZH_REQUIRE(fr == 2, "source target must be rank 2 or 3");
+    p.fdims[0] = 1;
+    p.fdims[1] = static_cast<int>(field.dims[0]);
+    p.fdims[2] = static_cast<int>(field.dims[1]);
+  }
+  p.coeff = static_cast<const float*>(c.data);
+  p.wave = static_cast<const float*>(grp.waves->data);
+  p.starts = static_cast<const int32_t*>(grp.starts->data);
+  p.n = n;
+  for (int d = 0; d < 3; ++d) p.sdims[d] = static_cast<int>(c.dims[1 + d]);
+  p.T = static_cast<int>(grp.waves->dims[1]);
+  p.step0 = step0;
+  p.k = k;
+  if (timing != 0) {
+    int lo[3], hi[3];
+    mask_ends(comp, metallic, g.is3d, lo, hi);
+    for (int d = 0; d < 3; ++d) {
+      p.mlo[d] = lo[d];
+      p.mhi[d] = hi[d];
+    }
+  }
+  if (grp.coincident) {
+    p.slab0 = 0;
+    p.nslabs = n;
+    out.push_back(make_launch(Kind::kSource, 0, p.dtype, p));
+  } else {
+    // overlapping slabs are applied in declaration order, one launch each
+    for (int s = 0; s < n; ++s) {
+      p.slab0 = s;
+      p.nslabs = 1;
+      out.push_back(make_launch(Kind::kSource, 0, p.dtype, p));
+    }
+  }
+}
+
+// ---- monitors ----------------------------------------------------------------------
+struct MonitorIn {
+  const bzh_buffer* b[12];
+};
+
+DftParams make_dft(const MonitorIn& in, const bzh_buffer* const* bank, const Grid& g,
+                   float dt, int monitor_count) {
+  DftParams p;
+  std::memset(&p, 0, sizeof(p));
+  const bzh_buffer& idx = *in.b[0];
+  const bzh_buffer& w = *in.b[1];
+  const bzh_buffer& fr = *in.b[2];
+  BZH_REQUIRE(idx.element_type == BZH_S32 && idx.rank == 4 && idx.dims[1] == 6,
+              "monitor indices must be s32[M, 6, P, K]");
+  BZH_REQUIRE(w.element_type == BZH_F32 && dims_equal(w, 4, idx.dims),
+              "monitor weights must be f32 with the shape of the indices");
+  p.M = static_cast<int>(idx.dims[0]);
+  BZH_REQUIRE(monitor_count <= 0 || monitor_count == p.M, "monitor_count does not match");
+  p.Pmax = static_cast<int>(idx.dims[2]);
+  p.K = static_cast<int>(idx.dims[3]);
+  BZH_REQUIRE(fr.rank == 2 && fr.dims[0] == p.M, "monitor frequencies must be [M, F]");
+  BZH_REQUIRE(fr.element_type == BZH_F32 || fr.element_type == BZH_F64,
+              "monitor frequencies must be f32 or f64");
+  p.Fmax = static_cast<int>(fr.dims[1]);
+  p.freqs = fr.data;
+  p.freqs_f64 = fr.element_type == BZH_F64;
+  BZH_REQUIRE(in.b[3]->element_type == BZH_F32 && numel(*in.b[3]) == 6 * p.M,
+              "component masks must be f32[M, 6]");
+  BZH_REQUIRE(in.b[4]->element_type == BZH_S32 && numel(*in.b[4]) == 5 * p.M,
+              "monitor counts must be s32[M, 5]");
+  BZH_REQUIRE(in.b[5]->element_type == BZH_S32 && numel(*in.b[5]) == 2 * p.M,
+              "monitor codes must be s32[M, 2]");
+  BZH_REQUIRE(in.b[6]->element_type == BZH_F32 && numel(*in.b[6]) == 3 * p.M,
+              "monitor windows must be f32[M, 3]");
+  p.indices = static_cast<const int32_t*>(idx.data);
+  p.weights = static_cast<const float*>(w.data);
+  p.masks = static_cast<const float*>(in.b[3]->data);
+  p.counts = static_cast<const int32_t*>(in.b[4]->data);
+  p.codes = static_cast<const int32_t*>(in.b[5]->data);
+  p.windows = static_cast<const float*>(in.b[6]->data);
+  const bzh_buffer& re = *in.b[7];
+  const bzh_buffer& im = *in.b[8];
+  const bzh_buffer& ws = *in.b[9];
+  BZH_REQUIRE(re.element_type == im.element_type && re.element_type == ws.element_type &&
+                  (re.element_type == BZH_F64 || re.element_type == BZH_F32),
+              "DFT accumulators must all be f64 (or all f32)");
+  BZH_REQUIRE(numel(re) == numel(im), "DFT re/im arenas differ in length");
+  p.re = re.data;
+  p.im = im.data;
+  p.wsum = ws.data;
+  p.acc_f64 = re.element_type == BZH_F64;
+  p.re_len = numel(re);
+  p.w_len = numel(ws);
+  const bzh_buffer& t = *in.b[10];
+  BZH_REQUIRE((t.element_type == BZH_F64 || t.element_type == BZH_F32) && numel(t) == 1,
+              "monitor time must be an f32 or f64 scalar");
+  p.time = t.data;
+  p.time_f64 = t.element_type == BZH_F64;
+  BZH_REQUIRE(in.b[11]->element_type == BZH_S32 && numel(*in.b[11]) == 1,
+              "current_step must be an s32 scalar");
+  p.step0 = static_cast<const int32_t*>(in.b[11]->data);
+  p.dt = dt;
+  p.dtype = g.dtype;
+  // bank order Hx..Ez -> DFT order Ex, Ey, Ez, Hx, Hy, Hz
+  const int order[6] = {kEx, kEy, kEz, kHx, kHy, kHz};
+  for (int c = 0; c < 6; ++c) {
+    const bzh_buffer& f = *bank[order[c]];
+    const bool active = g.is3d || order[c] == kEz || order[c] == kHx || order[c] == kHy;
+    p.field[c] = active ? f.data : nullptr;
+    p.fsize[c] = active ? static_cast<int32_t>(numel(f)) : 0;
+    p.inv_scale[c] = 1.0f;
+  }
+  return p;
+}
+
+// ---- program ------------------------------------------------------------------
+struct ProgramIn {
+  const bzh_buffer* bank_a[6];   // Hx Hy Hz Ex Ey Ez
+  const bzh_buffer* bank_b[6];   // workspace (temporal layouts) or nullptr
+  PhaseIn h;                     // targets/sources bound per step below
+  PhaseIn e;
+  SourceGroupIn groups[9];
+  int ngroups;
+  const bzh_buffer* step;        // s32 scalar for the sources (may be null without sources)
+  MonitorIn mon;
+  bool has_monitors;
+  int monitor_count;
+  int nsteps;
+  float dt, resolution;
+  int metallic, uniform_l;
+  int flags;
+};
+
+void run_program(hipStream_t s, ProgramIn& in) {
+  BZH_REQUIRE(in.nsteps >= 1, "step count must be positive");
+  Grid g = infer_grid(in.bank_a);
+  const bool two_banks = in.bank_b[0] != nullptr;
+  if (two_banks) {
+    const Grid gb = infer_grid(in.bank_b);
+    BZH_REQUIRE(gb.nz == g.nz && gb.ny == g.ny && gb.nx == g.nx && gb.dtype == g.dtype,
+                "workspace bank does not match the field bank");
+  }
+  const int32_t* step0 = in.step ? static_cast<const int32_t*>(in.step->data) : nullptr;
+  std::vector<Launch> launches;
+  launches.reserve(static_cast<size_t>(in.nsteps) * 8);
+  const Kind pk = g.is3d ? Kind::kPhase3d : Kind::kPhaseTm;
+  for (int k = 0; k < in.nsteps; ++k) {
+    // bank roles: in-place uses A for everything; two banks ping-pong A->B->A
+    const bzh_buffer* const* cur = (two_banks && (k % 2 == 1)) ? in.bank_b : in.bank_a;
+    const bzh_buffer* const* nxt = two_banks ? ((k % 2 == 1) ? in.bank_a : in.bank_b) : cur;
+    PhaseIn h = in.h, e = in.e;
+    for (int c = 0; c < 3; ++c) {
+      h.tin[c] = cur[kHx + c];
+      h.tout[c] = nxt[kHx + c];
+      h.src[c] = cur[kEx + c];
+      e.tin[c] = cur[kEx + c];
+      e.tout[c] = nxt[kEx + c];
+      e.src[c] = nxt[kHx + c];
+    }
+    // pre_e sources act on the E bank read by this step
+    for (int c = 0; c < 3 && c < in.ngroups; ++c)
+      add_source_launches(launches, in.groups[c], 0, c, cur, g, in.metallic, step0, k);
+    PhaseParams hp = make_phase(h, BZH_PHASE_H, g, in.dt, in.resolution, in.metallic, in.uniform_l);
+    launches.push_back(make_launch(pk, BZH_PHASE_H, g.dtype, hp));
+    for (int c = 0; c < 3 && 3 + c < in.ngroups; ++c)
+      add_source_launches(launches, in.groups[3 + c], 1, c, nxt, g, in.metallic, step0, k);
+    PhaseParams ep = make_phase(e, BZH_PHASE_E, g, in.dt, in.resolution, in.metallic, in.uniform_l);
+    launches.push_back(make_launch(pk, BZH_PHASE_E, g.dtype, ep));
+    for (int c = 0; c < 3 && 6 + c < in.ngroups; ++c)
+      add_source_launches(launches, in.groups[6 + c], 2, c, nxt, g, in.metallic, step0, k);
+    if (in.has_monitors) {
+      DftParams dp = make_dft(in.mon, nxt, g, in.dt, in.monitor_count);
+      dp.k = k;
+      launches.push_back(make_launch(Kind::kDft, 0, g.dtype, dp));
+    }
+  }
+  graph_cache().run(s, launches, (in.flags & BZH_FLAG_GRAPH_CACHE) != 0);
+}
+
+// ---- positional decoding (ffi_handler.cc:336-708) ---------------------------------
+const bzh_buffer kEmpty = {nullptr, 1, BZH_F32, {0, 0, 0, 0}};
+
+const bzh_buffer* arg(const bzh_buffer* args, int nargs, int i) {
+  BZH_REQUIRE(i >= 0 && i < nargs, "positional argument " + std::to_string(i) +
+                                       " missing (got " + std::to_string(nargs) + ")");
+  return &args[i];
+}
+
+void decode_cpml_payload(const bzh_buffer* args, int nargs, int off, PhaseIn& ph,
+                         bool cpml) {
+  // off: first material of the phase payload
+  for (int m = 0; m < 6; ++m) ph.mat[m] = arg(args, nargs, off + m);
+  if (!cpml) {
+    ph.nterms = 0;
+    return;
+  }
+  // the CPML payload always carries 18 term arrays and 6 psi slots; a TMxy
+  // phase uses the first two terms (metadata s32[2, 5])
+  ph.meta = arg(args, nargs, off + 6);
+  BZH_REQUIRE(ph.meta->rank == 2 && (ph.meta->dims[0] == 6 || ph.meta->dims[0] == 2),
+              "term metadata must be s32[6, 5] (3D) or s32[2, 5] (TMxy)");
+  ph.nterms = static_cast<int>(ph.meta->dims[0]);
+  for (int t = 0; t < 18; ++t) ph.terms[t] = arg(args, nargs, off + 7 + t);
+  for (int t = 0; t < 6; ++t) {
+    ph.psi_in[t] = arg(args, nargs, off + 25 + t);
+    ph.psi_out[t] = nullptr;
+  }
+}
+
+void clear_phase(PhaseIn& ph) {
+  std::memset(&ph, 0, sizeof(ph));
+  for (int t = 0; t < 18; ++t) ph.terms[t] = &kEmpty;
+  for (int t = 0; t < 6; ++t) ph.psi_in[t] = ph.psi_out[t] = nullptr;
+  for (int a = 0; a < 3; ++a) ph.metric[a] = nullptr;
+}
+
+int decode_program(const bzh_buffer* args, int nargs, const bzh_program_attrs& at,
+                   ProgramIn& in) {
+  std::memset(&in, 0, sizeof(in));
+  clear_phase(in.h);
+  clear_phase(in.e);
+  in.nsteps = at.nsteps;
+  in.dt = at.dt;
+  in.resolution = at.resolution;
+  in.metallic = at.boundary_code & 0xFF;
+  in.uniform_l = at.boundary_code >> 8;
+  in.flags = at.flags;
+  in.monitor_count = at.monitor_count;
+  const int layout = at.program_layout;
+  const bool cpml = at.cpml_enabled != 0;
+  const bool temporal = layout == BZH_LAYOUT_YEE_TEMPORAL ||
+                        layout == BZH_LAYOUT_SOURCE_TEMPORAL_CPML ||
+                        layout == BZH_LAYOUT_MONITOR_TEMPORAL_CPML;
+  BZH_REQUIRE(layout >= 0 && layout <= 6, "unknown program layout");
+  if (layout == BZH_LAYOUT_CPML_IN_PLACE) BZH_REQUIRE(cpml, "layout 2 requires cpml_enabled");
+  if (layout == BZH_LAYOUT_YEE_IN_PLACE || layout == BZH_LAYOUT_YEE_TEMPORAL)
+    BZH_REQUIRE(!cpml, "Yee layouts carry no CPML payload");
+  int next = 0;
+  if (layout == BZH_LAYOUT_YEE_TEMPORAL) {
+    // fields [0-5], workspace [6-11], materials [12-23], metrics [24-29]
+    for (int c = 0; c < 6; ++c) in.bank_a[c] = arg(args, nargs, c);
+    for (int c = 0; c < 6; ++c) in.bank_b[c] = arg(args, nargs, 6 + c);
+    for (int m = 0; m < 6; ++m) in.h.mat[m] = arg(args, nargs, 12 + m);
+    for (int m = 0; m < 6; ++m) in.e.mat[m] = arg(args, nargs, 18 + m);
+    for (int a = 0; a < 3; ++a) in.h.metric[a] = arg(args, nargs, 24 + a);
+    for (int a = 0; a < 3; ++a) in.e.metric[a] = arg(args, nargs, 27 + a);
+    next = 30;
+  } else if (!cpml) {
+    // fields [0-5], H materials [6-11], E materials [12-17], metrics [18-23]
+    for (int c = 0; c < 6; ++c) in.bank_a[c] = arg(args, nargs, c);
+    for (int m = 0; m < 6; ++m) in.h.mat[m] = arg(args, nargs, 6 + m);
+    for (int m = 0; m < 6; ++m) in.e.mat[m] = arg(args, nargs, 12 + m);
+    for (int a = 0; a < 3; ++a) in.h.metric[a] = arg(args, nargs, 18 + a);
+    for (int a = 0; a < 3; ++a) in.e.metric[a] = arg(args, nargs, 21 + a);
+    next = BZH_YEE_GRAPH_INPUT_COUNT;
+    BZH_REQUIRE(!temporal, "temporal layouts 4 and 6 require CPML");
+  } else {
+    for (int c = 0; c < 6; ++c) in.bank_a[c] = arg(args, nargs, c);
+    decode_cpml_payload(args, nargs, 6, in.h, true);
+    decode_cpml_payload(args, nargs, BZH_CPML_E_INPUT_OFFSET, in.e, true);
+    for (int a = 0; a < 3; ++a) in.h.metric[a] = arg(args, nargs, BZH_CPML_H_METRIC_OFFSET + a);
+    for (int a = 0; a < 3; ++a) in.e.metric[a] = arg(args, nargs, BZH_CPML_E_METRIC_OFFSET + a);
+    next = BZH_CPML_GRAPH_INPUT_COUNT;
+    if (temporal) {
+      for (int c = 0; c < 6; ++c) in.bank_b[c] = arg(args, nargs, BZH_TEMPORAL_FIELD_WORKSPACE_INPUT + c);
+      // the per-phase schedule updates psi in place; the psi workspace
+      // (args 80-91) receives a copy of the result on odd step counts
+      next = BZH_TEMPORAL_PSI_WORKSPACE_INPUT + 12;
+    }
+  }
+  if (layout == BZH_LAYOUT_SOURCE_IN_PLACE || layout == BZH_LAYOUT_SOURCE_TEMPORAL_CPML ||
+      layout == BZH_LAYOUT_MONITOR_IN_PLACE || layout == BZH_LAYOUT_MONITOR_TEMPORAL_CPML) {
+    for (int gi = 0; gi < BZH_SOURCE_GROUP_COUNT; ++gi) {
+      in.groups[gi].coeffs = arg(args, nargs, next + 3 * gi + 0);
+      in.groups[gi].waves = arg(args, nargs, next + 3 * gi + 1);
+      in.groups[gi].starts = arg(args, nargs, next + 3 * gi + 2);
+      in.groups[gi].coincident = (at.coincident_source_group_mask >> gi) & 1;
+    }
+    in.ngroups = BZH_SOURCE_GROUP_COUNT;
+    next += 3 * BZH_SOURCE_GROUP_COUNT;
+  }
+  if (layout == BZH_LAYOUT_SOURCE_IN_PLACE || layout == BZH_LAYOUT_SOURCE_TEMPORAL_CPML) {
+    in.step = arg(args, nargs, next++);
+  }
+  if (layout == BZH_LAYOUT_MONITOR_IN_PLACE || layout == BZH_LAYOUT_MONITOR_TEMPORAL_CPML) {
+    for (int q = 0; q < 12; ++q) in.mon.b[q] = arg(args, nargs, next + q);
+    in.step = in.mon.b[11];
+    in.has_monitors = true;
+    next += 12;
+  }
+  BZH_REQUIRE(next == nargs, "layout " + std::to_string(layout) + " expects " +
+                                 std::to_string(next) + " arguments, got " + std::to_string(nargs));
+  if (in.ngroups > 0) BZH_REQUIRE(in.step && in.step->element_type == BZH_S32,
+                                  "current_step must be s32");
+  return temporal ? 1 : 0;
+}
+
+void copy_psi_workspace(hipStream_t s, const bzh_buffer* args, int nargs) {
+  // result banks of the temporal layouts: psi of an odd step count lives in
+  // the workspace (runtime.py:393-440)
+  for (int t = 0; t < 12; ++t) {
+    const bzh_buffer* src = arg(args, nargs, (t < 6 ? BZH_CPML_H_PSI_INPUT_OFFSET : BZH_CPML_E_PSI_INPUT_OFFSET) + (t % 6));
+    const bzh_buffer* dst = arg(args, nargs, BZH_TEMPORAL_PSI_WORKSPACE_INPUT + t);
+    BZH_REQUIRE(numel(*src) == numel(*dst) && src->element_type == dst->element_type,
+                "psi workspace does not match psi");
+    const size_t bytes = static_cast<size_t>(numel(*src)) * (src->element_type == BZH_BF16 ? 2 : 4);
+    if (bytes) BZH_HIP(hipMemcpyAsync(dst->data, src->data, bytes, hipMemcpyDeviceToDevice, s));
+  }
+}
+
+template <typename F>
+int32_t guarded(F&& f) {
+  clear_error();
+  try {
+    f();
+    return 0;
+  } catch (const Failure& e) {
+    set_error(e.message);
+    return 1;
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return 1;
+  } catch (...) {
+    set_error("unknown native failure");
+    return 1;
+  }
+}
+
+}  // namespace
+}  // namespace bzh
+
+using namespace bzh;
+
+extern "C" {
+
+int32_t bzh_abi_version(void) { return BZH_ABI_VERSION; }
+const char* bzh_component_version(void) { return BZH_COMPONENT_VERSION; }
+const char* bzh_targets(void) { return "beamz_hip_streamed,beamz_hip_program"; }
+const char* bzh_last_error(void) { return bzh::g_error.c_str(); }
+
+int32_t bzh_launch_phase(void* stream, const bzh_launch* launch) {
+  return guarded([&] {
+    BZH_REQUIRE(launch != nullptr, "launch descriptor is null");
+    BZH_REQUIRE(launch->abi_version == BZH_ABI_VERSION, "ABI version mismatch");
+    BZH_REQUIRE(launch->phase == BZH_PHASE_H || launch->phase == BZH_PHASE_E, "phase must be 0 (H) or 1 (E)");
+    PhaseIn ph;
+    clear_phase(ph);
+    const bzh_buffer* bank[6];
+    for (int c = 0; c < 3; ++c) {
+      ph.tin[c] = &launch->inputs[c];
+      ph.tout[c] = &launch->outputs[c];
+      ph.src[c] = &launch->inputs[3 + c];
+    }
+    for (int m = 0; m < 6; ++m) ph.mat[m] = &launch->inputs[6 + m];
+    ph.nterms = launch->nterms;
+    if (launch->nterms > 0) {
+      ph.meta = &launch->inputs[12];
+      for (int t = 0; t < 18; ++t) ph.terms[t] = &launch->inputs[13 + t];
+      for (int t = 0; t < 6; ++t) {
+        ph.psi_in[t] = &launch->inputs[31 + t];
+        ph.psi_out[t] = &launch->outputs[3 + t];
+      }
+    }
+    for (int a = 0; a < 3; ++a) ph.metric[a] = &launch->metrics[a];
+    const bool hphase = launch->phase == BZH_PHASE_H;
+    for (int c = 0; c < 3; ++c) {
+      bank[(hphase ? kHx : kEx) + c] = &launch->inputs[c];
+      bank[(hphase ? kEx : kHx) + c] = &launch->inputs[3 + c];
+    }
+    const Grid g = infer_grid(bank);
+    const PhaseParams p = make_phase(ph, launch->phase, g, launch->dt, launch->resolution,
+                                     launch->metallic_edges, launch->uniform_cpml_thickness);
+    run_phase(static_cast<hipStream_t>(stream), p, launch->phase, g);
+  });
+}
+
+int32_t bzh_streamed(void* stream, const bzh_buffer* args, int32_t nargs,
+                     const bzh_phase_attrs* attrs) {
+  return guarded([&] {
+    BZH_REQUIRE(attrs != nullptr && args != nullptr, "null arguments");
+    BZH_REQUIRE(attrs->abi_version == BZH_ABI_VERSION, "ABI version mismatch");
+    BZH_REQUIRE(attrs->phase == BZH_PHASE_H || attrs->phase == BZH_PHASE_E, "phase must be 0 (H) or 1 (E)");
+    BZH_REQUIRE(attrs->nterms == 0 || attrs->nterms == 6 || attrs->nterms == 2,
+                "nterms must be 0, 2 (TMxy) or 6 (3D)");
+    PhaseIn ph;
+    clear_phase(ph);
+    for (int c = 0; c < 3; ++c) {
+      ph.tin[c] = arg(args, nargs, c);
+      ph.tout[c] = nullptr;
+      ph.src[c] = arg(args, nargs, 3 + c);
+    }
+    for (int m = 0; m < 6; ++m) ph.mat[m] = arg(args, nargs, 6 + m);
+    int next = 12;
+    ph.nterms = attrs->nterms;
+    if (attrs->nterms > 0) {
+      ph.meta = arg(args, nargs, next++);
+      for (int t = 0; t < 3 * attrs->nterms; ++t) ph.terms[t] = arg(args, nargs, next++);
+      for (int t = 0; t < attrs->nterms; ++t) ph.psi_in[t] = arg(args, nargs, next++);
+    }
+    for (int a = 0; a < 3; ++a) ph.metric[a] = arg(args, nargs, next++);
+    BZH_REQUIRE(next == nargs, "streamed call expects " + std::to_string(next) + " arguments");
+    const bool hphase = attrs->phase == BZH_PHASE_H;
+    const bzh_buffer* bank[6];
+    for (int c = 0; c < 3; ++c) {
+      bank[(hphase ? kHx : kEx) + c] = ph.tin[c];
+      bank[(hphase ? kEx : kHx) + c] = ph.src[c];
+    }
+    const Grid g = infer_grid(bank);
+    const PhaseParams p = make_phase(ph, attrs->phase, g, attrs->dt, attrs->resolution,
+                                     attrs->boundary_code & 0xFF, attrs->boundary_code >> 8);
+    run_phase(static_cast<hipStream_t>(stream), p, attrs->phase, g);
+  });
+}
+
+int32_t bzh_program(void* stream, const bzh_buffer* args, int32_t nargs,
+                    const bzh_program_attrs* attrs) {
+  return guarded([&] {
+    BZH_REQUIRE(attrs != nullptr && args != nullptr, "null arguments");
+    BZH_REQUIRE(attrs->abi_version == BZH_ABI_VERSION, "ABI version mismatch");
+    ProgramIn in;
+    const int temporal = decode_program(args, nargs, *attrs, in);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    run_program(s, in);
+    if (temporal && attrs->cpml_enabled && (attrs->nsteps % 2 == 1)) copy_psi_workspace(s, args, nargs);
+  });
+}
+
+int32_t bzh_launch_program(void* stream, const bzh_program_launch* program) {
+  return guarded([&] {
+    BZH_REQUIRE(program != nullptr, "program descriptor is null");
+    BZH_REQUIRE(program->field_bank_count == 1 || program->field_bank_count == 2,
+                "field_bank_count must be 1 or 2");
+    ProgramIn in;
+    std::memset(&in, 0, sizeof(in));
+    clear_phase(in.h);
+    clear_phase(in.e);
+    const bzh_launch& h = program->h_ab;
+    const bzh_launch& e = program->e_ab;
+    BZH_REQUIRE(h.abi_version == BZH_ABI_VERSION && e.abi_version == BZH_ABI_VERSION,
+                "ABI version mismatch");
+    for (int c = 0; c < 3; ++c) {
+      in.bank_a[kHx + c] = &h.inputs[c];
+      in.bank_a[kEx + c] = &e.inputs[c];
+      if (program->field_bank_count == 2) {
+        in.bank_b[kHx + c] = &h.outputs[c];
+        in.bank_b[kEx + c] = &e.outputs[c];
+      }
+    }
+    auto fill = [](PhaseIn& ph, const bzh_launch& l) {
+      for (int m = 0; m < 6; ++m) ph.mat[m] = &l.inputs[6 + m];
+      ph.nterms = l.nterms;
+      if (l.nterms > 0) {
+        ph.meta = &l.inputs[12];
+        for (int t = 0; t < 18; ++t) ph.terms[t] = &l.inputs[13 + t];
+        for (int t = 0; t < 6; ++t) ph.psi_in[t] = &l.inputs[31 + t];
+      }
+      for (int a = 0; a < 3; ++a) ph.metric[a] = &l.metrics[a];
+    };
+    fill(in.h, h);
+    fill(in.e, e);
+    in.nsteps = program->nsteps;
+    in.dt = h.dt;
+    in.resolution = h.resolution;
+    in.metallic = h.metallic_edges;
+    in.uniform_l = h.uniform_cpml_thickness;
+    in.flags = h.flags;
+    BZH_REQUIRE(program->source_group_count == 0 || program->source_group_count == 9,
+                "source_group_count must be 0 or 9");
+    in.ngroups = program->source_group_count;
+    for (int gi = 0; gi < in.ngroups; ++gi) {
+      const bzh_source_group& sg = program->source_groups[gi];
+      in.groups[gi].coeffs = &sg.coefficients;
+      in.groups[gi].waves = &sg.waveforms;
+      in.groups[gi].starts = &sg.starts;
+      in.groups[gi].coincident = sg.coincident;
+      if (sg.current_step.data) in.step = &sg.current_step;
+    }
+    if (program->monitors != nullptr) {
+      const bzh_dft_group& m = *program->monitors;
+      const bzh_buffer* list[12] = {&m.indices, &m.weights, &m.frequencies, &m.component_masks,
+                                    &m.counts, &m.codes, &m.windows, &m.dft_re, &m.dft_im,
+                                    &m.dft_weight, &m.time, &m.current_step};
+      for (int q = 0; q < 12; ++q) in.mon.b[q] = list[q];
+      in.has_monitors = true;
+      in.monitor_count = m.monitor_count;
+      if (in.step == nullptr) in.step = &m.current_step;
+    }
+    run_program(static_cast<hipStream_t>(stream), in);
+  });
+}
+
+}  // extern "C"

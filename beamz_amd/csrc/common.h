@@ -1,0 +1,164 @@
+// Internal definitions shared by the gfx950 kernels and the C-ABI layer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+
+#include <cstdint>
+#include <string>
+
+#include "beamz_hip.h"
+
+namespace bzh {
+
+// ---- error reporting -------------------------------------------------------
+void set_error(const std::string& message);
+void clear_error();
+
+struct Failure {
+  std::string message;
+};
+
+#define BZH_REQUIRE(cond, msg)                                  \
+  do {                                                          \
+    if (!(cond)) throw ::bzh::Failure{std::string(msg)};        \
+  } while (0)
+
+#define BZH_HIP(expr)                                                          \
+  do {                                                                         \
+    hipError_t bzh_status_ = (expr);                                           \
+    if (bzh_status_ != hipSuccess)                                             \
+      throw ::bzh::Failure{std::string(#expr) + ": " +                         \
+                           hipGetErrorString(bzh_status_)};                    \
+  } while (0)
+
+// ---- storage types ---------------------------------------------------------
+// Fields may be stored as f32, bf16 or f16; arithmetic is always f32.
+struct bf16_t {
+  uint16_t bits;
+};
+
+__device__ __forceinline__ float load_f(const float* p, int64_t i) { return p[i]; }
+__device__ __forceinline__ float load_f(const bf16_t* p, int64_t i) {
+  return __uint_as_float(static_cast<uint32_t>(p[i].bits) << 16);
+}
+__device__ __forceinline__ float load_f(const __half* p, int64_t i) {
+  return __half2float(p[i]);
+}
+__device__ __forceinline__ void store_f(float* p, int64_t i, float v) { p[i] = v; }
+__device__ __forceinline__ void store_f(bf16_t* p, int64_t i, float v) {
+  // round to nearest even; NaN stays NaN
+  uint32_t u = __float_as_uint(v);
+  uint32_t r = (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+  if ((u & 0x7FFFFFFFu) > 0x7F800000u) r = (u >> 16) | 0x40u;
+  p[i].bits = static_cast<uint16_t>(r);
+}
+__device__ __forceinline__ void store_f(__half* p, int64_t i, float v) {
+  p[i] = __float2half(v);
+}
+
+// ---- phase parameters --------------------------------------------------------
+// One packed CPML derivative term on the device.
+struct DevTerm {
+  void* psi;          // packed recurrence memory (f32 or bf16), target shape with
+                      // the derivative axis cut to low + high (only the sum
+                      // low + high, the psi extent, is used by the kernels)
+  const float* a;     // packed along the axis, length low + high
+  const float* b;
+  const float* ik;
+  int32_t low;
+  int32_t high;
+  int32_t n;          // target extent along the axis
+  int32_t psi_bf16;   // psi stored as bf16
+};
+
+// Fields of one phase: three targets (updated in place, or written to an out
+// bank) and the three fields of the other phase.  Shapes are derived from the
+// material grid (nz, ny, nx) by the complete-Yee rule and validated on the host.
+struct PhaseParams {
+  const void* tin[3];   // targets read
+  void* tout[3];        // targets written (== tin for in-place launches)
+  const void* src[3];   // other-phase fields
+  const float* dec[3];  // decay coefficients (nullptr -> dec_s); element o * dec_st
+  const float* scf[3];  // source coefficients (nullptr -> scf_s); element o * scf_st
+  int32_t dec_st[3];    // 1 for a per-cell array, 0 for a device scalar
+  int32_t scf_st[3];
+  float dec_s[3];
+  float scf_s[3];
+  float ratio;          // target storage scale / source storage scale
+  const float* metric[3];  // (z, y, x) inverse distances (nullptr -> inv_dx)
+  int32_t met_st[3];       // 1 per-coordinate array, 0 device scalar
+  float inv_dx;
+  int32_t metallic;     // bit mask of PEC faces
+  int32_t nz, ny, nx;   // material grid
+  int32_t nterms;       // 0 or 6 (3D) / 0 or 2 (TMxy)
+  int32_t uniform_l;    // uniform CPML thickness (0 -> read metadata)
+  const int32_t* meta;  // term metadata s32[nterms, 5] (device)
+  DevTerm term[6];
+};
+
+// Face bits (runtime.py:38-40): front, back, bottom, top, left, right.
+enum : int32_t {
+  kFront = 1,
+  kBack = 2,
+  kBottom = 4,
+  kTop = 8,
+  kLeft = 16,
+  kRight = 32
+};
+
+// Launchers implemented in the .hip files.
+void launch_phase3d(hipStream_t s, const PhaseParams& p, int phase, int dtype);
+void launch_phase_tm(hipStream_t s, const PhaseParams& p, int phase, int dtype);
+
+// Slab-source group on one field component.
+struct SourceParams {
+  void* field;           // the target field (storage dtype)
+  int32_t dtype;
+  float scale;           // field storage scale
+  int32_t fdims[3];      // field shape (z, y, x); 2D uses fdims[0] = 1
+  const float* coeff;    // [n, sz, sy, sx]
+  const float* wave;     // [n, T]
+  const int32_t* starts; // [n, 3] (z, y, x)
+  int32_t n;
+  int32_t sdims[3];      // slab shape
+  int32_t T;
+  const int32_t* step0;  // device scalar: step index of step 0 of this call
+  int32_t k;             // step within this call
+  int32_t slab0;         // first slab handled by this launch
+  int32_t nslabs;        // slabs summed per cell by one thread (coincident group)
+  int32_t mlo[3];        // component masked at the low end of axis (z, y, x)
+  int32_t mhi[3];        // ... at the high end (only for post-update timings)
+};
+void launch_source(hipStream_t s, const SourceParams& p);
+
+// Packed DFT monitors.
+struct DftParams {
+  const void* field[6];  // Ex, Ey, Ez, Hx, Hy, Hz (storage dtype; nullptr absent)
+  float inv_scale[6];    // 1 / storage scale per component
+  int32_t dtype;
+  const int32_t* indices;  // [M, 6, Pmax, K]
+  const float* weights;    // [M, 6, Pmax, K]
+  const void* freqs;       // [M, Fmax]
+  int32_t freqs_f64;
+  const float* masks;      // [M, 6]
+  const int32_t* counts;   // [M, 5]
+  const int32_t* codes;    // [M, 2]
+  const float* windows;    // [M, 3]
+  void* re;
+  void* im;
+  void* wsum;
+  int32_t acc_f64;
+  const void* time;        // device scalar: t at the start of the call
+  int32_t time_f64;
+  const int32_t* step0;
+  int32_t k;
+  double dt;
+  int32_t M, Pmax, K, Fmax;
+  int32_t fsize[6];        // element count of each component (gather bound)
+  int64_t re_len;          // arena lengths (accumulation bound)
+  int64_t w_len;
+};
+void launch_dft(hipStream_t s, const DftParams& p);
+
+}  // namespace bzh

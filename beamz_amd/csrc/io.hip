@@ -1,0 +1,153 @@
+// Source injection and DFT-monitor accumulation kernels for gfx950.
+//
+//   slab sources: beamz/simulation/execute.py:374-461 (field[start:start+size]
+//                 += coeff * waveform[clip(step, 0, T-1)]); post-update timings
+//                 skip PEC-constrained cells (equivalent to re-masking,
+//                 execute.py:488-521)
+//   DFT monitors: beamz/simulation/observe.py:315-357 (gating, window),
+//                 observe.py:529-585 (weighted gather and accumulation)
+//
+// The DFT accumulators and their phase factors are float64: the phase
+// 2*pi*f*t_phys is evaluated in float64 from t_phys = t + (k + 1) dt.
+
+#include "common.h"
+
+namespace bzh {
+namespace {
+
+template <typename T>
+__global__ void __launch_bounds__(256) source_kernel(SourceParams p) {
+  const int cx = blockIdx.x * 64 + threadIdx.x;
+  const int cy = blockIdx.y * 4 + threadIdx.y;
+  const int cz = blockIdx.z;
+  if (cx >= p.sdims[2] || cy >= p.sdims[1] || cz >= p.sdims[0]) return;
+  int widx = *p.step0 + p.k;
+  widx = widx < 0 ? 0 : (widx > p.T - 1 ? p.T - 1 : widx);
+  const int slab_cells = p.sdims[0] * p.sdims[1] * p.sdims[2];
+  const int c = (cz * p.sdims[1] + cy) * p.sdims[2] + cx;
+  T* f = static_cast<T*>(p.field);
+  for (int s = p.slab0; s < p.slab0 + p.nslabs; ++s) {
+    const int z = p.starts[s * 3 + 0] + cz;
+    const int y = p.starts[s * 3 + 1] + cy;
+    const int x = p.starts[s * 3 + 2] + cx;
+    if (z < 0 || y < 0 || x < 0 || z >= p.fdims[0] || y >= p.fdims[1] || x >= p.fdims[2]) continue;
+    if ((p.mlo[0] && z == 0) || (p.mhi[0] && z == p.fdims[0] - 1) ||
+        (p.mlo[1] && y == 0) || (p.mhi[1] && y == p.fdims[1] - 1) ||
+        (p.mlo[2] && x == 0) || (p.mhi[2] && x == p.fdims[2] - 1))
+      continue;
+    const int o = (z * p.fdims[1] + y) * p.fdims[2] + x;
+    const float add = p.coeff[static_cast<int64_t>(s) * slab_cells + c] *
+                      p.wave[static_cast<int64_t>(s) * p.T + widx] * p.scale;
+    store_f(f, o, load_f(f, o) + add);
+  }
+}
+
+constexpr int kDftBlock = 128;
+constexpr int kMaxFreq = 256;
+
+template <typename T, typename A>
+__global__ void __launch_bounds__(kDftBlock) dft_kernel(DftParams p) {
+  const int m = blockIdx.z / 6;
+  const int c = blockIdx.z % 6;
+  const int P = p.counts[m * 5 + 1];
+  const int F = p.counts[m * 5 + 0];
+  const int interval = p.counts[m * 5 + 2] > 1 ? p.counts[m * 5 + 2] : 1;
+  const int64_t voff = p.counts[m * 5 + 3];
+  const int64_t woff = p.counts[m * 5 + 4];
+  const int step = *p.step0 + p.k;
+  const double t0 = p.time_f64 ? *static_cast<const double*>(p.time)
+                               : static_cast<double>(*static_cast<const float*>(p.time));
+  const double t = t0 + static_cast<double>(p.k + 1) * p.dt;
+  const double ts = p.windows[m * 3 + 0];
+  const double te = p.windows[m * 3 + 1];
+  // observe.py:315-338: gate on the time window and the sampling interval
+  if (!(t >= ts && t <= te) || (step % interval) != 0) return;
+  double window = 1.0;
+  if (p.codes[m * 2 + 0] == 1 && isfinite(te) && te > ts) {
+    double tau = (t - ts) / fmax(te - ts, 1e-30);
+    tau = fmin(fmax(tau, 0.0), 1.0);
+    window = 0.5 * (1.0 - cos(2.0 * M_PI * tau));
+  }
+  double scale = fmax(window, 0.0);
+  if (p.codes[m * 2 + 1] == 1) {
+    // observe.py:341-357 physical normalization
+    const double lu = p.windows[m * 3 + 2];
+    scale *= (p.dt * interval * 299792458.0 / lu) / sqrt(2.0 * M_PI);
+  }
+  __shared__ double s_cos[kMaxFreq];
+  __shared__ double s_sin[kMaxFreq];
+  const int nf = F < p.Fmax ? F : p.Fmax;
+  for (int f = threadIdx.x; f < nf && f < kMaxFreq; f += kDftBlock) {
+    const double fr = p.freqs_f64 ? static_cast<const double*>(p.freqs)[m * p.Fmax + f]
+                                  : static_cast<double>(static_cast<const float*>(p.freqs)[m * p.Fmax + f]);
+    double sn, cs;
+    sincos(2.0 * M_PI * fr * t, &sn, &cs);
+    s_cos[f] = cs;
+    s_sin[f] = sn;
+  }
+  __syncthreads();
+  // weight sum: one thread per monitor adds the window for each frequency
+  if (c == 0 && blockIdx.x == 0 && threadIdx.x == 0) {
+    A* w = static_cast<A*>(p.wsum);
+    for (int f = 0; f < nf; ++f)
+      if (woff + f < p.w_len) w[woff + f] += static_cast<A>(window);
+  }
+  const float cmask = p.masks[m * 6 + c];
+  const int pt = blockIdx.x * kDftBlock + threadIdx.x;
+  if (cmask == 0.0f || pt >= P || pt >= p.Pmax || p.field[c] == nullptr) return;
+  const T* fld = static_cast<const T*>(p.field[c]);
+  const int64_t base = ((static_cast<int64_t>(m) * 6 + c) * p.Pmax + pt) * p.K;
+  float v = 0.0f;
+  for (int q = 0; q < p.K; ++q) {
+    const int idx = p.indices[base + q];
+    const float w = p.weights[base + q];
+    if (w != 0.0f && idx >= 0 && idx < p.fsize[c]) v += w * load_f(fld, idx);
+  }
+  const double dv = static_cast<double>(v) * p.inv_scale[c] * cmask * scale;
+  A* re = static_cast<A*>(p.re);
+  A* im = static_cast<A*>(p.im);
+  for (int f = 0; f < nf && f < kMaxFreq; ++f) {
+    const int64_t o = voff + (static_cast<int64_t>(c) * F + f) * P + pt;
+    if (o < p.re_len) {
+      re[o] += static_cast<A>(dv * s_cos[f]);
+      im[o] += static_cast<A>(dv * s_sin[f]);
+    }
+  }
+}
+
+template <typename T>
+void dft_dispatch(hipStream_t s, const DftParams& p) {
+  dim3 block(kDftBlock, 1, 1);
+  dim3 grid((p.Pmax + kDftBlock - 1) / kDftBlock, 1, p.M * 6);
+  if (p.acc_f64) hipLaunchKernelGGL((dft_kernel<T, double>), grid, block, 0, s, p);
+  else hipLaunchKernelGGL((dft_kernel<T, float>), grid, block, 0, s, p);
+}
+
+}  // namespace
+
+void launch_source(hipStream_t s, const SourceParams& p) {
+  if (p.sdims[0] <= 0 || p.sdims[1] <= 0 || p.sdims[2] <= 0 || p.nslabs <= 0) return;
+  dim3 block(64, 4, 1);
+  dim3 grid((p.sdims[2] + 63) / 64, (p.sdims[1] + 3) / 4, p.sdims[0]);
+  switch (p.dtype) {
+    case BZH_F32: hipLaunchKernelGGL((source_kernel<float>), grid, block, 0, s, p); break;
+    case BZH_BF16: hipLaunchKernelGGL((source_kernel<bf16_t>), grid, block, 0, s, p); break;
+    case BZH_F16: hipLaunchKernelGGL((source_kernel<__half>), grid, block, 0, s, p); break;
+    default: throw Failure{"unsupported source field dtype"};
+  }
+  BZH_HIP(hipGetLastError());
+}
+
+void launch_dft(hipStream_t s, const DftParams& p) {
+  if (p.M <= 0 || p.Pmax <= 0) return;
+  BZH_REQUIRE(p.Fmax <= kMaxFreq, "DFT monitors support at most 256 frequencies");
+  switch (p.dtype) {
+    case BZH_F32: dft_dispatch<float>(s, p); break;
+    case BZH_BF16: dft_dispatch<bf16_t>(s, p); break;
+    case BZH_F16: dft_dispatch<__half>(s, p); break;
+    default: throw Failure{"unsupported DFT field dtype"};
+  }
+  BZH_HIP(hipGetLastError());
+}
+
+}  // namespace bzh

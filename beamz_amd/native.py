@@ -1,0 +1,179 @@
+"""ctypes binding of ``libbeamz_hip.so`` (declared in ``include/beamz_hip.h``).
+
+The library is the only compute path: there is no CPU or PyTorch fallback.
+When it is missing or its ABI does not match, :func:`library` raises
+:class:`HipBackendUnavailable` (the counterpart of the reference's
+``CudaBackendUnavailable``, ``beamz/simulation/backend.py:35-36``).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import torch
+
+from . import _abi as abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("BEAMZ_HIP_LIBRARY", os.path.join(HERE, "libbeamz_hip.so"))
+
+F32, S32, BF16, F16, F64 = 0, 1, 2, 3, 4
+_DTYPE_CODE = {
+    torch.float32: F32,
+    torch.int32: S32,
+    torch.bfloat16: BF16,
+    torch.float16: F16,
+    torch.float64: F64,
+}
+
+
+class HipBackendUnavailable(RuntimeError):
+    """The HIP library could not be loaded or reports another ABI."""
+
+
+class HipLaunchError(RuntimeError):
+    """A native call rejected its arguments or failed to enqueue work."""
+
+
+class Buffer(ctypes.Structure):
+    _fields_ = [
+        ("data", ctypes.c_void_p),
+        ("rank", ctypes.c_int32),
+        ("element_type", ctypes.c_int32),
+        ("dims", ctypes.c_int64 * 4),
+    ]
+
+
+class PhaseAttrs(ctypes.Structure):
+    _fields_ = [
+        ("abi_version", ctypes.c_int32),
+        ("flags", ctypes.c_int32),
+        ("phase", ctypes.c_int32),
+        ("nterms", ctypes.c_int32),
+        ("dt", ctypes.c_float),
+        ("resolution", ctypes.c_float),
+        ("boundary_code", ctypes.c_int32),
+        ("metric_kind", ctypes.c_int32),
+    ]
+
+
+class ProgramAttrs(ctypes.Structure):
+    _fields_ = [
+        ("abi_version", ctypes.c_int32),
+        ("flags", ctypes.c_int32),
+        ("nsteps", ctypes.c_int32),
+        ("dt", ctypes.c_float),
+        ("resolution", ctypes.c_float),
+        ("boundary_code", ctypes.c_int32),
+        ("metric_kind", ctypes.c_int32),
+        ("program_layout", ctypes.c_int32),
+        ("cpml_enabled", ctypes.c_int32),
+        ("monitor_count", ctypes.c_int32),
+        ("coincident_source_group_mask", ctypes.c_int32),
+    ]
+
+
+# Every exported symbol and its (restype, argtypes); checked by the tests
+# against the declarations of include/beamz_hip.h.
+SIGNATURES = {
+    "bzh_abi_version": (ctypes.c_int32, []),
+    "bzh_component_version": (ctypes.c_char_p, []),
+    "bzh_targets": (ctypes.c_char_p, []),
+    "bzh_last_error": (ctypes.c_char_p, []),
+    "bzh_launch_phase": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p]),
+    "bzh_launch_program": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p]),
+    "bzh_streamed": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(Buffer), ctypes.c_int32,
+                                      ctypes.POINTER(PhaseAttrs)]),
+    "bzh_program": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(Buffer), ctypes.c_int32,
+                                     ctypes.POINTER(ProgramAttrs)]),
+}
+
+_LOCK = threading.Lock()
+_LIB = None
+
+
+def library():
+    """Load the library once and check its ABI version."""
+    global _LIB
+    with _LOCK:
+        if _LIB is not None:
+            return _LIB
+        if not os.path.exists(LIB_PATH):
+            raise HipBackendUnavailable(
+                f"{LIB_PATH} is missing; build it with `python -m beamz_amd.build`")
+        try:
+            lib = ctypes.CDLL(LIB_PATH)
+        except OSError as exc:
+            raise HipBackendUnavailable(f"cannot load {LIB_PATH}: {exc}") from exc
+        for name, (restype, argtypes) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = restype
+            fn.argtypes = argtypes
+        version = int(lib.bzh_abi_version())
+        if version != abi.ABI_VERSION:
+            raise HipBackendUnavailable(
+                f"libbeamz_hip ABI {version} does not match the host ABI {abi.ABI_VERSION}")
+        _LIB = lib
+        return lib
+
+
+def status():
+    """``(available, reason)`` without raising."""
+    try:
+        library()
+    except HipBackendUnavailable as exc:
+        return False, str(exc)
+    return True, ""
+
+
+def buffer(t) -> Buffer:
+    """Describe a contiguous tensor (or ``None`` for an empty slot)."""
+    b = Buffer()
+    if t is None:
+        b.data = None
+        b.rank = 1
+        b.element_type = F32
+        b.dims[0] = 0
+        return b
+    if not t.is_contiguous():
+        raise HipLaunchError("native buffers must be contiguous")
+    if t.dim() > 4:
+        raise HipLaunchError("native buffers have rank <= 4")
+    b.data = t.data_ptr() if t.numel() else None
+    b.rank = t.dim()
+    b.element_type = _DTYPE_CODE[t.dtype]
+    for d, n in enumerate(t.shape):
+        b.dims[d] = int(n)
+    return b
+
+
+def buffers(tensors):
+    arr = (Buffer * len(tensors))()
+    for i, t in enumerate(tensors):
+        arr[i] = buffer(t)
+    return arr
+
+
+def _check(rc, lib):
+    if rc != 0:
+        raise HipLaunchError(lib.bzh_last_error().decode())
+
+
+def current_stream_handle():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def call_streamed(args, attrs: PhaseAttrs, stream=None):
+    lib = library()
+    arr = buffers(args)
+    s = current_stream_handle() if stream is None else stream
+    _check(lib.bzh_streamed(s, arr, len(args), ctypes.byref(attrs)), lib)
+
+
+def call_program(args, attrs: ProgramAttrs, stream=None):
+    lib = library()
+    arr = buffers(args)
+    s = current_stream_handle() if stream is None else stream
+    _check(lib.bzh_program(s, arr, len(args), ctypes.byref(attrs)), lib)

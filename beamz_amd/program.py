@@ -1,0 +1,240 @@
+"""Host-side lowering of a simulation into an immutable device program and the
+initial state.
+
+References: ``beamz/simulation/compile.py:540-793`` (``compile_simulation``),
+``:126-143`` (uniform-coefficient elision), ``:563-733`` (coefficients),
+``beamz/simulation/kernels.py:704-728`` (update coefficients),
+``beamz/simulation/model.py:127-324`` (state, coefficients, CPML terms),
+``beamz/simulation/execute.py:814-905`` (initial state).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import NamedTuple
+
+import numpy as np
+import torch
+
+from . import _abi as abi
+from .boundaries import (PML, edge_mask, extrude_through_cpml, lower_cpml,
+                         resolve_metallic_edges)
+from .constants import EPS_0, MU_0
+from .lattice import component_shapes, sample_e_site_3d, sample_site_tm
+from .monitors import assign_offsets, lower_monitor, pack_monitors
+from .sources import CustomSource, SoftSource, group_sources, lower_soft_source
+
+_COMPONENT_CODE = {"Hx": 0, "Hy": 1, "Hz": 2, "Ex": 0, "Ey": 1, "Ez": 2}
+
+
+class SimulationState(NamedTuple):
+    """The step carry (``model.py:127-257``) as device tensors, updated in place
+    by the HIP library.  2D TMxy states hold empty ``hz``, ``ex``, ``ey``."""
+
+    hx: torch.Tensor
+    hy: torch.Tensor
+    hz: torch.Tensor
+    ex: torch.Tensor
+    ey: torch.Tensor
+    ez: torch.Tensor
+    psi_h: tuple
+    psi_e: tuple
+    dft_re: torch.Tensor
+    dft_im: torch.Tensor
+    dft_w: torch.Tensor
+    t: torch.Tensor
+    current_step: torch.Tensor
+
+    def fields(self):
+        return (self.hx, self.hy, self.hz, self.ex, self.ey, self.ez)
+
+
+@dataclass
+class CompiledProgram:
+    grid: tuple
+    is_3d: bool
+    resolution: float
+    dt: float
+    t0: float
+    num_steps: int
+    shapes: dict
+    metallic: frozenset
+    boundary_code: int
+    cpml: object
+    coeffs: dict
+    h_meta: torch.Tensor
+    e_meta: torch.Tensor
+    h_term_arrays: tuple
+    e_term_arrays: tuple
+    source_groups: list
+    coincident_mask: int
+    monitors: list
+    packed_monitors: tuple
+    device: torch.device
+    field_dtype: torch.dtype
+    flags: int
+    eps_cells: np.ndarray = field(repr=False, default=None)
+    eps_site: dict = field(repr=False, default=None)
+
+    @property
+    def cells(self) -> int:
+        """Material cells: the GCUPS numerator (``execute.py:1111-1118``)."""
+        return int(np.prod(self.grid))
+
+    @property
+    def cpml_enabled(self) -> bool:
+        return self.cpml is not None
+
+    @property
+    def nterms(self) -> int:
+        return 0 if self.cpml is None else len(self.cpml.h_terms)
+
+
+def _dev(x, device, dtype=torch.float32):
+    return torch.as_tensor(np.asarray(x), dtype=dtype).to(device).contiguous()
+
+
+def _elide(arr, device):
+    """A uniform coefficient becomes a rank-0 tensor (``compile.py:137-143``)."""
+    a = np.asarray(arr, dtype=np.float64)
+    a32 = a.astype(np.float32)
+    if a32.size and np.all(a32 == a32.flat[0]):
+        return torch.tensor(float(a32.flat[0]), dtype=torch.float32, device=device)
+    return _dev(a32, device)
+
+
+def compile_program(permittivity, resolution, dt, *, boundaries=(), sources=(), monitors=(),
+                    num_steps=1, t0=0.0, conductivity=None, device=None,
+                    field_dtype=torch.float32, flags=abi.DEFAULT_FLAGS):
+    """Lower ``(permittivity raster, dx, dt, boundaries, sources, monitors)``."""
+    device = torch.device(device or ("cuda" if torch.cuda.is_available() else "cpu"))
+    eps = np.asarray(permittivity, dtype=np.float64)
+    if eps.ndim not in (2, 3):
+        raise ValueError("permittivity must be a 2D (y, x) or 3D (z, y, x) raster")
+    is_3d = eps.ndim == 3
+    grid = tuple(int(v) for v in eps.shape)
+    shapes = component_shapes(grid)
+    sigma = np.zeros(grid) if conductivity is None else np.broadcast_to(
+        np.asarray(conductivity, dtype=np.float64), grid).copy()
+    pmls = [b for b in boundaries if isinstance(b, PML)]
+    if len(pmls) > 1:
+        raise NotImplementedError("one PML specification per simulation")
+    cpml = lower_cpml(pmls[0], grid, shapes, resolution, dt) if pmls else None
+    if cpml is not None:
+        eps = extrude_through_cpml(eps, cpml.cells, cpml.faces, is_3d)
+        sigma = extrude_through_cpml(sigma, cpml.cells, cpml.faces, is_3d)
+    metallic = resolve_metallic_edges(boundaries, is_3d)
+
+    # update coefficients (kernels.py:704-728), evaluated in float64
+    coeffs = {}
+    eps_site = {}
+    e_comps = ("Ex", "Ey", "Ez") if is_3d else ("Ez",)
+    for ax in "xyz":
+        comp = "E" + ax
+        if comp in e_comps:
+            es = sample_e_site_3d(eps, comp) if is_3d else sample_site_tm(eps, comp)
+            ss = sample_e_site_3d(sigma, comp) if is_3d else sample_site_tm(sigma, comp)
+            beta = ss * dt / (2.0 * EPS_0 * es)
+            eps_site[comp] = es
+            coeffs[f"e_decay_{ax}"] = _elide((1.0 - beta) / (1.0 + beta), device)
+            coeffs[f"e_source_{ax}"] = _elide((dt / (EPS_0 * es)) / (1.0 + beta), device)
+        else:
+            coeffs[f"e_decay_{ax}"] = torch.tensor(1.0, device=device)
+            coeffs[f"e_source_{ax}"] = torch.tensor(0.0, device=device)
+        coeffs[f"h_decay_{ax}"] = torch.tensor(1.0, device=device)
+        coeffs[f"h_source_{ax}"] = torch.tensor(float(np.float32(dt / MU_0)), device=device)
+
+    # CPML terms (compile.py:241-305) -> metadata and packed coefficient arrays
+    def term_payload(terms):
+        if not terms:
+            return torch.zeros((0, 5), dtype=torch.int32, device=device), ()
+        meta = np.asarray([(_COMPONENT_CODE[t.component], t.axis, t.low, t.high,
+                            1 if t.sign > 0 else -1) for t in terms], dtype=np.int32)
+        arrays = []
+        for t in terms:
+            arrays += [_dev(t.a, device), _dev(t.b, device), _dev(t.inv_kappa, device)]
+        while len(arrays) < 18:
+            arrays.append(torch.zeros(0, device=device))
+        return _dev(meta, device, torch.int32), tuple(arrays)
+
+    h_meta, h_arrays = term_payload(cpml.h_terms if cpml else ())
+    e_meta, e_arrays = term_payload(cpml.e_terms if cpml else ())
+    thickness = 0
+    if cpml is not None:
+        allterms = cpml.h_terms + cpml.e_terms
+        first = allterms[0].low
+        if first > 0 and all(t.low == first and t.high == first for t in allterms):
+            thickness = first
+    boundary_code = edge_mask(metallic) | (thickness << 8)
+
+    # sources -> nine (timing, component) groups
+    customs = []
+    for s in sources:
+        if isinstance(s, SoftSource):
+            customs.append(lower_soft_source(s, eps_site[s.component], dt, t0, num_steps))
+        elif isinstance(s, CustomSource):
+            customs.append(s)
+        else:
+            raise TypeError(f"unsupported source {type(s).__name__}")
+    groups = []
+    coincident = 0
+    for gi, g in enumerate(group_sources(customs, is_3d)):
+        if g is None:
+            groups.append(None)
+            continue
+        c, w, st, co = g
+        groups.append((_dev(c, device), _dev(w, device), _dev(st, device, torch.int32)))
+        coincident |= (1 << gi) if co else 0
+
+    compiled = [lower_monitor(m, grid, shapes, resolution, i) for i, m in enumerate(monitors)]
+    assign_offsets(compiled)
+    packed = ()
+    if compiled:
+        idx, wts, fr, masks, counts, codes, windows = pack_monitors(compiled)
+        packed = (_dev(idx, device, torch.int32), _dev(wts, device), _dev(fr, device, torch.float64),
+                  _dev(masks, device), _dev(counts, device, torch.int32),
+                  _dev(codes, device, torch.int32), _dev(windows, device))
+
+    return CompiledProgram(
+        grid=grid, is_3d=is_3d, resolution=float(resolution), dt=float(dt), t0=float(t0),
+        num_steps=int(num_steps), shapes=shapes, metallic=metallic,
+        boundary_code=boundary_code, cpml=cpml, coeffs=coeffs, h_meta=h_meta, e_meta=e_meta,
+        h_term_arrays=h_arrays, e_term_arrays=e_arrays, source_groups=groups,
+        coincident_mask=coincident, monitors=compiled, packed_monitors=packed,
+        device=device, field_dtype=field_dtype, flags=int(flags), eps_cells=eps,
+        eps_site=eps_site)
+
+
+def initial_state(prog: CompiledProgram, fields=None) -> SimulationState:
+    """Zero fields and recurrences (``execute.py:814-905``); ``fields`` may give
+    initial values by lower-case component name."""
+    dev, dt_ = prog.device, prog.field_dtype
+    out = {}
+    for c in ("Hx", "Hy", "Hz", "Ex", "Ey", "Ez"):
+        if c in prog.shapes:
+            out[c.lower()] = torch.zeros(prog.shapes[c], dtype=dt_, device=dev)
+        else:
+            out[c.lower()] = torch.zeros(0, dtype=dt_, device=dev)
+    if fields:
+        for k, v in fields.items():
+            out[k].copy_(torch.as_tensor(np.asarray(v), dtype=dt_))
+
+    def psi(terms):
+        res = []
+        for t in terms:
+            shape = list(prog.shapes[t.component])
+            shape[t.axis] = t.low + t.high
+            res.append(torch.zeros(shape, dtype=torch.float32, device=dev))
+        return tuple(res)
+
+    psi_h = psi(prog.cpml.h_terms) if prog.cpml else ()
+    psi_e = psi(prog.cpml.e_terms) if prog.cpml else ()
+    nv = sum(6 * m.freqs.size * m.points for m in prog.monitors)
+    nw = sum(m.freqs.size for m in prog.monitors)
+    return SimulationState(
+        **out, psi_h=psi_h, psi_e=psi_e,
+        dft_re=torch.zeros(nv, dtype=torch.float64, device=dev),
+        dft_im=torch.zeros(nv, dtype=torch.float64, device=dev),
+        dft_w=torch.zeros(nw, dtype=torch.float64, device=dev),
+        t=torch.tensor(prog.t0, dtype=torch.float64, device=dev),
+        current_step=torch.tensor(0, dtype=torch.int32, device=dev))

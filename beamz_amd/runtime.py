@@ -1,0 +1,165 @@
+"""Lower Yee phases and multi-step programs to the HIP library's positional
+entry points.
+
+This module mirrors ``beamz/simulation/cuda/runtime.py`` function for function:
+``update_h``/``update_e`` (``runtime.py:184-280``, one phase through
+``bzh_streamed``), ``run_steps`` (``:685-750``), ``run_source_group_steps``
+(``:496-527``) and ``run_program_steps`` (``:621-682``) through
+``bzh_program`` with the same positional layouts (``abi_layout.json``).  State
+tensors are updated in place (the reference's ``input_output_aliases``) and
+the same state object is returned.
+"""
+
+from __future__ import annotations
+
+import torch
+
+from . import _abi as abi
+from . import native
+from .program import CompiledProgram, SimulationState
+
+_EMPTY_GROUP = None
+
+
+def _empty(device):
+    return torch.zeros(0, dtype=torch.float32, device=device)
+
+
+def _metrics(prog):
+    # isotropic uniform grids: no metric arrays (metric_kind 0)
+    e = _empty(prog.device)
+    return (e, e, e)
+
+
+def _materials(prog, phase):
+    c = prog.coeffs
+    p = "h" if phase == abi.PHASE_H else "e"
+    return (c[f"{p}_decay_x"], c[f"{p}_decay_y"], c[f"{p}_decay_z"],
+            c[f"{p}_source_x"], c[f"{p}_source_y"], c[f"{p}_source_z"])
+
+
+def _psi_slots(psi, device):
+    slots = list(psi)
+    while len(slots) < abi.CPML_TERM_COUNT:
+        slots.append(_empty(device))
+    return tuple(slots)
+
+
+def _cpml_payload(prog, state, phase):
+    if phase == abi.PHASE_H:
+        return (*_materials(prog, phase), prog.h_meta, *prog.h_term_arrays,
+                *_psi_slots(state.psi_h, prog.device))
+    return (*_materials(prog, phase), prog.e_meta, *prog.e_term_arrays,
+            *_psi_slots(state.psi_e, prog.device))
+
+
+def _graph_io(prog: CompiledProgram, state: SimulationState):
+    """Positional inputs of the 24-input (Yee) or 74-input (CPML) graph
+    (``runtime.py:284-381``)."""
+    if prog.cpml_enabled:
+        return (*state.fields(), *_cpml_payload(prog, state, abi.PHASE_H),
+                *_cpml_payload(prog, state, abi.PHASE_E), *_metrics(prog), *_metrics(prog))
+    return (*state.fields(), *_materials(prog, abi.PHASE_H), *_materials(prog, abi.PHASE_E),
+            *_metrics(prog), *_metrics(prog))
+
+
+def _source_group_arguments(prog):
+    out = []
+    for g in prog.source_groups:
+        if g is None:
+            dev = prog.device
+            out += [torch.zeros((0, 1, 1, 1), device=dev), torch.zeros((0, 1), device=dev),
+                    torch.zeros((0, 3), dtype=torch.int32, device=dev)]
+        else:
+            out += list(g)
+    return tuple(out)
+
+
+def _program_attrs(prog, nsteps, layout, monitor_count=0):
+    return native.ProgramAttrs(
+        abi_version=abi.ABI_VERSION, flags=prog.flags, nsteps=int(nsteps), dt=prog.dt,
+        resolution=prog.resolution, boundary_code=prog.boundary_code, metric_kind=0,
+        program_layout=layout, cpml_enabled=int(prog.cpml_enabled),
+        monitor_count=int(monitor_count), coincident_source_group_mask=prog.coincident_mask)
+
+
+def _phase(prog, state, phase):
+    if phase == abi.PHASE_H:
+        targets, sources = (state.hx, state.hy, state.hz), (state.ex, state.ey, state.ez)
+    else:
+        targets, sources = (state.ex, state.ey, state.ez), (state.hx, state.hy, state.hz)
+    args = [*targets, *sources, *_materials(prog, phase)]
+    if prog.cpml_enabled:
+        meta = prog.h_meta if phase == abi.PHASE_H else prog.e_meta
+        arrays = prog.h_term_arrays if phase == abi.PHASE_H else prog.e_term_arrays
+        psi = state.psi_h if phase == abi.PHASE_H else state.psi_e
+        n = prog.nterms
+        args += [meta, *arrays[:3 * n], *psi[:n]]
+    args += list(_metrics(prog))
+    attrs = native.PhaseAttrs(
+        abi_version=abi.ABI_VERSION, flags=prog.flags, phase=phase, nterms=prog.nterms,
+        dt=prog.dt, resolution=prog.resolution, boundary_code=prog.boundary_code, metric_kind=0)
+    native.call_streamed(args, attrs)
+    return state
+
+
+def update_h(state: SimulationState, prog: CompiledProgram) -> SimulationState:
+    """Advance the magnetic fields and their CPML memory (``runtime.py:184-231``)."""
+    return _phase(prog, state, abi.PHASE_H)
+
+
+def update_e(state: SimulationState, prog: CompiledProgram) -> SimulationState:
+    """Advance the electric fields and their CPML memory (``runtime.py:234-280``)."""
+    return _phase(prog, state, abi.PHASE_E)
+
+
+def run_steps(state, prog, nsteps):
+    """Source-free, monitor-free steps (``runtime.py:685-750``)."""
+    layout = abi.PROGRAM_LAYOUT_CPML_IN_PLACE if prog.cpml_enabled else abi.PROGRAM_LAYOUT_YEE_IN_PLACE
+    native.call_program(_graph_io(prog, state), _program_attrs(prog, nsteps, layout))
+    return state
+
+
+def run_source_group_steps(state, prog, nsteps):
+    """Steps with the nine slab-source groups (``runtime.py:496-527``)."""
+    args = (*_graph_io(prog, state), *_source_group_arguments(prog), state.current_step)
+    native.call_program(args, _program_attrs(prog, nsteps, abi.PROGRAM_LAYOUT_SOURCE_IN_PLACE))
+    return state
+
+
+def run_program_steps(state, prog, nsteps):
+    """Steps with sources and packed DFT monitors (``runtime.py:621-682``)."""
+    if state.dft_re.dtype != torch.float64:
+        raise ValueError("the HIP program accumulates DFT monitors in float64")
+    args = (*_graph_io(prog, state), *_source_group_arguments(prog), *prog.packed_monitors,
+            state.dft_re, state.dft_im, state.dft_w, state.t, state.current_step)
+    native.call_program(args, _program_attrs(prog, nsteps, abi.PROGRAM_LAYOUT_MONITOR_IN_PLACE,
+                                             monitor_count=len(prog.monitors)))
+    return state
+
+
+def run_chunk(state, prog, nsteps):
+    """Pick the layout the program needs (``execute.py:595-681``)."""
+    if prog.monitors:
+        return run_program_steps(state, prog, nsteps)
+    if any(g is not None for g in prog.source_groups):
+        return run_source_group_steps(state, prog, nsteps)
+    return run_steps(state, prog, nsteps)
+
+
+GRAPH_MAX_STEPS = 256  # execute.py:64
+
+
+def advance(state, prog, nsteps, *, chunk=GRAPH_MAX_STEPS):
+    """Advance ``nsteps`` in chunks of at most ``chunk`` steps; the clocks are
+    recomputed from the run origin after every chunk (``execute.py:622-657``)."""
+    done = int(state.current_step.item()) if nsteps > 0 else 0
+    remaining = int(nsteps)
+    while remaining > 0:
+        n = min(chunk, remaining)
+        run_chunk(state, prog, n)
+        done += n
+        remaining -= n
+        state.current_step.fill_(done)
+        state.t.fill_(prog.t0 + done * prog.dt)
+    return state
