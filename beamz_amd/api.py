@@ -96,7 +96,7 @@ class Simulation:
     def __init__(self, permittivity, resolution, *, dt=None, courant=0.99, boundaries=(),
                  sources=(), monitors=(), steps=1, t0=0.0, conductivity=None,
                  backend="auto", field_dtype=torch.float32, device=None,
-                 flags=abi.DEFAULT_FLAGS):
+                 flags=abi.DEFAULT_FLAGS | abi.FLAG_FUSED_STEP):
         self.permittivity = np.asarray(permittivity, dtype=np.float64)
         self.resolution = float(resolution)
         self.dt = float(dt) if dt is not None else courant_dt(resolution, self.permittivity.ndim, courant)
@@ -135,7 +135,7 @@ class Simulation:
         state = self.initial_state() if state is None else state
         torch.cuda.synchronize()
         t0 = _time.perf_counter()
-        runtime.advance(state, prog, self.steps, chunk=chunk)
+        state = runtime.advance(state, prog, self.steps, chunk=chunk)
         torch.cuda.synchronize()
         perf = SimulationPerformance(prog.cells, self.steps, _time.perf_counter() - t0)
         return SimulationResults(state, prog, perf)

@@ -10,6 +10,8 @@
 // The per-step order is execute.py:467-541: pre_e sources, H, h sources,
 // (H masks, folded into the kernels), E, e sources, (E masks), monitors.
 
+#include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -355,8 +357,12 @@ void enqueue(hipStream_t s, const Launch& l) {
       launch_dft(s, p);
       break;
     }
-    case Kind::kFused:
-      throw Failure{"fused schedule not available"};
+    case Kind::kFused: {
+      FusedParams p;
+      std::memcpy(&p, l.bytes.data(), sizeof(p));
+      launch_fused3d(s, p, l.dtype);
+      break;
+    }
   }
 }
 
@@ -595,6 +601,7 @@ DftParams make_dft(const MonitorIn& in, const bzh_buffer* const* bank, const Gri
 struct ProgramIn {
   const bzh_buffer* bank_a[6];   // Hx Hy Hz Ex Ey Ez
   const bzh_buffer* bank_b[6];   // workspace (temporal layouts) or nullptr
+  const bzh_buffer* psi_b[12];   // psi workspace (H terms then E terms) or nullptr
   PhaseIn h;                     // targets/sources bound per step below
   PhaseIn e;
   SourceGroupIn groups[9];
@@ -609,7 +616,33 @@ struct ProgramIn {
   int flags;
 };
 
-void run_program(hipStream_t s, ProgramIn& in) {
+// z-chunking of the fused kernel: enough workgroups to fill 256 CUs several
+// times over, chunks of at most 32 planes (the prologue recomputes one plane).
+void fused_tiling(const Grid& g, FusedParams& fp) {
+  fp.tiles_x = (g.nx + 1 + 63) / 64;
+  fp.tiles_y = (g.ny + 1 + 3) / 4;
+  const int planes = g.nz + 1;
+  const int tiles = fp.tiles_x * fp.tiles_y;
+  int kz = 32;
+  if (const char* env = std::getenv("BZH_FUSED_KZ")) {
+    kz = std::max(1, std::atoi(env));  // explicit chunk length (tuning and tests)
+  } else {
+    while (kz > 4 && static_cast<int64_t>(tiles) * ((planes + kz - 1) / kz) < 4096) kz /= 2;
+  }
+  const int nzc = (planes + kz - 1) / kz;
+  fp.kz = (planes + nzc - 1) / nzc;
+  fp.nzc = (planes + fp.kz - 1) / fp.kz;
+}
+
+bool metrics_empty(const PhaseIn& ph) {
+  for (int a = 0; a < 3; ++a)
+    if (ph.metric[a] && ph.metric[a]->data && numel(*ph.metric[a]) > 0) return false;
+  return true;
+}
+
+// Returns true when the fused two-bank kernel carried the whole schedule (psi
+// then ping-pongs with the fields); false for the per-phase schedule.
+bool run_program(hipStream_t s, ProgramIn& in) {
   BZH_REQUIRE(in.nsteps >= 1, "step count must be positive");
   Grid g = infer_grid(in.bank_a);
   const bool two_banks = in.bank_b[0] != nullptr;
@@ -618,14 +651,23 @@ void run_program(hipStream_t s, ProgramIn& in) {
     BZH_REQUIRE(gb.nz == g.nz && gb.ny == g.ny && gb.nx == g.nx && gb.dtype == g.dtype,
                 "workspace bank does not match the field bank");
   }
+  bool h_sources = false;
+  for (int c = 0; c < 3 && 3 + c < in.ngroups; ++c) {
+    const bzh_buffer* cb = in.groups[3 + c].coeffs;
+    if (cb && cb->data && cb->rank >= 1 && cb->dims[0] > 0) h_sources = true;
+  }
+  const bool psi_ws = in.h.nterms == 0 || (in.psi_b[0] != nullptr);
+  const bool fused = two_banks && (in.flags & BZH_FLAG_FUSED_STEP) && g.is3d && !h_sources &&
+                     metrics_empty(in.h) && metrics_empty(in.e) && psi_ws;
   const int32_t* step0 = in.step ? static_cast<const int32_t*>(in.step->data) : nullptr;
   std::vector<Launch> launches;
   launches.reserve(static_cast<size_t>(in.nsteps) * 8);
   const Kind pk = g.is3d ? Kind::kPhase3d : Kind::kPhaseTm;
   for (int k = 0; k < in.nsteps; ++k) {
     // bank roles: in-place uses A for everything; two banks ping-pong A->B->A
-    const bzh_buffer* const* cur = (two_banks && (k % 2 == 1)) ? in.bank_b : in.bank_a;
-    const bzh_buffer* const* nxt = two_banks ? ((k % 2 == 1) ? in.bank_a : in.bank_b) : cur;
+    const bool odd = two_banks && (k % 2 == 1);
+    const bzh_buffer* const* cur = odd ? in.bank_b : in.bank_a;
+    const bzh_buffer* const* nxt = two_banks ? (odd ? in.bank_a : in.bank_b) : cur;
     PhaseIn h = in.h, e = in.e;
     for (int c = 0; c < 3; ++c) {
       h.tin[c] = cur[kHx + c];
@@ -635,15 +677,43 @@ void run_program(hipStream_t s, ProgramIn& in) {
       e.tout[c] = nxt[kEx + c];
       e.src[c] = nxt[kHx + c];
     }
+    if (fused && in.h.nterms > 0) {
+      for (int t = 0; t < 6; ++t) {
+        if (odd) {
+          h.psi_in[t] = in.psi_b[t];
+          e.psi_in[t] = in.psi_b[6 + t];
+        }
+        h.psi_out[t] = nullptr;
+        e.psi_out[t] = nullptr;
+      }
+    }
     // pre_e sources act on the E bank read by this step
     for (int c = 0; c < 3 && c < in.ngroups; ++c)
       add_source_launches(launches, in.groups[c], 0, c, cur, g, in.metallic, step0, k);
-    PhaseParams hp = make_phase(h, BZH_PHASE_H, g, in.dt, in.resolution, in.metallic, in.uniform_l);
-    launches.push_back(make_launch(pk, BZH_PHASE_H, g.dtype, hp));
-    for (int c = 0; c < 3 && 3 + c < in.ngroups; ++c)
-      add_source_launches(launches, in.groups[3 + c], 1, c, nxt, g, in.metallic, step0, k);
-    PhaseParams ep = make_phase(e, BZH_PHASE_E, g, in.dt, in.resolution, in.metallic, in.uniform_l);
-    launches.push_back(make_launch(pk, BZH_PHASE_E, g.dtype, ep));
+    if (fused) {
+      FusedParams fp;
+      std::memset(&fp, 0, sizeof(fp));
+      fp.h = make_phase(h, BZH_PHASE_H, g, in.dt, in.resolution, in.metallic, in.uniform_l);
+      fp.e = make_phase(e, BZH_PHASE_E, g, in.dt, in.resolution, in.metallic, in.uniform_l);
+      for (int t = 0; t < 6 && in.h.nterms > 0; ++t) {
+        const bzh_buffer* ho = odd ? in.h.psi_in[t] : in.psi_b[t];
+        const bzh_buffer* eo = odd ? in.e.psi_in[t] : in.psi_b[6 + t];
+        BZH_REQUIRE(numel(*ho) == numel(*h.psi_in[t]) && ho->element_type == h.psi_in[t]->element_type &&
+                        numel(*eo) == numel(*e.psi_in[t]) && eo->element_type == e.psi_in[t]->element_type,
+                    "psi workspace does not match psi");
+        fp.psi_h_out[t] = ho->data;
+        fp.psi_e_out[t] = eo->data;
+      }
+      fused_tiling(g, fp);
+      launches.push_back(make_launch(Kind::kFused, 0, g.dtype, fp));
+    } else {
+      PhaseParams hp = make_phase(h, BZH_PHASE_H, g, in.dt, in.resolution, in.metallic, in.uniform_l);
+      launches.push_back(make_launch(pk, BZH_PHASE_H, g.dtype, hp));
+      for (int c = 0; c < 3 && 3 + c < in.ngroups; ++c)
+        add_source_launches(launches, in.groups[3 + c], 1, c, nxt, g, in.metallic, step0, k);
+      PhaseParams ep = make_phase(e, BZH_PHASE_E, g, in.dt, in.resolution, in.metallic, in.uniform_l);
+      launches.push_back(make_launch(pk, BZH_PHASE_E, g.dtype, ep));
+    }
     for (int c = 0; c < 3 && 6 + c < in.ngroups; ++c)
       add_source_launches(launches, in.groups[6 + c], 2, c, nxt, g, in.metallic, step0, k);
     if (in.has_monitors) {
@@ -653,6 +723,7 @@ void run_program(hipStream_t s, ProgramIn& in) {
     }
   }
   graph_cache().run(s, launches, (in.flags & BZH_FLAG_GRAPH_CACHE) != 0);
+  return fused;
 }
 
 // ---- positional decoding (ffi_handler.cc:336-708) ---------------------------------
@@ -731,7 +802,12 @@ int decode_program(const bzh_buffer* args, int nargs, const bzh_program_attrs& a
     for (int a = 0; a < 3; ++a) in.h.metric[a] = arg(args, nargs, 18 + a);
     for (int a = 0; a < 3; ++a) in.e.metric[a] = arg(args, nargs, 21 + a);
     next = BZH_YEE_GRAPH_INPUT_COUNT;
-    BZH_REQUIRE(!temporal, "temporal layouts 4 and 6 require CPML");
+    if (temporal) {
+      // extension: layouts 4 and 6 without CPML take the 24-input Yee graph
+      // followed by the six-field workspace bank
+      for (int c = 0; c < 6; ++c) in.bank_b[c] = arg(args, nargs, next + c);
+      next += 6;
+    }
   } else {
     for (int c = 0; c < 6; ++c) in.bank_a[c] = arg(args, nargs, c);
     decode_cpml_payload(args, nargs, 6, in.h, true);
@@ -741,8 +817,10 @@ int decode_program(const bzh_buffer* args, int nargs, const bzh_program_attrs& a
     next = BZH_CPML_GRAPH_INPUT_COUNT;
     if (temporal) {
       for (int c = 0; c < 6; ++c) in.bank_b[c] = arg(args, nargs, BZH_TEMPORAL_FIELD_WORKSPACE_INPUT + c);
-      // the per-phase schedule updates psi in place; the psi workspace
-      // (args 80-91) receives a copy of the result on odd step counts
+      // psi workspace (args 80-91: six H terms, then six E terms).  The fused
+      // schedule ping-pongs psi with the fields; the per-phase schedule
+      // updates bank A in place and copies it here after an odd step count.
+      for (int t = 0; t < 12; ++t) in.psi_b[t] = arg(args, nargs, BZH_TEMPORAL_PSI_WORKSPACE_INPUT + t);
       next = BZH_TEMPORAL_PSI_WORKSPACE_INPUT + 12;
     }
   }
@@ -898,8 +976,9 @@ int32_t bzh_program(void* stream, const bzh_buffer* args, int32_t nargs,
     ProgramIn in;
     const int temporal = decode_program(args, nargs, *attrs, in);
     hipStream_t s = static_cast<hipStream_t>(stream);
-    run_program(s, in);
-    if (temporal && attrs->cpml_enabled && (attrs->nsteps % 2 == 1)) copy_psi_workspace(s, args, nargs);
+    const bool fused = run_program(s, in);
+    if (temporal && attrs->cpml_enabled && !fused && (attrs->nsteps % 2 == 1))
+      copy_psi_workspace(s, args, nargs);
   });
 }
 
@@ -922,6 +1001,13 @@ int32_t bzh_launch_program(void* stream, const bzh_program_launch* program) {
       if (program->field_bank_count == 2) {
         in.bank_b[kHx + c] = &h.outputs[c];
         in.bank_b[kEx + c] = &e.outputs[c];
+      }
+    }
+    if (program->field_bank_count == 2 && h.nterms > 0) {
+      // two banks: outputs[3..8] of the *_ab launches are the psi workspace
+      for (int t = 0; t < 6; ++t) {
+        in.psi_b[t] = &h.outputs[3 + t];
+        in.psi_b[6 + t] = &e.outputs[3 + t];
       }
     }
     auto fill = [](PhaseIn& ph, const bzh_launch& l) {
@@ -963,7 +1049,20 @@ int32_t bzh_launch_program(void* stream, const bzh_program_launch* program) {
       in.monitor_count = m.monitor_count;
       if (in.step == nullptr) in.step = &m.current_step;
     }
-    run_program(static_cast<hipStream_t>(stream), in);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const bool fused = run_program(s, in);
+    if (program->field_bank_count == 2 && h.nterms > 0 && !fused && (in.nsteps % 2 == 1)) {
+      // per-phase schedule: psi was updated in place; the result convention
+      // puts it in the workspace after an odd step count
+      for (int t = 0; t < 12; ++t) {
+        const bzh_buffer* src = t < 6 ? in.h.psi_in[t] : in.e.psi_in[t - 6];
+        const bzh_buffer* dst = in.psi_b[t];
+        BZH_REQUIRE(numel(*src) == numel(*dst) && src->element_type == dst->element_type,
+                    "psi workspace does not match psi");
+        const size_t bytes = static_cast<size_t>(numel(*src)) * (src->element_type == BZH_BF16 ? 2 : 4);
+        if (bytes) BZH_HIP(hipMemcpyAsync(dst->data, src->data, bytes, hipMemcpyDeviceToDevice, s));
+      }
+    }
   });
 }
 

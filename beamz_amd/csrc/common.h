@@ -107,7 +107,22 @@ enum : int32_t {
   kRight = 32
 };
 
+// One fused leapfrog step reading bank A and writing bank B (fields and CPML
+// memories).  h.tin = H_A, h.tout = H_B, h.src = E_A; e.tin = E_A,
+// e.tout = E_B; term[t].psi point at bank A, psi_*_out at bank B.
+struct FusedParams {
+  PhaseParams h;
+  PhaseParams e;
+  void* psi_h_out[6];
+  void* psi_e_out[6];
+  int32_t kz;        // z planes per workgroup
+  int32_t nzc;       // number of z chunks
+  int32_t tiles_x;   // tiles along x
+  int32_t tiles_y;   // tiles along y
+};
+
 // Launchers implemented in the .hip files.
+void launch_fused3d(hipStream_t s, const FusedParams& p, int dtype);
 void launch_phase3d(hipStream_t s, const PhaseParams& p, int phase, int dtype);
 void launch_phase_tm(hipStream_t s, const PhaseParams& p, int phase, int dtype);
 

@@ -113,8 +113,49 @@ def update_e(state: SimulationState, prog: CompiledProgram) -> SimulationState:
     return _phase(prog, state, abi.PHASE_E)
 
 
+def uses_two_banks(prog) -> bool:
+    """The fused two-bank step applies to 3D programs without H-timed sources
+    when the fused flag is set (``abi.FLAG_FUSED_STEP``)."""
+    return (prog.is_3d and bool(prog.flags & abi.FLAG_FUSED_STEP)
+            and all(g is None for g in prog.source_groups[3:6]))
+
+
+def _workspace(prog, state):
+    """The second field (and CPML memory) bank, allocated once per program."""
+    ws = getattr(prog, "_workspace", None)
+    if ws is None or ws[0][0].shape != state.hx.shape or ws[0][0].dtype != state.hx.dtype:
+        fields = tuple(torch.empty_like(t) for t in state.fields())
+        psi = tuple(torch.empty_like(t) for t in (*state.psi_h, *state.psi_e))
+        ws = (fields, psi)
+        prog._workspace = ws
+    return ws
+
+
+def _swap_banks(prog, state, nsteps):
+    """After an odd step count the result lives in the workspace bank
+    (``runtime.py:393-440``): exchange the bank roles."""
+    if nsteps % 2 == 0:
+        return state
+    fields, psi = prog._workspace
+    nh = len(state.psi_h)
+    new = state._replace(hx=fields[0], hy=fields[1], hz=fields[2], ex=fields[3], ey=fields[4],
+                         ez=fields[5], psi_h=tuple(psi[:nh]), psi_e=tuple(psi[nh:]))
+    prog._workspace = (state.fields(), (*state.psi_h, *state.psi_e))
+    return new
+
+
+def _temporal_io(prog, state):
+    fields, psi = _workspace(prog, state)
+    if prog.cpml_enabled:
+        return (*_graph_io(prog, state), *fields, *psi)
+    return (*_graph_io(prog, state), *fields)
+
+
 def run_steps(state, prog, nsteps):
-    """Source-free, monitor-free steps (``runtime.py:685-750``)."""
+    """Source-free, monitor-free steps (``runtime.py:685-750``).  Returns the
+    state holding the result (the bank roles swap after an odd count)."""
+    if uses_two_banks(prog):
+        return run_source_group_steps(state, prog, nsteps)
     layout = abi.PROGRAM_LAYOUT_CPML_IN_PLACE if prog.cpml_enabled else abi.PROGRAM_LAYOUT_YEE_IN_PLACE
     native.call_program(_graph_io(prog, state), _program_attrs(prog, nsteps, layout))
     return state
@@ -122,6 +163,10 @@ def run_steps(state, prog, nsteps):
 
 def run_source_group_steps(state, prog, nsteps):
     """Steps with the nine slab-source groups (``runtime.py:496-527``)."""
+    if uses_two_banks(prog):
+        args = (*_temporal_io(prog, state), *_source_group_arguments(prog), state.current_step)
+        native.call_program(args, _program_attrs(prog, nsteps, abi.PROGRAM_LAYOUT_SOURCE_TEMPORAL_CPML))
+        return _swap_banks(prog, state, nsteps)
     args = (*_graph_io(prog, state), *_source_group_arguments(prog), state.current_step)
     native.call_program(args, _program_attrs(prog, nsteps, abi.PROGRAM_LAYOUT_SOURCE_IN_PLACE))
     return state
@@ -131,15 +176,22 @@ def run_program_steps(state, prog, nsteps):
     """Steps with sources and packed DFT monitors (``runtime.py:621-682``)."""
     if state.dft_re.dtype != torch.float64:
         raise ValueError("the HIP program accumulates DFT monitors in float64")
-    args = (*_graph_io(prog, state), *_source_group_arguments(prog), *prog.packed_monitors,
+    tail = (*_source_group_arguments(prog), *prog.packed_monitors,
             state.dft_re, state.dft_im, state.dft_w, state.t, state.current_step)
-    native.call_program(args, _program_attrs(prog, nsteps, abi.PROGRAM_LAYOUT_MONITOR_IN_PLACE,
-                                             monitor_count=len(prog.monitors)))
+    if uses_two_banks(prog):
+        native.call_program((*_temporal_io(prog, state), *tail),
+                            _program_attrs(prog, nsteps, abi.PROGRAM_LAYOUT_MONITOR_TEMPORAL_CPML,
+                                           monitor_count=len(prog.monitors)))
+        return _swap_banks(prog, state, nsteps)
+    native.call_program((*_graph_io(prog, state), *tail),
+                        _program_attrs(prog, nsteps, abi.PROGRAM_LAYOUT_MONITOR_IN_PLACE,
+                                       monitor_count=len(prog.monitors)))
     return state
 
 
 def run_chunk(state, prog, nsteps):
-    """Pick the layout the program needs (``execute.py:595-681``)."""
+    """Pick the layout the program needs (``execute.py:595-681``); returns the
+    state that holds the result."""
     if prog.monitors:
         return run_program_steps(state, prog, nsteps)
     if any(g is not None for g in prog.source_groups):
@@ -152,14 +204,57 @@ GRAPH_MAX_STEPS = 256  # execute.py:64
 
 def advance(state, prog, nsteps, *, chunk=GRAPH_MAX_STEPS):
     """Advance ``nsteps`` in chunks of at most ``chunk`` steps; the clocks are
-    recomputed from the run origin after every chunk (``execute.py:622-657``)."""
+    recomputed from the run origin after every chunk (``execute.py:622-657``).
+    Returns the state holding the result."""
     done = int(state.current_step.item()) if nsteps > 0 else 0
     remaining = int(nsteps)
     while remaining > 0:
         n = min(chunk, remaining)
-        run_chunk(state, prog, n)
+        state = run_chunk(state, prog, n)
         done += n
         remaining -= n
         state.current_step.fill_(done)
         state.t.fill_(prog.t0 + done * prog.dt)
     return state
+
+
+def step_kernel_name(prog):
+    if uses_two_banks(prog):
+        return "fused_step"
+    return "h_phase_3d+e_phase_3d" if prog.is_3d else "h_phase_tm+e_phase_tm"
+
+
+def time_step_kernel(state, prog, *, launches=20, stream=None):
+    """Average device time of one bare field step (no sources or monitors),
+    measured with events on ``stream``: two steps per call so the banks return
+    to their starting roles."""
+    stream = stream or torch.cuda.current_stream()
+    with torch.cuda.stream(stream):
+        layout_state = state
+        start = torch.cuda.Event(enable_timing=True)
+        stop = torch.cuda.Event(enable_timing=True)
+        if uses_two_banks(prog):
+            args = (*_temporal_io(prog, layout_state), *_source_group_arguments_empty(prog),
+                    layout_state.current_step)
+            attrs = _program_attrs(prog, 2, abi.PROGRAM_LAYOUT_SOURCE_TEMPORAL_CPML)
+            attrs.coincident_source_group_mask = 0
+        else:
+            layout = abi.PROGRAM_LAYOUT_CPML_IN_PLACE if prog.cpml_enabled else abi.PROGRAM_LAYOUT_YEE_IN_PLACE
+            args = _graph_io(prog, layout_state)
+            attrs = _program_attrs(prog, 2, layout)
+        native.call_program(args, attrs)  # warm (captures the graph)
+        start.record(stream)
+        for _ in range(launches):
+            native.call_program(args, attrs)
+        stop.record(stream)
+    torch.cuda.synchronize()
+    return start.elapsed_time(stop) / (2 * launches)
+
+
+def _source_group_arguments_empty(prog):
+    dev = prog.device
+    out = []
+    for _ in range(abi.SOURCE_GROUP_COUNT):
+        out += [torch.zeros((0, 1, 1, 1), device=dev), torch.zeros((0, 1), device=dev),
+                torch.zeros((0, 3), dtype=torch.int32, device=dev)]
+    return tuple(out)

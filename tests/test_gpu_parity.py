@@ -42,7 +42,7 @@ def _require_gpu():
 
 
 def _setup(grid, *, seed, cpml_cells=None, faces=None, metallic=None, kappa_max=2.0,
-           sources=(), monitors=(), steps=1, dt_scale=0.9):
+           sources=(), monitors=(), steps=1, dt_scale=0.9, fused=True):
     import beamz_amd as bz
     eps = eps_raster(grid, seed)
     dx = 40e-9
@@ -79,8 +79,9 @@ def _setup(grid, *, seed, cpml_cells=None, faces=None, metallic=None, kappa_max=
     prob = o.build_problem(eps, dx, dt, cpml=spec,
                            metallic=frozenset(metallic) if metallic else None,
                            sources=sources, monitors=omons)
+    flags = bz._abi.DEFAULT_FLAGS | (bz._abi.FLAG_FUSED_STEP if fused else 0)
     sim = bz.Simulation(eps, dx, dt=dt, boundaries=bounds, sources=customs, monitors=monitors,
-                        steps=steps)
+                        steps=steps, flags=flags)
     prog = sim.compile()
     assert prog.metallic == prob.metallic
     fields = random_fields(prob.shapes, seed + 1)
@@ -143,8 +144,9 @@ def _plane_source(grid, dt, steps, seed):
     return o.Source("Ez", "pre_e", (2, 2, 6), coeff, wave)
 
 
-@pytest.mark.parametrize("cpml", [True, False])
-def test_program_parity_3d_with_sources_and_monitors(cpml):
+@pytest.mark.parametrize("fused", [True, False], ids=["fused", "per_phase"])
+@pytest.mark.parametrize("cpml", [True, False], ids=["cpml", "pec"])
+def test_program_parity_3d_with_sources_and_monitors(cpml, fused):
     """Mirrors tests/hardware/test_cuda_backends.py:26-61: an 18x20x35 grid,
     3-cell CPML, a source crossing its peak and DFT monitors, 32 steps."""
     _require_gpu()
@@ -158,12 +160,15 @@ def test_program_parity_3d_with_sources_and_monitors(cpml):
     freqs = np.array([1.8e14, 2.0e14, 2.2e14])
     mons = [bz.FluxMonitor("x", 28, freqs),
             bz.FluxMonitor("z", 9, freqs[:2], ranges={"y": (3, 17), "x": (4, 30)})]
+    srcs = [src, e_src] if fused else [src, h_src, e_src]   # H-timed sources use the per-phase schedule
     sim, prog, state, prob, ost = _setup(grid, seed=21, cpml_cells=3 if cpml else None,
-                                         sources=[src, h_src, e_src], monitors=mons, steps=steps)
-    print(f"case program_parity_3d[cpml={cpml}] grid={grid} steps={steps}", flush=True)
+                                         sources=srcs, monitors=mons, steps=steps, fused=fused)
+    from beamz_amd import runtime
+    assert runtime.uses_two_banks(prog) == fused
+    print(f"case program_parity_3d[cpml={cpml},fused={fused}] grid={grid} steps={steps}", flush=True)
     s = torch.cuda.Stream()
     with torch.cuda.stream(s):
-        sim.advance(state, steps, chunk=12)
+        state = sim.advance(state, steps, chunk=12)
     torch.cuda.synchronize()
     o.run(prob, ost, steps)
     _compare_state(prog, state, prob, ost, dft=True)
@@ -187,7 +192,7 @@ def test_program_parity_tm2d():
     print(f"case program_parity_tm2d grid={grid} steps={steps}", flush=True)
     s = torch.cuda.Stream()
     with torch.cuda.stream(s):
-        sim.advance(state, steps, chunk=64)
+        state = sim.advance(state, steps, chunk=64)
     torch.cuda.synchronize()
     o.run(prob, ost, steps)
     _compare_state(prog, state, prob, ost, dft=True)
@@ -207,10 +212,30 @@ def test_graph_replay_matches_direct_enqueue_bitwise():
         if use_stream:
             s = torch.cuda.Stream()
             with torch.cuda.stream(s):
-                sim.advance(state, 24, chunk=8)   # 3 chunks: capture once, replay twice
+                state = sim.advance(state, 24, chunk=8)   # 3 chunks: capture once, replay twice
         else:
-            sim.advance(state, 24, chunk=8)
+            state = sim.advance(state, 24, chunk=8)
         torch.cuda.synchronize()
         runs.append([t.cpu().numpy().copy() for t in state.fields()])
     for a, b in zip(*runs):
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("grid,kz", [((13, 9, 130), "4"), ((40, 11, 65), "7"), ((6, 5, 64), "32")])
+@pytest.mark.parametrize("bc", ["cpml", "pec", "mixed"])
+def test_fused_step_parity_odd_shapes(grid, kz, bc, monkeypatch):
+    """Fused two-bank kernel on shapes that cut tiles and z-chunks in every
+    place (partial x/y tiles, chunk boundaries, odd step counts)."""
+    _require_gpu()
+    monkeypatch.setenv("BZH_FUSED_KZ", kz)
+    kw = {"cpml": dict(cpml_cells=2), "pec": {},
+          "mixed": dict(cpml_cells=2, faces=("left", "back", "top"), metallic=("right", "front"))}[bc]
+    steps = 7
+    sim, prog, state, prob, ost = _setup(grid, seed=51, steps=steps, **kw)
+    print(f"case fused_step_parity grid={grid} kz={kz} bc={bc}", flush=True)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        state = sim.advance(state, steps, chunk=steps)
+    torch.cuda.synchronize()
+    o.run(prob, ost, steps)
+    _compare_state(prog, state, prob, ost)
