@@ -616,22 +616,134 @@ struct ProgramIn {
   int flags;
 };
 
-// z-chunking of the fused kernel: enough workgroups to fill 256 CUs several
-// times over, chunks of at most 32 planes (the prologue recomputes one plane).
-void fused_tiling(const Grid& g, FusedParams& fp) {
-  fp.tiles_x = (g.nx + 1 + 63) / 64;
-  fp.tiles_y = (g.ny + 1 + 3) / 4;
-  const int planes = g.nz + 1;
-  const int tiles = fp.tiles_x * fp.tiles_y;
-  int kz = 32;
-  if (const char* env = std::getenv("BZH_FUSED_KZ")) {
-    kz = std::max(1, std::atoi(env));  // explicit chunk length (tuning and tests)
+// Work split of the fused step: an interior box of (x, y) tiles over the
+// z-range whose every cell and halo lies outside the CPML slabs (kernel
+// without CPML code), and a list of shell tiles (CPML-capable kernel) -- the
+// counterpart of the reference's shell/core block queue (update.cu:655-755).
+struct FusedSplit {
+  int tx0, ty0, ntx, nty, z0, z1, kz, nzc;
+  const int4* shell;
+  int nshell;
+};
+
+int chunk_len(int64_t tiles, int planes, int target) {
+  int kz = 64;
+  if (const char* env = std::getenv("BZH_FUSED_KZ")) return std::max(1, std::atoi(env));
+  while (kz > 4 && tiles * ((planes + kz - 1) / kz) < target) kz /= 2;
+  return kz;
+}
+
+FusedSplit fused_split(const Grid& g, int L) {
+  static std::mutex mu;
+  static std::map<std::vector<int>, FusedSplit> cache;
+  int device = 0;
+  BZH_HIP(hipGetDevice(&device));
+  const int TY = fused_rows();
+  const int Tx = (g.nx + 1 + 61) / 62, Ty = (g.ny + 1 + TY - 1) / TY, planes = g.nz + 1;
+  int kz_env = 0;
+  if (const char* env = std::getenv("BZH_FUSED_KZ")) kz_env = std::atoi(env);
+  const std::vector<int> key = {device, g.nz, g.ny, g.nx, L, TY, kz_env};
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  FusedSplit sp{};
+  if (L <= 0) {
+    sp.tx0 = sp.ty0 = 0;
+    sp.ntx = Tx;
+    sp.nty = Ty;
+    sp.z0 = 0;
+    sp.z1 = planes;
   } else {
-    while (kz > 4 && static_cast<int64_t>(tiles) * ((planes + kz - 1) / kz) < 4096) kz /= 2;
+    // lanes x = i0-1 .. i0+61 and rows y = j0-1 .. j0+TY-1 must avoid the slabs
+    const int tx0 = (L + 1 + 61) / 62;
+    const int tx1 = (g.nx - L - 62) >= 0 ? (g.nx - L - 62) / 62 + 1 : 0;
+    const int ty0 = (L + 1 + TY - 1) / TY;
+    const int ty1 = (g.ny - L - TY) >= 0 ? (g.ny - L - TY) / TY + 1 : 0;
+    sp.tx0 = tx0;
+    sp.ty0 = ty0;
+    sp.ntx = std::max(0, tx1 - tx0);
+    sp.nty = std::max(0, ty1 - ty0);
+    sp.z0 = L + 1;
+    sp.z1 = std::max(L + 1, g.nz - L);
+    if (sp.ntx == 0 || sp.nty == 0 || sp.z1 <= sp.z0) {
+      sp.ntx = sp.nty = 0;
+      sp.z0 = sp.z1 = 0;
+    }
   }
-  const int nzc = (planes + kz - 1) / kz;
-  fp.kz = (planes + nzc - 1) / nzc;
-  fp.nzc = (planes + fp.kz - 1) / fp.kz;
+  const int64_t inner = static_cast<int64_t>(sp.ntx) * sp.nty;
+  sp.kz = inner > 0 ? chunk_len(inner, sp.z1 - sp.z0, 16384) : 1;
+  sp.nzc = inner > 0 ? (sp.z1 - sp.z0 + sp.kz - 1) / sp.kz : 0;
+  // shell list
+  std::vector<int4> shell;
+  if (L > 0) {
+    const int kzs = chunk_len(static_cast<int64_t>(Tx) * Ty - inner, planes, 8192);
+    auto push_range = [&](int bx, int by, int z0, int z1) {
+      for (int z = z0; z < z1; z += kzs) shell.push_back(make_int4(bx, by, z, std::min(z + kzs, z1)));
+    };
+    for (int by = 0; by < Ty; ++by)
+      for (int bx = 0; bx < Tx; ++bx) {
+        const bool in_box = inner > 0 && bx >= sp.tx0 && bx < sp.tx0 + sp.ntx && by >= sp.ty0 &&
+                            by < sp.ty0 + sp.nty;
+        if (in_box) {
+          push_range(bx, by, 0, sp.z0);
+          push_range(bx, by, sp.z1, planes);
+        } else {
+          push_range(bx, by, 0, planes);
+        }
+      }
+  }
+  sp.nshell = static_cast<int>(shell.size());
+  sp.shell = nullptr;
+  if (!shell.empty()) {
+    int4* d = nullptr;
+    BZH_HIP(hipMalloc(&d, shell.size() * sizeof(int4)));
+    BZH_HIP(hipMemcpy(d, shell.data(), shell.size() * sizeof(int4), hipMemcpyHostToDevice));
+    sp.shell = d;
+  }
+  cache[key] = sp;
+  return sp;
+}
+
+size_t elem_bytes(int t) { return t == BZH_BF16 || t == BZH_F16 ? 2 : (t == BZH_F64 ? 8 : 4); }
+
+// Can the fused kernel take this phase pair?  Fills the CPML part of ``fp``.
+bool fused_cpml_layout(const PhaseIn& h, const PhaseIn& e, const bzh_buffer* const* psi_b,
+                       int uniform_l, FusedParams& fp) {
+  fp.L = 0;
+  fp.tab = nullptr;
+  if (h.nterms == 0 && e.nterms == 0) return true;
+  if (h.nterms != 6 || e.nterms != 6 || uniform_l <= 0) return false;
+  const int64_t L2 = 2 * uniform_l;
+  const char* base = static_cast<const char*>(h.terms[0]->data);
+  for (int T = 0; T < 12; ++T) {
+    const PhaseIn& ph = T < 6 ? h : e;
+    for (int q = 0; q < 3; ++q) {
+      const bzh_buffer* b = ph.terms[(T % 6) * 3 + q];
+      if (b->element_type != BZH_F32 || numel(*b) != L2) return false;
+      if (static_cast<const char*>(b->data) != base + ((T * 3 + q) * L2) * 4) return false;
+    }
+  }
+  int64_t delta = 0;
+  int ptype = -1;
+  for (int T = 0; T < 12; ++T) {
+    const bzh_buffer* a = T < 6 ? h.psi_in[T] : e.psi_in[T - 6];
+    const bzh_buffer* b = psi_b[T];
+    if (a == nullptr || b == nullptr || a->data == nullptr || b->data == nullptr) return false;
+    if (numel(*a) != numel(*b) || a->element_type != b->element_type) return false;
+    if (ptype < 0) ptype = a->element_type;
+    if (a->element_type != ptype) return false;
+    const int64_t diff = static_cast<const char*>(b->data) - static_cast<const char*>(a->data);
+    const int64_t sz = static_cast<int64_t>(elem_bytes(ptype));
+    if (diff % sz != 0) return false;
+    if (T == 0) delta = diff / sz;
+    else if (diff / sz != delta) return false;
+    fp.psi[T] = a->data;
+  }
+  fp.L = uniform_l;
+  fp.tab = reinterpret_cast<const float*>(base);
+  fp.psi_delta = delta;
+  fp.psi_bf16 = ptype == BZH_BF16;
+  return true;
 }
 
 bool metrics_empty(const PhaseIn& ph) {
@@ -656,9 +768,14 @@ bool run_program(hipStream_t s, ProgramIn& in) {
     const bzh_buffer* cb = in.groups[3 + c].coeffs;
     if (cb && cb->data && cb->rank >= 1 && cb->dims[0] > 0) h_sources = true;
   }
-  const bool psi_ws = in.h.nterms == 0 || (in.psi_b[0] != nullptr);
+  bool h_scalar = true;
+  for (int q = 0; q < 6; ++q)
+    if (in.h.mat[q] == nullptr || numel(*in.h.mat[q]) != 1) h_scalar = false;
+  FusedParams cpml_probe;
+  std::memset(&cpml_probe, 0, sizeof(cpml_probe));
   const bool fused = two_banks && (in.flags & BZH_FLAG_FUSED_STEP) && g.is3d && !h_sources &&
-                     metrics_empty(in.h) && metrics_empty(in.e) && psi_ws;
+                     h_scalar && metrics_empty(in.h) && metrics_empty(in.e) &&
+                     fused_cpml_layout(in.h, in.e, in.psi_b, in.uniform_l, cpml_probe);
   const int32_t* step0 = in.step ? static_cast<const int32_t*>(in.step->data) : nullptr;
   std::vector<Launch> launches;
   launches.reserve(static_cast<size_t>(in.nsteps) * 8);
@@ -677,35 +794,60 @@ bool run_program(hipStream_t s, ProgramIn& in) {
       e.tout[c] = nxt[kEx + c];
       e.src[c] = nxt[kHx + c];
     }
-    if (fused && in.h.nterms > 0) {
-      for (int t = 0; t < 6; ++t) {
-        if (odd) {
-          h.psi_in[t] = in.psi_b[t];
-          e.psi_in[t] = in.psi_b[6 + t];
-        }
-        h.psi_out[t] = nullptr;
-        e.psi_out[t] = nullptr;
-      }
-    }
     // pre_e sources act on the E bank read by this step
     for (int c = 0; c < 3 && c < in.ngroups; ++c)
       add_source_launches(launches, in.groups[c], 0, c, cur, g, in.metallic, step0, k);
     if (fused) {
-      FusedParams fp;
-      std::memset(&fp, 0, sizeof(fp));
-      fp.h = make_phase(h, BZH_PHASE_H, g, in.dt, in.resolution, in.metallic, in.uniform_l);
-      fp.e = make_phase(e, BZH_PHASE_E, g, in.dt, in.resolution, in.metallic, in.uniform_l);
-      for (int t = 0; t < 6 && in.h.nterms > 0; ++t) {
-        const bzh_buffer* ho = odd ? in.h.psi_in[t] : in.psi_b[t];
-        const bzh_buffer* eo = odd ? in.e.psi_in[t] : in.psi_b[6 + t];
-        BZH_REQUIRE(numel(*ho) == numel(*h.psi_in[t]) && ho->element_type == h.psi_in[t]->element_type &&
-                        numel(*eo) == numel(*e.psi_in[t]) && eo->element_type == e.psi_in[t]->element_type,
-                    "psi workspace does not match psi");
-        fp.psi_h_out[t] = ho->data;
-        fp.psi_e_out[t] = eo->data;
+      // validate both phases through the generic decoder, then repack
+      const PhaseParams hp = make_phase(h, BZH_PHASE_H, g, in.dt, in.resolution, in.metallic, in.uniform_l);
+      const PhaseParams ep = make_phase(e, BZH_PHASE_E, g, in.dt, in.resolution, in.metallic, in.uniform_l);
+      FusedParams fp = cpml_probe;
+      for (int c = 0; c < 3; ++c) {
+        fp.h_in[c] = hp.tin[c];
+        fp.h_out[c] = hp.tout[c];
+        fp.e_in[c] = ep.tin[c];
+        fp.e_out[c] = ep.tout[c];
+        fp.hdec[c] = hp.dec[c];
+        fp.hsrc[c] = hp.scf[c];
+        fp.edec[c] = ep.dec[c];
+        fp.esrc[c] = ep.scf[c];
+        fp.hdec_st[c] = hp.dec_st[c];
+        fp.hsrc_st[c] = hp.scf_st[c];
+        fp.edec_st[c] = ep.dec_st[c];
+        fp.esrc_st[c] = ep.scf_st[c];
       }
-      fused_tiling(g, fp);
-      launches.push_back(make_launch(Kind::kFused, 0, g.dtype, fp));
+      fp.inv_dx = hp.inv_dx;
+      fp.ratio_h = 1.0f;
+      fp.ratio_e = 1.0f;
+      fp.nz = g.nz;
+      fp.ny = g.ny;
+      fp.nx = g.nx;
+      fp.metallic = in.metallic;
+      if (fp.L > 0 && odd) {
+        // bank B is read on odd steps: psi roles swap
+        for (int t = 0; t < 12; ++t)
+          fp.psi[t] = static_cast<char*>(fp.psi[t]) + fp.psi_delta * (fp.psi_bf16 ? 2 : 4);
+        fp.psi_delta = -fp.psi_delta;
+      }
+      const FusedSplit sp = fused_split(g, fp.L);
+      fp.tiles = nullptr;
+      fp.box_tx0 = sp.tx0;
+      fp.box_ty0 = sp.ty0;
+      fp.box_ntx = sp.ntx;
+      fp.box_nty = sp.nty;
+      fp.box_z0 = sp.z0;
+      fp.box_z1 = sp.z1;
+      fp.box_kz = sp.kz;
+      fp.box_nzc = sp.nzc;
+      fp.cpml_kernel = 0;
+      if (sp.ntx > 0 && sp.nty > 0 && sp.nzc > 0)
+        launches.push_back(make_launch(Kind::kFused, 0, g.dtype, fp));
+      if (sp.nshell > 0) {
+        fp.tiles = sp.shell;
+        fp.ntiles = sp.nshell;
+        fp.cpml_kernel = 1;
+        launches.push_back(make_launch(Kind::kFused, 0, g.dtype, fp));
+      }
     } else {
       PhaseParams hp = make_phase(h, BZH_PHASE_H, g, in.dt, in.resolution, in.metallic, in.uniform_l);
       launches.push_back(make_launch(pk, BZH_PHASE_H, g.dtype, hp));

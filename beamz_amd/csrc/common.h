@@ -108,18 +108,38 @@ enum : int32_t {
 };
 
 // One fused leapfrog step reading bank A and writing bank B (fields and CPML
-// memories).  h.tin = H_A, h.tout = H_B, h.src = E_A; e.tin = E_A,
-// e.tout = E_B; term[t].psi point at bank A, psi_*_out at bank B.
+// memories).  Requires uniform H coefficients, isotropic metrics and, with
+// CPML, one slab thickness L on every term, the 36 term arrays packed as
+// tab[12][3][2L] (H terms then E terms; a, b, 1/kappa) and each psi of bank B
+// at psi + psi_delta (elements).
 struct FusedParams {
-  PhaseParams h;
-  PhaseParams e;
-  void* psi_h_out[6];
-  void* psi_e_out[6];
-  int32_t kz;        // z planes per workgroup
-  int32_t nzc;       // number of z chunks
-  int32_t tiles_x;   // tiles along x
-  int32_t tiles_y;   // tiles along y
+  const void* e_in[3];
+  const void* h_in[3];
+  void* e_out[3];
+  void* h_out[3];
+  const float* hdec[3];
+  const float* hsrc[3];
+  const float* edec[3];
+  const float* esrc[3];
+  int32_t hdec_st[3], hsrc_st[3], edec_st[3], esrc_st[3];
+  float inv_dx;
+  float ratio_h;   // H storage scale / E storage scale
+  float ratio_e;   // E storage scale / H storage scale
+  int32_t nz, ny, nx;
+  int32_t metallic;
+  int32_t L;             // uniform CPML thickness (0: no CPML)
+  const float* tab;
+  void* psi[12];         // bank A: H terms 0..5, E terms 6..11
+  int64_t psi_delta;
+  int32_t psi_bf16;
+  // work assignment: either a box of tiles (tiles == nullptr) or a list
+  const int4* tiles;     // (bx, by, kz0, kz1) per workgroup
+  int32_t ntiles;
+  int32_t box_tx0, box_ty0, box_ntx, box_nty;  // box of (x, y) tiles
+  int32_t box_z0, box_z1, box_kz, box_nzc;     // z range [z0, z1) in chunks of kz
+  int32_t cpml_kernel;   // 1: CPML-capable kernel (shell tiles)
 };
+int fused_rows();
 
 // Launchers implemented in the .hip files.
 void launch_fused3d(hipStream_t s, const FusedParams& p, int dtype);

@@ -1,272 +1,388 @@
 // Fused full leapfrog step (H then E) for gfx950, two field banks.
 //
-// One launch reads bank A (E, H, CPML memories) and writes bank B, so no tile
-// ever reads a value another tile is overwriting.  Each workgroup owns a
-// 64 (x) x kTY (y) column of cells and marches along z over a chunk of planes:
+// One launch reads bank A (E, H, CPML memories) and writes bank B, so no
+// workgroup ever reads a value another is overwriting.
 //
-//   plane k:  H_new(k) at its own cells        -> registers, LDS, bank B
-//             H_new(k) on a one-cell low halo  -> LDS only (recomputed, the
-//             (row j0-1 and column i0-1)          owning tile writes it)
-//             barrier
-//             E_new(k) at its own cells from H_new(k) (LDS: own, j-1, i-1) and
-//             H_new(k-1) (registers: same (j, i)) -> bank B
-//             barrier
+// Mapping: one wave64 per workgroup.  A wave owns a (x, y) tile of 62 x kTY
+// output cells and marches along z over a chunk of planes.  Lane l sits at
+// x = i0 - 1 + l: lanes 1..62 own outputs, lanes 0 and 63 are halo columns.
+// Each lane keeps kTY + 2 rows (y = j0 - 1 .. j0 + kTY) of E in registers.
 //
-// Every E/H/coefficient element is read from HBM once per step and every
-// output written once (halo re-reads are served by the caches), which is the
-// compulsory-traffic bound the roofline is priced on.
+//   plane k:  issue every load of the plane (E(k+1), H_A(k), coefficients)
+//             H_new(k) on rows j0-1 .. j0+kTY-1, lanes 0..62 (row j0-1 and
+//               lane 0 are recomputed halos; only owned cells are stored)
+//             E_new(k) on owned cells from H_new(k) (own, y-1 in registers,
+//               x-1 by cross-lane permute) and H_new(k-1) (carried along z)
+//
+// There is no LDS traffic and no barrier: x-neighbours cross lanes through
+// ds_bpermute, y-neighbours are in the lane's own registers, z-neighbours
+// are carried from the previous plane.  Every E/H/coefficient element is read
+// from HBM once per step and every output written once (halo re-reads hit the
+// caches): the compulsory-traffic bound the roofline is priced on.
 //
 // Numerical contract: identical to phase.hip (kernels.py:458-626,
-// lattice.py:523-575, kernels.py:184-264, _boundary_compile.py:118-160);
-// arithmetic is float32 in the same operation order.
+// lattice.py:523-575, kernels.py:184-264 with a uniform slab thickness L,
+// _boundary_compile.py:118-160); arithmetic is float32.
 
 #include "common.h"
 
 namespace bzh {
 namespace {
 
-constexpr int kTX = 64;
-constexpr int kTY = 4;
-
 template <typename T>
 __device__ __forceinline__ float ldf(const void* p, int i) {
   return load_f(static_cast<const T*>(p), i);
 }
 
-__device__ __forceinline__ float cf(const float* arr, int st, int i) { return arr[i * st]; }
-
-// Packed CPML correction reading psi from bank A and, for owned cells, writing
-// bank B.  ``E`` is the psi extent along the term's axis (2L for uniform L).
-template <bool kWrite, typename F>
-__device__ __forceinline__ float corr(const PhaseParams& p, void* const* psi_out, int t, float d,
-                                      int c, F lin) {
-  const DevTerm& tm = p.term[t];
-  int low, high;
-  if (p.uniform_l > 0) {
-    low = high = p.uniform_l;
-  } else {
-    low = p.meta[t * 5 + 2];
-    high = p.meta[t * 5 + 3];
-  }
-  int q = -1;
-  if (c < low) q = c;
-  else if (c >= tm.n - high) q = low + c - (tm.n - high);
-  if (q < 0 || q >= tm.low + tm.high) return d;
-  const int idx = lin(q, tm.low + tm.high);
-  float psi = tm.psi_bf16 ? load_f(static_cast<const bf16_t*>(tm.psi), idx)
-                          : static_cast<const float*>(tm.psi)[idx];
-  psi = tm.b[q] * psi + tm.a[q] * d;
-  if constexpr (kWrite) {
-    if (tm.psi_bf16) store_f(static_cast<bf16_t*>(psi_out[t]), idx, psi);
-    else static_cast<float*>(psi_out[t])[idx] = psi;
-  }
-  return d * tm.ik[q] + psi;
+// Value of the lane above / below (x + 1 / x - 1).
+__device__ __forceinline__ float from_right(float v, int addr_dn) {
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(addr_dn, __float_as_int(v)));
+}
+__device__ __forceinline__ float from_left(float v, int addr_up) {
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(addr_up, __float_as_int(v)));
 }
 
-// ---- H updates at one cell (reads bank A); bounds are the caller's ----------
-template <typename T, bool kCpml, bool kWrite>
-__device__ __forceinline__ float h_x(const FusedParams& f, int k, int j, int i) {
-  const PhaseParams& p = f.h;
-  const int ny = p.ny, nx = p.nx;
-  const int ez = (k * (ny + 1) + j) * (nx + 1) + i;
-  const int ey = (k * ny + j) * (nx + 1) + i;
-  float d0 = (ldf<T>(p.src[2], ez + nx + 1) - ldf<T>(p.src[2], ez)) * p.inv_dx;
-  float d1 = (ldf<T>(p.src[1], ey + ny * (nx + 1)) - ldf<T>(p.src[1], ey)) * p.inv_dx;
-  if constexpr (kCpml) {
-    d0 = corr<kWrite>(p, f.psi_h_out, 0, d0, j, [&](int q, int E) { return (k * E + q) * (nx + 1) + i; });
-    d1 = corr<kWrite>(p, f.psi_h_out, 1, d1, k, [&](int q, int) { return (q * ny + j) * (nx + 1) + i; });
-  }
-  const int o = (k * ny + j) * (nx + 1) + i;
-  float h = cf(p.dec[0], p.dec_st[0], o) * ldf<T>(p.tin[0], o) -
-            cf(p.scf[0], p.scf_st[0], o) * p.ratio * (d0 - d1);
-  if (((p.metallic & kLeft) && i == 0) || ((p.metallic & kRight) && i == nx)) h = 0.0f;
-  return h;
-}
-
-template <typename T, bool kCpml, bool kWrite>
-__device__ __forceinline__ float h_y(const FusedParams& f, int k, int j, int i) {
-  const PhaseParams& p = f.h;
-  const int ny = p.ny, nx = p.nx;
-  const int ex = (k * (ny + 1) + j) * nx + i;
-  const int ez = (k * (ny + 1) + j) * (nx + 1) + i;
-  float d2 = (ldf<T>(p.src[0], ex + (ny + 1) * nx) - ldf<T>(p.src[0], ex)) * p.inv_dx;
-  float d3 = (ldf<T>(p.src[2], ez + 1) - ldf<T>(p.src[2], ez)) * p.inv_dx;
-  if constexpr (kCpml) {
-    d2 = corr<kWrite>(p, f.psi_h_out, 2, d2, k, [&](int q, int) { return (q * (ny + 1) + j) * nx + i; });
-    d3 = corr<kWrite>(p, f.psi_h_out, 3, d3, i, [&](int q, int E) { return (k * (ny + 1) + j) * E + q; });
-  }
-  const int o = ex;
-  float h = cf(p.dec[1], p.dec_st[1], o) * ldf<T>(p.tin[1], o) -
-            cf(p.scf[1], p.scf_st[1], o) * p.ratio * (d2 - d3);
-  if (((p.metallic & kBottom) && j == 0) || ((p.metallic & kTop) && j == ny)) h = 0.0f;
-  return h;
-}
-
-template <typename T, bool kCpml, bool kWrite>
-__device__ __forceinline__ float h_z(const FusedParams& f, int k, int j, int i) {
-  const PhaseParams& p = f.h;
-  const int nz = p.nz, ny = p.ny, nx = p.nx;
-  const int ey = (k * ny + j) * (nx + 1) + i;
-  const int ex = (k * (ny + 1) + j) * nx + i;
-  float d4 = (ldf<T>(p.src[1], ey + 1) - ldf<T>(p.src[1], ey)) * p.inv_dx;
-  float d5 = (ldf<T>(p.src[0], ex + nx) - ldf<T>(p.src[0], ex)) * p.inv_dx;
-  if constexpr (kCpml) {
-    d4 = corr<kWrite>(p, f.psi_h_out, 4, d4, i, [&](int q, int E) { return (k * ny + j) * E + q; });
-    d5 = corr<kWrite>(p, f.psi_h_out, 5, d5, j, [&](int q, int E) { return (k * E + q) * nx + i; });
-  }
-  const int o = (k * ny + j) * nx + i;
-  float h = cf(p.dec[2], p.dec_st[2], o) * ldf<T>(p.tin[2], o) -
-            cf(p.scf[2], p.scf_st[2], o) * p.ratio * (d4 - d5);
-  if (((p.metallic & kFront) && k == 0) || ((p.metallic & kBack) && k == nz)) h = 0.0f;
-  return h;
-}
-
-// Backward difference with the ghost rule from already-loaded neighbours:
-// ``cur`` is H at c (valid when c < N), ``prev`` is H at c - 1 (valid when c > 0).
+// Ghost-rule backward difference from loaded neighbours (lattice.py:523-575).
 __device__ __forceinline__ float gdiff(float cur, float prev, int c, int N, bool mlo, bool mhi) {
   const float up = (c < N) ? cur : (mhi ? 0.0f : prev);
   const float lo = (c > 0) ? prev : (mlo ? 0.0f : cur);
   return up - lo;
 }
 
-template <typename T, bool kCpml>
-__global__ void __launch_bounds__(kTX * kTY) fused_step(FusedParams f) {
-  __shared__ float sHx[kTY + 1][kTX + 1];
-  __shared__ float sHy[kTY + 1][kTX + 1];
-  __shared__ float sHz[kTY + 1][kTX + 1];
+// Packed slab coordinate for a uniform thickness L on an axis of extent n.
+__device__ __forceinline__ int slab_q(int c, int n, int L) {
+  return c < L ? c : (c >= n - L ? c - (n - 2 * L) : -1);
+}
 
-  const PhaseParams& hp = f.h;
-  const PhaseParams& ep = f.e;
-  const int nz = hp.nz, ny = hp.ny, nx = hp.nx;
-  const int m = hp.metallic;
-  const int tx = threadIdx.x, ty = threadIdx.y;
+struct Ctx {
+  int nz, ny, nx, L, m;
+  float idx;
+};
 
-  // tile order: x fastest, then y, then z-chunk
-  int b = blockIdx.x;
-  const int bx = b % f.tiles_x;
-  b /= f.tiles_x;
-  const int by = b % f.tiles_y;
-  const int bz = b / f.tiles_y;
-  const int i0 = bx * kTX, j0 = by * kTY;
-  const int i = i0 + tx, j = j0 + ty;
-  const int kz0 = bz * f.kz;
-  const int kz1 = min(kz0 + f.kz, nz + 1);
-
-  // H_new(k-1) at this cell, carried along z
-  float prev_hx = 0.0f, prev_hy = 0.0f;
-  if (kz0 > 0) {
-    const int k = kz0 - 1;  // < nz
-    if (j < ny && i <= nx) prev_hx = h_x<T, kCpml, false>(f, k, j, i);
-    if (j <= ny && i < nx) prev_hy = h_y<T, kCpml, false>(f, k, j, i);
+// Apply one packed CPML term: psi read from bank A (f.psi[t]), written to
+// bank B (psi + delta) when ``own``.  ``lin`` is the psi linear index.
+template <int kT>
+__device__ __forceinline__ float cpml(const FusedParams& f, float d, int q, int lin, bool own) {
+  const int L2 = 2 * f.L;
+  const float* tab = f.tab + kT * 3 * L2;
+  if (f.psi_bf16) {
+    const bf16_t* pin = static_cast<const bf16_t*>(f.psi[kT]);
+    float psi = tab[L2 + q] * load_f(pin, lin) + tab[q] * d;
+    if (own) store_f(const_cast<bf16_t*>(pin) + f.psi_delta, lin, psi);
+    return d * tab[2 * L2 + q] + psi;
   }
+  const float* pin = static_cast<const float*>(f.psi[kT]);
+  float psi = tab[L2 + q] * pin[lin] + tab[q] * d;
+  if (own) const_cast<float*>(pin)[f.psi_delta + lin] = psi;
+  return d * tab[2 * L2 + q] + psi;
+}
 
-  for (int k = kz0; k < kz1; ++k) {
-    // ---- H at owned cells ----
-    float hx = 0.0f, hy = 0.0f, hz = 0.0f;
-    const bool vhx = k < nz && j < ny && i <= nx;
-    const bool vhy = k < nz && j <= ny && i < nx;
-    const bool vhz = j < ny && i < nx;  // k <= nz always here
-    if (vhx) {
-      hx = h_x<T, kCpml, true>(f, k, j, i);
-      store_f(static_cast<T*>(hp.tout[0]), (k * ny + j) * (nx + 1) + i, hx);
-    }
-    if (vhy) {
-      hy = h_y<T, kCpml, true>(f, k, j, i);
-      store_f(static_cast<T*>(hp.tout[1]), (k * (ny + 1) + j) * nx + i, hy);
-    }
-    if (vhz) {
-      hz = h_z<T, kCpml, true>(f, k, j, i);
-      store_f(static_cast<T*>(hp.tout[2]), (k * ny + j) * nx + i, hz);
-    }
-    sHx[ty + 1][tx + 1] = hx;
-    sHy[ty + 1][tx + 1] = hy;
-    sHz[ty + 1][tx + 1] = hz;
-    // ---- halo: row j0-1 (Hx, Hz) and column i0-1 (Hy, Hz), recomputed ----
-    if (ty == 0 && j0 > 0) {
-      const int jh = j0 - 1;
-      sHx[0][tx + 1] = (k < nz && i <= nx) ? h_x<T, kCpml, false>(f, k, jh, i) : 0.0f;
-      sHz[0][tx + 1] = (i < nx) ? h_z<T, kCpml, false>(f, k, jh, i) : 0.0f;
-    }
-    if (tx == 0 && i0 > 0) {
-      const int ih = i0 - 1;
-      sHy[ty + 1][0] = (k < nz && j <= ny) ? h_y<T, kCpml, false>(f, k, j, ih) : 0.0f;
-      sHz[ty + 1][0] = (j < ny) ? h_z<T, kCpml, false>(f, k, j, ih) : 0.0f;
-    }
-    __syncthreads();
+template <typename T, int kTY>
+struct Plane {
+  // E(k) on rows 0..kTY+1 (y = j0-1+r), this lane's column
+  float ex[kTY + 2], ey[kTY + 2], ez[kTY + 2];
+};
 
-    // ---- E at owned cells ----
-    // Ex (nz+1, ny+1, nx): d(Hz)/dy (+), d(Hy)/dz (-)
-    if (j <= ny && i < nx) {
-      const int o = (k * (ny + 1) + j) * nx + i;
-      float d0 = gdiff(sHz[ty + 1][tx + 1], sHz[ty][tx + 1], j, ny, m & kBottom, m & kTop) * ep.inv_dx;
-      float d1 = gdiff(hy, prev_hy, k, nz, m & kFront, m & kBack) * ep.inv_dx;
-      if constexpr (kCpml) {
-        d0 = corr<true>(ep, f.psi_e_out, 0, d0, j, [&](int q, int E) { return (k * E + q) * nx + i; });
-        d1 = corr<true>(ep, f.psi_e_out, 1, d1, k, [&](int q, int) { return (q * (ny + 1) + j) * nx + i; });
-      }
-      float e = cf(ep.dec[0], ep.dec_st[0], o) * ldf<T>(ep.tin[0], o) +
-                cf(ep.scf[0], ep.scf_st[0], o) * ep.ratio * (d0 - d1);
-      if (((m & kFront) && k == 0) || ((m & kBack) && k == nz) ||
-          ((m & kBottom) && j == 0) || ((m & kTop) && j == ny))
-        e = 0.0f;
-      store_f(static_cast<T*>(ep.tout[0]), o, e);
-    }
-    // Ey (nz+1, ny, nx+1): d(Hx)/dz (+), d(Hz)/dx (-)
-    if (j < ny && i <= nx) {
-      const int o = (k * ny + j) * (nx + 1) + i;
-      float d2 = gdiff(hx, prev_hx, k, nz, m & kFront, m & kBack) * ep.inv_dx;
-      float d3 = gdiff(sHz[ty + 1][tx + 1], sHz[ty + 1][tx], i, nx, m & kLeft, m & kRight) * ep.inv_dx;
-      if constexpr (kCpml) {
-        d2 = corr<true>(ep, f.psi_e_out, 2, d2, k, [&](int q, int) { return (q * ny + j) * (nx + 1) + i; });
-        d3 = corr<true>(ep, f.psi_e_out, 3, d3, i, [&](int q, int E) { return (k * ny + j) * E + q; });
-      }
-      float e = cf(ep.dec[1], ep.dec_st[1], o) * ldf<T>(ep.tin[1], o) +
-                cf(ep.scf[1], ep.scf_st[1], o) * ep.ratio * (d2 - d3);
-      if (((m & kFront) && k == 0) || ((m & kBack) && k == nz) ||
-          ((m & kLeft) && i == 0) || ((m & kRight) && i == nx))
-        e = 0.0f;
-      store_f(static_cast<T*>(ep.tout[1]), o, e);
-    }
-    // Ez (nz, ny+1, nx+1): d(Hy)/dx (+), d(Hx)/dy (-)
-    if (k < nz && j <= ny && i <= nx) {
-      const int o = (k * (ny + 1) + j) * (nx + 1) + i;
-      float d4 = gdiff(sHy[ty + 1][tx + 1], sHy[ty + 1][tx], i, nx, m & kLeft, m & kRight) * ep.inv_dx;
-      float d5 = gdiff(sHx[ty + 1][tx + 1], sHx[ty][tx + 1], j, ny, m & kBottom, m & kTop) * ep.inv_dx;
-      if constexpr (kCpml) {
-        d4 = corr<true>(ep, f.psi_e_out, 4, d4, i, [&](int q, int E) { return (k * (ny + 1) + j) * E + q; });
-        d5 = corr<true>(ep, f.psi_e_out, 5, d5, j, [&](int q, int E) { return (k * E + q) * (nx + 1) + i; });
-      }
-      float e = cf(ep.dec[2], ep.dec_st[2], o) * ldf<T>(ep.tin[2], o) +
-                cf(ep.scf[2], ep.scf_st[2], o) * ep.ratio * (d4 - d5);
-      if (((m & kBottom) && j == 0) || ((m & kTop) && j == ny) ||
-          ((m & kLeft) && i == 0) || ((m & kRight) && i == nx))
-        e = 0.0f;
-      store_f(static_cast<T*>(ep.tout[2]), o, e);
-    }
-    __syncthreads();
-    prev_hx = hx;
-    prev_hy = hy;
+template <typename T, int kTY>
+__device__ __forceinline__ void load_e(const FusedParams& f, const Ctx& c, int k, int j0, int x,
+                                       Plane<T, kTY>& P) {
+  const int nz = c.nz, ny = c.ny, nx = c.nx;
+#pragma unroll
+  for (int r = 0; r < kTY + 2; ++r) {
+    const int y = j0 - 1 + r;
+    const bool okx = k <= nz && y >= 0 && y <= ny && x >= 0 && x < nx;
+    const bool oky = k <= nz && y >= 0 && y < ny && x >= 0 && x <= nx;
+    const bool okz = k < nz && y >= 0 && y <= ny && x >= 0 && x <= nx;
+    // clamp to a valid element so every load is unconditional; the value of an
+    // out-of-support sample is replaced by zero
+    const int kx = min(max(k, 0), nz), yx = min(max(y, 0), ny), xx = min(max(x, 0), nx - 1);
+    const int yy = min(max(y, 0), ny - 1), xy = min(max(x, 0), nx);
+    const int kz = min(max(k, 0), nz - 1);
+    const float vx = ldf<T>(f.e_in[0], (kx * (ny + 1) + yx) * nx + xx);
+    const float vy = ldf<T>(f.e_in[1], (kx * ny + yy) * (nx + 1) + xy);
+    const float vz = ldf<T>(f.e_in[2], (kz * (ny + 1) + yx) * (nx + 1) + xy);
+    P.ex[r] = okx ? vx : 0.0f;
+    P.ey[r] = oky ? vy : 0.0f;
+    P.ez[r] = okz ? vz : 0.0f;
   }
 }
 
+// H_new(k) rows 0..kTY at this lane.  ``own`` selects the cells written to
+// bank B.  kCpml enables the packed CPML terms.
+template <typename T, int kTY, bool kCpml>
+__device__ __forceinline__ void h_plane(const FusedParams& f, const Ctx& c, int k, int j0, int x,
+                                        bool lane_own, bool write, const Plane<T, kTY>& E0,
+                                        const Plane<T, kTY>& E1, int dn, float* hx, float* hy,
+                                        float* hz, const float hc[6]) {
+  const int nz = c.nz, ny = c.ny, nx = c.nx, m = c.m;
+  const float idx = c.idx;
+  // H_A(k) loads (clamped, unconditional)
+  float ha[3][kTY + 1];
+#pragma unroll
+  for (int r = 0; r <= kTY; ++r) {
+    const int y = j0 - 1 + r;
+    const int kk = min(k, nz - 1), ky = min(max(y, 0), ny - 1), kyy = min(max(y, 0), ny);
+    const int xi = min(max(x, 0), nx), xo = min(max(x, 0), nx - 1);
+    ha[0][r] = ldf<T>(f.h_in[0], (kk * ny + ky) * (nx + 1) + xi);
+    ha[1][r] = ldf<T>(f.h_in[1], (kk * (ny + 1) + kyy) * nx + xo);
+    ha[2][r] = ldf<T>(f.h_in[2], (min(k, nz) * ny + ky) * nx + xo);
+  }
+  // per-cell H coefficients (magnetic loss) are rare: scalars live in hc[]
+#pragma unroll
+  for (int r = 0; r <= kTY; ++r) {
+    const int y = j0 - 1 + r;
+    const bool own = write && lane_own && r >= 1;
+    // Hx (nz, ny, nx+1): dEz/dy (+), dEy/dz (-)
+    {
+      const bool ok = k < nz && y >= 0 && y < ny && x >= 0 && x <= nx;
+      float d0 = (E0.ez[r + 1] - E0.ez[r]) * idx;
+      float d1 = (E1.ey[r] - E0.ey[r]) * idx;
+      if constexpr (kCpml) {
+        if (ok) {
+          const int q0 = slab_q(y, ny, c.L);
+          if (q0 >= 0) d0 = cpml<0>(f, d0, q0, (k * 2 * c.L + q0) * (nx + 1) + x, own);
+          const int q1 = slab_q(k, nz, c.L);
+          if (q1 >= 0) d1 = cpml<1>(f, d1, q1, (q1 * ny + y) * (nx + 1) + x, own);
+        }
+      }
+      float h = hc[0] * ha[0][r] - hc[3] * (d0 - d1);
+      if (((m & kLeft) && x == 0) || ((m & kRight) && x == nx)) h = 0.0f;
+      hx[r] = ok ? h : 0.0f;
+    }
+    // Hy (nz, ny+1, nx): dEx/dz (+), dEz/dx (-)
+    {
+      const bool ok = k < nz && y >= 0 && y <= ny && x >= 0 && x < nx;
+      float d2 = (E1.ex[r] - E0.ex[r]) * idx;
+      float d3 = (from_right(E0.ez[r], dn) - E0.ez[r]) * idx;
+      if constexpr (kCpml) {
+        if (ok) {
+          const int q2 = slab_q(k, nz, c.L);
+          if (q2 >= 0) d2 = cpml<2>(f, d2, q2, (q2 * (ny + 1) + y) * nx + x, own);
+          const int q3 = slab_q(x, nx, c.L);
+          if (q3 >= 0) d3 = cpml<3>(f, d3, q3, (k * (ny + 1) + y) * 2 * c.L + q3, own);
+        }
+      }
+      float h = hc[1] * ha[1][r] - hc[4] * (d2 - d3);
+      if (((m & kBottom) && y == 0) || ((m & kTop) && y == ny)) h = 0.0f;
+      hy[r] = ok ? h : 0.0f;
+    }
+    // Hz (nz+1, ny, nx): dEy/dx (+), dEx/dy (-)
+    {
+      const bool ok = k <= nz && y >= 0 && y < ny && x >= 0 && x < nx;
+      float d4 = (from_right(E0.ey[r], dn) - E0.ey[r]) * idx;
+      float d5 = (E0.ex[r + 1] - E0.ex[r]) * idx;
+      if constexpr (kCpml) {
+        if (ok) {
+          const int q4 = slab_q(x, nx, c.L);
+          if (q4 >= 0) d4 = cpml<4>(f, d4, q4, (k * ny + y) * 2 * c.L + q4, own);
+          const int q5 = slab_q(y, ny, c.L);
+          if (q5 >= 0) d5 = cpml<5>(f, d5, q5, (k * 2 * c.L + q5) * nx + x, own);
+        }
+      }
+      float h = hc[2] * ha[2][r] - hc[5] * (d4 - d5);
+      if (((m & kFront) && k == 0) || ((m & kBack) && k == nz)) h = 0.0f;
+      hz[r] = ok ? h : 0.0f;
+    }
+  }
+}
+
+template <typename T, int kTY, bool kCpml>
+__device__ __forceinline__ void plane_step(const FusedParams& f, const Ctx& c, int k, int j0, int x,
+                                           bool lane_own, const Plane<T, kTY>& E0,
+                                           const Plane<T, kTY>& E1, int dn, int up,
+                                           float* prev_hx, float* prev_hy, const float hc[6],
+                                           const float ec[6]) {
+  const int nz = c.nz, ny = c.ny, nx = c.nx, m = c.m;
+  const float idx = c.idx;
+  // E coefficients of this plane's owned rows (per-cell arrays or scalars)
+  float es[3][kTY];
+#pragma unroll
+  for (int r = 0; r < kTY; ++r) {
+    const int y = j0 + r;
+    const int kk = min(k, nz), yy1 = min(y, ny), yy0 = min(y, ny - 1), kz = min(k, nz - 1);
+    const int xo = min(max(x, 0), nx - 1), xi = min(max(x, 0), nx);
+    es[0][r] = f.esrc_st[0] ? f.esrc[0][(kk * (ny + 1) + yy1) * nx + xo] : ec[3];
+    es[1][r] = f.esrc_st[1] ? f.esrc[1][(kk * ny + yy0) * (nx + 1) + xi] : ec[4];
+    es[2][r] = f.esrc_st[2] ? f.esrc[2][(kz * (ny + 1) + yy1) * (nx + 1) + xi] : ec[5];
+  }
+  float hx[kTY + 1], hy[kTY + 1], hz[kTY + 1];
+  h_plane<T, kTY, kCpml>(f, c, k, j0, x, lane_own, true, E0, E1, dn, hx, hy, hz, hc);
+  // store owned H
+#pragma unroll
+  for (int r = 1; r <= kTY; ++r) {
+    const int y = j0 - 1 + r;
+    if (lane_own) {
+      if (k < nz && y < ny && x <= nx) store_f(static_cast<T*>(f.h_out[0]), (k * ny + y) * (nx + 1) + x, hx[r]);
+      if (k < nz && y <= ny && x < nx) store_f(static_cast<T*>(f.h_out[1]), (k * (ny + 1) + y) * nx + x, hy[r]);
+      if (y < ny && x < nx) store_f(static_cast<T*>(f.h_out[2]), (k * ny + y) * nx + x, hz[r]);
+    }
+  }
+  // E_new(k) on owned rows
+#pragma unroll
+  for (int r = 1; r <= kTY; ++r) {
+    const int y = j0 - 1 + r;
+    const float hz_l = from_left(hz[r], up);
+    const float hy_l = from_left(hy[r], up);
+    // Ex (nz+1, ny+1, nx): dHz/dy (+), dHy/dz (-)
+    if (lane_own && y <= ny && x < nx) {
+      float d0 = gdiff(hz[r], hz[r - 1], y, ny, m & kBottom, m & kTop) * idx;
+      float d1 = gdiff(hy[r], prev_hy[r - 1], k, nz, m & kFront, m & kBack) * idx;
+      if constexpr (kCpml) {
+        const int q0 = slab_q(y, ny + 1, c.L);
+        if (q0 >= 0) d0 = cpml<6>(f, d0, q0, (k * 2 * c.L + q0) * nx + x, true);
+        const int q1 = slab_q(k, nz + 1, c.L);
+        if (q1 >= 0) d1 = cpml<7>(f, d1, q1, (q1 * (ny + 1) + y) * nx + x, true);
+      }
+      const int o = (k * (ny + 1) + y) * nx + x;
+      float e = (f.edec_st[0] ? f.edec[0][o] : ec[0]) * E0.ex[r] + es[0][r - 1] * f.ratio_e * (d0 - d1);
+      if (((m & kFront) && k == 0) || ((m & kBack) && k == nz) || ((m & kBottom) && y == 0) ||
+          ((m & kTop) && y == ny))
+        e = 0.0f;
+      store_f(static_cast<T*>(f.e_out[0]), o, e);
+    }
+    // Ey (nz+1, ny, nx+1): dHx/dz (+), dHz/dx (-)
+    if (lane_own && y < ny && x <= nx) {
+      float d2 = gdiff(hx[r], prev_hx[r - 1], k, nz, m & kFront, m & kBack) * idx;
+      float d3 = gdiff(hz[r], hz_l, x, nx, m & kLeft, m & kRight) * idx;
+      if constexpr (kCpml) {
+        const int q2 = slab_q(k, nz + 1, c.L);
+        if (q2 >= 0) d2 = cpml<8>(f, d2, q2, (q2 * ny + y) * (nx + 1) + x, true);
+        const int q3 = slab_q(x, nx + 1, c.L);
+        if (q3 >= 0) d3 = cpml<9>(f, d3, q3, (k * ny + y) * 2 * c.L + q3, true);
+      }
+      const int o = (k * ny + y) * (nx + 1) + x;
+      float e = (f.edec_st[1] ? f.edec[1][o] : ec[1]) * E0.ey[r] + es[1][r - 1] * f.ratio_e * (d2 - d3);
+      if (((m & kFront) && k == 0) || ((m & kBack) && k == nz) || ((m & kLeft) && x == 0) ||
+          ((m & kRight) && x == nx))
+        e = 0.0f;
+      store_f(static_cast<T*>(f.e_out[1]), o, e);
+    }
+    // Ez (nz, ny+1, nx+1): dHy/dx (+), dHx/dy (-)
+    if (lane_own && k < nz && y <= ny && x <= nx) {
+      float d4 = gdiff(hy[r], hy_l, x, nx, m & kLeft, m & kRight) * idx;
+      float d5 = gdiff(hx[r], hx[r - 1], y, ny, m & kBottom, m & kTop) * idx;
+      if constexpr (kCpml) {
+        const int q4 = slab_q(x, nx + 1, c.L);
+        if (q4 >= 0) d4 = cpml<10>(f, d4, q4, (k * (ny + 1) + y) * 2 * c.L + q4, true);
+        const int q5 = slab_q(y, ny + 1, c.L);
+        if (q5 >= 0) d5 = cpml<11>(f, d5, q5, (k * 2 * c.L + q5) * (nx + 1) + x, true);
+      }
+      const int o = (k * (ny + 1) + y) * (nx + 1) + x;
+      float e = (f.edec_st[2] ? f.edec[2][o] : ec[2]) * E0.ez[r] + es[2][r - 1] * f.ratio_e * (d4 - d5);
+      if (((m & kBottom) && y == 0) || ((m & kTop) && y == ny) || ((m & kLeft) && x == 0) ||
+          ((m & kRight) && x == nx))
+        e = 0.0f;
+      store_f(static_cast<T*>(f.e_out[2]), o, e);
+    }
+  }
+#pragma unroll
+  for (int r = 1; r <= kTY; ++r) {
+    prev_hx[r - 1] = hx[r];
+    prev_hy[r - 1] = hy[r];
+  }
+}
+
+template <typename T, int kTY, bool kCpml>
+__global__ void __launch_bounds__(64) fused_step(FusedParams f) {
+  Ctx c;
+  c.nz = f.nz;
+  c.ny = f.ny;
+  c.nx = f.nx;
+  c.L = f.L;
+  c.m = f.metallic;
+  c.idx = f.inv_dx;
+  const int lane = threadIdx.x;
+  const int dn = ((lane + 1) & 63) << 2;
+  const int up = ((lane - 1) & 63) << 2;
+
+  int bx, by, kz0, kz1;
+  if (f.tiles != nullptr) {
+    const int4 t = f.tiles[blockIdx.x];
+    bx = t.x;
+    by = t.y;
+    kz0 = t.z;
+    kz1 = t.w;
+  } else {
+    int b = blockIdx.x;
+    bx = f.box_tx0 + b % f.box_ntx;
+    b /= f.box_ntx;
+    by = f.box_ty0 + b % f.box_nty;
+    const int bz = b / f.box_nty;
+    kz0 = f.box_z0 + bz * f.box_kz;
+    kz1 = min(kz0 + f.box_kz, f.box_z1);
+  }
+  const int i0 = bx * 62, j0 = by * kTY;
+  const int x = i0 - 1 + lane;
+  const bool lane_own = lane >= 1 && lane <= 62 && x <= c.nx;
+
+  // uniform coefficients (rank-0 device scalars) are read once
+  float hc[6], ec[6];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    hc[q] = f.hdec_st[q] ? 0.0f : f.hdec[q][0];
+    hc[3 + q] = (f.hsrc_st[q] ? 0.0f : f.hsrc[q][0]) * f.ratio_h;
+    ec[q] = f.edec_st[q] ? 0.0f : f.edec[q][0];
+    ec[3 + q] = f.esrc_st[q] ? 0.0f : f.esrc[q][0];
+  }
+
+  Plane<T, kTY> E0, E1;
+  float prev_hx[kTY], prev_hy[kTY];
+#pragma unroll
+  for (int r = 0; r < kTY; ++r) prev_hx[r] = prev_hy[r] = 0.0f;
+
+  if (kz0 > 0) {
+    // H_new(kz0 - 1) on this lane's rows (not stored: the chunk below owns it)
+    load_e<T, kTY>(f, c, kz0 - 1, j0, x, E1);
+    load_e<T, kTY>(f, c, kz0, j0, x, E0);
+    float hx[kTY + 1], hy[kTY + 1], hz[kTY + 1];
+    h_plane<T, kTY, kCpml>(f, c, kz0 - 1, j0, x, lane_own, false, E1, E0, dn, hx, hy, hz, hc);
+#pragma unroll
+    for (int r = 1; r <= kTY; ++r) {
+      prev_hx[r - 1] = hx[r];
+      prev_hy[r - 1] = hy[r];
+    }
+  } else {
+    load_e<T, kTY>(f, c, kz0, j0, x, E0);
+  }
+
+  for (int k = kz0; k < kz1; ++k) {
+    load_e<T, kTY>(f, c, k + 1, j0, x, E1);
+    plane_step<T, kTY, kCpml>(f, c, k, j0, x, lane_own, E0, E1, dn, up, prev_hx, prev_hy, hc, ec);
+    E0 = E1;
+  }
+}
+
+constexpr int kRows = 4;
+
 template <typename T>
-void dispatch(hipStream_t s, const FusedParams& p) {
-  dim3 block(kTX, kTY, 1);
-  dim3 grid(p.tiles_x * p.tiles_y * p.nzc, 1, 1);
-  if (p.h.nterms > 0) hipLaunchKernelGGL((fused_step<T, true>), grid, block, 0, s, p);
-  else hipLaunchKernelGGL((fused_step<T, false>), grid, block, 0, s, p);
+void dispatch(hipStream_t s, const FusedParams& p, int nblocks) {
+  dim3 block(64, 1, 1);
+  dim3 grid(nblocks, 1, 1);
+  if (p.cpml_kernel) hipLaunchKernelGGL((fused_step<T, kRows, true>), grid, block, 0, s, p);
+  else hipLaunchKernelGGL((fused_step<T, kRows, false>), grid, block, 0, s, p);
 }
 
 }  // namespace
 
+int fused_rows() { return kRows; }
+
 void launch_fused3d(hipStream_t s, const FusedParams& p, int dtype) {
-  BZH_REQUIRE(p.tiles_x == (p.h.nx + 1 + kTX - 1) / kTX && p.tiles_y == (p.h.ny + 1 + kTY - 1) / kTY,
-              "fused tiling does not match the grid");
-  BZH_REQUIRE(p.kz >= 1 && p.nzc * p.kz >= p.h.nz + 1, "fused z chunks do not cover the grid");
+  int nblocks;
+  if (p.tiles != nullptr) {
+    nblocks = p.ntiles;
+  } else {
+    BZH_REQUIRE(p.box_ntx >= 0 && p.box_nty >= 0 && p.box_kz >= 1, "fused box is malformed");
+    BZH_REQUIRE(p.box_tx0 >= 0 && (p.box_tx0 + p.box_ntx - 1) * 62 <= p.nx + 62 &&
+                    p.box_ty0 >= 0 && p.box_z0 >= 0 && p.box_z1 <= p.nz + 1,
+                "fused box exceeds the grid");
+    nblocks = p.box_ntx * p.box_nty * p.box_nzc;
+  }
+  BZH_REQUIRE(p.cpml_kernel || p.L == 0 || p.tiles == nullptr, "shell tiles need the CPML kernel");
+  if (nblocks <= 0) return;
   switch (dtype) {
-    case BZH_F32: dispatch<float>(s, p); break;
-    case BZH_BF16: dispatch<bf16_t>(s, p); break;
-    case BZH_F16: dispatch<__half>(s, p); break;
+    case BZH_F32: dispatch<float>(s, p, nblocks); break;
+    case BZH_BF16: dispatch<bf16_t>(s, p, nblocks); break;
+    case BZH_F16: dispatch<__half>(s, p, nblocks); break;
     default: throw Failure{"unsupported field dtype"};
   }
   BZH_HIP(hipGetLastError());

@@ -44,6 +44,8 @@ class SimulationState(NamedTuple):
     dft_w: torch.Tensor
     t: torch.Tensor
     current_step: torch.Tensor
+    fields_b: tuple = ()   # second field bank of the two-bank schedule
+    psi_b: tuple = ()      # second CPML bank: H terms then E terms
 
     def fields(self):
         return (self.hx, self.hy, self.hz, self.ex, self.ey, self.ez)
@@ -145,26 +147,36 @@ def compile_program(permittivity, resolution, dt, *, boundaries=(), sources=(), 
         coeffs[f"h_source_{ax}"] = torch.tensor(float(np.float32(dt / MU_0)), device=device)
 
     # CPML terms (compile.py:241-305) -> metadata and packed coefficient arrays
-    def term_payload(terms):
-        if not terms:
-            return torch.zeros((0, 5), dtype=torch.int32, device=device), ()
-        meta = np.asarray([(_COMPONENT_CODE[t.component], t.axis, t.low, t.high,
-                            1 if t.sign > 0 else -1) for t in terms], dtype=np.int32)
-        arrays = []
-        for t in terms:
-            arrays += [_dev(t.a, device), _dev(t.b, device), _dev(t.inv_kappa, device)]
-        while len(arrays) < 18:
-            arrays.append(torch.zeros(0, device=device))
-        return _dev(meta, device, torch.int32), tuple(arrays)
-
-    h_meta, h_arrays = term_payload(cpml.h_terms if cpml else ())
-    e_meta, e_arrays = term_payload(cpml.e_terms if cpml else ())
     thickness = 0
     if cpml is not None:
         allterms = cpml.h_terms + cpml.e_terms
         first = allterms[0].low
         if first > 0 and all(t.low == first and t.high == first for t in allterms):
             thickness = first
+    table = None
+    if thickness:
+        # one contiguous [H terms, E terms] x (a, b, 1/kappa) x 2L table; the
+        # 36 term arrays are views of it (the fused kernel reads the table)
+        tab = np.stack([np.stack([t.a, t.b, t.inv_kappa]) for t in cpml.h_terms + cpml.e_terms])
+        table = _dev(tab.astype(np.float32), device)
+
+    def term_payload(terms, first):
+        if not terms:
+            return torch.zeros((0, 5), dtype=torch.int32, device=device), ()
+        meta = np.asarray([(_COMPONENT_CODE[t.component], t.axis, t.low, t.high,
+                            1 if t.sign > 0 else -1) for t in terms], dtype=np.int32)
+        arrays = []
+        for i, t in enumerate(terms):
+            if table is not None:
+                arrays += [table[first + i, 0], table[first + i, 1], table[first + i, 2]]
+            else:
+                arrays += [_dev(t.a, device), _dev(t.b, device), _dev(t.inv_kappa, device)]
+        while len(arrays) < 18:
+            arrays.append(torch.zeros(0, device=device))
+        return _dev(meta, device, torch.int32), tuple(arrays)
+
+    h_meta, h_arrays = term_payload(cpml.h_terms if cpml else (), 0)
+    e_meta, e_arrays = term_payload(cpml.e_terms if cpml else (), len(cpml.h_terms) if cpml else 0)
     boundary_code = edge_mask(metallic) | (thickness << 8)
 
     # sources -> nine (timing, component) groups
@@ -219,16 +231,25 @@ def initial_state(prog: CompiledProgram, fields=None) -> SimulationState:
         for k, v in fields.items():
             out[k].copy_(torch.as_tensor(np.asarray(v), dtype=dt_))
 
-    def psi(terms):
-        res = []
-        for t in terms:
-            shape = list(prog.shapes[t.component])
-            shape[t.axis] = t.low + t.high
-            res.append(torch.zeros(shape, dtype=torch.float32, device=dev))
-        return tuple(res)
-
-    psi_h = psi(prog.cpml.h_terms) if prog.cpml else ()
-    psi_e = psi(prog.cpml.e_terms) if prog.cpml else ()
+    # both CPML banks live in one arena [2, total] so that every psi of bank B
+    # sits at the same element distance from its bank-A twin
+    terms = (prog.cpml.h_terms + prog.cpml.e_terms) if prog.cpml else ()
+    shapes = []
+    for t in terms:
+        shape = list(prog.shapes[t.component])
+        shape[t.axis] = t.low + t.high
+        shapes.append(tuple(shape))
+    total = int(sum(int(np.prod(sh)) for sh in shapes))
+    arena = torch.zeros((2, max(total, 1)), dtype=torch.float32, device=dev)
+    banks = ([], [])
+    off = 0
+    for sh in shapes:
+        n = int(np.prod(sh))
+        for b in (0, 1):
+            banks[b].append(arena[b, off:off + n].view(sh))
+        off += n
+    nh = len(prog.cpml.h_terms) if prog.cpml else 0
+    psi_h, psi_e = tuple(banks[0][:nh]), tuple(banks[0][nh:])
     nv = sum(6 * m.freqs.size * m.points for m in prog.monitors)
     nw = sum(m.freqs.size for m in prog.monitors)
     return SimulationState(
@@ -237,4 +258,5 @@ def initial_state(prog: CompiledProgram, fields=None) -> SimulationState:
         dft_im=torch.zeros(nv, dtype=torch.float64, device=dev),
         dft_w=torch.zeros(nw, dtype=torch.float64, device=dev),
         t=torch.tensor(prog.t0, dtype=torch.float64, device=dev),
-        current_step=torch.tensor(0, dtype=torch.int32, device=dev))
+        current_step=torch.tensor(0, dtype=torch.int32, device=dev),
+        fields_b=(), psi_b=tuple(banks[1]))

@@ -120,35 +120,34 @@ def uses_two_banks(prog) -> bool:
             and all(g is None for g in prog.source_groups[3:6]))
 
 
-def _workspace(prog, state):
-    """The second field (and CPML memory) bank, allocated once per program."""
-    ws = getattr(prog, "_workspace", None)
-    if ws is None or ws[0][0].shape != state.hx.shape or ws[0][0].dtype != state.hx.dtype:
-        fields = tuple(torch.empty_like(t) for t in state.fields())
-        psi = tuple(torch.empty_like(t) for t in (*state.psi_h, *state.psi_e))
-        ws = (fields, psi)
-        prog._workspace = ws
-    return ws
+def with_workspace(state):
+    """Allocate the second field bank on first use (the CPML bank is allocated
+    with the state, in the same arena as the first)."""
+    if state.fields_b:
+        return state
+    fields_b = tuple(torch.empty_like(t) for t in state.fields())
+    psi_b = state.psi_b
+    if len(psi_b) != len(state.psi_h) + len(state.psi_e):
+        psi_b = tuple(torch.empty_like(t) for t in (*state.psi_h, *state.psi_e))
+    return state._replace(fields_b=fields_b, psi_b=psi_b)
 
 
-def _swap_banks(prog, state, nsteps):
+def _swap_banks(state, nsteps):
     """After an odd step count the result lives in the workspace bank
     (``runtime.py:393-440``): exchange the bank roles."""
     if nsteps % 2 == 0:
         return state
-    fields, psi = prog._workspace
+    f = state.fields_b
     nh = len(state.psi_h)
-    new = state._replace(hx=fields[0], hy=fields[1], hz=fields[2], ex=fields[3], ey=fields[4],
-                         ez=fields[5], psi_h=tuple(psi[:nh]), psi_e=tuple(psi[nh:]))
-    prog._workspace = (state.fields(), (*state.psi_h, *state.psi_e))
-    return new
+    return state._replace(hx=f[0], hy=f[1], hz=f[2], ex=f[3], ey=f[4], ez=f[5],
+                          psi_h=tuple(state.psi_b[:nh]), psi_e=tuple(state.psi_b[nh:]),
+                          fields_b=state.fields(), psi_b=(*state.psi_h, *state.psi_e))
 
 
 def _temporal_io(prog, state):
-    fields, psi = _workspace(prog, state)
     if prog.cpml_enabled:
-        return (*_graph_io(prog, state), *fields, *psi)
-    return (*_graph_io(prog, state), *fields)
+        return (*_graph_io(prog, state), *state.fields_b, *state.psi_b)
+    return (*_graph_io(prog, state), *state.fields_b)
 
 
 def run_steps(state, prog, nsteps):
@@ -164,9 +163,10 @@ def run_steps(state, prog, nsteps):
 def run_source_group_steps(state, prog, nsteps):
     """Steps with the nine slab-source groups (``runtime.py:496-527``)."""
     if uses_two_banks(prog):
+        state = with_workspace(state)
         args = (*_temporal_io(prog, state), *_source_group_arguments(prog), state.current_step)
         native.call_program(args, _program_attrs(prog, nsteps, abi.PROGRAM_LAYOUT_SOURCE_TEMPORAL_CPML))
-        return _swap_banks(prog, state, nsteps)
+        return _swap_banks(state, nsteps)
     args = (*_graph_io(prog, state), *_source_group_arguments(prog), state.current_step)
     native.call_program(args, _program_attrs(prog, nsteps, abi.PROGRAM_LAYOUT_SOURCE_IN_PLACE))
     return state
@@ -179,10 +179,11 @@ def run_program_steps(state, prog, nsteps):
     tail = (*_source_group_arguments(prog), *prog.packed_monitors,
             state.dft_re, state.dft_im, state.dft_w, state.t, state.current_step)
     if uses_two_banks(prog):
+        state = with_workspace(state)
         native.call_program((*_temporal_io(prog, state), *tail),
                             _program_attrs(prog, nsteps, abi.PROGRAM_LAYOUT_MONITOR_TEMPORAL_CPML,
                                            monitor_count=len(prog.monitors)))
-        return _swap_banks(prog, state, nsteps)
+        return _swap_banks(state, nsteps)
     native.call_program((*_graph_io(prog, state), *tail),
                         _program_attrs(prog, nsteps, abi.PROGRAM_LAYOUT_MONITOR_IN_PLACE,
                                        monitor_count=len(prog.monitors)))
@@ -234,6 +235,7 @@ def time_step_kernel(state, prog, *, launches=20, stream=None):
         start = torch.cuda.Event(enable_timing=True)
         stop = torch.cuda.Event(enable_timing=True)
         if uses_two_banks(prog):
+            layout_state = with_workspace(layout_state)
             args = (*_temporal_io(prog, layout_state), *_source_group_arguments_empty(prog),
                     layout_state.current_step)
             attrs = _program_attrs(prog, 2, abi.PROGRAM_LAYOUT_SOURCE_TEMPORAL_CPML)
