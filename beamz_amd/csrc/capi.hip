@@ -633,7 +633,7 @@ int chunk_len(int64_t tiles, int planes, int target) {
   return kz;
 }
 
-FusedSplit fused_split(const Grid& g, int L) {
+FusedSplit fused_split(const Grid& g, int L, bool fast_ok) {
   static std::mutex mu;
   static std::map<std::vector<int>, FusedSplit> cache;
   int device = 0;
@@ -642,62 +642,45 @@ FusedSplit fused_split(const Grid& g, int L) {
   const int Tx = (g.nx + 1 + 61) / 62, Ty = (g.ny + 1 + TY - 1) / TY, planes = g.nz + 1;
   int kz_env = 0;
   if (const char* env = std::getenv("BZH_FUSED_KZ")) kz_env = std::atoi(env);
-  const std::vector<int> key = {device, g.nz, g.ny, g.nx, L, TY, kz_env};
+  const std::vector<int> key = {device, g.nz, g.ny, g.nx, L, TY, kz_env, fast_ok ? 1 : 0};
   std::lock_guard<std::mutex> lock(mu);
   auto it = cache.find(key);
   if (it != cache.end()) return it->second;
   FusedSplit sp{};
-  if (L <= 0) {
-    sp.tx0 = sp.ty0 = 0;
-    sp.ntx = Tx;
-    sp.nty = Ty;
-    sp.z0 = 0;
-    sp.z1 = planes;
-  } else {
-    // lanes x = i0-1 .. i0+61 and rows y = j0-1 .. j0+TY-1 must avoid the slabs
-    const int tx0 = (L + 1 + 61) / 62;
-    const int tx1 = (g.nx - L - 62) >= 0 ? (g.nx - L - 62) / 62 + 1 : 0;
-    const int ty0 = (L + 1 + TY - 1) / TY;
-    const int ty1 = (g.ny - L - TY) >= 0 ? (g.ny - L - TY) / TY + 1 : 0;
-    sp.tx0 = tx0;
-    sp.ty0 = ty0;
-    sp.ntx = std::max(0, tx1 - tx0);
-    sp.nty = std::max(0, ty1 - ty0);
-    sp.z0 = L + 1;
-    sp.z1 = std::max(L + 1, g.nz - L);
-    if (sp.ntx == 0 || sp.nty == 0 || sp.z1 <= sp.z0) {
-      sp.ntx = sp.nty = 0;
-      sp.z0 = sp.z1 = 0;
-    }
-  }
+  // fast tiles: lanes x = i0-1 .. i0+62 and rows y = j0-1 .. j0+TY inside
+  // [M, n - M) with M = max(L, 1): no x/y face, ghost or CPML slab is touched
+  const int M = std::max(L, 1);
+  const int tx0 = (M + 1 + 61) / 62;
+  const int tx1 = (g.nx - M - 63) >= 0 ? (g.nx - M - 63) / 62 + 1 : 0;
+  const int ty0 = (M + 1 + TY - 1) / TY;
+  const int ty1 = (g.ny - M - TY) >= 0 ? (g.ny - M - TY) / TY + 1 : 0;
+  sp.tx0 = tx0;
+  sp.ty0 = ty0;
+  sp.ntx = fast_ok ? std::max(0, tx1 - tx0) : 0;
+  sp.nty = fast_ok ? std::max(0, ty1 - ty0) : 0;
+  if (sp.ntx == 0 || sp.nty == 0) sp.ntx = sp.nty = 0;
+  sp.z0 = 0;
+  sp.z1 = planes;
   const int64_t inner = static_cast<int64_t>(sp.ntx) * sp.nty;
-  sp.kz = inner > 0 ? chunk_len(inner, sp.z1 - sp.z0, 16384) : 1;
-  sp.nzc = inner > 0 ? (sp.z1 - sp.z0 + sp.kz - 1) / sp.kz : 0;
-  // shell list
-  std::vector<int4> shell;
-  if (L > 0) {
-    const int kzs = chunk_len(static_cast<int64_t>(Tx) * Ty - inner, planes, 8192);
-    auto push_range = [&](int bx, int by, int z0, int z1) {
-      for (int z = z0; z < z1; z += kzs) shell.push_back(make_int4(bx, by, z, std::min(z + kzs, z1)));
-    };
+  sp.kz = inner > 0 ? chunk_len(inner, planes, 16384) : 1;
+  sp.nzc = inner > 0 ? (planes + sp.kz - 1) / sp.kz : 0;
+  // edge list: every other tile over the whole z range
+  std::vector<int4> edge;
+  const int64_t outer = static_cast<int64_t>(Tx) * Ty - inner;
+  const int kze = chunk_len(std::max<int64_t>(outer, 1), planes, 8192);
+  for (int z = 0; z < planes; z += kze)
     for (int by = 0; by < Ty; ++by)
       for (int bx = 0; bx < Tx; ++bx) {
-        const bool in_box = inner > 0 && bx >= sp.tx0 && bx < sp.tx0 + sp.ntx && by >= sp.ty0 &&
-                            by < sp.ty0 + sp.nty;
-        if (in_box) {
-          push_range(bx, by, 0, sp.z0);
-          push_range(bx, by, sp.z1, planes);
-        } else {
-          push_range(bx, by, 0, planes);
-        }
+        const bool in_box = inner > 0 && bx >= sp.tx0 && bx < sp.tx0 + sp.ntx &&
+                            by >= sp.ty0 && by < sp.ty0 + sp.nty;
+        if (!in_box) edge.push_back(make_int4(bx, by, z, std::min(z + kze, planes)));
       }
-  }
-  sp.nshell = static_cast<int>(shell.size());
+  sp.nshell = static_cast<int>(edge.size());
   sp.shell = nullptr;
-  if (!shell.empty()) {
+  if (!edge.empty()) {
     int4* d = nullptr;
-    BZH_HIP(hipMalloc(&d, shell.size() * sizeof(int4)));
-    BZH_HIP(hipMemcpy(d, shell.data(), shell.size() * sizeof(int4), hipMemcpyHostToDevice));
+    BZH_HIP(hipMalloc(&d, edge.size() * sizeof(int4)));
+    BZH_HIP(hipMemcpy(d, edge.data(), edge.size() * sizeof(int4), hipMemcpyHostToDevice));
     sp.shell = d;
   }
   cache[key] = sp;
@@ -829,7 +812,17 @@ bool run_program(hipStream_t s, ProgramIn& in) {
           fp.psi[t] = static_cast<char*>(fp.psi[t]) + fp.psi_delta * (fp.psi_bf16 ? 2 : 4);
         fp.psi_delta = -fp.psi_delta;
       }
-      const FusedSplit sp = fused_split(g, fp.L);
+      bool fast_ok = true;
+      for (int c = 0; c < 3; ++c) fast_ok = fast_ok && fp.edec_st[c] == 0;
+      fast_ok = fast_ok && (fp.esrc_st[0] == fp.esrc_st[1] && fp.esrc_st[1] == fp.esrc_st[2]);
+      for (int c = 0; c < 6; ++c) {
+        // 32-bit byte offsets: every field array must stay below 4 GiB
+        int64_t sh[3];
+        yee_shape3d(c, g.nz, g.ny, g.nx, sh);
+        fast_ok = fast_ok && sh[0] * sh[1] * sh[2] * static_cast<int64_t>(elem_bytes(g.dtype)) <
+                                 (int64_t{1} << 32);
+      }
+      const FusedSplit sp = fused_split(g, fp.L, fast_ok);
       fp.tiles = nullptr;
       fp.box_tx0 = sp.tx0;
       fp.box_ty0 = sp.ty0;
@@ -839,13 +832,13 @@ bool run_program(hipStream_t s, ProgramIn& in) {
       fp.box_z1 = sp.z1;
       fp.box_kz = sp.kz;
       fp.box_nzc = sp.nzc;
-      fp.cpml_kernel = 0;
+      fp.cpml_kernel = fp.L > 0 ? 1 : 0;
       if (sp.ntx > 0 && sp.nty > 0 && sp.nzc > 0)
         launches.push_back(make_launch(Kind::kFused, 0, g.dtype, fp));
       if (sp.nshell > 0) {
         fp.tiles = sp.shell;
         fp.ntiles = sp.nshell;
-        fp.cpml_kernel = 1;
+        fp.cpml_kernel = fp.L > 0 ? 1 : 0;
         launches.push_back(make_launch(Kind::kFused, 0, g.dtype, fp));
       }
     } else {

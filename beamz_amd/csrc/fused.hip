@@ -24,6 +24,8 @@
 // lattice.py:523-575, kernels.py:184-264 with a uniform slab thickness L,
 // _boundary_compile.py:118-160); arithmetic is float32.
 
+#include <cstdlib>
+
 #include "common.h"
 
 namespace bzh {
@@ -282,6 +284,16 @@ __device__ __forceinline__ void plane_step(const FusedParams& f, const Ctx& c, i
   }
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin (g % 8); give each XCD
+// a contiguous range of logical tiles so that neighbouring tiles, which share
+// halo rows and 128-byte lines, run against the same L2.
+__device__ __forceinline__ int xcd_remap(int g, int n) {
+  constexpr int kXcd = 8;
+  const int c = g % kXcd, m = g / kXcd;
+  const int base = n / kXcd, rem = n % kXcd;
+  return c * base + min(c, rem) + m;
+}
+
 template <typename T, int kTY, bool kCpml>
 __global__ void __launch_bounds__(64) fused_step(FusedParams f) {
   Ctx c;
@@ -296,14 +308,15 @@ __global__ void __launch_bounds__(64) fused_step(FusedParams f) {
   const int up = ((lane - 1) & 63) << 2;
 
   int bx, by, kz0, kz1;
+  const int g = xcd_remap(blockIdx.x, gridDim.x);
   if (f.tiles != nullptr) {
-    const int4 t = f.tiles[blockIdx.x];
+    const int4 t = f.tiles[g];
     bx = t.x;
     by = t.y;
     kz0 = t.z;
     kz1 = t.w;
   } else {
-    int b = blockIdx.x;
+    int b = g;
     bx = f.box_tx0 + b % f.box_ntx;
     b /= f.box_ntx;
     by = f.box_ty0 + b % f.box_nty;
@@ -352,19 +365,266 @@ __global__ void __launch_bounds__(64) fused_step(FusedParams f) {
   }
 }
 
-constexpr int kRows = 4;
+// ---------------------------------------------------------------------------
+// Fast path: tiles whose x/y footprint (halos included) avoids every domain
+// face and the x/y CPML slabs.  All loads and stores are unconditional, the
+// z faces and the z CPML slabs are plane-uniform branches, and every address
+// is a 32-bit byte offset on a uniform base (global_load ... saddr).
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ float ldT(const void* b, uint32_t e) {
+  return load_f(reinterpret_cast<const T*>(static_cast<const char*>(b) + e * uint32_t(sizeof(T))), 0);
+}
+template <typename T>
+__device__ __forceinline__ void stT(void* b, uint32_t e, float v) {
+  store_f(reinterpret_cast<T*>(static_cast<char*>(b) + e * uint32_t(sizeof(T))), 0, v);
+}
+__device__ __forceinline__ float ldF(const float* b, uint32_t e) {
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(b) + e * 4u);
+}
+
+// z-term CPML (kT: 1 Hx_z, 2 Hy_z, 7 Ex_z, 8 Ey_z) at element offset e of psi
+template <int kT>
+__device__ __forceinline__ float zcpml(const FusedParams& f, float d, int q, uint32_t e, bool own) {
+  const int L2 = 2 * f.L;
+  const float* tab = f.tab + kT * 3 * L2;
+  const float a = tab[q], b = tab[L2 + q], ik = tab[2 * L2 + q];
+  if (f.psi_bf16) {
+    const bf16_t* pin = static_cast<const bf16_t*>(f.psi[kT]);
+    const float psi = b * load_f(pin, e) + a * d;
+    if (own) store_f(const_cast<bf16_t*>(pin) + f.psi_delta, e, psi);
+    return d * ik + psi;
+  }
+  const float* pin = static_cast<const float*>(f.psi[kT]);
+  const float psi = b * ldF(pin, e) + a * d;
+  if (own) *reinterpret_cast<float*>(reinterpret_cast<char*>(const_cast<float*>(pin)) +
+                                     (static_cast<int64_t>(e) + f.psi_delta) * 4) = psi;
+  return d * ik + psi;
+}
+
+template <typename T, int kTY, bool kEArr>
+__global__ void __launch_bounds__(64) fused_fast(FusedParams f) {
+  const int lane = threadIdx.x;
+  const int g = xcd_remap(blockIdx.x, gridDim.x);
+  int b = g;
+  const int bx = f.box_tx0 + b % f.box_ntx;
+  b /= f.box_ntx;
+  const int by = f.box_ty0 + b % f.box_nty;
+  const int bz = b / f.box_nty;
+  const int kz0 = f.box_z0 + bz * f.box_kz;
+  const int kz1 = min(kz0 + f.box_kz, f.box_z1);
+
+  const int nz = f.nz, ny = f.ny, nx = f.nx, L = f.L, m = f.metallic;
+  const float idx = f.inv_dx;
+  const int j0 = by * kTY;
+  const int x = bx * 62 - 1 + lane;
+  const bool own = lane >= 1 && lane <= 62;
+
+  // element geometry of the four layout classes (plane stride, row stride)
+  //   X: Ex, Hy, e_source_x    (ny+1) x nx
+  //   Y: Ey, Hx, e_source_y     ny    x (nx+1)
+  //   Z: Ez, e_source_z        (ny+1) x (nx+1)
+  //   W: Hz                     ny    x nx
+  const uint32_t rX = nx, rY = nx + 1, rZ = nx + 1, rW = nx;
+  const uint32_t pX = (ny + 1) * nx, pY = ny * (nx + 1), pZ = (ny + 1) * (nx + 1), pW = ny * nx;
+  const uint32_t y0 = j0 - 1;
+  const uint32_t bX = y0 * rX + x, bY = y0 * rY + x, bZ = y0 * rZ + x, bW = y0 * rW + x;
+
+  float hc[6], ec[6];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    hc[q] = f.hdec[q][0];
+    hc[3 + q] = f.hsrc[q][0] * f.ratio_h;
+    ec[q] = f.edec[q][0];
+    ec[3 + q] = kEArr ? 0.0f : f.esrc[q][0] * f.ratio_e;
+  }
+  const int dn = ((lane + 1) & 63) << 2;
+  const int up = ((lane - 1) & 63) << 2;
+
+  float ex0[kTY + 2], ey0[kTY + 2], ez0[kTY + 2];
+  float ex1[kTY + 2], ey1[kTY + 2], ez1[kTY + 2];
+  float phx[kTY], phy[kTY];
+
+  auto load_plane = [&](int k, float* ex, float* ey, float* ez) {
+    // Ex/Ey exist on planes 0..nz, Ez on 0..nz-1; a missing plane reads as 0
+    const bool vxy = k <= nz, vz = k < nz;
+    const uint32_t kx = min(k, nz), kz = min(k, nz - 1);
+    const uint32_t oX = kx * pX + bX, oY = kx * pY + bY, oZ = kz * pZ + bZ;
+#pragma unroll
+    for (int r = 0; r < kTY + 2; ++r) {
+      const float vx = ldT<T>(f.e_in[0], oX + r * rX);
+      const float vy = ldT<T>(f.e_in[1], oY + r * rY);
+      const float vzz = ldT<T>(f.e_in[2], oZ + r * rZ);
+      ex[r] = vxy ? vx : 0.0f;
+      ey[r] = vxy ? vy : 0.0f;
+      ez[r] = vz ? vzz : 0.0f;
+    }
+  };
+
+  // H_new(k) on rows 0..kTY (row 0 = y0 halo) at lanes 0..62
+  auto h_rows = [&](int k, const float* ex, const float* ey, const float* ez, const float* exn,
+                    const float* eyn, float* hx, float* hy, float* hz, bool write) {
+    const bool vxy = k < nz;  // Hx, Hy exist on planes 0..nz-1
+    const uint32_t kk = min(k, nz - 1);
+    const uint32_t oY = kk * pY + bY, oX = kk * pX + bX, oW = k * pW + bW;
+    float ha[3][kTY + 1];
+#pragma unroll
+    for (int r = 0; r <= kTY; ++r) {
+      ha[0][r] = ldT<T>(f.h_in[0], oY + r * rY);
+      ha[1][r] = ldT<T>(f.h_in[1], oX + r * rX);
+      ha[2][r] = ldT<T>(f.h_in[2], oW + r * rW);
+    }
+    const int qz = L > 0 && vxy ? slab_q(k, nz, L) : -1;
+    const bool hz_pec = ((m & kFront) && k == 0) || ((m & kBack) && k == nz);
+#pragma unroll
+    for (int r = 0; r <= kTY; ++r) {
+      const bool w = write && own && r >= 1;
+      float d0 = (ez[r + 1] - ez[r]) * idx;
+      float d1 = (eyn[r] - ey[r]) * idx;
+      float d2 = (exn[r] - ex[r]) * idx;
+      float d3 = (from_right(ez[r], dn) - ez[r]) * idx;
+      float d4 = (from_right(ey[r], dn) - ey[r]) * idx;
+      float d5 = (ex[r + 1] - ex[r]) * idx;
+      if (qz >= 0) {
+        d1 = zcpml<1>(f, d1, qz, qz * pY + bY + r * rY, w);
+        d2 = zcpml<2>(f, d2, qz, qz * pX + bX + r * rX, w);
+      }
+      hx[r] = vxy ? hc[0] * ha[0][r] - hc[3] * (d0 - d1) : 0.0f;
+      hy[r] = vxy ? hc[1] * ha[1][r] - hc[4] * (d2 - d3) : 0.0f;
+      hz[r] = hz_pec ? 0.0f : hc[2] * ha[2][r] - hc[5] * (d4 - d5);
+    }
+  };
+
+#pragma unroll
+  for (int r = 0; r < kTY; ++r) phx[r] = phy[r] = 0.0f;
+  if (kz0 > 0) {
+    float hx[kTY + 1], hy[kTY + 1], hz[kTY + 1];
+    load_plane(kz0 - 1, ex1, ey1, ez1);
+    load_plane(kz0, ex0, ey0, ez0);
+    h_rows(kz0 - 1, ex1, ey1, ez1, ex0, ey0, hx, hy, hz, false);
+#pragma unroll
+    for (int r = 0; r < kTY; ++r) {
+      phx[r] = hx[r + 1];
+      phy[r] = hy[r + 1];
+    }
+  } else {
+    load_plane(kz0, ex0, ey0, ez0);
+  }
+
+  for (int k = kz0; k < kz1; ++k) {
+    load_plane(k + 1, ex1, ey1, ez1);
+    float es[3][kTY];
+    if constexpr (kEArr) {
+      const uint32_t kx = min(k, nz), kz = min(k, nz - 1);
+      const uint32_t oX = kx * pX + bX + rX, oY = kx * pY + bY + rY, oZ = kz * pZ + bZ + rZ;
+#pragma unroll
+      for (int r = 0; r < kTY; ++r) {
+        es[0][r] = ldF(f.esrc[0], oX + r * rX) * f.ratio_e;
+        es[1][r] = ldF(f.esrc[1], oY + r * rY) * f.ratio_e;
+        es[2][r] = ldF(f.esrc[2], oZ + r * rZ) * f.ratio_e;
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < kTY; ++r) {
+        es[0][r] = ec[3];
+        es[1][r] = ec[4];
+        es[2][r] = ec[5];
+      }
+    }
+    float hx[kTY + 1], hy[kTY + 1], hz[kTY + 1];
+    h_rows(k, ex0, ey0, ez0, ex1, ey1, hx, hy, hz, true);
+
+    const bool vxy = k < nz;
+    const uint32_t kk = min(k, nz - 1);
+    const uint32_t hoY = kk * pY + bY, hoX = kk * pX + bX, hoW = k * pW + bW;
+    if (own) {
+#pragma unroll
+      for (int r = 1; r <= kTY; ++r) {
+        if (vxy) {
+          stT<T>(f.h_out[0], hoY + r * rY, hx[r]);
+          stT<T>(f.h_out[1], hoX + r * rX, hy[r]);
+        }
+        stT<T>(f.h_out[2], hoW + r * rW, hz[r]);
+      }
+    }
+    // E_new(k): z ghost rule and z-face PEC are plane-uniform
+    const int qz = L > 0 ? slab_q(k, nz + 1, L) : -1;
+    const bool exy_pec = ((m & kFront) && k == 0) || ((m & kBack) && k == nz);
+    const bool mlo = m & kFront, mhi = m & kBack;
+    const uint32_t kx = min(k, nz), kz = min(k, nz - 1);
+    const uint32_t eoX = kx * pX + bX, eoY = kx * pY + bY, eoZ = kz * pZ + bZ;
+#pragma unroll
+    for (int r = 1; r <= kTY; ++r) {
+      const float hz_l = from_left(hz[r], up);
+      const float hy_l = from_left(hy[r], up);
+      float d0 = (hz[r] - hz[r - 1]) * idx;
+      float d1 = gdiff(hy[r], phy[r - 1], k, nz, mlo, mhi) * idx;
+      float d2 = gdiff(hx[r], phx[r - 1], k, nz, mlo, mhi) * idx;
+      float d3 = (hz[r] - hz_l) * idx;
+      float d4 = (hy[r] - hy_l) * idx;
+      float d5 = (hx[r] - hx[r - 1]) * idx;
+      if (qz >= 0) {
+        d1 = zcpml<7>(f, d1, qz, qz * pX + bX + r * rX, own);
+        d2 = zcpml<8>(f, d2, qz, qz * pY + bY + r * rY, own);
+      }
+      const float nex = exy_pec ? 0.0f : ec[0] * ex0[r] + es[0][r - 1] * (d0 - d1);
+      const float ney = exy_pec ? 0.0f : ec[1] * ey0[r] + es[1][r - 1] * (d2 - d3);
+      const float nez = ec[2] * ez0[r] + es[2][r - 1] * (d4 - d5);
+      if (own) {
+        stT<T>(f.e_out[0], eoX + r * rX, nex);
+        stT<T>(f.e_out[1], eoY + r * rY, ney);
+        if (k < nz) stT<T>(f.e_out[2], eoZ + r * rZ, nez);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < kTY; ++r) {
+      phx[r] = hx[r + 1];
+      phy[r] = hy[r + 1];
+    }
+#pragma unroll
+    for (int r = 0; r < kTY + 2; ++r) {
+      ex0[r] = ex1[r];
+      ey0[r] = ey1[r];
+      ez0[r] = ez1[r];
+    }
+  }
+}
+
+template <typename T, int kR>
+void dispatch_rows(hipStream_t s, const FusedParams& p, int nblocks) {
+  dim3 block(64, 1, 1);
+  dim3 grid(nblocks, 1, 1);
+  if (p.tiles == nullptr) {
+    bool earr = p.esrc_st[0] && p.esrc_st[1] && p.esrc_st[2];
+    if (earr) hipLaunchKernelGGL((fused_fast<T, kR, true>), grid, block, 0, s, p);
+    else hipLaunchKernelGGL((fused_fast<T, kR, false>), grid, block, 0, s, p);
+  } else if (p.cpml_kernel) {
+    hipLaunchKernelGGL((fused_step<T, kR, true>), grid, block, 0, s, p);
+  } else {
+    hipLaunchKernelGGL((fused_step<T, kR, false>), grid, block, 0, s, p);
+  }
+}
+
+// Rows per wave (y extent of a tile): 2, 4 or 8; BZH_FUSED_ROWS overrides the
+// default for tuning sweeps.
+int rows_setting() {
+  int r = 4;
+  if (const char* env = std::getenv("BZH_FUSED_ROWS")) r = std::atoi(env);
+  return (r == 2 || r == 4 || r == 8) ? r : 4;
+}
 
 template <typename T>
 void dispatch(hipStream_t s, const FusedParams& p, int nblocks) {
-  dim3 block(64, 1, 1);
-  dim3 grid(nblocks, 1, 1);
-  if (p.cpml_kernel) hipLaunchKernelGGL((fused_step<T, kRows, true>), grid, block, 0, s, p);
-  else hipLaunchKernelGGL((fused_step<T, kRows, false>), grid, block, 0, s, p);
+  switch (rows_setting()) {
+    case 2: dispatch_rows<T, 2>(s, p, nblocks); break;
+    case 8: dispatch_rows<T, 8>(s, p, nblocks); break;
+    default: dispatch_rows<T, 4>(s, p, nblocks); break;
+  }
 }
 
 }  // namespace
 
-int fused_rows() { return kRows; }
+int fused_rows() { return rows_setting(); }
 
 void launch_fused3d(hipStream_t s, const FusedParams& p, int dtype) {
   int nblocks;
@@ -377,7 +637,7 @@ void launch_fused3d(hipStream_t s, const FusedParams& p, int dtype) {
                 "fused box exceeds the grid");
     nblocks = p.box_ntx * p.box_nty * p.box_nzc;
   }
-  BZH_REQUIRE(p.cpml_kernel || p.L == 0 || p.tiles == nullptr, "shell tiles need the CPML kernel");
+  BZH_REQUIRE(p.cpml_kernel || p.L == 0, "CPML tiles need the CPML-capable kernel");
   if (nblocks <= 0) return;
   switch (dtype) {
     case BZH_F32: dispatch<float>(s, p, nblocks); break;

@@ -97,7 +97,29 @@ def main():
     ap.add_argument("--no-monitors", action="store_true")
     ap.add_argument("--per-phase", action="store_true", help="use the per-phase kernels")
     ap.add_argument("--chunk", type=int, default=256)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a 1 GiB device-to-device copy (the byte-counter calibration "
+                         "kernel for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = 1 << 28  # 2^28 float32 = 1 GiB
+        src = torch.ones(n, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        t0.record()
+        for _ in range(10):
+            dst.copy_(src)
+        t1.record()
+        torch.cuda.synchronize()
+        ms = t0.elapsed_time(t1) / 10
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 13,
+                          "copy_ms": round(ms, 4), "copy_GBps": round(8 * n / ms / 1e6, 1)}))
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(the z-slab sharding design is described in DESIGN.md)", file=sys.stderr)
