@@ -616,74 +616,45 @@ struct ProgramIn {
   int flags;
 };
 
-// Work split of the fused step: an interior box of (x, y) tiles over the
-// z-range whose every cell and halo lies outside the CPML slabs (kernel
-// without CPML code), and a list of shell tiles (CPML-capable kernel) -- the
-// counterpart of the reference's shell/core block queue (update.cu:655-755).
+// Work split of the fused step -- the counterpart of the reference's
+// shell/core block queue (update.cu:655-755).  Every (x, y) tile over every
+// z chunk belongs to one of two boxes: the x-interior box, whose lanes
+// x = i0-1 .. i0+62 all lie in [M, nx - M) with M = max(L, 1) (no x face, x
+// ghost or x CPML slab, so the kernel has no per-lane x checks), and the
+// x-edge box (the tiles left of and right of it).  y and z boundaries are
+// handled by row- and plane-uniform branches in both.
+struct FusedBox {
+  int tx0, ntx, tsplit, tskip, nty, kz, nzc;
+};
 struct FusedSplit {
-  int tx0, ty0, ntx, nty, z0, z1, kz, nzc;
-  const int4* shell;
-  int nshell;
+  FusedBox interior, edge;
 };
 
+// z chunk length: the longest of 64, 32, ... 4 planes that still gives at
+// least ``target`` workgroups (BZH_FUSED_KZ overrides it for tuning sweeps).
 int chunk_len(int64_t tiles, int planes, int target) {
-  int kz = 64;
   if (const char* env = std::getenv("BZH_FUSED_KZ")) return std::max(1, std::atoi(env));
+  int kz = 64;
   while (kz > 4 && tiles * ((planes + kz - 1) / kz) < target) kz /= 2;
   return kz;
 }
 
-FusedSplit fused_split(const Grid& g, int L, bool fast_ok) {
-  static std::mutex mu;
-  static std::map<std::vector<int>, FusedSplit> cache;
-  int device = 0;
-  BZH_HIP(hipGetDevice(&device));
+FusedSplit fused_split(const Grid& g, int L) {
   const int TY = fused_rows();
   const int Tx = (g.nx + 1 + 61) / 62, Ty = (g.ny + 1 + TY - 1) / TY, planes = g.nz + 1;
-  int kz_env = 0;
-  if (const char* env = std::getenv("BZH_FUSED_KZ")) kz_env = std::atoi(env);
-  const std::vector<int> key = {device, g.nz, g.ny, g.nx, L, TY, kz_env, fast_ok ? 1 : 0};
-  std::lock_guard<std::mutex> lock(mu);
-  auto it = cache.find(key);
-  if (it != cache.end()) return it->second;
-  FusedSplit sp{};
-  // fast tiles: lanes x = i0-1 .. i0+62 and rows y = j0-1 .. j0+TY inside
-  // [M, n - M) with M = max(L, 1): no x/y face, ghost or CPML slab is touched
   const int M = std::max(L, 1);
-  const int tx0 = (M + 1 + 61) / 62;
-  const int tx1 = (g.nx - M - 63) >= 0 ? (g.nx - M - 63) / 62 + 1 : 0;
-  const int ty0 = (M + 1 + TY - 1) / TY;
-  const int ty1 = (g.ny - M - TY) >= 0 ? (g.ny - M - TY) / TY + 1 : 0;
-  sp.tx0 = tx0;
-  sp.ty0 = ty0;
-  sp.ntx = fast_ok ? std::max(0, tx1 - tx0) : 0;
-  sp.nty = fast_ok ? std::max(0, ty1 - ty0) : 0;
-  if (sp.ntx == 0 || sp.nty == 0) sp.ntx = sp.nty = 0;
-  sp.z0 = 0;
-  sp.z1 = planes;
-  const int64_t inner = static_cast<int64_t>(sp.ntx) * sp.nty;
-  sp.kz = inner > 0 ? chunk_len(inner, planes, 16384) : 1;
-  sp.nzc = inner > 0 ? (planes + sp.kz - 1) / sp.kz : 0;
-  // edge list: every other tile over the whole z range
-  std::vector<int4> edge;
-  const int64_t outer = static_cast<int64_t>(Tx) * Ty - inner;
-  const int kze = chunk_len(std::max<int64_t>(outer, 1), planes, 8192);
-  for (int z = 0; z < planes; z += kze)
-    for (int by = 0; by < Ty; ++by)
-      for (int bx = 0; bx < Tx; ++bx) {
-        const bool in_box = inner > 0 && bx >= sp.tx0 && bx < sp.tx0 + sp.ntx &&
-                            by >= sp.ty0 && by < sp.ty0 + sp.nty;
-        if (!in_box) edge.push_back(make_int4(bx, by, z, std::min(z + kze, planes)));
-      }
-  sp.nshell = static_cast<int>(edge.size());
-  sp.shell = nullptr;
-  if (!edge.empty()) {
-    int4* d = nullptr;
-    BZH_HIP(hipMalloc(&d, edge.size() * sizeof(int4)));
-    BZH_HIP(hipMemcpy(d, edge.data(), edge.size() * sizeof(int4), hipMemcpyHostToDevice));
-    sp.shell = d;
+  int tx0 = (M + 1 + 61) / 62;                                  // 62 tx0 - 1 >= M
+  int tx1 = (g.nx - M - 63) >= 0 ? (g.nx - M - 63) / 62 + 1 : 0;  // 62 (tx1-1) + 62 <= nx-M-1
+  if (tx1 <= tx0) tx0 = tx1 = 0;
+  FusedSplit sp{};
+  sp.interior = {tx0, tx1 - tx0, tx1 - tx0, 0, Ty, 1, 0};
+  sp.edge = {0, tx0 + (Tx - tx1), tx0, tx1 - tx0, Ty, 1, 0};
+  for (FusedBox* b : {&sp.interior, &sp.edge}) {
+    const int64_t tiles = static_cast<int64_t>(b->ntx) * b->nty;
+    if (tiles == 0) continue;
+    b->kz = chunk_len(tiles, planes, 65536);
+    b->nzc = (planes + b->kz - 1) / b->kz;
   }
-  cache[key] = sp;
   return sp;
 }
 
@@ -751,13 +722,30 @@ bool run_program(hipStream_t s, ProgramIn& in) {
     const bzh_buffer* cb = in.groups[3 + c].coeffs;
     if (cb && cb->data && cb->rank >= 1 && cb->dims[0] > 0) h_sources = true;
   }
-  bool h_scalar = true;
+  // the fused kernel takes scalar H coefficients and E decay, E source
+  // coefficients that are all arrays or all scalars, and arrays addressable
+  // with 32-bit element offsets; anything else runs the per-phase schedule
+  bool coef_ok = true;
   for (int q = 0; q < 6; ++q)
-    if (in.h.mat[q] == nullptr || numel(*in.h.mat[q]) != 1) h_scalar = false;
+    if (in.h.mat[q] == nullptr || numel(*in.h.mat[q]) != 1) coef_ok = false;
+  for (int q = 0; q < 3; ++q)
+    if (in.e.mat[q] == nullptr || numel(*in.e.mat[q]) != 1) coef_ok = false;
+  int n_e_arrays = 0;
+  for (int q = 3; q < 6; ++q)
+    if (in.e.mat[q] != nullptr && numel(*in.e.mat[q]) != 1) ++n_e_arrays;
+  coef_ok = coef_ok && (n_e_arrays == 0 || n_e_arrays == 3);
+  // 32-bit byte offsets: every field, coefficient (f32) and psi array (at most
+  // the size of its field) must stay below 4 GiB at 4 bytes per element
+  bool small = g.is3d;
+  for (int c = 0; c < 6 && small; ++c) {
+    int64_t sh[3];
+    yee_shape3d(c, g.nz, g.ny, g.nx, sh);
+    small = sh[0] * sh[1] * sh[2] * 4 < (int64_t{1} << 32);
+  }
   FusedParams cpml_probe;
   std::memset(&cpml_probe, 0, sizeof(cpml_probe));
   const bool fused = two_banks && (in.flags & BZH_FLAG_FUSED_STEP) && g.is3d && !h_sources &&
-                     h_scalar && metrics_empty(in.h) && metrics_empty(in.e) &&
+                     coef_ok && small && metrics_empty(in.h) && metrics_empty(in.e) &&
                      fused_cpml_layout(in.h, in.e, in.psi_b, in.uniform_l, cpml_probe);
   const int32_t* step0 = in.step ? static_cast<const int32_t*>(in.step->data) : nullptr;
   std::vector<Launch> launches;
@@ -812,33 +800,23 @@ bool run_program(hipStream_t s, ProgramIn& in) {
           fp.psi[t] = static_cast<char*>(fp.psi[t]) + fp.psi_delta * (fp.psi_bf16 ? 2 : 4);
         fp.psi_delta = -fp.psi_delta;
       }
-      bool fast_ok = true;
-      for (int c = 0; c < 3; ++c) fast_ok = fast_ok && fp.edec_st[c] == 0;
-      fast_ok = fast_ok && (fp.esrc_st[0] == fp.esrc_st[1] && fp.esrc_st[1] == fp.esrc_st[2]);
-      for (int c = 0; c < 6; ++c) {
-        // 32-bit byte offsets: every field array must stay below 4 GiB
-        int64_t sh[3];
-        yee_shape3d(c, g.nz, g.ny, g.nx, sh);
-        fast_ok = fast_ok && sh[0] * sh[1] * sh[2] * static_cast<int64_t>(elem_bytes(g.dtype)) <
-                                 (int64_t{1} << 32);
-      }
-      const FusedSplit sp = fused_split(g, fp.L, fast_ok);
-      fp.tiles = nullptr;
-      fp.box_tx0 = sp.tx0;
-      fp.box_ty0 = sp.ty0;
-      fp.box_ntx = sp.ntx;
-      fp.box_nty = sp.nty;
-      fp.box_z0 = sp.z0;
-      fp.box_z1 = sp.z1;
-      fp.box_kz = sp.kz;
-      fp.box_nzc = sp.nzc;
-      fp.cpml_kernel = fp.L > 0 ? 1 : 0;
-      if (sp.ntx > 0 && sp.nty > 0 && sp.nzc > 0)
-        launches.push_back(make_launch(Kind::kFused, 0, g.dtype, fp));
-      if (sp.nshell > 0) {
-        fp.tiles = sp.shell;
-        fp.ntiles = sp.nshell;
-        fp.cpml_kernel = fp.L > 0 ? 1 : 0;
+      const FusedSplit sp = fused_split(g, fp.L);
+      fp.rows = fused_rows();
+      fp.wg = fused_wg();
+      fp.box_ty0 = 0;
+      fp.box_z0 = 0;
+      fp.box_z1 = g.nz + 1;
+      for (int v = 0; v < 2; ++v) {
+        const FusedBox& b = v == 0 ? sp.interior : sp.edge;
+        if (b.ntx == 0 || b.nty == 0 || b.nzc == 0) continue;
+        fp.xedge = v;
+        fp.box_tx0 = b.tx0;
+        fp.box_ntx = b.ntx;
+        fp.box_tsplit = b.tsplit;
+        fp.box_tskip = b.tskip;
+        fp.box_nty = b.nty;
+        fp.box_kz = b.kz;
+        fp.box_nzc = b.nzc;
         launches.push_back(make_launch(Kind::kFused, 0, g.dtype, fp));
       }
     } else {

@@ -132,14 +132,18 @@ struct FusedParams {
   void* psi[12];         // bank A: H terms 0..5, E terms 6..11
   int64_t psi_delta;
   int32_t psi_bf16;
-  // work assignment: either a box of tiles (tiles == nullptr) or a list
-  const int4* tiles;     // (bx, by, kz0, kz1) per workgroup
-  int32_t ntiles;
-  int32_t box_tx0, box_ty0, box_ntx, box_nty;  // box of (x, y) tiles
+  // work assignment: a box of (x tile, y tile, z chunk) workgroups.  Logical
+  // x tile b in [0, box_ntx) maps to box_tx0 + b, plus box_tskip when
+  // b >= box_tsplit (the two x-edge ranges of the grid in one launch).
+  int32_t box_tx0, box_ntx, box_tsplit, box_tskip;
+  int32_t box_ty0, box_nty;
   int32_t box_z0, box_z1, box_kz, box_nzc;     // z range [z0, z1) in chunks of kz
-  int32_t cpml_kernel;   // 1: CPML-capable kernel (shell tiles)
+  int32_t xedge;         // 1: tiles touching an x face or x CPML slab
+  int32_t rows;          // tile height (y rows) the box was cut with
+  int32_t wg;            // 1: workgroup-row kernel, 0: wave-tile kernel
 };
 int fused_rows();
+int fused_wg();
 
 // Launchers implemented in the .hip files.
 void launch_fused3d(hipStream_t s, const FusedParams& p, int dtype);

@@ -8,280 +8,67 @@
 // x = i0 - 1 + l: lanes 1..62 own outputs, lanes 0 and 63 are halo columns.
 // Each lane keeps kTY + 2 rows (y = j0 - 1 .. j0 + kTY) of E in registers.
 //
-//   plane k:  issue every load of the plane (E(k+1), H_A(k), coefficients)
+//   plane k:  issue every load of the plane in one batch: E(k+1), H_A(k),
+//               E coefficients and the CPML memories of every active term
 //             H_new(k) on rows j0-1 .. j0+kTY-1, lanes 0..62 (row j0-1 and
 //               lane 0 are recomputed halos; only owned cells are stored)
 //             E_new(k) on owned cells from H_new(k) (own, y-1 in registers,
 //               x-1 by cross-lane permute) and H_new(k-1) (carried along z)
 //
-// There is no LDS traffic and no barrier: x-neighbours cross lanes through
-// ds_bpermute, y-neighbours are in the lane's own registers, z-neighbours
-// are carried from the previous plane.  Every E/H/coefficient element is read
-// from HBM once per step and every output written once (halo re-reads hit the
-// caches): the compulsory-traffic bound the roofline is priced on.
+// Boundary handling costs nothing where it does not apply.  Everything that
+// depends on z (faces, ghost rule, CPML z-terms) is a plane-uniform branch,
+// everything that depends on y a row-uniform branch, and only the tiles that
+// touch an x face or an x CPML slab run the kXEdge variant with per-lane x
+// checks.  There is no LDS traffic and no barrier.  Every address is a
+// uniform 64-bit row base plus a per-lane 32-bit byte offset
+// (global_load ... saddr), so a load costs no address arithmetic.
 //
 // Numerical contract: identical to phase.hip (kernels.py:458-626,
 // lattice.py:523-575, kernels.py:184-264 with a uniform slab thickness L,
 // _boundary_compile.py:118-160); arithmetic is float32.
 
 #include <cstdlib>
+#include <string>
+#include <type_traits>
 
 #include "common.h"
 
 namespace bzh {
 namespace {
 
+// Constant-address-space view: uniform indices become scalar (s_load) reads.
+using cfloat = const __attribute__((address_space(4))) float;
+__device__ __forceinline__ cfloat* as_const(const float* p) { return (cfloat*)(p); }
+
+// Element ``row`` (uniform) of ``base`` plus a per-lane byte offset.
 template <typename T>
-__device__ __forceinline__ float ldf(const void* p, int i) {
-  return load_f(static_cast<const T*>(p), i);
+__device__ __forceinline__ float ldr(const void* base, uint32_t row, uint32_t xb) {
+  const char* p = static_cast<const char*>(base) + static_cast<size_t>(row) * sizeof(T);
+  return load_f(reinterpret_cast<const T*>(p + xb), 0);
+}
+template <typename T>
+__device__ __forceinline__ void str(void* base, uint32_t row, uint32_t xb, float v) {
+  char* p = static_cast<char*>(base) + static_cast<size_t>(row) * sizeof(T);
+  store_f(reinterpret_cast<T*>(p + xb), 0, v);
 }
 
-// Value of the lane above / below (x + 1 / x - 1).
-__device__ __forceinline__ float from_right(float v, int addr_dn) {
-  return __int_as_float(__builtin_amdgcn_ds_bpermute(addr_dn, __float_as_int(v)));
-}
-__device__ __forceinline__ float from_left(float v, int addr_up) {
-  return __int_as_float(__builtin_amdgcn_ds_bpermute(addr_up, __float_as_int(v)));
+// Value of the lane to the right / left (x + 1 / x - 1).
+__device__ __forceinline__ float lane_shift(float v, int addr) {
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(v)));
 }
 
-// Ghost-rule backward difference from loaded neighbours (lattice.py:523-575).
+// Ghost-rule backward difference from loaded neighbours (lattice.py:523-575):
+// ``cur`` is the sample at c, ``prev`` at c - 1, N the sample count.
 __device__ __forceinline__ float gdiff(float cur, float prev, int c, int N, bool mlo, bool mhi) {
   const float up = (c < N) ? cur : (mhi ? 0.0f : prev);
   const float lo = (c > 0) ? prev : (mlo ? 0.0f : cur);
   return up - lo;
 }
 
-// Packed slab coordinate for a uniform thickness L on an axis of extent n.
+// Packed slab coordinate for a uniform thickness L on an axis of n samples
+// (kernels.py:184-222); -1 outside the slabs.  Requires 0 <= c < n.
 __device__ __forceinline__ int slab_q(int c, int n, int L) {
-  return c < L ? c : (c >= n - L ? c - (n - 2 * L) : -1);
-}
-
-struct Ctx {
-  int nz, ny, nx, L, m;
-  float idx;
-};
-
-// Apply one packed CPML term: psi read from bank A (f.psi[t]), written to
-// bank B (psi + delta) when ``own``.  ``lin`` is the psi linear index.
-template <int kT>
-__device__ __forceinline__ float cpml(const FusedParams& f, float d, int q, int lin, bool own) {
-  const int L2 = 2 * f.L;
-  const float* tab = f.tab + kT * 3 * L2;
-  if (f.psi_bf16) {
-    const bf16_t* pin = static_cast<const bf16_t*>(f.psi[kT]);
-    float psi = tab[L2 + q] * load_f(pin, lin) + tab[q] * d;
-    if (own) store_f(const_cast<bf16_t*>(pin) + f.psi_delta, lin, psi);
-    return d * tab[2 * L2 + q] + psi;
-  }
-  const float* pin = static_cast<const float*>(f.psi[kT]);
-  float psi = tab[L2 + q] * pin[lin] + tab[q] * d;
-  if (own) const_cast<float*>(pin)[f.psi_delta + lin] = psi;
-  return d * tab[2 * L2 + q] + psi;
-}
-
-template <typename T, int kTY>
-struct Plane {
-  // E(k) on rows 0..kTY+1 (y = j0-1+r), this lane's column
-  float ex[kTY + 2], ey[kTY + 2], ez[kTY + 2];
-};
-
-template <typename T, int kTY>
-__device__ __forceinline__ void load_e(const FusedParams& f, const Ctx& c, int k, int j0, int x,
-                                       Plane<T, kTY>& P) {
-  const int nz = c.nz, ny = c.ny, nx = c.nx;
-#pragma unroll
-  for (int r = 0; r < kTY + 2; ++r) {
-    const int y = j0 - 1 + r;
-    const bool okx = k <= nz && y >= 0 && y <= ny && x >= 0 && x < nx;
-    const bool oky = k <= nz && y >= 0 && y < ny && x >= 0 && x <= nx;
-    const bool okz = k < nz && y >= 0 && y <= ny && x >= 0 && x <= nx;
-    // clamp to a valid element so every load is unconditional; the value of an
-    // out-of-support sample is replaced by zero
-    const int kx = min(max(k, 0), nz), yx = min(max(y, 0), ny), xx = min(max(x, 0), nx - 1);
-    const int yy = min(max(y, 0), ny - 1), xy = min(max(x, 0), nx);
-    const int kz = min(max(k, 0), nz - 1);
-    const float vx = ldf<T>(f.e_in[0], (kx * (ny + 1) + yx) * nx + xx);
-    const float vy = ldf<T>(f.e_in[1], (kx * ny + yy) * (nx + 1) + xy);
-    const float vz = ldf<T>(f.e_in[2], (kz * (ny + 1) + yx) * (nx + 1) + xy);
-    P.ex[r] = okx ? vx : 0.0f;
-    P.ey[r] = oky ? vy : 0.0f;
-    P.ez[r] = okz ? vz : 0.0f;
-  }
-}
-
-// H_new(k) rows 0..kTY at this lane.  ``own`` selects the cells written to
-// bank B.  kCpml enables the packed CPML terms.
-template <typename T, int kTY, bool kCpml>
-__device__ __forceinline__ void h_plane(const FusedParams& f, const Ctx& c, int k, int j0, int x,
-                                        bool lane_own, bool write, const Plane<T, kTY>& E0,
-                                        const Plane<T, kTY>& E1, int dn, float* hx, float* hy,
-                                        float* hz, const float hc[6]) {
-  const int nz = c.nz, ny = c.ny, nx = c.nx, m = c.m;
-  const float idx = c.idx;
-  // H_A(k) loads (clamped, unconditional)
-  float ha[3][kTY + 1];
-#pragma unroll
-  for (int r = 0; r <= kTY; ++r) {
-    const int y = j0 - 1 + r;
-    const int kk = min(k, nz - 1), ky = min(max(y, 0), ny - 1), kyy = min(max(y, 0), ny);
-    const int xi = min(max(x, 0), nx), xo = min(max(x, 0), nx - 1);
-    ha[0][r] = ldf<T>(f.h_in[0], (kk * ny + ky) * (nx + 1) + xi);
-    ha[1][r] = ldf<T>(f.h_in[1], (kk * (ny + 1) + kyy) * nx + xo);
-    ha[2][r] = ldf<T>(f.h_in[2], (min(k, nz) * ny + ky) * nx + xo);
-  }
-  // per-cell H coefficients (magnetic loss) are rare: scalars live in hc[]
-#pragma unroll
-  for (int r = 0; r <= kTY; ++r) {
-    const int y = j0 - 1 + r;
-    const bool own = write && lane_own && r >= 1;
-    // Hx (nz, ny, nx+1): dEz/dy (+), dEy/dz (-)
-    {
-      const bool ok = k < nz && y >= 0 && y < ny && x >= 0 && x <= nx;
-      float d0 = (E0.ez[r + 1] - E0.ez[r]) * idx;
-      float d1 = (E1.ey[r] - E0.ey[r]) * idx;
-      if constexpr (kCpml) {
-        if (ok) {
-          const int q0 = slab_q(y, ny, c.L);
-          if (q0 >= 0) d0 = cpml<0>(f, d0, q0, (k * 2 * c.L + q0) * (nx + 1) + x, own);
-          const int q1 = slab_q(k, nz, c.L);
-          if (q1 >= 0) d1 = cpml<1>(f, d1, q1, (q1 * ny + y) * (nx + 1) + x, own);
-        }
-      }
-      float h = hc[0] * ha[0][r] - hc[3] * (d0 - d1);
-      if (((m & kLeft) && x == 0) || ((m & kRight) && x == nx)) h = 0.0f;
-      hx[r] = ok ? h : 0.0f;
-    }
-    // Hy (nz, ny+1, nx): dEx/dz (+), dEz/dx (-)
-    {
-      const bool ok = k < nz && y >= 0 && y <= ny && x >= 0 && x < nx;
-      float d2 = (E1.ex[r] - E0.ex[r]) * idx;
-      float d3 = (from_right(E0.ez[r], dn) - E0.ez[r]) * idx;
-      if constexpr (kCpml) {
-        if (ok) {
-          const int q2 = slab_q(k, nz, c.L);
-          if (q2 >= 0) d2 = cpml<2>(f, d2, q2, (q2 * (ny + 1) + y) * nx + x, own);
-          const int q3 = slab_q(x, nx, c.L);
-          if (q3 >= 0) d3 = cpml<3>(f, d3, q3, (k * (ny + 1) + y) * 2 * c.L + q3, own);
-        }
-      }
-      float h = hc[1] * ha[1][r] - hc[4] * (d2 - d3);
-      if (((m & kBottom) && y == 0) || ((m & kTop) && y == ny)) h = 0.0f;
-      hy[r] = ok ? h : 0.0f;
-    }
-    // Hz (nz+1, ny, nx): dEy/dx (+), dEx/dy (-)
-    {
-      const bool ok = k <= nz && y >= 0 && y < ny && x >= 0 && x < nx;
-      float d4 = (from_right(E0.ey[r], dn) - E0.ey[r]) * idx;
-      float d5 = (E0.ex[r + 1] - E0.ex[r]) * idx;
-      if constexpr (kCpml) {
-        if (ok) {
-          const int q4 = slab_q(x, nx, c.L);
-          if (q4 >= 0) d4 = cpml<4>(f, d4, q4, (k * ny + y) * 2 * c.L + q4, own);
-          const int q5 = slab_q(y, ny, c.L);
-          if (q5 >= 0) d5 = cpml<5>(f, d5, q5, (k * 2 * c.L + q5) * nx + x, own);
-        }
-      }
-      float h = hc[2] * ha[2][r] - hc[5] * (d4 - d5);
-      if (((m & kFront) && k == 0) || ((m & kBack) && k == nz)) h = 0.0f;
-      hz[r] = ok ? h : 0.0f;
-    }
-  }
-}
-
-template <typename T, int kTY, bool kCpml>
-__device__ __forceinline__ void plane_step(const FusedParams& f, const Ctx& c, int k, int j0, int x,
-                                           bool lane_own, const Plane<T, kTY>& E0,
-                                           const Plane<T, kTY>& E1, int dn, int up,
-                                           float* prev_hx, float* prev_hy, const float hc[6],
-                                           const float ec[6]) {
-  const int nz = c.nz, ny = c.ny, nx = c.nx, m = c.m;
-  const float idx = c.idx;
-  // E coefficients of this plane's owned rows (per-cell arrays or scalars)
-  float es[3][kTY];
-#pragma unroll
-  for (int r = 0; r < kTY; ++r) {
-    const int y = j0 + r;
-    const int kk = min(k, nz), yy1 = min(y, ny), yy0 = min(y, ny - 1), kz = min(k, nz - 1);
-    const int xo = min(max(x, 0), nx - 1), xi = min(max(x, 0), nx);
-    es[0][r] = f.esrc_st[0] ? f.esrc[0][(kk * (ny + 1) + yy1) * nx + xo] : ec[3];
-    es[1][r] = f.esrc_st[1] ? f.esrc[1][(kk * ny + yy0) * (nx + 1) + xi] : ec[4];
-    es[2][r] = f.esrc_st[2] ? f.esrc[2][(kz * (ny + 1) + yy1) * (nx + 1) + xi] : ec[5];
-  }
-  float hx[kTY + 1], hy[kTY + 1], hz[kTY + 1];
-  h_plane<T, kTY, kCpml>(f, c, k, j0, x, lane_own, true, E0, E1, dn, hx, hy, hz, hc);
-  // store owned H
-#pragma unroll
-  for (int r = 1; r <= kTY; ++r) {
-    const int y = j0 - 1 + r;
-    if (lane_own) {
-      if (k < nz && y < ny && x <= nx) store_f(static_cast<T*>(f.h_out[0]), (k * ny + y) * (nx + 1) + x, hx[r]);
-      if (k < nz && y <= ny && x < nx) store_f(static_cast<T*>(f.h_out[1]), (k * (ny + 1) + y) * nx + x, hy[r]);
-      if (y < ny && x < nx) store_f(static_cast<T*>(f.h_out[2]), (k * ny + y) * nx + x, hz[r]);
-    }
-  }
-  // E_new(k) on owned rows
-#pragma unroll
-  for (int r = 1; r <= kTY; ++r) {
-    const int y = j0 - 1 + r;
-    const float hz_l = from_left(hz[r], up);
-    const float hy_l = from_left(hy[r], up);
-    // Ex (nz+1, ny+1, nx): dHz/dy (+), dHy/dz (-)
-    if (lane_own && y <= ny && x < nx) {
-      float d0 = gdiff(hz[r], hz[r - 1], y, ny, m & kBottom, m & kTop) * idx;
-      float d1 = gdiff(hy[r], prev_hy[r - 1], k, nz, m & kFront, m & kBack) * idx;
-      if constexpr (kCpml) {
-        const int q0 = slab_q(y, ny + 1, c.L);
-        if (q0 >= 0) d0 = cpml<6>(f, d0, q0, (k * 2 * c.L + q0) * nx + x, true);
-        const int q1 = slab_q(k, nz + 1, c.L);
-        if (q1 >= 0) d1 = cpml<7>(f, d1, q1, (q1 * (ny + 1) + y) * nx + x, true);
-      }
-      const int o = (k * (ny + 1) + y) * nx + x;
-      float e = (f.edec_st[0] ? f.edec[0][o] : ec[0]) * E0.ex[r] + es[0][r - 1] * f.ratio_e * (d0 - d1);
-      if (((m & kFront) && k == 0) || ((m & kBack) && k == nz) || ((m & kBottom) && y == 0) ||
-          ((m & kTop) && y == ny))
-        e = 0.0f;
-      store_f(static_cast<T*>(f.e_out[0]), o, e);
-    }
-    // Ey (nz+1, ny, nx+1): dHx/dz (+), dHz/dx (-)
-    if (lane_own && y < ny && x <= nx) {
-      float d2 = gdiff(hx[r], prev_hx[r - 1], k, nz, m & kFront, m & kBack) * idx;
-      float d3 = gdiff(hz[r], hz_l, x, nx, m & kLeft, m & kRight) * idx;
-      if constexpr (kCpml) {
-        const int q2 = slab_q(k, nz + 1, c.L);
-        if (q2 >= 0) d2 = cpml<8>(f, d2, q2, (q2 * ny + y) * (nx + 1) + x, true);
-        const int q3 = slab_q(x, nx + 1, c.L);
-        if (q3 >= 0) d3 = cpml<9>(f, d3, q3, (k * ny + y) * 2 * c.L + q3, true);
-      }
-      const int o = (k * ny + y) * (nx + 1) + x;
-      float e = (f.edec_st[1] ? f.edec[1][o] : ec[1]) * E0.ey[r] + es[1][r - 1] * f.ratio_e * (d2 - d3);
-      if (((m & kFront) && k == 0) || ((m & kBack) && k == nz) || ((m & kLeft) && x == 0) ||
-          ((m & kRight) && x == nx))
-        e = 0.0f;
-      store_f(static_cast<T*>(f.e_out[1]), o, e);
-    }
-    // Ez (nz, ny+1, nx+1): dHy/dx (+), dHx/dy (-)
-    if (lane_own && k < nz && y <= ny && x <= nx) {
-      float d4 = gdiff(hy[r], hy_l, x, nx, m & kLeft, m & kRight) * idx;
-      float d5 = gdiff(hx[r], hx[r - 1], y, ny, m & kBottom, m & kTop) * idx;
-      if constexpr (kCpml) {
-        const int q4 = slab_q(x, nx + 1, c.L);
-        if (q4 >= 0) d4 = cpml<10>(f, d4, q4, (k * (ny + 1) + y) * 2 * c.L + q4, true);
-        const int q5 = slab_q(y, ny + 1, c.L);
-        if (q5 >= 0) d5 = cpml<11>(f, d5, q5, (k * 2 * c.L + q5) * (nx + 1) + x, true);
-      }
-      const int o = (k * (ny + 1) + y) * (nx + 1) + x;
-      float e = (f.edec_st[2] ? f.edec[2][o] : ec[2]) * E0.ez[r] + es[2][r - 1] * f.ratio_e * (d4 - d5);
-      if (((m & kBottom) && y == 0) || ((m & kTop) && y == ny) || ((m & kLeft) && x == 0) ||
-          ((m & kRight) && x == nx))
-        e = 0.0f;
-      store_f(static_cast<T*>(f.e_out[2]), o, e);
-    }
-  }
-#pragma unroll
-  for (int r = 1; r <= kTY; ++r) {
-    prev_hx[r - 1] = hx[r];
-    prev_hy[r - 1] = hy[r];
-  }
+  return L <= 0 ? -1 : (c < L ? c : (c >= n - L ? c - (n - 2 * L) : -1));
 }
 
 // Workgroups are dispatched to the 8 XCDs round-robin (g % 8); give each XCD
@@ -294,210 +81,213 @@ __device__ __forceinline__ int xcd_remap(int g, int n) {
   return c * base + min(c, rem) + m;
 }
 
-template <typename T, int kTY, bool kCpml>
-__global__ void __launch_bounds__(64) fused_step(FusedParams f) {
-  Ctx c;
-  c.nz = f.nz;
-  c.ny = f.ny;
-  c.nx = f.nx;
-  c.L = f.L;
-  c.m = f.metallic;
-  c.idx = f.inv_dx;
+struct Coef {
+  float a, b, ik;
+};
+// Term t's recurrence coefficients at packed coordinate q; tab is [12][3][2L].
+__device__ __forceinline__ Coef coef_u(cfloat* tab, int t, int L2, int q) {
+  return {tab[(t * 3 + 0) * L2 + q], tab[(t * 3 + 1) * L2 + q], tab[(t * 3 + 2) * L2 + q]};
+}
+__device__ __forceinline__ Coef coef_v(const float* tab, int t, int L2, int q) {
+  return {tab[(t * 3 + 0) * L2 + q], tab[(t * 3 + 1) * L2 + q], tab[(t * 3 + 2) * L2 + q]};
+}
+// psi' = b psi + a d;  returns d / kappa + psi'  (kernels.py:237-264)
+__device__ __forceinline__ float cpml_step(const Coef& c, float d, float psi, float& out) {
+  out = c.b * psi + c.a * d;
+  return d * c.ik + out;
+}
+
+template <typename T, typename P, int kTY, bool kEArr, bool kXEdge>
+__global__ void __launch_bounds__(64) fused_tile(FusedParams f) {
   const int lane = threadIdx.x;
-  const int dn = ((lane + 1) & 63) << 2;
-  const int up = ((lane - 1) & 63) << 2;
-
-  int bx, by, kz0, kz1;
-  const int g = xcd_remap(blockIdx.x, gridDim.x);
-  if (f.tiles != nullptr) {
-    const int4 t = f.tiles[g];
-    bx = t.x;
-    by = t.y;
-    kz0 = t.z;
-    kz1 = t.w;
-  } else {
-    int b = g;
-    bx = f.box_tx0 + b % f.box_ntx;
-    b /= f.box_ntx;
-    by = f.box_ty0 + b % f.box_nty;
-    const int bz = b / f.box_nty;
-    kz0 = f.box_z0 + bz * f.box_kz;
-    kz1 = min(kz0 + f.box_kz, f.box_z1);
-  }
-  const int i0 = bx * 62, j0 = by * kTY;
-  const int x = i0 - 1 + lane;
-  const bool lane_own = lane >= 1 && lane <= 62 && x <= c.nx;
-
-  // uniform coefficients (rank-0 device scalars) are read once
-  float hc[6], ec[6];
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    hc[q] = f.hdec_st[q] ? 0.0f : f.hdec[q][0];
-    hc[3 + q] = (f.hsrc_st[q] ? 0.0f : f.hsrc[q][0]) * f.ratio_h;
-    ec[q] = f.edec_st[q] ? 0.0f : f.edec[q][0];
-    ec[3 + q] = f.esrc_st[q] ? 0.0f : f.esrc[q][0];
-  }
-
-  Plane<T, kTY> E0, E1;
-  float prev_hx[kTY], prev_hy[kTY];
-#pragma unroll
-  for (int r = 0; r < kTY; ++r) prev_hx[r] = prev_hy[r] = 0.0f;
-
-  if (kz0 > 0) {
-    // H_new(kz0 - 1) on this lane's rows (not stored: the chunk below owns it)
-    load_e<T, kTY>(f, c, kz0 - 1, j0, x, E1);
-    load_e<T, kTY>(f, c, kz0, j0, x, E0);
-    float hx[kTY + 1], hy[kTY + 1], hz[kTY + 1];
-    h_plane<T, kTY, kCpml>(f, c, kz0 - 1, j0, x, lane_own, false, E1, E0, dn, hx, hy, hz, hc);
-#pragma unroll
-    for (int r = 1; r <= kTY; ++r) {
-      prev_hx[r - 1] = hx[r];
-      prev_hy[r - 1] = hy[r];
-    }
-  } else {
-    load_e<T, kTY>(f, c, kz0, j0, x, E0);
-  }
-
-  for (int k = kz0; k < kz1; ++k) {
-    load_e<T, kTY>(f, c, k + 1, j0, x, E1);
-    plane_step<T, kTY, kCpml>(f, c, k, j0, x, lane_own, E0, E1, dn, up, prev_hx, prev_hy, hc, ec);
-    E0 = E1;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Fast path: tiles whose x/y footprint (halos included) avoids every domain
-// face and the x/y CPML slabs.  All loads and stores are unconditional, the
-// z faces and the z CPML slabs are plane-uniform branches, and every address
-// is a 32-bit byte offset on a uniform base (global_load ... saddr).
-// ---------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ float ldT(const void* b, uint32_t e) {
-  return load_f(reinterpret_cast<const T*>(static_cast<const char*>(b) + e * uint32_t(sizeof(T))), 0);
-}
-template <typename T>
-__device__ __forceinline__ void stT(void* b, uint32_t e, float v) {
-  store_f(reinterpret_cast<T*>(static_cast<char*>(b) + e * uint32_t(sizeof(T))), 0, v);
-}
-__device__ __forceinline__ float ldF(const float* b, uint32_t e) {
-  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(b) + e * 4u);
-}
-
-// z-term CPML (kT: 1 Hx_z, 2 Hy_z, 7 Ex_z, 8 Ey_z) at element offset e of psi
-template <int kT>
-__device__ __forceinline__ float zcpml(const FusedParams& f, float d, int q, uint32_t e, bool own) {
-  const int L2 = 2 * f.L;
-  const float* tab = f.tab + kT * 3 * L2;
-  const float a = tab[q], b = tab[L2 + q], ik = tab[2 * L2 + q];
-  if (f.psi_bf16) {
-    const bf16_t* pin = static_cast<const bf16_t*>(f.psi[kT]);
-    const float psi = b * load_f(pin, e) + a * d;
-    if (own) store_f(const_cast<bf16_t*>(pin) + f.psi_delta, e, psi);
-    return d * ik + psi;
-  }
-  const float* pin = static_cast<const float*>(f.psi[kT]);
-  const float psi = b * ldF(pin, e) + a * d;
-  if (own) *reinterpret_cast<float*>(reinterpret_cast<char*>(const_cast<float*>(pin)) +
-                                     (static_cast<int64_t>(e) + f.psi_delta) * 4) = psi;
-  return d * ik + psi;
-}
-
-template <typename T, int kTY, bool kEArr>
-__global__ void __launch_bounds__(64) fused_fast(FusedParams f) {
-  const int lane = threadIdx.x;
-  const int g = xcd_remap(blockIdx.x, gridDim.x);
-  int b = g;
-  const int bx = f.box_tx0 + b % f.box_ntx;
+  int b = xcd_remap(blockIdx.x, gridDim.x);
+  int bx = b % f.box_ntx;
   b /= f.box_ntx;
+  if (bx >= f.box_tsplit) bx += f.box_tskip;
+  bx += f.box_tx0;
   const int by = f.box_ty0 + b % f.box_nty;
-  const int bz = b / f.box_nty;
-  const int kz0 = f.box_z0 + bz * f.box_kz;
+  const int kz0 = f.box_z0 + (b / f.box_nty) * f.box_kz;
   const int kz1 = min(kz0 + f.box_kz, f.box_z1);
 
-  const int nz = f.nz, ny = f.ny, nx = f.nx, L = f.L, m = f.metallic;
+  const int nz = f.nz, ny = f.ny, nx = f.nx, L = f.L, L2 = 2 * f.L;
+  const int m = f.metallic;
+  const bool mf = m & kFront, mk = m & kBack, mb = m & kBottom, mt = m & kTop;
+  const bool ml = m & kLeft, mr = m & kRight;
   const float idx = f.inv_dx;
   const int j0 = by * kTY;
   const int x = bx * 62 - 1 + lane;
-  const bool own = lane >= 1 && lane <= 62;
+  const int dn = ((lane + 1) & 63) << 2;  // permute address of lane + 1
+  const int up = ((lane - 1) & 63) << 2;  // ... of lane - 1
+  cfloat* tab = as_const(f.tab);
 
-  // element geometry of the four layout classes (plane stride, row stride)
-  //   X: Ex, Hy, e_source_x    (ny+1) x nx
-  //   Y: Ey, Hx, e_source_y     ny    x (nx+1)
-  //   Z: Ez, e_source_z        (ny+1) x (nx+1)
-  //   W: Hz                     ny    x nx
-  const uint32_t rX = nx, rY = nx + 1, rZ = nx + 1, rW = nx;
-  const uint32_t pX = (ny + 1) * nx, pY = ny * (nx + 1), pZ = (ny + 1) * (nx + 1), pW = ny * nx;
-  const uint32_t y0 = j0 - 1;
-  const uint32_t bX = y0 * rX + x, bY = y0 * rY + x, bZ = y0 * rZ + x, bW = y0 * rW + x;
+  // ---- x: per-lane, fixed for the whole wave ----
+  // X-class arrays (Ex, Hy, Hz) have nx columns, Y-class (Ey, Ez, Hx) nx + 1
+  bool vxX = true, vxY = true, xface = false;
+  int xX = x, xY = x, qxH = -1, qxE = -1;
+  if constexpr (kXEdge) {
+    vxX = x >= 0 && x < nx;
+    vxY = x >= 0 && x <= nx;
+    xX = min(max(x, 0), nx - 1);
+    xY = min(max(x, 0), nx);
+    qxH = vxX ? slab_q(x, nx, L) : -1;
+    qxE = vxY ? slab_q(x, nx + 1, L) : -1;
+    xface = (ml && x == 0) || (mr && x == nx);
+  }
+  const bool own = lane >= 1 && lane <= 62 && (!kXEdge || x <= nx);
+  const uint32_t tX = xX * sizeof(T), tY = xY * sizeof(T);
+  const uint32_t pX = xX * sizeof(P), pY = xY * sizeof(P);
+  const uint32_t cX = xX * 4u, cY = xY * 4u;
+  const uint32_t qH = max(qxH, 0) * sizeof(P), qE = max(qxE, 0) * sizeof(P);
+  Coef c3{}, c4{}, c9{}, c10{};
+  if constexpr (kXEdge) {
+    if (qxH >= 0) {
+      c3 = coef_v(f.tab, 3, L2, qxH);
+      c4 = coef_v(f.tab, 4, L2, qxH);
+    }
+    if (qxE >= 0) {
+      c9 = coef_v(f.tab, 9, L2, qxE);
+      c10 = coef_v(f.tab, 10, L2, qxE);
+    }
+  }
 
+  // bank-B twin of each CPML memory
+  void* pout[12];
+#pragma unroll
+  for (int t = 0; t < 12; ++t)
+    pout[t] = static_cast<char*>(f.psi[t]) + f.psi_delta * static_cast<int64_t>(sizeof(P));
+
+  // uniform coefficients (rank-0 device scalars)
   float hc[6], ec[6];
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
-    hc[q] = f.hdec[q][0];
-    hc[3 + q] = f.hsrc[q][0] * f.ratio_h;
-    ec[q] = f.edec[q][0];
-    ec[3 + q] = kEArr ? 0.0f : f.esrc[q][0] * f.ratio_e;
+    hc[q] = as_const(f.hdec[q])[0];
+    hc[3 + q] = as_const(f.hsrc[q])[0] * f.ratio_h;
+    ec[q] = as_const(f.edec[q])[0];
+    ec[3 + q] = kEArr ? 0.0f : as_const(f.esrc[q])[0] * f.ratio_e;
   }
-  const int dn = ((lane + 1) & 63) << 2;
-  const int up = ((lane - 1) & 63) << 2;
+
+  // ---- y: per row, uniform ----
+  // rows r = 0 .. kTY + 1 hold y = j0 - 1 + r; X-rows (Ex, Hy, Ez) run 0..ny,
+  // Y-rows (Ey, Hx, Hz) 0..ny-1.  Out-of-support rows load a clamped row.
+  int yX[kTY + 2], yY[kTY + 2];
+  bool vyX[kTY + 2], vyY[kTY + 2];
+#pragma unroll
+  for (int r = 0; r < kTY + 2; ++r) {
+    const int y = j0 - 1 + r;
+    vyX[r] = y >= 0 && y <= ny;
+    vyY[r] = y >= 0 && y < ny;
+    yX[r] = min(max(y, 0), ny);
+    yY[r] = min(max(y, 0), ny - 1);
+  }
 
   float ex0[kTY + 2], ey0[kTY + 2], ez0[kTY + 2];
   float ex1[kTY + 2], ey1[kTY + 2], ez1[kTY + 2];
-  float phx[kTY], phy[kTY];
+  float phx[kTY], phy[kTY];  // H_new(k - 1) of the owned rows
 
+  // E(k) rows 0..kTY+1; Ex/Ey exist on planes 0..nz, Ez on 0..nz-1
   auto load_plane = [&](int k, float* ex, float* ey, float* ez) {
-    // Ex/Ey exist on planes 0..nz, Ez on 0..nz-1; a missing plane reads as 0
     const bool vxy = k <= nz, vz = k < nz;
     const uint32_t kx = min(k, nz), kz = min(k, nz - 1);
-    const uint32_t oX = kx * pX + bX, oY = kx * pY + bY, oZ = kz * pZ + bZ;
 #pragma unroll
     for (int r = 0; r < kTY + 2; ++r) {
-      const float vx = ldT<T>(f.e_in[0], oX + r * rX);
-      const float vy = ldT<T>(f.e_in[1], oY + r * rY);
-      const float vzz = ldT<T>(f.e_in[2], oZ + r * rZ);
-      ex[r] = vxy ? vx : 0.0f;
-      ey[r] = vxy ? vy : 0.0f;
-      ez[r] = vz ? vzz : 0.0f;
+      const float a = ldr<T>(f.e_in[0], (kx * (ny + 1) + yX[r]) * nx, tX);
+      const float c = ldr<T>(f.e_in[1], (kx * ny + yY[r]) * (nx + 1), tY);
+      const float d = ldr<T>(f.e_in[2], (kz * (ny + 1) + yX[r]) * (nx + 1), tY);
+      ex[r] = (vxy && vyX[r] && vxX) ? a : 0.0f;
+      ey[r] = (vxy && vyY[r] && vxY) ? c : 0.0f;
+      ez[r] = (vz && vyX[r] && vxY) ? d : 0.0f;
     }
   };
 
-  // H_new(k) on rows 0..kTY (row 0 = y0 halo) at lanes 0..62
+  // H_new(k) on rows 0..kTY (y = j0-1 .. j0+kTY-1) at this lane.  With
+  // ``write`` the CPML memories of owned cells are stored to bank B.
+  // Hx, Hy exist on planes 0..nz-1, Hz on 0..nz.
   auto h_rows = [&](int k, const float* ex, const float* ey, const float* ez, const float* exn,
                     const float* eyn, float* hx, float* hy, float* hz, bool write) {
-    const bool vxy = k < nz;  // Hx, Hy exist on planes 0..nz-1
+    const bool vxy = k < nz;
     const uint32_t kk = min(k, nz - 1);
-    const uint32_t oY = kk * pY + bY, oX = kk * pX + bX, oW = k * pW + bW;
-    float ha[3][kTY + 1];
+    const int qz = vxy ? slab_q(k, nz, L) : -1;
+    float ha0[kTY + 1], ha1[kTY + 1], ha2[kTY + 1];
+    float p0[kTY + 1], p1[kTY + 1], p2[kTY + 1], p3[kTY + 1], p4[kTY + 1], p5[kTY + 1];
+    int qy[kTY + 1];
 #pragma unroll
     for (int r = 0; r <= kTY; ++r) {
-      ha[0][r] = ldT<T>(f.h_in[0], oY + r * rY);
-      ha[1][r] = ldT<T>(f.h_in[1], oX + r * rX);
-      ha[2][r] = ldT<T>(f.h_in[2], oW + r * rW);
-    }
-    const int qz = L > 0 && vxy ? slab_q(k, nz, L) : -1;
-    const bool hz_pec = ((m & kFront) && k == 0) || ((m & kBack) && k == nz);
-#pragma unroll
-    for (int r = 0; r <= kTY; ++r) {
-      const bool w = write && own && r >= 1;
-      float d0 = (ez[r + 1] - ez[r]) * idx;
-      float d1 = (eyn[r] - ey[r]) * idx;
-      float d2 = (exn[r] - ex[r]) * idx;
-      float d3 = (from_right(ez[r], dn) - ez[r]) * idx;
-      float d4 = (from_right(ey[r], dn) - ey[r]) * idx;
-      float d5 = (ex[r + 1] - ex[r]) * idx;
-      if (qz >= 0) {
-        d1 = zcpml<1>(f, d1, qz, qz * pY + bY + r * rY, w);
-        d2 = zcpml<2>(f, d2, qz, qz * pX + bX + r * rX, w);
+      ha0[r] = ldr<T>(f.h_in[0], (kk * ny + yY[r]) * (nx + 1), tY);
+      ha1[r] = ldr<T>(f.h_in[1], (kk * (ny + 1) + yX[r]) * nx, tX);
+      ha2[r] = ldr<T>(f.h_in[2], (k * ny + yY[r]) * nx, tX);
+      qy[r] = vyY[r] ? slab_q(j0 - 1 + r, ny, L) : -1;  // Hx_y and Hz_y: ny samples
+      p0[r] = p1[r] = p2[r] = p3[r] = p4[r] = p5[r] = 0.0f;
+      if (qy[r] >= 0) {
+        p0[r] = ldr<P>(f.psi[0], (kk * L2 + qy[r]) * (nx + 1), pY);
+        p5[r] = ldr<P>(f.psi[5], (k * L2 + qy[r]) * nx, pX);
       }
-      hx[r] = vxy ? hc[0] * ha[0][r] - hc[3] * (d0 - d1) : 0.0f;
-      hy[r] = vxy ? hc[1] * ha[1][r] - hc[4] * (d2 - d3) : 0.0f;
-      hz[r] = hz_pec ? 0.0f : hc[2] * ha[2][r] - hc[5] * (d4 - d5);
+      if (qz >= 0) {
+        p1[r] = ldr<P>(f.psi[1], (qz * ny + yY[r]) * (nx + 1), pY);
+        p2[r] = ldr<P>(f.psi[2], (qz * (ny + 1) + yX[r]) * nx, pX);
+      }
+      if constexpr (kXEdge) {
+        if (qxH >= 0) {
+          p3[r] = ldr<P>(f.psi[3], (kk * (ny + 1) + yX[r]) * L2, qH);
+          p4[r] = ldr<P>(f.psi[4], (k * ny + yY[r]) * L2, qH);
+        }
+      }
+    }
+    Coef c1{}, c2{};
+    if (qz >= 0) {
+      c1 = coef_u(tab, 1, L2, qz);
+      c2 = coef_u(tab, 2, L2, qz);
+    }
+    const bool hz_pec = (mf && k == 0) || (mk && k == nz);
+#pragma unroll
+    for (int r = 0; r <= kTY; ++r) {
+      const int y = j0 - 1 + r;
+      const bool w = write && own && r >= 1;
+      const bool okx = vxy && vyY[r] && vxY;  // Hx
+      const bool oky = vxy && vyX[r] && vxX;  // Hy
+      const bool okz = vyY[r] && vxX;         // Hz
+      float d0 = (ez[r + 1] - ez[r]) * idx;                 // dEz/dy
+      float d1 = (eyn[r] - ey[r]) * idx;                    // dEy/dz
+      float d2 = (exn[r] - ex[r]) * idx;                    // dEx/dz
+      float d3 = (lane_shift(ez[r], dn) - ez[r]) * idx;     // dEz/dx
+      float d4 = (lane_shift(ey[r], dn) - ey[r]) * idx;     // dEy/dx
+      float d5 = (ex[r + 1] - ex[r]) * idx;                 // dEx/dy
+      float n;
+      if (qy[r] >= 0) {
+        d0 = cpml_step(coef_u(tab, 0, L2, qy[r]), d0, p0[r], n);
+        if (w && okx) str<P>(pout[0], (k * L2 + qy[r]) * (nx + 1), pY, n);
+        d5 = cpml_step(coef_u(tab, 5, L2, qy[r]), d5, p5[r], n);
+        if (w && okz) str<P>(pout[5], (k * L2 + qy[r]) * nx, pX, n);
+      }
+      if (qz >= 0) {
+        d1 = cpml_step(c1, d1, p1[r], n);
+        if (w && okx) str<P>(pout[1], (qz * ny + yY[r]) * (nx + 1), pY, n);
+        d2 = cpml_step(c2, d2, p2[r], n);
+        if (w && oky) str<P>(pout[2], (qz * (ny + 1) + yX[r]) * nx, pX, n);
+      }
+      if constexpr (kXEdge) {
+        if (qxH >= 0) {
+          d3 = cpml_step(c3, d3, p3[r], n);
+          if (w && oky) str<P>(pout[3], (k * (ny + 1) + yX[r]) * L2, qH, n);
+          d4 = cpml_step(c4, d4, p4[r], n);
+          if (w && okz) str<P>(pout[4], (k * ny + yY[r]) * L2, qH, n);
+        }
+      }
+      float vx = hc[0] * ha0[r] - hc[3] * (d0 - d1);
+      float vy = hc[1] * ha1[r] - hc[4] * (d2 - d3);
+      float vz = hc[2] * ha2[r] - hc[5] * (d4 - d5);
+      if (kXEdge && xface) vx = 0.0f;
+      if ((mb && y == 0) || (mt && y == ny)) vy = 0.0f;
+      if (hz_pec) vz = 0.0f;
+      hx[r] = okx ? vx : 0.0f;
+      hy[r] = oky ? vy : 0.0f;
+      hz[r] = okz ? vz : 0.0f;
     }
   };
 
 #pragma unroll
   for (int r = 0; r < kTY; ++r) phx[r] = phy[r] = 0.0f;
   if (kz0 > 0) {
+    // H_new(kz0 - 1) on this lane's rows (not stored: the chunk below owns it)
     float hx[kTY + 1], hy[kTY + 1], hz[kTY + 1];
     load_plane(kz0 - 1, ex1, ey1, ez1);
     load_plane(kz0, ex0, ey0, ez0);
@@ -513,68 +303,110 @@ __global__ void __launch_bounds__(64) fused_fast(FusedParams f) {
 
   for (int k = kz0; k < kz1; ++k) {
     load_plane(k + 1, ex1, ey1, ez1);
-    float es[3][kTY];
-    if constexpr (kEArr) {
-      const uint32_t kx = min(k, nz), kz = min(k, nz - 1);
-      const uint32_t oX = kx * pX + bX + rX, oY = kx * pY + bY + rY, oZ = kz * pZ + bZ + rZ;
+    const bool vxy = k <= nz, vz = k < nz;  // Ex/Ey planes, Ez planes
+    const uint32_t kx = min(k, nz), kz = min(k, nz - 1);
+    const int qz = slab_q(k, nz + 1, L);
+    // E coefficients and E-term CPML memories of the owned rows (s = r - 1)
+    float es0[kTY], es1[kTY], es2[kTY];
+    float p6[kTY], p7[kTY], p8[kTY], p9[kTY], p10[kTY], p11[kTY];
+    int qy[kTY];
 #pragma unroll
-      for (int r = 0; r < kTY; ++r) {
-        es[0][r] = ldF(f.esrc[0], oX + r * rX) * f.ratio_e;
-        es[1][r] = ldF(f.esrc[1], oY + r * rY) * f.ratio_e;
-        es[2][r] = ldF(f.esrc[2], oZ + r * rZ) * f.ratio_e;
+    for (int s = 0; s < kTY; ++s) {
+      const int r = s + 1;
+      if constexpr (kEArr) {
+        es0[s] = ldr<float>(f.esrc[0], (kx * (ny + 1) + yX[r]) * nx, cX) * f.ratio_e;
+        es1[s] = ldr<float>(f.esrc[1], (kx * ny + yY[r]) * (nx + 1), cY) * f.ratio_e;
+        es2[s] = ldr<float>(f.esrc[2], (kz * (ny + 1) + yX[r]) * (nx + 1), cY) * f.ratio_e;
+      } else {
+        es0[s] = ec[3];
+        es1[s] = ec[4];
+        es2[s] = ec[5];
       }
-    } else {
-#pragma unroll
-      for (int r = 0; r < kTY; ++r) {
-        es[0][r] = ec[3];
-        es[1][r] = ec[4];
-        es[2][r] = ec[5];
+      qy[s] = vyX[r] ? slab_q(j0 + s, ny + 1, L) : -1;  // Ex_y and Ez_y: ny + 1 samples
+      p6[s] = p7[s] = p8[s] = p9[s] = p10[s] = p11[s] = 0.0f;
+      if (qy[s] >= 0) {
+        p6[s] = ldr<P>(f.psi[6], (kx * L2 + qy[s]) * nx, pX);
+        p11[s] = ldr<P>(f.psi[11], (kz * L2 + qy[s]) * (nx + 1), pY);
+      }
+      if (qz >= 0) {
+        p7[s] = ldr<P>(f.psi[7], (qz * (ny + 1) + yX[r]) * nx, pX);
+        p8[s] = ldr<P>(f.psi[8], (qz * ny + yY[r]) * (nx + 1), pY);
+      }
+      if constexpr (kXEdge) {
+        if (qxE >= 0) {
+          p9[s] = ldr<P>(f.psi[9], (kx * ny + yY[r]) * L2, qE);
+          p10[s] = ldr<P>(f.psi[10], (kz * (ny + 1) + yX[r]) * L2, qE);
+        }
       }
     }
     float hx[kTY + 1], hy[kTY + 1], hz[kTY + 1];
     h_rows(k, ex0, ey0, ez0, ex1, ey1, hx, hy, hz, true);
-
-    const bool vxy = k < nz;
-    const uint32_t kk = min(k, nz - 1);
-    const uint32_t hoY = kk * pY + bY, hoX = kk * pX + bX, hoW = k * pW + bW;
     if (own) {
 #pragma unroll
       for (int r = 1; r <= kTY; ++r) {
-        if (vxy) {
-          stT<T>(f.h_out[0], hoY + r * rY, hx[r]);
-          stT<T>(f.h_out[1], hoX + r * rX, hy[r]);
-        }
-        stT<T>(f.h_out[2], hoW + r * rW, hz[r]);
+        if (vz && vyY[r] && vxY) str<T>(f.h_out[0], (k * ny + yY[r]) * (nx + 1), tY, hx[r]);
+        if (vz && vyX[r] && vxX) str<T>(f.h_out[1], (k * (ny + 1) + yX[r]) * nx, tX, hy[r]);
+        if (vyY[r] && vxX) str<T>(f.h_out[2], (k * ny + yY[r]) * nx, tX, hz[r]);
       }
     }
-    // E_new(k): z ghost rule and z-face PEC are plane-uniform
-    const int qz = L > 0 ? slab_q(k, nz + 1, L) : -1;
-    const bool exy_pec = ((m & kFront) && k == 0) || ((m & kBack) && k == nz);
-    const bool mlo = m & kFront, mhi = m & kBack;
-    const uint32_t kx = min(k, nz), kz = min(k, nz - 1);
-    const uint32_t eoX = kx * pX + bX, eoY = kx * pY + bY, eoZ = kz * pZ + bZ;
+    Coef c7{}, c8{};
+    if (qz >= 0) {
+      c7 = coef_u(tab, 7, L2, qz);
+      c8 = coef_u(tab, 8, L2, qz);
+    }
+    const bool exy_pec = (mf && k == 0) || (mk && k == nz);
 #pragma unroll
     for (int r = 1; r <= kTY; ++r) {
-      const float hz_l = from_left(hz[r], up);
-      const float hy_l = from_left(hy[r], up);
-      float d0 = (hz[r] - hz[r - 1]) * idx;
-      float d1 = gdiff(hy[r], phy[r - 1], k, nz, mlo, mhi) * idx;
-      float d2 = gdiff(hx[r], phx[r - 1], k, nz, mlo, mhi) * idx;
-      float d3 = (hz[r] - hz_l) * idx;
-      float d4 = (hy[r] - hy_l) * idx;
-      float d5 = (hx[r] - hx[r - 1]) * idx;
+      const int s = r - 1;
+      const int y = j0 - 1 + r;
+      const float hz_l = lane_shift(hz[r], up);
+      const float hy_l = lane_shift(hy[r], up);
+      float d0 = gdiff(hz[r], hz[r - 1], y, ny, mb, mt) * idx;  // dHz/dy
+      float d1 = gdiff(hy[r], phy[s], k, nz, mf, mk) * idx;     // dHy/dz
+      float d2 = gdiff(hx[r], phx[s], k, nz, mf, mk) * idx;     // dHx/dz
+      float d3, d4;                                             // dHz/dx, dHy/dx
+      if constexpr (kXEdge) {
+        d3 = gdiff(hz[r], hz_l, x, nx, ml, mr) * idx;
+        d4 = gdiff(hy[r], hy_l, x, nx, ml, mr) * idx;
+      } else {
+        d3 = (hz[r] - hz_l) * idx;
+        d4 = (hy[r] - hy_l) * idx;
+      }
+      float d5 = gdiff(hx[r], hx[r - 1], y, ny, mb, mt) * idx;  // dHx/dy
+      const bool okx = own && vxy && vyX[r] && vxX;  // Ex
+      const bool oky = own && vxy && vyY[r] && vxY;  // Ey
+      const bool okz = own && vz && vyX[r] && vxY;   // Ez
+      float n;
+      if (qy[s] >= 0) {
+        d0 = cpml_step(coef_u(tab, 6, L2, qy[s]), d0, p6[s], n);
+        if (okx) str<P>(pout[6], (kx * L2 + qy[s]) * nx, pX, n);
+        d5 = cpml_step(coef_u(tab, 11, L2, qy[s]), d5, p11[s], n);
+        if (okz) str<P>(pout[11], (kz * L2 + qy[s]) * (nx + 1), pY, n);
+      }
       if (qz >= 0) {
-        d1 = zcpml<7>(f, d1, qz, qz * pX + bX + r * rX, own);
-        d2 = zcpml<8>(f, d2, qz, qz * pY + bY + r * rY, own);
+        d1 = cpml_step(c7, d1, p7[s], n);
+        if (okx) str<P>(pout[7], (qz * (ny + 1) + yX[r]) * nx, pX, n);
+        d2 = cpml_step(c8, d2, p8[s], n);
+        if (oky) str<P>(pout[8], (qz * ny + yY[r]) * (nx + 1), pY, n);
       }
-      const float nex = exy_pec ? 0.0f : ec[0] * ex0[r] + es[0][r - 1] * (d0 - d1);
-      const float ney = exy_pec ? 0.0f : ec[1] * ey0[r] + es[1][r - 1] * (d2 - d3);
-      const float nez = ec[2] * ez0[r] + es[2][r - 1] * (d4 - d5);
-      if (own) {
-        stT<T>(f.e_out[0], eoX + r * rX, nex);
-        stT<T>(f.e_out[1], eoY + r * rY, ney);
-        if (k < nz) stT<T>(f.e_out[2], eoZ + r * rZ, nez);
+      if constexpr (kXEdge) {
+        if (qxE >= 0) {
+          d3 = cpml_step(c9, d3, p9[s], n);
+          if (oky) str<P>(pout[9], (kx * ny + yY[r]) * L2, qE, n);
+          d4 = cpml_step(c10, d4, p10[s], n);
+          if (okz) str<P>(pout[10], (kz * (ny + 1) + yX[r]) * L2, qE, n);
+        }
       }
+      float nex = ec[0] * ex0[r] + es0[s] * (d0 - d1);
+      float ney = ec[1] * ey0[r] + es1[s] * (d2 - d3);
+      float nez = ec[2] * ez0[r] + es2[s] * (d4 - d5);
+      const bool y_pec = (mb && y == 0) || (mt && y == ny);
+      if (exy_pec || y_pec) nex = 0.0f;
+      if (exy_pec || (kXEdge && xface)) ney = 0.0f;
+      if (y_pec || (kXEdge && xface)) nez = 0.0f;
+      if (okx) str<T>(f.e_out[0], (kx * (ny + 1) + yX[r]) * nx, tX, nex);
+      if (oky) str<T>(f.e_out[1], (kx * ny + yY[r]) * (nx + 1), tY, ney);
+      if (okz) str<T>(f.e_out[2], (kz * (ny + 1) + yX[r]) * (nx + 1), tY, nez);
     }
 #pragma unroll
     for (int r = 0; r < kTY; ++r) {
@@ -590,59 +422,455 @@ __global__ void __launch_bounds__(64) fused_fast(FusedParams f) {
   }
 }
 
-template <typename T, int kR>
-void dispatch_rows(hipStream_t s, const FusedParams& p, int nblocks) {
-  dim3 block(64, 1, 1);
-  dim3 grid(nblocks, 1, 1);
-  if (p.tiles == nullptr) {
-    bool earr = p.esrc_st[0] && p.esrc_st[1] && p.esrc_st[2];
-    if (earr) hipLaunchKernelGGL((fused_fast<T, kR, true>), grid, block, 0, s, p);
-    else hipLaunchKernelGGL((fused_fast<T, kR, false>), grid, block, 0, s, p);
-  } else if (p.cpml_kernel) {
-    hipLaunchKernelGGL((fused_step<T, kR, true>), grid, block, 0, s, p);
+// ---------------------------------------------------------------------------
+// Workgroup-row variant: a workgroup of kR + 1 waves owns a 62 x kR tile and
+// marches along z.  Wave w sits on row y = j0 - 1 + w: waves 1..kR own rows
+// j0 .. j0+kR-1, wave 0 computes the halo row j0-1 of H (Hx, Hz, needed by
+// the E update of row j0) and also loads the top halo row j0+kR of E (Ez,
+// Ex, needed by the H update of row j0+kR-1).  Rows cross through LDS:
+//
+//   lE[b][s] = (Ez, Ex) of E(k) on row j0-1+s  (s = 0 .. kR+1), b = k & 1
+//   lH[b][w] = (Hz, Hx) of H_new(k) on row j0-1+w
+//
+// One barrier per plane: E(k+1) rows are published to the other buffer
+// before the barrier of plane k and read after it, and the buffers are reused
+// only two planes later, after another barrier.  Each lane holds one row, so
+// the per-lane state is a few dozen registers and many waves per SIMD keep
+// the loads in flight.
+// ---------------------------------------------------------------------------
+// Global-address-space views: the row base stays a scalar pair and the load
+// takes the ``global_load v, v_off, s[base]`` form.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+
+// Array base (kernel argument, uniform) plus a 32-bit byte offset formed next
+// to the access: ``row * sizeof(T) + xb`` with the per-lane byte offset xb.
+// Every array the fused kernel touches is below 4 GiB (checked on the host),
+// so the offset fits and the access selects ``global_load v, v_off, s[base]``.
+template <typename T>
+__device__ __forceinline__ gptr<const char> at(const void* base, uint32_t row, uint32_t xb) {
+  const uint32_t off = row * static_cast<uint32_t>(sizeof(T)) + xb;
+  return (gptr<const char>)(base) + off;
+}
+__device__ __forceinline__ float ld_g16(gptr<const char> p, bool bf) {
+  const uint16_t u = *reinterpret_cast<gptr<const uint16_t>>(p);
+  return bf ? __uint_as_float(static_cast<uint32_t>(u) << 16) : __half2float(__ushort_as_half(u));
+}
+template <typename T>
+__device__ __forceinline__ float ldu(const void* base, uint32_t row, uint32_t xb) {
+  gptr<const char> p = at<T>(base, row, xb);
+  if constexpr (sizeof(T) == 4) return *reinterpret_cast<gptr<const float>>(p);
+  else return ld_g16(p, std::is_same<T, bf16_t>::value);
+}
+template <typename T>
+__device__ __forceinline__ void stu(void* base, uint32_t row, uint32_t xb, float v) {
+  gptr<char> p = (gptr<char>)(at<T>(base, row, xb));
+  if constexpr (sizeof(T) == 4) {
+    *reinterpret_cast<gptr<float>>(p) = v;
   } else {
-    hipLaunchKernelGGL((fused_step<T, kR, false>), grid, block, 0, s, p);
+    T tmp;
+    store_f(&tmp, 0, v);
+    uint16_t u;
+    __builtin_memcpy(&u, &tmp, 2);
+    *reinterpret_cast<gptr<uint16_t>>(p) = u;
+  }
+}
+// Bank-B twin of CPML memory t (psi_delta elements past the bank-A array).
+template <typename P>
+__device__ __forceinline__ void* pout_base(const FusedParams& f, int t) {
+  return static_cast<char*>(f.psi[t]) + f.psi_delta * static_cast<int64_t>(sizeof(P));
+}
+
+template <typename T, typename P, int kR, bool kEArr, bool kXEdge>
+__global__ void __launch_bounds__(64 * (kR + 1)) fused_wg(FusedParams f) {
+  __shared__ float lE[2][kR + 2][2][64];
+  __shared__ float lH[2][kR + 1][2][64];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const bool halo = w == 0;
+
+  int b = xcd_remap(blockIdx.x, gridDim.x);
+  int bx = b % f.box_ntx;
+  b /= f.box_ntx;
+  if (bx >= f.box_tsplit) bx += f.box_tskip;
+  bx += f.box_tx0;
+  const int by = f.box_ty0 + b % f.box_nty;
+  const int kz0 = f.box_z0 + (b / f.box_nty) * f.box_kz;
+  const int kz1 = min(kz0 + f.box_kz, f.box_z1);
+
+  const int nz = f.nz, ny = f.ny, nx = f.nx, L = f.L, L2 = 2 * f.L;
+  const int m = f.metallic;
+  const bool mf = m & kFront, mk = m & kBack, mb = m & kBottom, mt = m & kTop;
+  const bool ml = m & kLeft, mr = m & kRight;
+  const float idx = f.inv_dx;
+  const int j0 = by * kR;
+  const int x = bx * 62 - 1 + lane;
+  const int dn = ((lane + 1) & 63) << 2;  // permute address of lane + 1
+  const int up = ((lane - 1) & 63) << 2;  // ... of lane - 1
+  cfloat* tab = as_const(f.tab);
+
+  // ---- x: per lane ----
+  bool vxX = true, vxY = true, xface = false;
+  int xX = x, xY = x, qxH = -1, qxE = -1;
+  if constexpr (kXEdge) {
+    vxX = x >= 0 && x < nx;
+    vxY = x >= 0 && x <= nx;
+    xX = min(max(x, 0), nx - 1);
+    xY = min(max(x, 0), nx);
+    qxH = vxX ? slab_q(x, nx, L) : -1;
+    qxE = vxY ? slab_q(x, nx + 1, L) : -1;
+    xface = (ml && x == 0) || (mr && x == nx);
+  }
+  const bool own = !halo && lane >= 1 && lane <= 62 && (!kXEdge || x <= nx);
+  const uint32_t tX = xX * sizeof(T), tY = xY * sizeof(T);
+  const uint32_t pX = xX * sizeof(P), pY = xY * sizeof(P);
+  const uint32_t cX = xX * 4u, cY = xY * 4u;
+  const uint32_t qH = max(qxH, 0) * sizeof(P), qE = max(qxE, 0) * sizeof(P);
+  Coef c3{}, c4{}, c9{}, c10{};
+  if constexpr (kXEdge) {
+    if (qxH >= 0) {
+      c3 = coef_v(f.tab, 3, L2, qxH);
+      c4 = coef_v(f.tab, 4, L2, qxH);
+    }
+    if (qxE >= 0) {
+      c9 = coef_v(f.tab, 9, L2, qxE);
+      c10 = coef_v(f.tab, 10, L2, qxE);
+    }
+  }
+
+  // ---- y: this wave's row (uniform) ----
+  const int y = j0 - 1 + w;
+  const bool vyX = y >= 0 && y <= ny, vyY = y >= 0 && y < ny;
+  const int yX = min(max(y, 0), ny), yY = min(max(y, 0), ny - 1);
+  const int qyH = vyY ? slab_q(y, ny, L) : -1;      // Hx_y, Hz_y: ny samples
+  const int qyE = vyX ? slab_q(y, ny + 1, L) : -1;  // Ex_y, Ez_y: ny + 1 samples
+  const bool y_pec_h = (mb && y == 0) || (mt && y == ny);  // Hy normal to a y face
+  // top halo row of E, loaded by wave 0
+  const int yt = j0 + kR;
+  const bool vtX = yt <= ny;
+  const int ytX = min(yt, ny);
+
+  float hc[6], ec[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    hc[q] = as_const(f.hdec[q])[0];
+    hc[3 + q] = as_const(f.hsrc[q])[0] * f.ratio_h;
+    ec[q] = as_const(f.edec[q])[0];
+  }
+  float es0 = 0.0f, es1 = 0.0f, es2 = 0.0f;
+  if constexpr (!kEArr) {
+    es0 = as_const(f.esrc[0])[0] * f.ratio_e;
+    es1 = as_const(f.esrc[1])[0] * f.ratio_e;
+    es2 = as_const(f.esrc[2])[0] * f.ratio_e;
+  }
+
+  // E(k) on this wave's row; Ex/Ey exist on planes 0..nz, Ez on 0..nz-1
+  auto load_e = [&](int k, float& ex, float& ey, float& ez) {
+    const bool vxy = k <= nz, vz = k < nz;
+    const uint32_t kx = min(k, nz), kz = min(k, nz - 1);
+    const float a = ldu<T>(f.e_in[0], (kx * (ny + 1) + yX) * nx, tX);
+    const float c = ldu<T>(f.e_in[1], (kx * ny + yY) * (nx + 1), tY);
+    const float d = ldu<T>(f.e_in[2], (kz * (ny + 1) + yX) * (nx + 1), tY);
+    ex = (vxy && vyX && vxX) ? a : 0.0f;
+    ey = (vxy && vyY && vxY) ? c : 0.0f;
+    ez = (vz && vyX && vxY) ? d : 0.0f;
+  };
+  auto load_top = [&](int k, float& ex, float& ez) {
+    const bool vxy = k <= nz, vz = k < nz;
+    const uint32_t kx = min(k, nz), kz = min(k, nz - 1);
+    const float a = ldu<T>(f.e_in[0], (kx * (ny + 1) + ytX) * nx, tX);
+    const float d = ldu<T>(f.e_in[2], (kz * (ny + 1) + ytX) * (nx + 1), tY);
+    ex = (vxy && vtX && vxX) ? a : 0.0f;
+    ez = (vz && vtX && vxY) ? d : 0.0f;
+  };
+
+  const int ks = kz0 > 0 ? kz0 - 1 : 0;  // plane kz0-1 only rebuilds H_new(kz0-1)
+  float ex, ey, ez;
+  load_e(ks, ex, ey, ez);
+  lE[ks & 1][w][0][lane] = ez;
+  lE[ks & 1][w][1][lane] = ex;
+  if (halo) {
+    float tx_, tz_;
+    load_top(ks, tx_, tz_);
+    lE[ks & 1][kR + 1][0][lane] = tz_;
+    lE[ks & 1][kR + 1][1][lane] = tx_;
+  }
+  __syncthreads();
+
+  float phx = 0.0f, phy = 0.0f;  // H_new(k - 1) on this row
+  for (int k = ks; k < kz1; ++k) {
+    const int bb = k & 1;
+    const bool st = k >= kz0;  // this chunk owns plane k
+    const bool vxy = k < nz;   // Hx/Hy planes
+    const uint32_t kk = min(k, nz - 1);
+    const uint32_t kx = min(k, nz), kzc = min(k, nz - 1);
+    const int qzH = vxy ? slab_q(k, nz, L) : -1;
+    const int qzE = slab_q(k, nz + 1, L);
+
+    // ---- one batch of loads: E(k+1), H_A(k), coefficients, CPML memories ----
+    float exn, eyn, ezn, etx = 0.0f, etz = 0.0f;
+    load_e(k + 1, exn, eyn, ezn);
+    if (halo) load_top(k + 1, etx, etz);
+    const float ha0 = ldu<T>(f.h_in[0], (kk * ny + yY) * (nx + 1), tY);
+    const float ha1 = halo ? 0.0f : ldu<T>(f.h_in[1], (kk * (ny + 1) + yX) * nx, tX);
+    const float ha2 = ldu<T>(f.h_in[2], (k * ny + yY) * nx, tX);
+    float p0 = 0, p1 = 0, p2 = 0, p3 = 0, p4 = 0, p5 = 0;
+    if (qyH >= 0) {
+      p0 = ldu<P>(f.psi[0], (kk * L2 + qyH) * (nx + 1), pY);
+      p5 = ldu<P>(f.psi[5], (k * L2 + qyH) * nx, pX);
+    }
+    if (qzH >= 0) {
+      p1 = ldu<P>(f.psi[1], (qzH * ny + yY) * (nx + 1), pY);
+      if (!halo) p2 = ldu<P>(f.psi[2], (qzH * (ny + 1) + yX) * nx, pX);
+    }
+    if constexpr (kXEdge) {
+      if (qxH >= 0) {
+        if (!halo) p3 = ldu<P>(f.psi[3], (kk * (ny + 1) + yX) * L2, qH);
+        p4 = ldu<P>(f.psi[4], (k * ny + yY) * L2, qH);
+      }
+    }
+    float p6 = 0, p7 = 0, p8 = 0, p9 = 0, p10 = 0, p11 = 0;
+    if (!halo && st) {
+      if constexpr (kEArr) {
+        es0 = ldu<float>(f.esrc[0], (kx * (ny + 1) + yX) * nx, cX) * f.ratio_e;
+        es1 = ldu<float>(f.esrc[1], (kx * ny + yY) * (nx + 1), cY) * f.ratio_e;
+        es2 = ldu<float>(f.esrc[2], (kzc * (ny + 1) + yX) * (nx + 1), cY) * f.ratio_e;
+      }
+      if (qyE >= 0) {
+        p6 = ldu<P>(f.psi[6], (kx * L2 + qyE) * nx, pX);
+        p11 = ldu<P>(f.psi[11], (kzc * L2 + qyE) * (nx + 1), pY);
+      }
+      if (qzE >= 0) {
+        p7 = ldu<P>(f.psi[7], (qzE * (ny + 1) + yX) * nx, pX);
+        p8 = ldu<P>(f.psi[8], (qzE * ny + yY) * (nx + 1), pY);
+      }
+      if constexpr (kXEdge) {
+        if (qxE >= 0) {
+          p9 = ldu<P>(f.psi[9], (kx * ny + yY) * L2, qE);
+          p10 = ldu<P>(f.psi[10], (kzc * (ny + 1) + yX) * L2, qE);
+        }
+      }
+    }
+    const float ez_up = lE[bb][w + 1][0][lane];
+    const float ex_up = lE[bb][w + 1][1][lane];
+
+    // ---- H_new(k) on this row ----
+    float d0 = (ez_up - ez) * idx;                   // dEz/dy
+    float d1 = (eyn - ey) * idx;                     // dEy/dz
+    float d2 = (exn - ex) * idx;                     // dEx/dz
+    float d3 = (lane_shift(ez, dn) - ez) * idx;      // dEz/dx
+    float d4 = (lane_shift(ey, dn) - ey) * idx;      // dEy/dx
+    float d5 = (ex_up - ex) * idx;                   // dEx/dy
+    const bool okhx = vxy && vyY && vxY;
+    const bool okhy = vxy && vyX && vxX;
+    const bool okhz = vyY && vxX;
+    const bool wh = own && st;
+    float n;
+    if (qyH >= 0) {
+      d0 = cpml_step(coef_u(tab, 0, L2, qyH), d0, p0, n);
+      if (wh && okhx) stu<P>(pout_base<P>(f, 0), (k * L2 + qyH) * (nx + 1), pY, n);
+      d5 = cpml_step(coef_u(tab, 5, L2, qyH), d5, p5, n);
+      if (wh && okhz) stu<P>(pout_base<P>(f, 5), (k * L2 + qyH) * nx, pX, n);
+    }
+    if (qzH >= 0) {
+      d1 = cpml_step(coef_u(tab, 1, L2, qzH), d1, p1, n);
+      if (wh && okhx) stu<P>(pout_base<P>(f, 1), (qzH * ny + yY) * (nx + 1), pY, n);
+      d2 = cpml_step(coef_u(tab, 2, L2, qzH), d2, p2, n);
+      if (wh && okhy) stu<P>(pout_base<P>(f, 2), (qzH * (ny + 1) + yX) * nx, pX, n);
+    }
+    if constexpr (kXEdge) {
+      if (qxH >= 0) {
+        d3 = cpml_step(c3, d3, p3, n);
+        if (wh && okhy) stu<P>(pout_base<P>(f, 3), (k * (ny + 1) + yX) * L2, qH, n);
+        d4 = cpml_step(c4, d4, p4, n);
+        if (wh && okhz) stu<P>(pout_base<P>(f, 4), (k * ny + yY) * L2, qH, n);
+      }
+    }
+    float hx = hc[0] * ha0 - hc[3] * (d0 - d1);
+    float hy = hc[1] * ha1 - hc[4] * (d2 - d3);
+    float hz = hc[2] * ha2 - hc[5] * (d4 - d5);
+    if (kXEdge && xface) hx = 0.0f;
+    if (y_pec_h) hy = 0.0f;
+    if ((mf && k == 0) || (mk && k == nz)) hz = 0.0f;
+    hx = okhx ? hx : 0.0f;
+    hy = okhy ? hy : 0.0f;
+    hz = okhz ? hz : 0.0f;
+
+    // publish H_new(k) and E(k+1) rows, store owned H
+    lH[bb][w][0][lane] = hz;
+    lH[bb][w][1][lane] = hx;
+    lE[bb ^ 1][w][0][lane] = ezn;
+    lE[bb ^ 1][w][1][lane] = exn;
+    if (halo) {
+      lE[bb ^ 1][kR + 1][0][lane] = etz;
+      lE[bb ^ 1][kR + 1][1][lane] = etx;
+    }
+    if (wh) {
+      if (okhx) stu<T>(f.h_out[0], (k * ny + yY) * (nx + 1), tY, hx);
+      if (okhy) stu<T>(f.h_out[1], (k * (ny + 1) + yX) * nx, tX, hy);
+      if (okhz) stu<T>(f.h_out[2], (k * ny + yY) * nx, tX, hz);
+    }
+    __syncthreads();
+
+    // ---- E_new(k) on this row (owned waves) ----
+    if (!halo && st) {
+      const float hz_dn = lH[bb][w - 1][0][lane];
+      const float hx_dn = lH[bb][w - 1][1][lane];
+      const float hz_l = lane_shift(hz, up);
+      const float hy_l = lane_shift(hy, up);
+      float e0 = gdiff(hz, hz_dn, y, ny, mb, mt) * idx;  // dHz/dy
+      float e1 = gdiff(hy, phy, k, nz, mf, mk) * idx;    // dHy/dz
+      float e2 = gdiff(hx, phx, k, nz, mf, mk) * idx;    // dHx/dz
+      float e3, e4;                                      // dHz/dx, dHy/dx
+      if constexpr (kXEdge) {
+        e3 = gdiff(hz, hz_l, x, nx, ml, mr) * idx;
+        e4 = gdiff(hy, hy_l, x, nx, ml, mr) * idx;
+      } else {
+        e3 = (hz - hz_l) * idx;
+        e4 = (hy - hy_l) * idx;
+      }
+      float e5 = gdiff(hx, hx_dn, y, ny, mb, mt) * idx;  // dHx/dy
+      const bool okx = own && k <= nz && vyX && vxX;
+      const bool oky = own && k <= nz && vyY && vxY;
+      const bool okz = own && k < nz && vyX && vxY;
+      if (qyE >= 0) {
+        e0 = cpml_step(coef_u(tab, 6, L2, qyE), e0, p6, n);
+        if (okx) stu<P>(pout_base<P>(f, 6), (kx * L2 + qyE) * nx, pX, n);
+        e5 = cpml_step(coef_u(tab, 11, L2, qyE), e5, p11, n);
+        if (okz) stu<P>(pout_base<P>(f, 11), (kzc * L2 + qyE) * (nx + 1), pY, n);
+      }
+      if (qzE >= 0) {
+        e1 = cpml_step(coef_u(tab, 7, L2, qzE), e1, p7, n);
+        if (okx) stu<P>(pout_base<P>(f, 7), (qzE * (ny + 1) + yX) * nx, pX, n);
+        e2 = cpml_step(coef_u(tab, 8, L2, qzE), e2, p8, n);
+        if (oky) stu<P>(pout_base<P>(f, 8), (qzE * ny + yY) * (nx + 1), pY, n);
+      }
+      if constexpr (kXEdge) {
+        if (qxE >= 0) {
+          e3 = cpml_step(c9, e3, p9, n);
+          if (oky) stu<P>(pout_base<P>(f, 9), (kx * ny + yY) * L2, qE, n);
+          e4 = cpml_step(c10, e4, p10, n);
+          if (okz) stu<P>(pout_base<P>(f, 10), (kzc * (ny + 1) + yX) * L2, qE, n);
+        }
+      }
+      float nex = ec[0] * ex + es0 * (e0 - e1);
+      float ney = ec[1] * ey + es1 * (e2 - e3);
+      float nez = ec[2] * ez + es2 * (e4 - e5);
+      const bool zf = (mf && k == 0) || (mk && k == nz);
+      const bool yf = (mb && y == 0) || (mt && y == ny);
+      if (zf || yf) nex = 0.0f;
+      if (zf || (kXEdge && xface)) ney = 0.0f;
+      if (yf || (kXEdge && xface)) nez = 0.0f;
+      if (okx) stu<T>(f.e_out[0], (kx * (ny + 1) + yX) * nx, tX, nex);
+      if (oky) stu<T>(f.e_out[1], (kx * ny + yY) * (nx + 1), tY, ney);
+      if (okz) stu<T>(f.e_out[2], (kzc * (ny + 1) + yX) * (nx + 1), tY, nez);
+    }
+    phx = hx;
+    phy = hy;
+    ex = exn;
+    ey = eyn;
+    ez = ezn;
   }
 }
 
-// Rows per wave (y extent of a tile): 2, 4 or 8; BZH_FUSED_ROWS overrides the
-// default for tuning sweeps.
+template <typename T, typename P, int kR>
+void launch_wg(hipStream_t s, const FusedParams& p, int nblocks) {
+  const dim3 block(64 * (kR + 1), 1, 1), grid(nblocks, 1, 1);
+  const bool earr = p.esrc_st[0] != 0;
+  if (p.xedge) {
+    if (earr) hipLaunchKernelGGL((fused_wg<T, P, kR, true, true>), grid, block, 0, s, p);
+    else hipLaunchKernelGGL((fused_wg<T, P, kR, false, true>), grid, block, 0, s, p);
+  } else {
+    if (earr) hipLaunchKernelGGL((fused_wg<T, P, kR, true, false>), grid, block, 0, s, p);
+    else hipLaunchKernelGGL((fused_wg<T, P, kR, false, false>), grid, block, 0, s, p);
+  }
+}
+
+template <typename T, typename P, int kR>
+void launch_rows(hipStream_t s, const FusedParams& p, int nblocks) {
+  const dim3 block(64, 1, 1), grid(nblocks, 1, 1);
+  const bool earr = p.esrc_st[0] != 0;
+  if (p.xedge) {
+    if (earr) hipLaunchKernelGGL((fused_tile<T, P, kR, true, true>), grid, block, 0, s, p);
+    else hipLaunchKernelGGL((fused_tile<T, P, kR, false, true>), grid, block, 0, s, p);
+  } else {
+    if (earr) hipLaunchKernelGGL((fused_tile<T, P, kR, true, false>), grid, block, 0, s, p);
+    else hipLaunchKernelGGL((fused_tile<T, P, kR, false, false>), grid, block, 0, s, p);
+  }
+}
+
+// Kernel and tile height: the workgroup-row kernel with kR = 8 rows by
+// default (4 or 15 selectable), or the wave-tile kernel with 2 or 4 rows per
+// wave.  BZH_FUSED_KERNEL (wg | tile) and BZH_FUSED_ROWS override the default
+// for tuning sweeps.
+bool wg_setting() {
+  const char* env = std::getenv("BZH_FUSED_KERNEL");
+  return !(env && std::string(env) == "tile");
+}
 int rows_setting() {
-  int r = 4;
+  int r = 0;
   if (const char* env = std::getenv("BZH_FUSED_ROWS")) r = std::atoi(env);
-  return (r == 2 || r == 4 || r == 8) ? r : 4;
+  if (wg_setting()) return (r == 4 || r == 15) ? r : 8;
+  return (r == 2) ? 2 : 4;
+}
+
+template <typename T, typename P>
+void launch_typed(hipStream_t s, const FusedParams& p, int nblocks) {
+  const int r = p.rows;
+  if (p.wg) {
+    if (r == 4) launch_wg<T, P, 4>(s, p, nblocks);
+    else if (r == 15) launch_wg<T, P, 15>(s, p, nblocks);
+    else launch_wg<T, P, 8>(s, p, nblocks);
+  } else {
+    if (r == 2) launch_rows<T, P, 2>(s, p, nblocks);
+    else launch_rows<T, P, 4>(s, p, nblocks);
+  }
 }
 
 template <typename T>
-void dispatch(hipStream_t s, const FusedParams& p, int nblocks) {
-  switch (rows_setting()) {
-    case 2: dispatch_rows<T, 2>(s, p, nblocks); break;
-    case 8: dispatch_rows<T, 8>(s, p, nblocks); break;
-    default: dispatch_rows<T, 4>(s, p, nblocks); break;
-  }
+void launch_field(hipStream_t s, const FusedParams& p, int nblocks) {
+  if (p.psi_bf16) launch_typed<T, bf16_t>(s, p, nblocks);
+  else launch_typed<T, float>(s, p, nblocks);
 }
 
 }  // namespace
 
 int fused_rows() { return rows_setting(); }
+int fused_wg() { return wg_setting() ? 1 : 0; }
 
 void launch_fused3d(hipStream_t s, const FusedParams& p, int dtype) {
-  int nblocks;
-  if (p.tiles != nullptr) {
-    nblocks = p.ntiles;
-  } else {
-    BZH_REQUIRE(p.box_ntx >= 0 && p.box_nty >= 0 && p.box_kz >= 1, "fused box is malformed");
-    BZH_REQUIRE(p.box_tx0 >= 0 && (p.box_tx0 + p.box_ntx - 1) * 62 <= p.nx + 62 &&
-                    p.box_ty0 >= 0 && p.box_z0 >= 0 && p.box_z1 <= p.nz + 1,
-                "fused box exceeds the grid");
-    nblocks = p.box_ntx * p.box_nty * p.box_nzc;
+  BZH_REQUIRE(p.box_ntx >= 0 && p.box_nty >= 0 && p.box_kz >= 1 && p.box_nzc >= 0,
+              "fused box is malformed");
+  BZH_REQUIRE(p.wg ? (p.rows == 4 || p.rows == 8 || p.rows == 15) : (p.rows == 2 || p.rows == 4),
+              "fused tile height is not one the kernel is built for");
+  const int ntx_total = (p.nx + 1 + 61) / 62, nty_total = (p.ny + 1 + p.rows - 1) / p.rows;
+  BZH_REQUIRE(p.box_tx0 >= 0 && p.box_tsplit >= 0 && p.box_tskip >= 0 &&
+                  p.box_tx0 + p.box_ntx + p.box_tskip <= ntx_total && p.box_ty0 >= 0 &&
+                  p.box_ty0 + p.box_nty <= nty_total && p.box_z0 >= 0 &&
+                  p.box_z1 <= p.nz + 1 && p.box_z0 + (p.box_nzc - 1) * p.box_kz < p.box_z1,
+              "fused box exceeds the grid");
+  if (!p.xedge) {
+    // the interior variant reads lanes x = i0-1 .. i0+62 unchecked and assumes
+    // no x face or x CPML slab under lanes 0..62
+    const int M = p.L > 0 ? p.L : 1;
+    const int lo = p.box_tx0 * 62 - 1;
+    const int hi = (p.box_tx0 + p.box_ntx + p.box_tskip - 1) * 62 + 62;
+    BZH_REQUIRE(p.box_ntx == 0 || (lo >= M && hi <= p.nx - M - 1),
+                "interior fused tiles overlap an x face or x CPML slab");
   }
-  BZH_REQUIRE(p.cpml_kernel || p.L == 0, "CPML tiles need the CPML-capable kernel");
+  BZH_REQUIRE(p.L == 0 || p.tab != nullptr, "CPML tiles need the coefficient table");
+  for (int c = 0; c < 3; ++c) {
+    BZH_REQUIRE(p.hdec_st[c] == 0 && p.hsrc_st[c] == 0 && p.edec_st[c] == 0,
+                "the fused step needs uniform H coefficients and E decay");
+    BZH_REQUIRE(p.esrc_st[c] == p.esrc_st[0], "E source coefficients must be all arrays or all scalars");
+  }
+  const int64_t nblocks = static_cast<int64_t>(p.box_ntx) * p.box_nty * p.box_nzc;
+  BZH_REQUIRE(nblocks < (int64_t{1} << 31), "fused grid too large");
   if (nblocks <= 0) return;
   switch (dtype) {
-    case BZH_F32: dispatch<float>(s, p, nblocks); break;
-    case BZH_BF16: dispatch<bf16_t>(s, p, nblocks); break;
-    case BZH_F16: dispatch<__half>(s, p, nblocks); break;
+    case BZH_F32: launch_field<float>(s, p, static_cast<int>(nblocks)); break;
+    case BZH_BF16: launch_field<bf16_t>(s, p, static_cast<int>(nblocks)); break;
+    case BZH_F16: launch_field<__half>(s, p, static_cast<int>(nblocks)); break;
     default: throw Failure{"unsupported field dtype"};
   }
   BZH_HIP(hipGetLastError());

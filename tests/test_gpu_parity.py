@@ -221,18 +221,22 @@ def test_graph_replay_matches_direct_enqueue_bitwise():
         np.testing.assert_array_equal(a, b)
 
 
+@pytest.mark.parametrize("kernel,rows", [("wg", "8"), ("wg", "15"), ("tile", "4")])
 @pytest.mark.parametrize("grid,kz", [((13, 9, 130), "4"), ((40, 11, 65), "7"), ((6, 5, 64), "32")])
 @pytest.mark.parametrize("bc", ["cpml", "pec", "mixed"])
-def test_fused_step_parity_odd_shapes(grid, kz, bc, monkeypatch):
+def test_fused_step_parity_odd_shapes(grid, kz, bc, kernel, rows, monkeypatch):
     """Fused two-bank kernel on shapes that cut tiles and z-chunks in every
-    place (partial x/y tiles, chunk boundaries, odd step counts)."""
+    place (partial x/y tiles, an x-interior box next to x-edge tiles, chunk
+    boundaries, odd step counts), for each kernel variant and tile height."""
     _require_gpu()
     monkeypatch.setenv("BZH_FUSED_KZ", kz)
+    monkeypatch.setenv("BZH_FUSED_KERNEL", kernel)
+    monkeypatch.setenv("BZH_FUSED_ROWS", rows)
     kw = {"cpml": dict(cpml_cells=2), "pec": {},
           "mixed": dict(cpml_cells=2, faces=("left", "back", "top"), metallic=("right", "front"))}[bc]
     steps = 7
     sim, prog, state, prob, ost = _setup(grid, seed=51, steps=steps, **kw)
-    print(f"case fused_step_parity grid={grid} kz={kz} bc={bc}", flush=True)
+    print(f"case fused_step_parity grid={grid} kz={kz} bc={bc} kernel={kernel}/{rows}", flush=True)
     s = torch.cuda.Stream()
     with torch.cuda.stream(s):
         state = sim.advance(state, steps, chunk=steps)
