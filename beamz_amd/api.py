@@ -73,7 +73,9 @@ class SimulationResults:
                         self.program.dt)
 
     def field(self, name):
-        return getattr(self.state, name.lower()).float().cpu().numpy()
+        """Field in physical units (half-precision storage is rescaled)."""
+        scale = (self.program.field_scale or {}).get(name[0].upper(), 1.0)
+        return getattr(self.state, name.lower()).double().cpu().numpy() * scale
 
 
 class Simulation:
@@ -91,12 +93,16 @@ class Simulation:
         ``SoftSource``/``CustomSource`` and ``FluxMonitor``.
     steps : int
         Number of time steps of ``run``.
+    field_dtype : torch.float32, torch.float16 or torch.bfloat16
+        Field storage.  Half-precision fields are stored scaled (E / s_e,
+        H / s_h with s_h = s_e / eta0); ``field_scale`` sets s_e, by default
+        the largest per-step source increment.
     """
 
     def __init__(self, permittivity, resolution, *, dt=None, courant=0.99, boundaries=(),
                  sources=(), monitors=(), steps=1, t0=0.0, conductivity=None,
                  backend="auto", field_dtype=torch.float32, device=None,
-                 flags=abi.DEFAULT_FLAGS | abi.FLAG_FUSED_STEP):
+                 flags=abi.DEFAULT_FLAGS | abi.FLAG_FUSED_STEP, field_scale=None):
         self.permittivity = np.asarray(permittivity, dtype=np.float64)
         self.resolution = float(resolution)
         self.dt = float(dt) if dt is not None else courant_dt(resolution, self.permittivity.ndim, courant)
@@ -110,6 +116,7 @@ class Simulation:
         self.field_dtype = field_dtype
         self.device = device
         self.flags = flags
+        self.field_scale = field_scale
         self._program = None
 
     def compile(self):
@@ -119,7 +126,7 @@ class Simulation:
                 self.permittivity, self.resolution, self.dt, boundaries=self.boundaries,
                 sources=self.sources, monitors=self.monitors, num_steps=self.steps,
                 t0=self.t0, conductivity=self.conductivity, device=self.device,
-                field_dtype=self.field_dtype, flags=self.flags)
+                field_dtype=self.field_dtype, flags=self.flags, field_scale=self.field_scale)
         return self._program
 
     def initial_state(self, fields=None):

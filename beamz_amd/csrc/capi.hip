@@ -171,14 +171,20 @@ const int kAxes3d[6] = {1, 0, 0, 2, 2, 1};
 const int kAxesTmH[2] = {1, 2};
 const int kAxesTmE[2] = {2, 1};
 
+// A coefficient is an f32 device scalar or an f32 array of the target's shape;
+// with ``f16`` non-null an f16 array is accepted too (the folded E source
+// coefficients of half-precision programs, read only by the fused step).
 void fill_coef(const bzh_buffer* b, int64_t expect, const float** ptr, int32_t* st,
-               float* s, const char* what) {
+               float* s, const char* what, int32_t* f16 = nullptr) {
   *ptr = nullptr;
   *st = 0;
   *s = 0.0f;
   if (b == nullptr) return;
-  BZH_REQUIRE(b->element_type == BZH_F32, std::string(what) + " must be f32");
   const int64_t n = numel(*b);
+  const bool half_array = f16 != nullptr && b->element_type == BZH_F16 && b->rank > 0 && n > 1;
+  BZH_REQUIRE(b->element_type == BZH_F32 || half_array,
+              std::string(what) + (f16 ? " must be f32 (or an f16 array)" : " must be f32"));
+  if (half_array) *f16 = 1;
   if (b->rank == 0 || n == 1) {
     BZH_REQUIRE(b->data != nullptr, std::string(what) + " scalar pointer is null");
     *ptr = static_cast<const float*>(b->data);
@@ -221,7 +227,8 @@ PhaseParams make_phase(const PhaseIn& in, int phase, const Grid& g, float dt,
                     "output field does not match its input");
       const int64_t n = numel(*in.tin[c]);
       fill_coef(in.mat[c], n, &p.dec[c], &p.dec_st[c], &p.dec_s[c], "decay coefficient");
-      fill_coef(in.mat[3 + c], n, &p.scf[c], &p.scf_st[c], &p.scf_s[c], "source coefficient");
+      fill_coef(in.mat[3 + c], n, &p.scf[c], &p.scf_st[c], &p.scf_s[c], "source coefficient",
+                phase == BZH_PHASE_E ? &p.scf_f16 : nullptr);
       BZH_REQUIRE(p.dec[c] != nullptr && p.scf[c] != nullptr,
                   "every active component needs decay and source coefficients");
     }
@@ -311,7 +318,7 @@ void run_phase(hipStream_t s, const PhaseParams& p, int phase, const Grid& g) {
 }
 
 // ---- schedule ------------------------------------------------------------------
-enum class Kind : int32_t { kPhase3d, kPhaseTm, kSource, kDft, kFused };
+enum class Kind : int32_t { kPhase3d, kPhaseTm, kSource, kDft, kFused, kBatch2d, kZero, kAdjoint };
 
 struct Launch {
   Kind kind;
@@ -349,6 +356,24 @@ void enqueue(hipStream_t s, const Launch& l) {
       SourceParams p;
       std::memcpy(&p, l.bytes.data(), sizeof(p));
       launch_source(s, p);
+      break;
+    }
+    case Kind::kBatch2d: {
+      Batch2dParams p;
+      std::memcpy(&p, l.bytes.data(), sizeof(p));
+      launch_batch2d(s, p);
+      break;
+    }
+    case Kind::kAdjoint: {
+      Adjoint2dParams p;
+      std::memcpy(&p, l.bytes.data(), sizeof(p));
+      launch_adjoint_step(s, p);
+      break;
+    }
+    case Kind::kZero: {
+      ZeroParams p;
+      std::memcpy(&p, l.bytes.data(), sizeof(p));
+      launch_zero_counters(s, p);
       break;
     }
     case Kind::kDft: {
@@ -630,13 +655,53 @@ struct FusedSplit {
   FusedBox interior, edge;
 };
 
-// z chunk length: the longest of 64, 32, ... 4 planes that still gives at
-// least ``target`` workgroups (BZH_FUSED_KZ overrides it for tuning sweeps).
-int chunk_len(int64_t tiles, int planes, int target) {
+// z chunk length (BZH_FUSED_KZ overrides it for tuning sweeps): 12 planes
+// (the recomputed first plane costs 1/12 of a chunk), shortened (down to 4)
+// while the launch has fewer than 2048 workgroups.
+int chunk_len(int64_t tiles, int planes) {
   if (const char* env = std::getenv("BZH_FUSED_KZ")) return std::max(1, std::atoi(env));
-  int kz = 64;
-  while (kz > 4 && tiles * ((planes + kz - 1) / kz) < target) kz /= 2;
+  int kz = 12;
+  while (kz > 4 && tiles * ((planes + kz - 1) / kz) < 2048) kz = std::max(4, kz / 2);
   return kz;
+}
+
+// Ring length of the buffered DFT: 16 steps (BZH_DFT_SLOTS overrides it;
+// 1 accumulates every step directly).  Never longer than the call.
+int dft_slots(int nsteps) {
+  int n = kDftMaxSlots;
+  if (const char* env = std::getenv("BZH_DFT_SLOTS")) n = std::atoi(env);
+  n = std::max(1, std::min(n, kDftMaxSlots));
+  return std::min(n, std::max(nsteps, 1));
+}
+
+// Staging ring of the buffered DFT, one per (device, stream) so that calls on
+// different streams never share it; calls on one stream are ordered by the
+// stream.  Grown on demand and kept for the life of the process.
+float* dft_stage(hipStream_t s, size_t floats) {
+  static std::mutex mu;
+  static std::map<std::pair<int, hipStream_t>, std::pair<float*, size_t>> rings;
+  int device = 0;
+  BZH_HIP(hipGetDevice(&device));
+  std::lock_guard<std::mutex> lock(mu);
+  auto& r = rings[{device, s}];
+  if (r.second < floats) {
+    if (r.first != nullptr) {
+      // a graph that still references the old ring may be replayed later:
+      // keep it alive (rings only grow, so this happens at most a few times)
+      r.first = nullptr;
+    }
+    void* d = nullptr;
+    BZH_HIP(hipMalloc(&d, floats * sizeof(float)));
+    r = {static_cast<float*>(d), floats};
+  }
+  return r.first;
+}
+
+// Band ordering of the fused workgroups (BZH_FUSED_ORDER=xcd selects the
+// plain contiguous-range order for comparison).
+bool fused_banded() {
+  const char* env = std::getenv("BZH_FUSED_ORDER");
+  return !(env && std::string(env) == "xcd");
 }
 
 FusedSplit fused_split(const Grid& g, int L) {
@@ -652,7 +717,7 @@ FusedSplit fused_split(const Grid& g, int L) {
   for (FusedBox* b : {&sp.interior, &sp.edge}) {
     const int64_t tiles = static_cast<int64_t>(b->ntx) * b->nty;
     if (tiles == 0) continue;
-    b->kz = chunk_len(tiles, planes, 65536);
+    b->kz = chunk_len(tiles, planes);
     b->nzc = (planes + b->kz - 1) / b->kz;
   }
   return sp;
@@ -748,6 +813,13 @@ bool run_program(hipStream_t s, ProgramIn& in) {
                      coef_ok && small && metrics_empty(in.h) && metrics_empty(in.e) &&
                      fused_cpml_layout(in.h, in.e, in.psi_b, in.uniform_l, cpml_probe);
   const int32_t* step0 = in.step ? static_cast<const int32_t*>(in.step->data) : nullptr;
+  int slots = 1;
+  float* stage = nullptr;
+  if (in.has_monitors) {
+    const DftParams probe = make_dft(in.mon, in.bank_a, g, in.dt, in.monitor_count);
+    slots = dft_slots(in.nsteps);
+    if (slots > 1) stage = dft_stage(s, static_cast<size_t>(slots) * probe.M * 6 * probe.Pmax);
+  }
   std::vector<Launch> launches;
   launches.reserve(static_cast<size_t>(in.nsteps) * 8);
   const Kind pk = g.is3d ? Kind::kPhase3d : Kind::kPhaseTm;
@@ -787,6 +859,7 @@ bool run_program(hipStream_t s, ProgramIn& in) {
         fp.edec_st[c] = ep.dec_st[c];
         fp.esrc_st[c] = ep.scf_st[c];
       }
+      fp.coef_f16 = ep.scf_f16;
       fp.inv_dx = hp.inv_dx;
       fp.ratio_h = 1.0f;
       fp.ratio_e = 1.0f;
@@ -802,7 +875,6 @@ bool run_program(hipStream_t s, ProgramIn& in) {
       }
       const FusedSplit sp = fused_split(g, fp.L);
       fp.rows = fused_rows();
-      fp.wg = fused_wg();
       fp.box_ty0 = 0;
       fp.box_z0 = 0;
       fp.box_z1 = g.nz + 1;
@@ -817,6 +889,7 @@ bool run_program(hipStream_t s, ProgramIn& in) {
         fp.box_nty = b.nty;
         fp.box_kz = b.kz;
         fp.box_nzc = b.nzc;
+        fp.box_band = fused_banded() ? (b.ntx * b.nty + 7) / 8 : 0;
         launches.push_back(make_launch(Kind::kFused, 0, g.dtype, fp));
       }
     } else {
@@ -832,7 +905,22 @@ bool run_program(hipStream_t s, ProgramIn& in) {
     if (in.has_monitors) {
       DftParams dp = make_dft(in.mon, nxt, g, in.dt, in.monitor_count);
       dp.k = k;
-      launches.push_back(make_launch(Kind::kDft, 0, g.dtype, dp));
+      if (slots > 1) {
+        // buffered: stage this step, fold the ring every ``slots`` steps and
+        // at the end of the call
+        dp.mode = 1;
+        dp.slots = slots;
+        dp.stage = stage;
+        launches.push_back(make_launch(Kind::kDft, 0, g.dtype, dp));
+        if ((k + 1) % slots == 0 || k == in.nsteps - 1) {
+          dp.mode = 2;
+          dp.nk = (k % slots) + 1;
+          dp.k0 = k + 1 - dp.nk;
+          launches.push_back(make_launch(Kind::kDft, 0, g.dtype, dp));
+        }
+      } else {
+        launches.push_back(make_launch(Kind::kDft, 0, g.dtype, dp));
+      }
     }
   }
   graph_cache().run(s, launches, (in.flags & BZH_FLAG_GRAPH_CACHE) != 0);
@@ -977,6 +1065,218 @@ void copy_psi_workspace(hipStream_t s, const bzh_buffer* args, int nargs) {
   }
 }
 
+// ---- batched 2D simulations ---------------------------------------------------
+// Device copies of the descriptor table and of the tile queue, keyed on their
+// bytes, and per-(device, stream) step counters.  They are kept for the life
+// of the process: a captured graph refers to them.
+template <typename V>
+const void* upload_cached(const std::vector<V>& host) {
+  static std::mutex mu;
+  static std::map<std::string, void*> cache;
+  int device = 0;
+  BZH_HIP(hipGetDevice(&device));
+  std::string key(reinterpret_cast<const char*>(&device), sizeof(device));
+  key.append(reinterpret_cast<const char*>(host.data()), host.size() * sizeof(V));
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  void* d = nullptr;
+  BZH_HIP(hipMalloc(&d, std::max<size_t>(host.size() * sizeof(V), 16)));
+  BZH_HIP(hipMemcpy(d, host.data(), host.size() * sizeof(V), hipMemcpyHostToDevice));
+  cache[key] = d;
+  return d;
+}
+
+int* step_counters(hipStream_t s, int n) {
+  static std::mutex mu;
+  static std::map<std::pair<int, hipStream_t>, std::pair<int*, int>> rings;
+  int device = 0;
+  BZH_HIP(hipGetDevice(&device));
+  std::lock_guard<std::mutex> lock(mu);
+  auto& r = rings[{device, s}];
+  if (r.second < n) {
+    void* d = nullptr;  // an older, smaller array stays alive for its graphs
+    BZH_HIP(hipMalloc(&d, static_cast<size_t>(n) * sizeof(int)));
+    r = {static_cast<int*>(d), n};
+  }
+  return r.first;
+}
+
+float* f32_ptr(const bzh_buffer& b, int64_t expect, const std::string& what) {
+  BZH_REQUIRE(b.element_type == BZH_F32 && b.data != nullptr, what + " must be a non-null f32 buffer");
+  BZH_REQUIRE(numel(b) == expect, what + " has " + std::to_string(numel(b)) + " elements, expected " +
+                                      std::to_string(expect));
+  return static_cast<float*>(b.data);
+}
+
+
+Sim2dDev describe_sim2d(const bzh_sim2d& in, int index) {
+  const std::string who = "simulation " + std::to_string(index) + ": ";
+  Sim2dDev d;
+  std::memset(&d, 0, sizeof(d));
+  BZH_REQUIRE(in.ez[0].rank == 2, who + "Ez must be rank 2");
+  d.ny = static_cast<int>(in.ez[0].dims[0]) - 1;
+  d.nx = static_cast<int>(in.ez[0].dims[1]) - 1;
+  BZH_REQUIRE(d.ny >= 1 && d.nx >= 1, who + "grid must have at least one cell");
+  BZH_REQUIRE(static_cast<int64_t>(d.ny + 1) * (d.nx + 1) < (int64_t{1} << 31), who + "grid too large");
+  const int64_t nez = static_cast<int64_t>(d.ny + 1) * (d.nx + 1);
+  const int64_t nhx = static_cast<int64_t>(d.ny) * (d.nx + 1);
+  const int64_t nhy = static_cast<int64_t>(d.ny + 1) * d.nx;
+  for (int b = 0; b < 2; ++b) {
+    d.ez[b] = f32_ptr(in.ez[b], nez, who + "Ez");
+    d.hx[b] = f32_ptr(in.hx[b], nhx, who + "Hx");
+    d.hy[b] = f32_ptr(in.hy[b], nhy, who + "Hy");
+  }
+  BZH_REQUIRE(d.ez[0] != d.ez[1] && d.hx[0] != d.hx[1] && d.hy[0] != d.hy[1],
+              who + "the two field banks must be distinct");
+  d.e_dec = f32_ptr(in.e_decay, 1, who + "E decay");
+  d.h_dec = f32_ptr(in.h_decay, 1, who + "H decay");
+  d.h_src = f32_ptr(in.h_source, 1, who + "H source");
+  d.e_src_st = numel(in.e_source) == 1 ? 0 : 1;
+  d.e_src = f32_ptr(in.e_source, d.e_src_st ? nez : 1, who + "E source");
+  BZH_REQUIRE(in.resolution > 0.0f, who + "resolution must be positive");
+  d.inv_dx = 1.0f / in.resolution;
+  d.metallic = in.metallic & (kBottom | kTop | kLeft | kRight);
+  d.L = in.cpml_cells;
+  BZH_REQUIRE(d.L >= 0 && 2 * d.L <= std::min(d.ny, d.nx), who + "CPML thicker than the grid");
+  if (d.L > 0) {
+    const int64_t L2 = 2 * d.L;
+    d.tab = f32_ptr(in.cpml_table, 12 * L2, who + "CPML table");
+    const int64_t sizes[4] = {L2 * (d.nx + 1), (d.ny + 1) * L2, (d.ny + 1) * L2, L2 * (d.nx + 1)};
+    for (int b = 0; b < 2; ++b)
+      for (int t = 0; t < 4; ++t)
+        d.psi[b][t] = f32_ptr(in.psi[b][t], sizes[t], who + "CPML memory " + std::to_string(t));
+  }
+  BZH_REQUIRE(in.nsrc >= 0 && in.nsrc <= 4, who + "at most four sources");
+  d.nsrc = in.nsrc;
+  for (int q = 0; q < in.nsrc; ++q) {
+    const bzh_buffer& c = in.src_coeff[q];
+    BZH_REQUIRE(c.rank == 2, who + "source coefficients must be (sy, sx)");
+    d.src_sy[q] = static_cast<int>(c.dims[0]);
+    d.src_sx[q] = static_cast<int>(c.dims[1]);
+    d.src_y0[q] = in.src_start[q][0];
+    d.src_x0[q] = in.src_start[q][1];
+    BZH_REQUIRE(d.src_y0[q] >= 0 && d.src_x0[q] >= 0 && d.src_y0[q] + d.src_sy[q] <= d.ny + 1 &&
+                    d.src_x0[q] + d.src_sx[q] <= d.nx + 1,
+                who + "source slab leaves the Ez support");
+    d.src_coeff[q] = f32_ptr(c, static_cast<int64_t>(d.src_sy[q]) * d.src_sx[q], who + "source coefficients");
+    d.src_T[q] = static_cast<int>(numel(in.src_wave[q]));
+    BZH_REQUIRE(d.src_T[q] >= 1, who + "source waveform is empty");
+    d.src_wave[q] = f32_ptr(in.src_wave[q], d.src_T[q], who + "source waveform");
+  }
+  return d;
+}
+
+void run_batch2d(hipStream_t s, const bzh_sim2d* sims, int nsims, const bzh_buffer* step,
+                 const bzh_batch_attrs& a) {
+  BZH_REQUIRE(sims != nullptr && nsims >= 1, "empty batch");
+  BZH_REQUIRE(a.nsteps >= 1, "step count must be positive");
+  BZH_REQUIRE(a.tile_rows == 4 || a.tile_rows == 8 || a.tile_rows == 16 || a.tile_rows == 32,
+              "tile_rows must be 4, 8, 16 or 32");
+  BZH_REQUIRE(step != nullptr && step->element_type == BZH_S32 && step->data && numel(*step) == 1,
+              "current_step must be an s32 scalar");
+  std::vector<Sim2dDev> desc(nsims);
+  for (int i = 0; i < nsims; ++i) desc[i] = describe_sim2d(sims[i], i);
+  // queue: largest simulations first, row-major tiles within one
+  std::vector<int> order(nsims);
+  for (int i = 0; i < nsims; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    return static_cast<int64_t>(desc[x].ny) * desc[x].nx > static_cast<int64_t>(desc[y].ny) * desc[y].nx;
+  });
+  std::vector<int4> tiles;
+  for (int i : order) {
+    const int ty = (desc[i].ny + 1 + a.tile_rows - 1) / a.tile_rows, tx = (desc[i].nx + 1 + 63) / 64;
+    for (int r = 0; r < ty; ++r)
+      for (int c = 0; c < tx; ++c) tiles.push_back(make_int4(i, r, c, 0));
+  }
+  BZH_REQUIRE(tiles.size() < (size_t{1} << 31), "too many tiles");
+  int workgroups = a.workgroups;
+  if (workgroups <= 0) {
+    int device = 0, cus = 0;
+    BZH_HIP(hipGetDevice(&device));
+    BZH_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    workgroups = 4 * std::max(cus, 1);
+  }
+  Batch2dParams p;
+  std::memset(&p, 0, sizeof(p));
+  p.sims = static_cast<const Sim2dDev*>(upload_cached(desc));
+  p.tiles = static_cast<const int4*>(upload_cached(tiles));
+  p.ntiles = static_cast<int>(tiles.size());
+  p.step0 = static_cast<const int32_t*>(step->data);
+  p.tile_rows = a.tile_rows;
+  p.workgroups = workgroups;
+  int* counters = step_counters(s, a.nsteps);
+  std::vector<Launch> launches;
+  ZeroParams z{counters, a.nsteps};
+  launches.push_back(make_launch(Kind::kZero, 0, 0, z));
+  for (int k = 0; k < a.nsteps; ++k) {
+    p.k = k;
+    p.bank = k & 1;
+    p.counter = counters + k;
+    launches.push_back(make_launch(Kind::kBatch2d, 0, BZH_F32, p));
+  }
+  graph_cache().run(s, launches, (a.flags & BZH_FLAG_GRAPH_CACHE) != 0);
+}
+
+// ---- adjoint TMxy -----------------------------------------------------------
+Adjoint2dParams describe_adjoint(const bzh_adjoint2d& in, int64_t* record_slots) {
+  Adjoint2dParams p;
+  std::memset(&p, 0, sizeof(p));
+  BZH_REQUIRE(in.a_ez.rank == 2, "adjoint Ez must be rank 2");
+  p.ny = static_cast<int>(in.a_ez.dims[0]) - 1;
+  p.nx = static_cast<int>(in.a_ez.dims[1]) - 1;
+  BZH_REQUIRE(p.ny >= 2 && p.nx >= 2, "adjoint grid too small");
+  const int64_t nez = static_cast<int64_t>(p.ny + 1) * (p.nx + 1);
+  const int64_t nhx = static_cast<int64_t>(p.ny) * (p.nx + 1);
+  const int64_t nhy = static_cast<int64_t>(p.ny + 1) * p.nx;
+  BZH_REQUIRE(nez < (int64_t{1} << 31), "adjoint grid too large");
+  p.a_ez = f32_ptr(in.a_ez, nez, "adjoint Ez");
+  p.a_hx = f32_ptr(in.a_hx, nhx, "adjoint Hx");
+  p.a_hy = f32_ptr(in.a_hy, nhy, "adjoint Hy");
+  p.w_e0 = f32_ptr(in.work[0], nez, "adjoint scratch 0");
+  p.w_e1 = f32_ptr(in.work[1], nez, "adjoint scratch 1");
+  p.w_hy = f32_ptr(in.work[2], nhy, "adjoint scratch 2");
+  p.w_hx = f32_ptr(in.work[3], nhx, "adjoint scratch 3");
+  p.e_dec = f32_ptr(in.e_decay, 1, "E decay");
+  p.h_dec = f32_ptr(in.h_decay, 1, "H decay");
+  p.h_src = f32_ptr(in.h_source, 1, "H source");
+  p.e_src_st = numel(in.e_source) == 1 ? 0 : 1;
+  p.e_src = f32_ptr(in.e_source, p.e_src_st ? nez : 1, "E source");
+  BZH_REQUIRE(in.resolution > 0.0f, "resolution must be positive");
+  p.inv_dx = 1.0f / in.resolution;
+  p.metallic = in.metallic & (kBottom | kTop | kLeft | kRight);
+  p.L = in.cpml_cells;
+  BZH_REQUIRE(p.L >= 0 && 2 * p.L <= std::min(p.ny, p.nx), "CPML thicker than the grid");
+  if (p.L > 0) {
+    const int64_t L2 = 2 * p.L;
+    p.tab = f32_ptr(in.cpml_table, 12 * L2, "CPML table");
+    const int64_t sizes[4] = {L2 * (p.nx + 1), (p.ny + 1) * L2, (p.ny + 1) * L2, L2 * (p.nx + 1)};
+    for (int t = 0; t < 4; ++t) p.a_psi[t] = f32_ptr(in.a_psi[t], sizes[t], "adjoint CPML memory " + std::to_string(t));
+  }
+  if (!is_empty(in.source_profile)) {
+    p.src_profile = f32_ptr(in.source_profile, nez, "adjoint source profile");
+    BZH_REQUIRE(in.source_time.element_type == BZH_F32 && in.source_time.data && numel(in.source_time) >= 1,
+                "adjoint source time factors must be f32");
+    p.src_time = static_cast<const float*>(in.source_time.data);
+  }
+  for (int q = 0; q < 4; ++q) p.design[q] = in.design[q];
+  const int dh = p.design[1] - p.design[0], dw = p.design[3] - p.design[2];
+  // the recorded curl is the plain one: the box keeps off CPML slabs and faces
+  const int M = std::max(p.L, 1);
+  BZH_REQUIRE(dh >= 1 && dw >= 1 && p.design[0] >= M && p.design[1] <= p.ny - M &&
+                  p.design[2] >= M && p.design[3] <= p.nx - M,
+              "design box must be non-empty and clear of CPML slabs and edges");
+  BZH_REQUIRE(in.record.element_type == BZH_F32 && in.record.data && in.record.rank == 3 &&
+                  in.record.dims[1] == dh && in.record.dims[2] == dw,
+              "record must be f32 [K, design rows, design columns]");
+  *record_slots = in.record.dims[0];
+  p.record = static_cast<float*>(in.record.data);
+  BZH_REQUIRE(in.grad.element_type == BZH_F64 && in.grad.data && numel(in.grad) == int64_t{dh} * dw,
+              "grad must be f64 [design rows, design columns]");
+  p.grad = static_cast<double*>(in.grad.data);
+  return p;
+}
+
 template <typename F>
 int32_t guarded(F&& f) {
   clear_error();
@@ -1092,6 +1392,49 @@ int32_t bzh_program(void* stream, const bzh_buffer* args, int32_t nargs,
     const bool fused = run_program(s, in);
     if (temporal && attrs->cpml_enabled && !fused && (attrs->nsteps % 2 == 1))
       copy_psi_workspace(s, args, nargs);
+  });
+}
+
+int32_t bzh_batch2d(void* stream, const bzh_sim2d* sims, int32_t nsims,
+                    const bzh_buffer* current_step, const bzh_batch_attrs* attrs) {
+  return guarded([&] {
+    BZH_REQUIRE(attrs != nullptr, "null attributes");
+    BZH_REQUIRE(attrs->abi_version == BZH_ABI_VERSION, "ABI version mismatch");
+    run_batch2d(static_cast<hipStream_t>(stream), sims, nsims, current_step, *attrs);
+  });
+}
+
+int32_t bzh_adjoint2d_record(void* stream, const bzh_adjoint2d* adj, const bzh_buffer* hx,
+                             const bzh_buffer* hy, int32_t slot) {
+  return guarded([&] {
+    BZH_REQUIRE(adj != nullptr && hx != nullptr && hy != nullptr, "null arguments");
+    int64_t slots = 0;
+    Adjoint2dParams p = describe_adjoint(*adj, &slots);
+    BZH_REQUIRE(slot >= 0 && slot < slots, "record slot out of range");
+    p.slot = slot;
+    const float* fhx = f32_ptr(*hx, static_cast<int64_t>(p.ny) * (p.nx + 1), "forward Hx");
+    const float* fhy = f32_ptr(*hy, static_cast<int64_t>(p.ny + 1) * p.nx, "forward Hy");
+    launch_adjoint_record(static_cast<hipStream_t>(stream), p, fhx, fhy);
+  });
+}
+
+int32_t bzh_adjoint2d_reverse(void* stream, const bzh_adjoint2d* adj, int32_t n_last,
+                              int32_t count, int32_t slot_last, int32_t flags) {
+  return guarded([&] {
+    BZH_REQUIRE(adj != nullptr, "null arguments");
+    int64_t slots = 0;
+    Adjoint2dParams p = describe_adjoint(*adj, &slots);
+    BZH_REQUIRE(count >= 1 && n_last - count + 1 >= 0, "reverse range out of bounds");
+    BZH_REQUIRE(slot_last < slots && slot_last - count + 1 >= 0, "record slots out of range");
+    if (p.src_profile)
+      BZH_REQUIRE(n_last < numel(adj->source_time), "source time factors shorter than the run");
+    std::vector<Launch> launches;
+    for (int c = 0; c < count; ++c) {
+      p.n = n_last - c;
+      p.slot = slot_last - c;
+      launches.push_back(make_launch(Kind::kAdjoint, 0, BZH_F32, p));
+    }
+    graph_cache().run(static_cast<hipStream_t>(stream), launches, (flags & BZH_FLAG_GRAPH_CACHE) != 0);
   });
 }
 

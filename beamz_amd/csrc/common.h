@@ -86,6 +86,7 @@ struct PhaseParams {
   float dec_s[3];
   float scf_s[3];
   float ratio;          // target storage scale / source storage scale
+  int32_t scf_f16;      // some source coefficient array is f16 (fused step only)
   const float* metric[3];  // (z, y, x) inverse distances (nullptr -> inv_dx)
   int32_t met_st[3];       // 1 per-coordinate array, 0 device scalar
   float inv_dx;
@@ -138,12 +139,12 @@ struct FusedParams {
   int32_t box_tx0, box_ntx, box_tsplit, box_tskip;
   int32_t box_ty0, box_nty;
   int32_t box_z0, box_z1, box_kz, box_nzc;     // z range [z0, z1) in chunks of kz
+  int32_t box_band;      // tiles per XCD band (0: contiguous ranges per XCD)
   int32_t xedge;         // 1: tiles touching an x face or x CPML slab
   int32_t rows;          // tile height (y rows) the box was cut with
-  int32_t wg;            // 1: workgroup-row kernel, 0: wave-tile kernel
+  int32_t coef_f16;      // E source coefficient arrays stored as float16
 };
 int fused_rows();
-int fused_wg();
 
 // Launchers implemented in the .hip files.
 void launch_fused3d(hipStream_t s, const FusedParams& p, int dtype);
@@ -197,7 +198,86 @@ struct DftParams {
   int32_t fsize[6];        // element count of each component (gather bound)
   int64_t re_len;          // arena lengths (accumulation bound)
   int64_t w_len;
+  // Buffered accumulation: mode 1 stages this step's gathered samples in
+  // stage[k % slots][M][6][Pmax] (f32), mode 2 folds steps k0 .. k0+nk-1 of
+  // the ring into the accumulators; mode 0 accumulates step k directly.
+  int32_t mode;
+  float* stage;
+  int32_t slots;
+  int32_t k0, nk;
 };
+constexpr int kDftMaxSlots = 16;
 void launch_dft(hipStream_t s, const DftParams& p);
+
+// One TMxy simulation of a batch on the device: two banks of (Ez, Hx, Hy),
+// uniform decays and H source, an E source array or scalar, CPML memories of
+// both banks (Hx_y, Hy_x, Ez_x, Ez_y), the coefficient table [4][3][2L] and
+// up to four pre-E Ez slab sources.
+struct Sim2dDev {
+  float* ez[2];
+  float* hx[2];
+  float* hy[2];
+  float* psi[2][4];
+  const float* e_src;   // (ny+1, nx+1), or a device scalar when e_src_st == 0
+  const float* e_dec;   // device scalars
+  const float* h_src;
+  const float* h_dec;
+  const float* tab;
+  const float* src_coeff[4];
+  const float* src_wave[4];
+  float inv_dx;
+  int32_t e_src_st;
+  int32_t src_y0[4], src_x0[4], src_sy[4], src_sx[4], src_T[4];
+  int32_t nsrc;
+  int32_t ny, nx, L, metallic;
+};
+
+struct Batch2dParams {
+  const Sim2dDev* sims;
+  const int4* tiles;     // (simulation, tile row, tile column, 0), queue order
+  int32_t ntiles;
+  int* counter;          // queue head of this step (zeroed before the call)
+  int32_t bank;          // bank read by this step
+  const int32_t* step0;  // device scalar: step index of step 0 of this call
+  int32_t k;             // step within this call
+  int32_t tile_rows;     // tile height (64-wide tiles)
+  int32_t workgroups;    // persistent grid size
+};
+void launch_batch2d(hipStream_t s, const Batch2dParams& p);
+
+// Reverse-time TMxy step (adjoint.hip).
+struct Adjoint2dParams {
+  int32_t ny, nx, L, metallic;
+  float inv_dx;
+  const float* e_dec;
+  const float* e_src;
+  int32_t e_src_st;
+  const float* h_dec;
+  const float* h_src;
+  const float* tab;          // forward CPML table [4][3][2L] (Hx_y, Hy_x, Ez_x, Ez_y)
+  float* a_ez;               // adjoint state (Ez, Hx, Hy shapes)
+  float* a_hx;
+  float* a_hy;
+  float* a_psi[4];           // adjoint CPML memories (shapes of the forward ones)
+  float* w_e0;               // scratch: Ez, Ez, Hy, Hx shapes
+  float* w_e1;
+  float* w_hy;
+  float* w_hx;
+  const float* src_profile;  // Ez-shaped adjoint source weights (nullptr: none)
+  const float* src_time;     // time factor per forward step
+  int32_t n;                 // forward step being reversed
+  int32_t design[4];         // Ez-node box [y0, y1) x [x0, x1)
+  float* record;             // forward curl per segment step [K][dy][dx]
+  int32_t slot;              // record slot of step n
+  double* grad;              // sum over steps of g * curl, [dy][dx]
+};
+void launch_adjoint_step(hipStream_t s, const Adjoint2dParams& p);
+void launch_adjoint_record(hipStream_t s, const Adjoint2dParams& p, const float* hx, const float* hy);
+
+struct ZeroParams {
+  int* counters;
+  int32_t n;
+};
+void launch_zero_counters(hipStream_t s, const ZeroParams& p);
 
 }  // namespace bzh

@@ -3,25 +3,30 @@
 // One launch reads bank A (E, H, CPML memories) and writes bank B, so no
 // workgroup ever reads a value another is overwriting.
 //
-// Mapping: one wave64 per workgroup.  A wave owns a (x, y) tile of 62 x kTY
-// output cells and marches along z over a chunk of planes.  Lane l sits at
+// Mapping: a workgroup of kR + 1 waves owns a tile of 62 (x) by kR (y) cells
+// and marches along z over a chunk of planes.  Lane l of every wave sits at
 // x = i0 - 1 + l: lanes 1..62 own outputs, lanes 0 and 63 are halo columns.
-// Each lane keeps kTY + 2 rows (y = j0 - 1 .. j0 + kTY) of E in registers.
+// Wave w sits on row y = j0 - 1 + w: waves 1..kR own rows j0 .. j0+kR-1,
+// wave 0 computes the halo row j0-1 of H (Hx, Hz, needed by the E update of
+// row j0) and loads the top halo row j0+kR of E (Ez, Ex, needed by the H
+// update of row j0+kR-1).  Per plane k each wave
 //
-//   plane k:  issue every load of the plane in one batch: E(k+1), H_A(k),
-//               E coefficients and the CPML memories of every active term
-//             H_new(k) on rows j0-1 .. j0+kTY-1, lanes 0..62 (row j0-1 and
-//               lane 0 are recomputed halos; only owned cells are stored)
-//             E_new(k) on owned cells from H_new(k) (own, y-1 in registers,
-//               x-1 by cross-lane permute) and H_new(k-1) (carried along z)
+//   issues one batch of loads: E(k+1), H_A(k), E coefficients and the CPML
+//     memories of every active term of its row,
+//   computes H_new(k) on its row (E(k) of the row above comes from LDS, x+1
+//     neighbours by cross-lane permute),
+//   publishes H_new(k) and E(k+1) of its row to LDS, stores owned H,
+//   barrier,
+//   computes E_new(k) on its row from H_new(k) (row below from LDS, x-1 by
+//     permute) and H_new(k-1) (carried in registers along z), stores owned E.
 //
-// Boundary handling costs nothing where it does not apply.  Everything that
+// Rows cross through double-buffered LDS with a single barrier per plane.
+// Boundary handling costs nothing where it does not apply: everything that
 // depends on z (faces, ghost rule, CPML z-terms) is a plane-uniform branch,
-// everything that depends on y a row-uniform branch, and only the tiles that
+// everything that depends on y a wave-uniform branch, and only the tiles that
 // touch an x face or an x CPML slab run the kXEdge variant with per-lane x
-// checks.  There is no LDS traffic and no barrier.  Every address is a
-// uniform 64-bit row base plus a per-lane 32-bit byte offset
-// (global_load ... saddr), so a load costs no address arithmetic.
+// checks.  Addresses are an array base (scalar) plus a 32-bit byte offset
+// (global_load v, v_off, s[base]); every array is below 4 GiB.
 //
 // Numerical contract: identical to phase.hip (kernels.py:458-626,
 // lattice.py:523-575, kernels.py:184-264 with a uniform slab thickness L,
@@ -39,18 +44,6 @@ namespace {
 // Constant-address-space view: uniform indices become scalar (s_load) reads.
 using cfloat = const __attribute__((address_space(4))) float;
 __device__ __forceinline__ cfloat* as_const(const float* p) { return (cfloat*)(p); }
-
-// Element ``row`` (uniform) of ``base`` plus a per-lane byte offset.
-template <typename T>
-__device__ __forceinline__ float ldr(const void* base, uint32_t row, uint32_t xb) {
-  const char* p = static_cast<const char*>(base) + static_cast<size_t>(row) * sizeof(T);
-  return load_f(reinterpret_cast<const T*>(p + xb), 0);
-}
-template <typename T>
-__device__ __forceinline__ void str(void* base, uint32_t row, uint32_t xb, float v) {
-  char* p = static_cast<char*>(base) + static_cast<size_t>(row) * sizeof(T);
-  store_f(reinterpret_cast<T*>(p + xb), 0, v);
-}
 
 // Value of the lane to the right / left (x + 1 / x - 1).
 __device__ __forceinline__ float lane_shift(float v, int addr) {
@@ -81,6 +74,36 @@ __device__ __forceinline__ int xcd_remap(int g, int n) {
   return c * base + min(c, rem) + m;
 }
 
+// Work item of this workgroup.  Without bands (box_band == 0) the logical
+// index is x tile fastest, then y tile, then z chunk, dealt to the XCDs in
+// contiguous ranges.  With bands, the (x, y) tiles are cut into 8 bands of
+// box_band tiles and XCD c walks band c chunk by chunk, so at any time all
+// XCDs stream the same z chunk of the grid (one contiguous region per array)
+// while tiles that share halo rows stay in one L2.  Indices past the last
+// tile of a band are padding; their workgroups return at once.
+__device__ __forceinline__ bool decode_tile(const FusedParams& f, int& bx, int& by, int& kz0,
+                                            int& kz1) {
+  const int lam = xcd_remap(blockIdx.x, gridDim.x);
+  int t, zc;
+  if (f.box_band > 0) {
+    const int per = f.box_band * f.box_nzc;
+    const int c = lam / per, rem = lam - c * per;
+    zc = rem / f.box_band;
+    t = c * f.box_band + (rem - zc * f.box_band);
+    if (t >= f.box_ntx * f.box_nty) return false;
+  } else {
+    t = lam % (f.box_ntx * f.box_nty);
+    zc = lam / (f.box_ntx * f.box_nty);
+  }
+  bx = t % f.box_ntx;
+  by = f.box_ty0 + t / f.box_ntx;
+  if (bx >= f.box_tsplit) bx += f.box_tskip;
+  bx += f.box_tx0;
+  kz0 = f.box_z0 + zc * f.box_kz;
+  kz1 = min(kz0 + f.box_kz, f.box_z1);
+  return true;
+}
+
 struct Coef {
   float a, b, ik;
 };
@@ -97,347 +120,6 @@ __device__ __forceinline__ float cpml_step(const Coef& c, float d, float psi, fl
   return d * c.ik + out;
 }
 
-template <typename T, typename P, int kTY, bool kEArr, bool kXEdge>
-__global__ void __launch_bounds__(64) fused_tile(FusedParams f) {
-  const int lane = threadIdx.x;
-  int b = xcd_remap(blockIdx.x, gridDim.x);
-  int bx = b % f.box_ntx;
-  b /= f.box_ntx;
-  if (bx >= f.box_tsplit) bx += f.box_tskip;
-  bx += f.box_tx0;
-  const int by = f.box_ty0 + b % f.box_nty;
-  const int kz0 = f.box_z0 + (b / f.box_nty) * f.box_kz;
-  const int kz1 = min(kz0 + f.box_kz, f.box_z1);
-
-  const int nz = f.nz, ny = f.ny, nx = f.nx, L = f.L, L2 = 2 * f.L;
-  const int m = f.metallic;
-  const bool mf = m & kFront, mk = m & kBack, mb = m & kBottom, mt = m & kTop;
-  const bool ml = m & kLeft, mr = m & kRight;
-  const float idx = f.inv_dx;
-  const int j0 = by * kTY;
-  const int x = bx * 62 - 1 + lane;
-  const int dn = ((lane + 1) & 63) << 2;  // permute address of lane + 1
-  const int up = ((lane - 1) & 63) << 2;  // ... of lane - 1
-  cfloat* tab = as_const(f.tab);
-
-  // ---- x: per-lane, fixed for the whole wave ----
-  // X-class arrays (Ex, Hy, Hz) have nx columns, Y-class (Ey, Ez, Hx) nx + 1
-  bool vxX = true, vxY = true, xface = false;
-  int xX = x, xY = x, qxH = -1, qxE = -1;
-  if constexpr (kXEdge) {
-    vxX = x >= 0 && x < nx;
-    vxY = x >= 0 && x <= nx;
-    xX = min(max(x, 0), nx - 1);
-    xY = min(max(x, 0), nx);
-    qxH = vxX ? slab_q(x, nx, L) : -1;
-    qxE = vxY ? slab_q(x, nx + 1, L) : -1;
-    xface = (ml && x == 0) || (mr && x == nx);
-  }
-  const bool own = lane >= 1 && lane <= 62 && (!kXEdge || x <= nx);
-  const uint32_t tX = xX * sizeof(T), tY = xY * sizeof(T);
-  const uint32_t pX = xX * sizeof(P), pY = xY * sizeof(P);
-  const uint32_t cX = xX * 4u, cY = xY * 4u;
-  const uint32_t qH = max(qxH, 0) * sizeof(P), qE = max(qxE, 0) * sizeof(P);
-  Coef c3{}, c4{}, c9{}, c10{};
-  if constexpr (kXEdge) {
-    if (qxH >= 0) {
-      c3 = coef_v(f.tab, 3, L2, qxH);
-      c4 = coef_v(f.tab, 4, L2, qxH);
-    }
-    if (qxE >= 0) {
-      c9 = coef_v(f.tab, 9, L2, qxE);
-      c10 = coef_v(f.tab, 10, L2, qxE);
-    }
-  }
-
-  // bank-B twin of each CPML memory
-  void* pout[12];
-#pragma unroll
-  for (int t = 0; t < 12; ++t)
-    pout[t] = static_cast<char*>(f.psi[t]) + f.psi_delta * static_cast<int64_t>(sizeof(P));
-
-  // uniform coefficients (rank-0 device scalars)
-  float hc[6], ec[6];
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    hc[q] = as_const(f.hdec[q])[0];
-    hc[3 + q] = as_const(f.hsrc[q])[0] * f.ratio_h;
-    ec[q] = as_const(f.edec[q])[0];
-    ec[3 + q] = kEArr ? 0.0f : as_const(f.esrc[q])[0] * f.ratio_e;
-  }
-
-  // ---- y: per row, uniform ----
-  // rows r = 0 .. kTY + 1 hold y = j0 - 1 + r; X-rows (Ex, Hy, Ez) run 0..ny,
-  // Y-rows (Ey, Hx, Hz) 0..ny-1.  Out-of-support rows load a clamped row.
-  int yX[kTY + 2], yY[kTY + 2];
-  bool vyX[kTY + 2], vyY[kTY + 2];
-#pragma unroll
-  for (int r = 0; r < kTY + 2; ++r) {
-    const int y = j0 - 1 + r;
-    vyX[r] = y >= 0 && y <= ny;
-    vyY[r] = y >= 0 && y < ny;
-    yX[r] = min(max(y, 0), ny);
-    yY[r] = min(max(y, 0), ny - 1);
-  }
-
-  float ex0[kTY + 2], ey0[kTY + 2], ez0[kTY + 2];
-  float ex1[kTY + 2], ey1[kTY + 2], ez1[kTY + 2];
-  float phx[kTY], phy[kTY];  // H_new(k - 1) of the owned rows
-
-  // E(k) rows 0..kTY+1; Ex/Ey exist on planes 0..nz, Ez on 0..nz-1
-  auto load_plane = [&](int k, float* ex, float* ey, float* ez) {
-    const bool vxy = k <= nz, vz = k < nz;
-    const uint32_t kx = min(k, nz), kz = min(k, nz - 1);
-#pragma unroll
-    for (int r = 0; r < kTY + 2; ++r) {
-      const float a = ldr<T>(f.e_in[0], (kx * (ny + 1) + yX[r]) * nx, tX);
-      const float c = ldr<T>(f.e_in[1], (kx * ny + yY[r]) * (nx + 1), tY);
-      const float d = ldr<T>(f.e_in[2], (kz * (ny + 1) + yX[r]) * (nx + 1), tY);
-      ex[r] = (vxy && vyX[r] && vxX) ? a : 0.0f;
-      ey[r] = (vxy && vyY[r] && vxY) ? c : 0.0f;
-      ez[r] = (vz && vyX[r] && vxY) ? d : 0.0f;
-    }
-  };
-
-  // H_new(k) on rows 0..kTY (y = j0-1 .. j0+kTY-1) at this lane.  With
-  // ``write`` the CPML memories of owned cells are stored to bank B.
-  // Hx, Hy exist on planes 0..nz-1, Hz on 0..nz.
-  auto h_rows = [&](int k, const float* ex, const float* ey, const float* ez, const float* exn,
-                    const float* eyn, float* hx, float* hy, float* hz, bool write) {
-    const bool vxy = k < nz;
-    const uint32_t kk = min(k, nz - 1);
-    const int qz = vxy ? slab_q(k, nz, L) : -1;
-    float ha0[kTY + 1], ha1[kTY + 1], ha2[kTY + 1];
-    float p0[kTY + 1], p1[kTY + 1], p2[kTY + 1], p3[kTY + 1], p4[kTY + 1], p5[kTY + 1];
-    int qy[kTY + 1];
-#pragma unroll
-    for (int r = 0; r <= kTY; ++r) {
-      ha0[r] = ldr<T>(f.h_in[0], (kk * ny + yY[r]) * (nx + 1), tY);
-      ha1[r] = ldr<T>(f.h_in[1], (kk * (ny + 1) + yX[r]) * nx, tX);
-      ha2[r] = ldr<T>(f.h_in[2], (k * ny + yY[r]) * nx, tX);
-      qy[r] = vyY[r] ? slab_q(j0 - 1 + r, ny, L) : -1;  // Hx_y and Hz_y: ny samples
-      p0[r] = p1[r] = p2[r] = p3[r] = p4[r] = p5[r] = 0.0f;
-      if (qy[r] >= 0) {
-        p0[r] = ldr<P>(f.psi[0], (kk * L2 + qy[r]) * (nx + 1), pY);
-        p5[r] = ldr<P>(f.psi[5], (k * L2 + qy[r]) * nx, pX);
-      }
-      if (qz >= 0) {
-        p1[r] = ldr<P>(f.psi[1], (qz * ny + yY[r]) * (nx + 1), pY);
-        p2[r] = ldr<P>(f.psi[2], (qz * (ny + 1) + yX[r]) * nx, pX);
-      }
-      if constexpr (kXEdge) {
-        if (qxH >= 0) {
-          p3[r] = ldr<P>(f.psi[3], (kk * (ny + 1) + yX[r]) * L2, qH);
-          p4[r] = ldr<P>(f.psi[4], (k * ny + yY[r]) * L2, qH);
-        }
-      }
-    }
-    Coef c1{}, c2{};
-    if (qz >= 0) {
-      c1 = coef_u(tab, 1, L2, qz);
-      c2 = coef_u(tab, 2, L2, qz);
-    }
-    const bool hz_pec = (mf && k == 0) || (mk && k == nz);
-#pragma unroll
-    for (int r = 0; r <= kTY; ++r) {
-      const int y = j0 - 1 + r;
-      const bool w = write && own && r >= 1;
-      const bool okx = vxy && vyY[r] && vxY;  // Hx
-      const bool oky = vxy && vyX[r] && vxX;  // Hy
-      const bool okz = vyY[r] && vxX;         // Hz
-      float d0 = (ez[r + 1] - ez[r]) * idx;                 // dEz/dy
-      float d1 = (eyn[r] - ey[r]) * idx;                    // dEy/dz
-      float d2 = (exn[r] - ex[r]) * idx;                    // dEx/dz
-      float d3 = (lane_shift(ez[r], dn) - ez[r]) * idx;     // dEz/dx
-      float d4 = (lane_shift(ey[r], dn) - ey[r]) * idx;     // dEy/dx
-      float d5 = (ex[r + 1] - ex[r]) * idx;                 // dEx/dy
-      float n;
-      if (qy[r] >= 0) {
-        d0 = cpml_step(coef_u(tab, 0, L2, qy[r]), d0, p0[r], n);
-        if (w && okx) str<P>(pout[0], (k * L2 + qy[r]) * (nx + 1), pY, n);
-        d5 = cpml_step(coef_u(tab, 5, L2, qy[r]), d5, p5[r], n);
-        if (w && okz) str<P>(pout[5], (k * L2 + qy[r]) * nx, pX, n);
-      }
-      if (qz >= 0) {
-        d1 = cpml_step(c1, d1, p1[r], n);
-        if (w && okx) str<P>(pout[1], (qz * ny + yY[r]) * (nx + 1), pY, n);
-        d2 = cpml_step(c2, d2, p2[r], n);
-        if (w && oky) str<P>(pout[2], (qz * (ny + 1) + yX[r]) * nx, pX, n);
-      }
-      if constexpr (kXEdge) {
-        if (qxH >= 0) {
-          d3 = cpml_step(c3, d3, p3[r], n);
-          if (w && oky) str<P>(pout[3], (k * (ny + 1) + yX[r]) * L2, qH, n);
-          d4 = cpml_step(c4, d4, p4[r], n);
-          if (w && okz) str<P>(pout[4], (k * ny + yY[r]) * L2, qH, n);
-        }
-      }
-      float vx = hc[0] * ha0[r] - hc[3] * (d0 - d1);
-      float vy = hc[1] * ha1[r] - hc[4] * (d2 - d3);
-      float vz = hc[2] * ha2[r] - hc[5] * (d4 - d5);
-      if (kXEdge && xface) vx = 0.0f;
-      if ((mb && y == 0) || (mt && y == ny)) vy = 0.0f;
-      if (hz_pec) vz = 0.0f;
-      hx[r] = okx ? vx : 0.0f;
-      hy[r] = oky ? vy : 0.0f;
-      hz[r] = okz ? vz : 0.0f;
-    }
-  };
-
-#pragma unroll
-  for (int r = 0; r < kTY; ++r) phx[r] = phy[r] = 0.0f;
-  if (kz0 > 0) {
-    // H_new(kz0 - 1) on this lane's rows (not stored: the chunk below owns it)
-    float hx[kTY + 1], hy[kTY + 1], hz[kTY + 1];
-    load_plane(kz0 - 1, ex1, ey1, ez1);
-    load_plane(kz0, ex0, ey0, ez0);
-    h_rows(kz0 - 1, ex1, ey1, ez1, ex0, ey0, hx, hy, hz, false);
-#pragma unroll
-    for (int r = 0; r < kTY; ++r) {
-      phx[r] = hx[r + 1];
-      phy[r] = hy[r + 1];
-    }
-  } else {
-    load_plane(kz0, ex0, ey0, ez0);
-  }
-
-  for (int k = kz0; k < kz1; ++k) {
-    load_plane(k + 1, ex1, ey1, ez1);
-    const bool vxy = k <= nz, vz = k < nz;  // Ex/Ey planes, Ez planes
-    const uint32_t kx = min(k, nz), kz = min(k, nz - 1);
-    const int qz = slab_q(k, nz + 1, L);
-    // E coefficients and E-term CPML memories of the owned rows (s = r - 1)
-    float es0[kTY], es1[kTY], es2[kTY];
-    float p6[kTY], p7[kTY], p8[kTY], p9[kTY], p10[kTY], p11[kTY];
-    int qy[kTY];
-#pragma unroll
-    for (int s = 0; s < kTY; ++s) {
-      const int r = s + 1;
-      if constexpr (kEArr) {
-        es0[s] = ldr<float>(f.esrc[0], (kx * (ny + 1) + yX[r]) * nx, cX) * f.ratio_e;
-        es1[s] = ldr<float>(f.esrc[1], (kx * ny + yY[r]) * (nx + 1), cY) * f.ratio_e;
-        es2[s] = ldr<float>(f.esrc[2], (kz * (ny + 1) + yX[r]) * (nx + 1), cY) * f.ratio_e;
-      } else {
-        es0[s] = ec[3];
-        es1[s] = ec[4];
-        es2[s] = ec[5];
-      }
-      qy[s] = vyX[r] ? slab_q(j0 + s, ny + 1, L) : -1;  // Ex_y and Ez_y: ny + 1 samples
-      p6[s] = p7[s] = p8[s] = p9[s] = p10[s] = p11[s] = 0.0f;
-      if (qy[s] >= 0) {
-        p6[s] = ldr<P>(f.psi[6], (kx * L2 + qy[s]) * nx, pX);
-        p11[s] = ldr<P>(f.psi[11], (kz * L2 + qy[s]) * (nx + 1), pY);
-      }
-      if (qz >= 0) {
-        p7[s] = ldr<P>(f.psi[7], (qz * (ny + 1) + yX[r]) * nx, pX);
-        p8[s] = ldr<P>(f.psi[8], (qz * ny + yY[r]) * (nx + 1), pY);
-      }
-      if constexpr (kXEdge) {
-        if (qxE >= 0) {
-          p9[s] = ldr<P>(f.psi[9], (kx * ny + yY[r]) * L2, qE);
-          p10[s] = ldr<P>(f.psi[10], (kz * (ny + 1) + yX[r]) * L2, qE);
-        }
-      }
-    }
-    float hx[kTY + 1], hy[kTY + 1], hz[kTY + 1];
-    h_rows(k, ex0, ey0, ez0, ex1, ey1, hx, hy, hz, true);
-    if (own) {
-#pragma unroll
-      for (int r = 1; r <= kTY; ++r) {
-        if (vz && vyY[r] && vxY) str<T>(f.h_out[0], (k * ny + yY[r]) * (nx + 1), tY, hx[r]);
-        if (vz && vyX[r] && vxX) str<T>(f.h_out[1], (k * (ny + 1) + yX[r]) * nx, tX, hy[r]);
-        if (vyY[r] && vxX) str<T>(f.h_out[2], (k * ny + yY[r]) * nx, tX, hz[r]);
-      }
-    }
-    Coef c7{}, c8{};
-    if (qz >= 0) {
-      c7 = coef_u(tab, 7, L2, qz);
-      c8 = coef_u(tab, 8, L2, qz);
-    }
-    const bool exy_pec = (mf && k == 0) || (mk && k == nz);
-#pragma unroll
-    for (int r = 1; r <= kTY; ++r) {
-      const int s = r - 1;
-      const int y = j0 - 1 + r;
-      const float hz_l = lane_shift(hz[r], up);
-      const float hy_l = lane_shift(hy[r], up);
-      float d0 = gdiff(hz[r], hz[r - 1], y, ny, mb, mt) * idx;  // dHz/dy
-      float d1 = gdiff(hy[r], phy[s], k, nz, mf, mk) * idx;     // dHy/dz
-      float d2 = gdiff(hx[r], phx[s], k, nz, mf, mk) * idx;     // dHx/dz
-      float d3, d4;                                             // dHz/dx, dHy/dx
-      if constexpr (kXEdge) {
-        d3 = gdiff(hz[r], hz_l, x, nx, ml, mr) * idx;
-        d4 = gdiff(hy[r], hy_l, x, nx, ml, mr) * idx;
-      } else {
-        d3 = (hz[r] - hz_l) * idx;
-        d4 = (hy[r] - hy_l) * idx;
-      }
-      float d5 = gdiff(hx[r], hx[r - 1], y, ny, mb, mt) * idx;  // dHx/dy
-      const bool okx = own && vxy && vyX[r] && vxX;  // Ex
-      const bool oky = own && vxy && vyY[r] && vxY;  // Ey
-      const bool okz = own && vz && vyX[r] && vxY;   // Ez
-      float n;
-      if (qy[s] >= 0) {
-        d0 = cpml_step(coef_u(tab, 6, L2, qy[s]), d0, p6[s], n);
-        if (okx) str<P>(pout[6], (kx * L2 + qy[s]) * nx, pX, n);
-        d5 = cpml_step(coef_u(tab, 11, L2, qy[s]), d5, p11[s], n);
-        if (okz) str<P>(pout[11], (kz * L2 + qy[s]) * (nx + 1), pY, n);
-      }
-      if (qz >= 0) {
-        d1 = cpml_step(c7, d1, p7[s], n);
-        if (okx) str<P>(pout[7], (qz * (ny + 1) + yX[r]) * nx, pX, n);
-        d2 = cpml_step(c8, d2, p8[s], n);
-        if (oky) str<P>(pout[8], (qz * ny + yY[r]) * (nx + 1), pY, n);
-      }
-      if constexpr (kXEdge) {
-        if (qxE >= 0) {
-          d3 = cpml_step(c9, d3, p9[s], n);
-          if (oky) str<P>(pout[9], (kx * ny + yY[r]) * L2, qE, n);
-          d4 = cpml_step(c10, d4, p10[s], n);
-          if (okz) str<P>(pout[10], (kz * (ny + 1) + yX[r]) * L2, qE, n);
-        }
-      }
-      float nex = ec[0] * ex0[r] + es0[s] * (d0 - d1);
-      float ney = ec[1] * ey0[r] + es1[s] * (d2 - d3);
-      float nez = ec[2] * ez0[r] + es2[s] * (d4 - d5);
-      const bool y_pec = (mb && y == 0) || (mt && y == ny);
-      if (exy_pec || y_pec) nex = 0.0f;
-      if (exy_pec || (kXEdge && xface)) ney = 0.0f;
-      if (y_pec || (kXEdge && xface)) nez = 0.0f;
-      if (okx) str<T>(f.e_out[0], (kx * (ny + 1) + yX[r]) * nx, tX, nex);
-      if (oky) str<T>(f.e_out[1], (kx * ny + yY[r]) * (nx + 1), tY, ney);
-      if (okz) str<T>(f.e_out[2], (kz * (ny + 1) + yX[r]) * (nx + 1), tY, nez);
-    }
-#pragma unroll
-    for (int r = 0; r < kTY; ++r) {
-      phx[r] = hx[r + 1];
-      phy[r] = hy[r + 1];
-    }
-#pragma unroll
-    for (int r = 0; r < kTY + 2; ++r) {
-      ex0[r] = ex1[r];
-      ey0[r] = ey1[r];
-      ez0[r] = ez1[r];
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Workgroup-row variant: a workgroup of kR + 1 waves owns a 62 x kR tile and
-// marches along z.  Wave w sits on row y = j0 - 1 + w: waves 1..kR own rows
-// j0 .. j0+kR-1, wave 0 computes the halo row j0-1 of H (Hx, Hz, needed by
-// the E update of row j0) and also loads the top halo row j0+kR of E (Ez,
-// Ex, needed by the H update of row j0+kR-1).  Rows cross through LDS:
-//
-//   lE[b][s] = (Ez, Ex) of E(k) on row j0-1+s  (s = 0 .. kR+1), b = k & 1
-//   lH[b][w] = (Hz, Hx) of H_new(k) on row j0-1+w
-//
-// One barrier per plane: E(k+1) rows are published to the other buffer
-// before the barrier of plane k and read after it, and the buffers are reused
-// only two planes later, after another barrier.  Each lane holds one row, so
-// the per-lane state is a few dozen registers and many waves per SIMD keep
-// the loads in flight.
-// ---------------------------------------------------------------------------
 // Global-address-space views: the row base stays a scalar pair and the load
 // takes the ``global_load v, v_off, s[base]`` form.
 template <typename T>
@@ -481,7 +163,7 @@ __device__ __forceinline__ void* pout_base(const FusedParams& f, int t) {
   return static_cast<char*>(f.psi[t]) + f.psi_delta * static_cast<int64_t>(sizeof(P));
 }
 
-template <typename T, typename P, int kR, bool kEArr, bool kXEdge>
+template <typename T, typename P, typename C, int kR, bool kEArr, bool kXEdge>
 __global__ void __launch_bounds__(64 * (kR + 1)) fused_wg(FusedParams f) {
   __shared__ float lE[2][kR + 2][2][64];
   __shared__ float lH[2][kR + 1][2][64];
@@ -489,14 +171,8 @@ __global__ void __launch_bounds__(64 * (kR + 1)) fused_wg(FusedParams f) {
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const bool halo = w == 0;
 
-  int b = xcd_remap(blockIdx.x, gridDim.x);
-  int bx = b % f.box_ntx;
-  b /= f.box_ntx;
-  if (bx >= f.box_tsplit) bx += f.box_tskip;
-  bx += f.box_tx0;
-  const int by = f.box_ty0 + b % f.box_nty;
-  const int kz0 = f.box_z0 + (b / f.box_nty) * f.box_kz;
-  const int kz1 = min(kz0 + f.box_kz, f.box_z1);
+  int bx, by, kz0, kz1;
+  if (!decode_tile(f, bx, by, kz0, kz1)) return;  // padding workgroup
 
   const int nz = f.nz, ny = f.ny, nx = f.nx, L = f.L, L2 = 2 * f.L;
   const int m = f.metallic;
@@ -524,7 +200,7 @@ __global__ void __launch_bounds__(64 * (kR + 1)) fused_wg(FusedParams f) {
   const bool own = !halo && lane >= 1 && lane <= 62 && (!kXEdge || x <= nx);
   const uint32_t tX = xX * sizeof(T), tY = xY * sizeof(T);
   const uint32_t pX = xX * sizeof(P), pY = xY * sizeof(P);
-  const uint32_t cX = xX * 4u, cY = xY * 4u;
+  const uint32_t cX = xX * sizeof(C), cY = xY * sizeof(C);
   const uint32_t qH = max(qxH, 0) * sizeof(P), qE = max(qxE, 0) * sizeof(P);
   Coef c3{}, c4{}, c9{}, c10{};
   if constexpr (kXEdge) {
@@ -632,9 +308,9 @@ __global__ void __launch_bounds__(64 * (kR + 1)) fused_wg(FusedParams f) {
     float p6 = 0, p7 = 0, p8 = 0, p9 = 0, p10 = 0, p11 = 0;
     if (!halo && st) {
       if constexpr (kEArr) {
-        es0 = ldu<float>(f.esrc[0], (kx * (ny + 1) + yX) * nx, cX) * f.ratio_e;
-        es1 = ldu<float>(f.esrc[1], (kx * ny + yY) * (nx + 1), cY) * f.ratio_e;
-        es2 = ldu<float>(f.esrc[2], (kzc * (ny + 1) + yX) * (nx + 1), cY) * f.ratio_e;
+        es0 = ldu<C>(f.esrc[0], (kx * (ny + 1) + yX) * nx, cX) * f.ratio_e;
+        es1 = ldu<C>(f.esrc[1], (kx * ny + yY) * (nx + 1), cY) * f.ratio_e;
+        es2 = ldu<C>(f.esrc[2], (kzc * (ny + 1) + yX) * (nx + 1), cY) * f.ratio_e;
       }
       if (qyE >= 0) {
         p6 = ldu<P>(f.psi[6], (kx * L2 + qyE) * nx, pX);
@@ -773,76 +449,59 @@ __global__ void __launch_bounds__(64 * (kR + 1)) fused_wg(FusedParams f) {
   }
 }
 
-template <typename T, typename P, int kR>
+template <typename T, typename P, typename C, int kR>
 void launch_wg(hipStream_t s, const FusedParams& p, int nblocks) {
   const dim3 block(64 * (kR + 1), 1, 1), grid(nblocks, 1, 1);
   const bool earr = p.esrc_st[0] != 0;
   if (p.xedge) {
-    if (earr) hipLaunchKernelGGL((fused_wg<T, P, kR, true, true>), grid, block, 0, s, p);
-    else hipLaunchKernelGGL((fused_wg<T, P, kR, false, true>), grid, block, 0, s, p);
+    if (earr) hipLaunchKernelGGL((fused_wg<T, P, C, kR, true, true>), grid, block, 0, s, p);
+    else hipLaunchKernelGGL((fused_wg<T, P, float, kR, false, true>), grid, block, 0, s, p);
   } else {
-    if (earr) hipLaunchKernelGGL((fused_wg<T, P, kR, true, false>), grid, block, 0, s, p);
-    else hipLaunchKernelGGL((fused_wg<T, P, kR, false, false>), grid, block, 0, s, p);
+    if (earr) hipLaunchKernelGGL((fused_wg<T, P, C, kR, true, false>), grid, block, 0, s, p);
+    else hipLaunchKernelGGL((fused_wg<T, P, float, kR, false, false>), grid, block, 0, s, p);
   }
 }
 
-template <typename T, typename P, int kR>
-void launch_rows(hipStream_t s, const FusedParams& p, int nblocks) {
-  const dim3 block(64, 1, 1), grid(nblocks, 1, 1);
-  const bool earr = p.esrc_st[0] != 0;
-  if (p.xedge) {
-    if (earr) hipLaunchKernelGGL((fused_tile<T, P, kR, true, true>), grid, block, 0, s, p);
-    else hipLaunchKernelGGL((fused_tile<T, P, kR, false, true>), grid, block, 0, s, p);
-  } else {
-    if (earr) hipLaunchKernelGGL((fused_tile<T, P, kR, true, false>), grid, block, 0, s, p);
-    else hipLaunchKernelGGL((fused_tile<T, P, kR, false, false>), grid, block, 0, s, p);
-  }
-}
-
-// Kernel and tile height: the workgroup-row kernel with kR = 8 rows by
-// default (4 or 15 selectable), or the wave-tile kernel with 2 or 4 rows per
-// wave.  BZH_FUSED_KERNEL (wg | tile) and BZH_FUSED_ROWS override the default
-// for tuning sweeps.
-bool wg_setting() {
-  const char* env = std::getenv("BZH_FUSED_KERNEL");
-  return !(env && std::string(env) == "tile");
-}
+// Rows per workgroup: 15 by default, 8 selectable (BZH_FUSED_ROWS) for
+// tuning sweeps.
 int rows_setting() {
   int r = 0;
   if (const char* env = std::getenv("BZH_FUSED_ROWS")) r = std::atoi(env);
-  if (wg_setting()) return (r == 4 || r == 15) ? r : 8;
-  return (r == 2) ? 2 : 4;
+  return r == 8 ? 8 : 15;
+}
+
+template <typename T, typename P, typename C>
+void launch_rows(hipStream_t s, const FusedParams& p, int nblocks) {
+  if (p.rows == 8) launch_wg<T, P, C, 8>(s, p, nblocks);
+  else launch_wg<T, P, C, 15>(s, p, nblocks);
 }
 
 template <typename T, typename P>
-void launch_typed(hipStream_t s, const FusedParams& p, int nblocks) {
-  const int r = p.rows;
-  if (p.wg) {
-    if (r == 4) launch_wg<T, P, 4>(s, p, nblocks);
-    else if (r == 15) launch_wg<T, P, 15>(s, p, nblocks);
-    else launch_wg<T, P, 8>(s, p, nblocks);
+void launch_coef(hipStream_t s, const FusedParams& p, int nblocks) {
+  if constexpr (std::is_same<T, float>::value) {
+    launch_rows<T, P, float>(s, p, nblocks);
   } else {
-    if (r == 2) launch_rows<T, P, 2>(s, p, nblocks);
-    else launch_rows<T, P, 4>(s, p, nblocks);
+    // half-precision fields carry float16 (folded) or float32 E coefficients
+    if (p.coef_f16) launch_rows<T, P, __half>(s, p, nblocks);
+    else launch_rows<T, P, float>(s, p, nblocks);
   }
 }
 
 template <typename T>
 void launch_field(hipStream_t s, const FusedParams& p, int nblocks) {
-  if (p.psi_bf16) launch_typed<T, bf16_t>(s, p, nblocks);
-  else launch_typed<T, float>(s, p, nblocks);
+  if (p.psi_bf16) launch_coef<T, bf16_t>(s, p, nblocks);
+  else launch_coef<T, float>(s, p, nblocks);
 }
 
 }  // namespace
 
 int fused_rows() { return rows_setting(); }
-int fused_wg() { return wg_setting() ? 1 : 0; }
 
 void launch_fused3d(hipStream_t s, const FusedParams& p, int dtype) {
   BZH_REQUIRE(p.box_ntx >= 0 && p.box_nty >= 0 && p.box_kz >= 1 && p.box_nzc >= 0,
               "fused box is malformed");
-  BZH_REQUIRE(p.wg ? (p.rows == 4 || p.rows == 8 || p.rows == 15) : (p.rows == 2 || p.rows == 4),
-              "fused tile height is not one the kernel is built for");
+  BZH_REQUIRE(p.rows == 8 || p.rows == 15, "fused tile height is not one the kernel is built for");
+  BZH_REQUIRE(!p.coef_f16 || dtype != BZH_F32, "float16 coefficients need half-precision fields");
   const int ntx_total = (p.nx + 1 + 61) / 62, nty_total = (p.ny + 1 + p.rows - 1) / p.rows;
   BZH_REQUIRE(p.box_tx0 >= 0 && p.box_tsplit >= 0 && p.box_tskip >= 0 &&
                   p.box_tx0 + p.box_ntx + p.box_tskip <= ntx_total && p.box_ty0 >= 0 &&
@@ -864,7 +523,10 @@ void launch_fused3d(hipStream_t s, const FusedParams& p, int dtype) {
                 "the fused step needs uniform H coefficients and E decay");
     BZH_REQUIRE(p.esrc_st[c] == p.esrc_st[0], "E source coefficients must be all arrays or all scalars");
   }
-  const int64_t nblocks = static_cast<int64_t>(p.box_ntx) * p.box_nty * p.box_nzc;
+  const int64_t tiles = static_cast<int64_t>(p.box_ntx) * p.box_nty;
+  BZH_REQUIRE(p.box_band == 0 || (p.box_band >= 1 && 8 * int64_t{p.box_band} >= tiles),
+              "fused bands do not cover the tiles");
+  const int64_t nblocks = (p.box_band > 0 ? 8 * int64_t{p.box_band} : tiles) * p.box_nzc;
   BZH_REQUIRE(nblocks < (int64_t{1} << 31), "fused grid too large");
   if (nblocks <= 0) return;
   switch (dtype) {

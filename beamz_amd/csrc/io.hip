@@ -45,6 +45,143 @@ __global__ void __launch_bounds__(256) source_kernel(SourceParams p) {
 constexpr int kDftBlock = 128;
 constexpr int kMaxFreq = 256;
 
+// Weighted gather of one monitor sample (observe.py:360-370, 529-585).
+template <typename T>
+__device__ __forceinline__ float gather(const DftParams& p, int m, int c, int pt) {
+  const T* fld = static_cast<const T*>(p.field[c]);
+  const int64_t base = ((static_cast<int64_t>(m) * 6 + c) * p.Pmax + pt) * p.K;
+  float v = 0.0f;
+  for (int q = 0; q < p.K; ++q) {
+    const int idx = p.indices[base + q];
+    const float w = p.weights[base + q];
+    if (w != 0.0f && idx >= 0 && idx < p.fsize[c]) v += w * load_f(fld, idx);
+  }
+  return v;
+}
+
+// Gate, window and amplitude factor of step k of this call for monitor m
+// (observe.py:315-357).  Returns false when the step is not sampled.
+__device__ __forceinline__ bool dft_gate(const DftParams& p, int m, int k, double& t,
+                                         double& window, double& scale) {
+  const int interval = p.counts[m * 5 + 2] > 1 ? p.counts[m * 5 + 2] : 1;
+  const int step = *p.step0 + k;
+  const double t0 = p.time_f64 ? *static_cast<const double*>(p.time)
+                               : static_cast<double>(*static_cast<const float*>(p.time));
+  t = t0 + static_cast<double>(k + 1) * p.dt;
+  const double ts = p.windows[m * 3 + 0];
+  const double te = p.windows[m * 3 + 1];
+  if (!(t >= ts && t <= te) || (step % interval) != 0) return false;
+  window = 1.0;
+  if (p.codes[m * 2 + 0] == 1 && isfinite(te) && te > ts) {
+    double tau = (t - ts) / fmax(te - ts, 1e-30);
+    tau = fmin(fmax(tau, 0.0), 1.0);
+    window = 0.5 * (1.0 - cos(2.0 * M_PI * tau));
+  }
+  scale = fmax(window, 0.0);
+  if (p.codes[m * 2 + 1] == 1) {
+    const double lu = p.windows[m * 3 + 2];
+    scale *= (p.dt * interval * 299792458.0 / lu) / sqrt(2.0 * M_PI);
+  }
+  return true;
+}
+
+// Buffered path, step k: stage the gathered sample (f32) in ring slot k % slots.
+template <typename T>
+__global__ void __launch_bounds__(kDftBlock) dft_stage_kernel(DftParams p) {
+  const int m = blockIdx.z / 6;
+  const int c = blockIdx.z % 6;
+  const int P = p.counts[m * 5 + 1];
+  double t, window, scale;
+  if (!dft_gate(p, m, p.k, t, window, scale)) return;
+  const int pt = blockIdx.x * kDftBlock + threadIdx.x;
+  if (p.masks[m * 6 + c] == 0.0f || pt >= P || pt >= p.Pmax || p.field[c] == nullptr) return;
+  const int64_t slot = p.k % p.slots;
+  p.stage[((slot * p.M + m) * 6 + c) * p.Pmax + pt] = gather<T>(p, m, c, pt);
+}
+
+// Buffered path: fold steps k0 .. k0+nk-1 of the ring into the accumulators.
+// Each step's term is formed exactly as in the direct path (f32 sample times
+// 1/scale, mask and amplitude in f64, times the f64 phase factor); the nk
+// terms of one (frequency, point) are summed in f64 and added once.
+template <typename A>
+__global__ void __launch_bounds__(kDftBlock) dft_flush_kernel(DftParams p) {
+  constexpr int kFt = 16;  // frequencies per LDS tile
+  const int m = blockIdx.z / 6;
+  const int c = blockIdx.z % 6;
+  const int P = p.counts[m * 5 + 1];
+  const int F = p.counts[m * 5 + 0];
+  const int64_t voff = p.counts[m * 5 + 3];
+  const int64_t woff = p.counts[m * 5 + 4];
+  const int nk = p.nk < kDftMaxSlots ? p.nk : kDftMaxSlots;
+  __shared__ double s_t[kDftMaxSlots], s_scale[kDftMaxSlots], s_win[kDftMaxSlots];
+  __shared__ int s_on[kDftMaxSlots];
+  __shared__ double s_cos[kDftMaxSlots][kFt], s_sin[kDftMaxSlots][kFt];
+  if (threadIdx.x < nk) {
+    double t = 0.0, window = 0.0, scale = 0.0;
+    const bool on = dft_gate(p, m, p.k0 + threadIdx.x, t, window, scale);
+    s_t[threadIdx.x] = t;
+    s_scale[threadIdx.x] = on ? scale : 0.0;
+    s_win[threadIdx.x] = on ? window : 0.0;
+    s_on[threadIdx.x] = on ? 1 : 0;
+  }
+  __syncthreads();
+  const int nf = F < p.Fmax ? F : p.Fmax;
+  if (c == 0 && blockIdx.x == 0 && threadIdx.x == 0) {
+    A* w = static_cast<A*>(p.wsum);
+    for (int f = 0; f < nf; ++f) {
+      double acc = 0.0;
+      for (int j = 0; j < nk; ++j) acc += s_win[j];
+      if (woff + f < p.w_len) w[woff + f] += static_cast<A>(acc);
+    }
+  }
+  const float cmask = p.masks[m * 6 + c];
+  const int pt = blockIdx.x * kDftBlock + threadIdx.x;
+  const bool live = cmask != 0.0f && pt < P && pt < p.Pmax && p.field[c] != nullptr;
+  double dv[kDftMaxSlots];
+#pragma unroll
+  for (int j = 0; j < kDftMaxSlots; ++j) {
+    dv[j] = 0.0;
+    if (live && j < nk && s_on[j]) {
+      const int64_t slot = (p.k0 + j) % p.slots;
+      const float v = p.stage[((slot * p.M + m) * 6 + c) * p.Pmax + pt];
+      dv[j] = static_cast<double>(v) * p.inv_scale[c] * cmask * s_scale[j];
+    }
+  }
+  A* re = static_cast<A*>(p.re);
+  A* im = static_cast<A*>(p.im);
+  for (int f0 = 0; f0 < nf; f0 += kFt) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < nk * kFt; e += kDftBlock) {
+      const int j = e / kFt, f = f0 + e % kFt;
+      double sn = 0.0, cs = 0.0;
+      if (f < nf) {
+        const double fr = p.freqs_f64 ? static_cast<const double*>(p.freqs)[m * p.Fmax + f]
+                                      : static_cast<double>(static_cast<const float*>(p.freqs)[m * p.Fmax + f]);
+        sincos(2.0 * M_PI * fr * s_t[j], &sn, &cs);
+      }
+      s_cos[j][e % kFt] = cs;
+      s_sin[j][e % kFt] = sn;
+    }
+    __syncthreads();
+    if (!live) continue;
+    for (int ff = 0; ff < kFt && f0 + ff < nf; ++ff) {
+      double ar = 0.0, ai = 0.0;
+#pragma unroll
+      for (int j = 0; j < kDftMaxSlots; ++j) {
+        if (j < nk) {
+          ar += dv[j] * s_cos[j][ff];
+          ai += dv[j] * s_sin[j][ff];
+        }
+      }
+      const int64_t o = voff + (static_cast<int64_t>(c) * F + f0 + ff) * P + pt;
+      if (o < p.re_len) {
+        re[o] += static_cast<A>(ar);
+        im[o] += static_cast<A>(ai);
+      }
+    }
+  }
+}
+
 template <typename T, typename A>
 __global__ void __launch_bounds__(kDftBlock) dft_kernel(DftParams p) {
   const int m = blockIdx.z / 6;
@@ -95,14 +232,7 @@ __global__ void __launch_bounds__(kDftBlock) dft_kernel(DftParams p) {
   const float cmask = p.masks[m * 6 + c];
   const int pt = blockIdx.x * kDftBlock + threadIdx.x;
   if (cmask == 0.0f || pt >= P || pt >= p.Pmax || p.field[c] == nullptr) return;
-  const T* fld = static_cast<const T*>(p.field[c]);
-  const int64_t base = ((static_cast<int64_t>(m) * 6 + c) * p.Pmax + pt) * p.K;
-  float v = 0.0f;
-  for (int q = 0; q < p.K; ++q) {
-    const int idx = p.indices[base + q];
-    const float w = p.weights[base + q];
-    if (w != 0.0f && idx >= 0 && idx < p.fsize[c]) v += w * load_f(fld, idx);
-  }
+  const float v = gather<T>(p, m, c, pt);
   const double dv = static_cast<double>(v) * p.inv_scale[c] * cmask * scale;
   A* re = static_cast<A*>(p.re);
   A* im = static_cast<A*>(p.im);
@@ -119,8 +249,15 @@ template <typename T>
 void dft_dispatch(hipStream_t s, const DftParams& p) {
   dim3 block(kDftBlock, 1, 1);
   dim3 grid((p.Pmax + kDftBlock - 1) / kDftBlock, 1, p.M * 6);
-  if (p.acc_f64) hipLaunchKernelGGL((dft_kernel<T, double>), grid, block, 0, s, p);
-  else hipLaunchKernelGGL((dft_kernel<T, float>), grid, block, 0, s, p);
+  if (p.mode == 1) {
+    hipLaunchKernelGGL((dft_stage_kernel<T>), grid, block, 0, s, p);
+  } else if (p.mode == 2) {
+    if (p.acc_f64) hipLaunchKernelGGL((dft_flush_kernel<double>), grid, block, 0, s, p);
+    else hipLaunchKernelGGL((dft_flush_kernel<float>), grid, block, 0, s, p);
+  } else {
+    if (p.acc_f64) hipLaunchKernelGGL((dft_kernel<T, double>), grid, block, 0, s, p);
+    else hipLaunchKernelGGL((dft_kernel<T, float>), grid, block, 0, s, p);
+  }
 }
 
 }  // namespace
@@ -141,6 +278,11 @@ void launch_source(hipStream_t s, const SourceParams& p) {
 void launch_dft(hipStream_t s, const DftParams& p) {
   if (p.M <= 0 || p.Pmax <= 0) return;
   BZH_REQUIRE(p.Fmax <= kMaxFreq, "DFT monitors support at most 256 frequencies");
+  if (p.mode != 0) {
+    BZH_REQUIRE(p.stage != nullptr && p.slots >= 1 && p.slots <= kDftMaxSlots,
+                "buffered DFT needs a staging ring of 1..16 slots");
+    BZH_REQUIRE(p.mode != 2 || (p.nk >= 1 && p.nk <= p.slots), "DFT flush range exceeds the ring");
+  }
   switch (p.dtype) {
     case BZH_F32: dft_dispatch<float>(s, p); break;
     case BZH_BF16: dft_dispatch<bf16_t>(s, p); break;

@@ -329,6 +329,7 @@ void dispatch_tm(hipStream_t s, const PhaseParams& p, int phase) {
 }  // namespace
 
 void launch_phase3d(hipStream_t s, const PhaseParams& p, int phase, int dtype) {
+  BZH_REQUIRE(!p.scf_f16, "f16 coefficient arrays are read only by the fused step");
   switch (dtype) {
     case BZH_F32: dispatch3d<float>(s, p, phase); break;
     case BZH_BF16: dispatch3d<bf16_t>(s, p, phase); break;
@@ -339,6 +340,7 @@ void launch_phase3d(hipStream_t s, const PhaseParams& p, int phase, int dtype) {
 }
 
 void launch_phase_tm(hipStream_t s, const PhaseParams& p, int phase, int dtype) {
+  BZH_REQUIRE(!p.scf_f16, "f16 coefficient arrays are read only by the fused step");
   switch (dtype) {
     case BZH_F32: dispatch_tm<float>(s, p, phase); break;
     case BZH_BF16: dispatch_tm<bf16_t>(s, p, phase); break;

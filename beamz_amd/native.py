@@ -75,6 +75,41 @@ class ProgramAttrs(ctypes.Structure):
     ]
 
 
+class Sim2d(ctypes.Structure):
+    """``bzh_sim2d``: one TMxy simulation of a batch (two field banks)."""
+    _fields_ = [
+        ("ez", Buffer * 2), ("hx", Buffer * 2), ("hy", Buffer * 2),
+        ("e_decay", Buffer), ("e_source", Buffer), ("h_decay", Buffer), ("h_source", Buffer),
+        ("cpml_table", Buffer), ("psi", (Buffer * 4) * 2),
+        ("src_coeff", Buffer * 4), ("src_wave", Buffer * 4),
+        ("src_start", (ctypes.c_int32 * 2) * 4), ("nsrc", ctypes.c_int32),
+        ("cpml_cells", ctypes.c_int32), ("metallic", ctypes.c_int32),
+        ("resolution", ctypes.c_float),
+    ]
+
+
+class BatchAttrs(ctypes.Structure):
+    _fields_ = [
+        ("abi_version", ctypes.c_int32),
+        ("flags", ctypes.c_int32),
+        ("nsteps", ctypes.c_int32),
+        ("tile_rows", ctypes.c_int32),
+        ("workgroups", ctypes.c_int32),
+    ]
+
+
+class Adjoint2d(ctypes.Structure):
+    """``bzh_adjoint2d``: the reverse-time TMxy problem and its adjoint state."""
+    _fields_ = [
+        ("e_decay", Buffer), ("e_source", Buffer), ("h_decay", Buffer), ("h_source", Buffer),
+        ("cpml_table", Buffer), ("cpml_cells", ctypes.c_int32), ("metallic", ctypes.c_int32),
+        ("resolution", ctypes.c_float),
+        ("a_ez", Buffer), ("a_hx", Buffer), ("a_hy", Buffer), ("a_psi", Buffer * 4),
+        ("work", Buffer * 4), ("source_profile", Buffer), ("source_time", Buffer),
+        ("design", ctypes.c_int32 * 4), ("record", Buffer), ("grad", Buffer),
+    ]
+
+
 # Every exported symbol and its (restype, argtypes); checked by the tests
 # against the declarations of include/beamz_hip.h.
 SIGNATURES = {
@@ -88,6 +123,14 @@ SIGNATURES = {
                                       ctypes.POINTER(PhaseAttrs)]),
     "bzh_program": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(Buffer), ctypes.c_int32,
                                      ctypes.POINTER(ProgramAttrs)]),
+    "bzh_batch2d": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(Sim2d), ctypes.c_int32,
+                                     ctypes.POINTER(Buffer), ctypes.POINTER(BatchAttrs)]),
+    "bzh_adjoint2d_record": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(Adjoint2d),
+                                              ctypes.POINTER(Buffer), ctypes.POINTER(Buffer),
+                                              ctypes.c_int32]),
+    "bzh_adjoint2d_reverse": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(Adjoint2d),
+                                               ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                               ctypes.c_int32]),
 }
 
 _LOCK = threading.Lock()
@@ -177,3 +220,27 @@ def call_program(args, attrs: ProgramAttrs, stream=None):
     arr = buffers(args)
     s = current_stream_handle() if stream is None else stream
     _check(lib.bzh_program(s, arr, len(args), ctypes.byref(attrs)), lib)
+
+
+def call_batch2d(sims, current_step, attrs: BatchAttrs, stream=None):
+    """``sims``: a ctypes array of :class:`Sim2d`; tensors it points to must
+    stay alive for the call."""
+    lib = library()
+    step = buffer(current_step)
+    s = current_stream_handle() if stream is None else stream
+    _check(lib.bzh_batch2d(s, sims, len(sims), ctypes.byref(step), ctypes.byref(attrs)), lib)
+
+
+def call_adjoint_record(adj, hx, hy, slot, stream=None):
+    lib = library()
+    s = current_stream_handle() if stream is None else stream
+    bx, by = buffer(hx), buffer(hy)
+    _check(lib.bzh_adjoint2d_record(s, ctypes.byref(adj), ctypes.byref(bx), ctypes.byref(by),
+                                    int(slot)), lib)
+
+
+def call_adjoint_reverse(adj, n_last, count, slot_last, flags, stream=None):
+    lib = library()
+    s = current_stream_handle() if stream is None else stream
+    _check(lib.bzh_adjoint2d_reverse(s, ctypes.byref(adj), int(n_last), int(count),
+                                     int(slot_last), int(flags)), lib)

@@ -19,7 +19,7 @@ import torch
 from . import _abi as abi
 from .boundaries import (PML, edge_mask, extrude_through_cpml, lower_cpml,
                          resolve_metallic_edges)
-from .constants import EPS_0, MU_0
+from .constants import EPS_0, ETA_0, MU_0
 from .lattice import component_shapes, sample_e_site_3d, sample_site_tm
 from .monitors import assign_offsets, lower_monitor, pack_monitors
 from .sources import CustomSource, SoftSource, group_sources, lower_soft_source
@@ -75,6 +75,9 @@ class CompiledProgram:
     device: torch.device
     field_dtype: torch.dtype
     flags: int
+    abi_resolution: float = None   # cell size handed to the kernels (1 when folded)
+    cpml_table: torch.Tensor = None  # [terms, 3, 2L] when every CPML term has thickness L
+    field_scale: dict = None       # storage scale of the E and H fields
     eps_cells: np.ndarray = field(repr=False, default=None)
     eps_site: dict = field(repr=False, default=None)
 
@@ -92,6 +95,31 @@ class CompiledProgram:
         return 0 if self.cpml is None else len(self.cpml.h_terms)
 
 
+HALF_DTYPES = (torch.bfloat16, torch.float16)
+
+
+def storage_scales(customs, field_dtype, field_scale=None):
+    """Storage scales ``(s_e, s_h)`` of the E and H fields.  float32 fields are
+    stored unscaled.  Half-precision fields are stored as E / s_e and H / s_h
+    with s_e the largest per-step source increment (|coefficient| x |waveform|,
+    H sources weighted by eta0) -- the order of the peak field a soft source
+    builds -- and s_h = s_e / eta0, so both stored fields are O(1) and far from
+    the float16 range limits."""
+    if field_dtype not in HALF_DTYPES:
+        return 1.0, 1.0
+    if field_scale is not None:
+        s_e = float(field_scale)
+        if not (s_e > 0.0 and np.isfinite(s_e)):
+            raise ValueError("field_scale must be a positive finite number")
+        return s_e, s_e / ETA_0
+    peak = 0.0
+    for c in customs:
+        amp = float(np.max(np.abs(c.coeff), initial=0.0)) * float(np.max(np.abs(c.waveform), initial=0.0))
+        peak = max(peak, amp * (ETA_0 if c.component[0] == "H" else 1.0))
+    s_e = peak if peak > 0.0 and np.isfinite(peak) else 1.0
+    return s_e, s_e / ETA_0
+
+
 def _dev(x, device, dtype=torch.float32):
     return torch.as_tensor(np.asarray(x), dtype=dtype).to(device).contiguous()
 
@@ -107,7 +135,7 @@ def _elide(arr, device):
 
 def compile_program(permittivity, resolution, dt, *, boundaries=(), sources=(), monitors=(),
                     num_steps=1, t0=0.0, conductivity=None, device=None,
-                    field_dtype=torch.float32, flags=abi.DEFAULT_FLAGS):
+                    field_dtype=torch.float32, flags=abi.DEFAULT_FLAGS, field_scale=None):
     """Lower ``(permittivity raster, dx, dt, boundaries, sources, monitors)``."""
     device = torch.device(device or ("cuda" if torch.cuda.is_available() else "cpu"))
     eps = np.asarray(permittivity, dtype=np.float64)
@@ -130,6 +158,7 @@ def compile_program(permittivity, resolution, dt, *, boundaries=(), sources=(), 
     # update coefficients (kernels.py:704-728), evaluated in float64
     coeffs = {}
     eps_site = {}
+    e_source_raw = {}
     e_comps = ("Ex", "Ey", "Ez") if is_3d else ("Ez",)
     for ax in "xyz":
         comp = "E" + ax
@@ -139,12 +168,11 @@ def compile_program(permittivity, resolution, dt, *, boundaries=(), sources=(), 
             beta = ss * dt / (2.0 * EPS_0 * es)
             eps_site[comp] = es
             coeffs[f"e_decay_{ax}"] = _elide((1.0 - beta) / (1.0 + beta), device)
-            coeffs[f"e_source_{ax}"] = _elide((dt / (EPS_0 * es)) / (1.0 + beta), device)
+            e_source_raw[ax] = (dt / (EPS_0 * es)) / (1.0 + beta)
         else:
             coeffs[f"e_decay_{ax}"] = torch.tensor(1.0, device=device)
-            coeffs[f"e_source_{ax}"] = torch.tensor(0.0, device=device)
+            e_source_raw[ax] = np.zeros(1)
         coeffs[f"h_decay_{ax}"] = torch.tensor(1.0, device=device)
-        coeffs[f"h_source_{ax}"] = torch.tensor(float(np.float32(dt / MU_0)), device=device)
 
     # CPML terms (compile.py:241-305) -> metadata and packed coefficient arrays
     thickness = 0
@@ -188,6 +216,24 @@ def compile_program(permittivity, resolution, dt, *, boundaries=(), sources=(), 
             customs.append(s)
         else:
             raise TypeError(f"unsupported source {type(s).__name__}")
+
+    # storage scaling of half-precision fields (no counterpart in the
+    # reference, whose fields are float32): stored E = E / s_e, stored
+    # H = H / s_h.  The scales and 1/dx are folded into the update coefficients
+    # (the kernels then run with a unit inverse cell size), the source
+    # coefficients are divided by the target's scale and the monitor component
+    # masks multiplied by it, so every kernel is unchanged and every monitor
+    # output is in physical units.
+    s_e, s_h = storage_scales(customs, field_dtype, field_scale)
+    half = field_dtype in HALF_DTYPES
+    fold = 1.0 / float(resolution) if half else 1.0
+    if half:
+        customs = [CustomSource(c.component, c.start,
+                                np.asarray(c.coeff, dtype=np.float64) / (s_e if c.component[0] == "E" else s_h),
+                                c.waveform, c.timing) for c in customs]
+    h_source = dt / MU_0 * (s_e / s_h) * fold
+    for ax in "xyz":
+        coeffs[f"h_source_{ax}"] = torch.tensor(float(np.float32(h_source)), device=device)
     groups = []
     coincident = 0
     for gi, g in enumerate(group_sources(customs, is_3d)):
@@ -198,17 +244,30 @@ def compile_program(permittivity, resolution, dt, *, boundaries=(), sources=(), 
         groups.append((_dev(c, device), _dev(w, device), _dev(st, device, torch.int32)))
         coincident |= (1 << gi) if co else 0
 
+    # E source coefficients: float32, or float16 when half-precision fields
+    # run the fused step (the folded values c dt / (eps dx) lie in [0.02, 1])
+    fused = bool(is_3d and (flags & abi.FLAG_FUSED_STEP) and all(g is None for g in groups[3:6]))
+    coef_dtype = torch.float16 if (half and fused) else torch.float32
+    for ax in "xyz":
+        arr = np.asarray(e_source_raw[ax], dtype=np.float64) * (s_h / s_e) * fold
+        t = _elide(arr, device)
+        coeffs[f"e_source_{ax}"] = t if t.dim() == 0 else t.to(coef_dtype)
+
     compiled = [lower_monitor(m, grid, shapes, resolution, i) for i, m in enumerate(monitors)]
     assign_offsets(compiled)
     packed = ()
     if compiled:
         idx, wts, fr, masks, counts, codes, windows = pack_monitors(compiled)
+        masks[:, :3] *= s_e    # DFT component order Ex, Ey, Ez, Hx, Hy, Hz
+        masks[:, 3:] *= s_h
         packed = (_dev(idx, device, torch.int32), _dev(wts, device), _dev(fr, device, torch.float64),
                   _dev(masks, device), _dev(counts, device, torch.int32),
                   _dev(codes, device, torch.int32), _dev(windows, device))
 
     return CompiledProgram(
         grid=grid, is_3d=is_3d, resolution=float(resolution), dt=float(dt), t0=float(t0),
+        abi_resolution=1.0 if half else float(resolution), field_scale={"E": s_e, "H": s_h},
+        cpml_table=table,
         num_steps=int(num_steps), shapes=shapes, metallic=metallic,
         boundary_code=boundary_code, cpml=cpml, coeffs=coeffs, h_meta=h_meta, e_meta=e_meta,
         h_term_arrays=h_arrays, e_term_arrays=e_arrays, source_groups=groups,
@@ -228,8 +287,15 @@ def initial_state(prog: CompiledProgram, fields=None) -> SimulationState:
         else:
             out[c.lower()] = torch.zeros(0, dtype=dt_, device=dev)
     if fields:
+        scale = prog.field_scale or {"E": 1.0, "H": 1.0}
+        limit = torch.finfo(dt_).max
         for k, v in fields.items():
-            out[k].copy_(torch.as_tensor(np.asarray(v), dtype=dt_))
+            stored = np.asarray(v, dtype=np.float64) / scale[k[0].upper()]
+            if stored.size and np.max(np.abs(stored)) >= limit:
+                raise ValueError(
+                    f"initial {k} exceeds the {dt_} storage range at field scale "
+                    f"{scale[k[0].upper()]:.3e}; pass field_scale= to the simulation")
+            out[k].copy_(torch.as_tensor(stored, dtype=dt_))
 
     # both CPML banks live in one arena [2, total] so that every psi of bank B
     # sits at the same element distance from its bank-A twin

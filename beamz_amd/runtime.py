@@ -78,7 +78,7 @@ def _source_group_arguments(prog):
 def _program_attrs(prog, nsteps, layout, monitor_count=0):
     return native.ProgramAttrs(
         abi_version=abi.ABI_VERSION, flags=prog.flags, nsteps=int(nsteps), dt=prog.dt,
-        resolution=prog.resolution, boundary_code=prog.boundary_code, metric_kind=0,
+        resolution=prog.abi_resolution, boundary_code=prog.boundary_code, metric_kind=0,
         program_layout=layout, cpml_enabled=int(prog.cpml_enabled),
         monitor_count=int(monitor_count), coincident_source_group_mask=prog.coincident_mask)
 
@@ -98,7 +98,7 @@ def _phase(prog, state, phase):
     args += list(_metrics(prog))
     attrs = native.PhaseAttrs(
         abi_version=abi.ABI_VERSION, flags=prog.flags, phase=phase, nterms=prog.nterms,
-        dt=prog.dt, resolution=prog.resolution, boundary_code=prog.boundary_code, metric_kind=0)
+        dt=prog.dt, resolution=prog.abi_resolution, boundary_code=prog.boundary_code, metric_kind=0)
     native.call_streamed(args, attrs)
     return state
 

@@ -53,12 +53,149 @@ def build_workload(dtype_name, monitors=True, fused=True):
     return sim, eps
 
 
+def build_batched_workload(seed=4, count=64, steps=3000):
+    """The batched 2D sweep: ``count`` TMxy simulations, each (ny, nx) drawn
+    uniformly from {256..1024}^2 at 20 nm, with 2-4 x-directed waveguides of
+    width U[400, 600] nm (eps 12.25 in 2.085), 10-cell CPML on every edge and
+    a soft Ez line source (Gaussian pulse at 1.55 um, 0.2 um bandwidth) across
+    every guide at x = 40 cells."""
+    import beamz_amd as bz
+    rng = np.random.default_rng(seed)
+    dx = 20e-9
+    dt = 0.99 * dx / (bz.LIGHT_SPEED * np.sqrt(2.0))
+    f0 = bz.LIGHT_SPEED / 1.55e-6
+    fw = bz.LIGHT_SPEED * 0.2e-6 / (1.55e-6) ** 2
+    sims = []
+    for _ in range(count):
+        ny, nx = (int(v) for v in rng.integers(256, 1025, size=2))
+        eps = np.full((ny, nx), 2.085)
+        nguides = int(rng.integers(2, 5))
+        centres = np.linspace(0, ny, nguides + 2)[1:-1]
+        srcs = []
+        for c in centres:
+            w = int(round(rng.uniform(400e-9, 600e-9) / dx))
+            y0 = int(c) - w // 2
+            eps[y0:y0 + w, :] = 12.25
+            srcs.append(bz.SoftSource("Ez", (y0, 40), (w, 1), bz.GaussianPulse(f0, fw)))
+        sims.append(bz.Simulation(eps, dx, dt=dt, boundaries=[bz.PML(cells=10)], sources=srcs[:4],
+                                  steps=steps))
+    return sims
+
+
+def run_batched(args):
+    import torch
+    from beamz_amd.batch import BatchedSimulation2D, autotune
+    print("[bench] building the batched 2D workload", file=sys.stderr, flush=True)
+    batch = BatchedSimulation2D(build_batched_workload())
+    state = batch.initial_state()
+    torch.cuda.synchronize()
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        tr, wg, timings = autotune(batch, steps=4, verbose=True)
+        batch.advance(state, args.warmup, chunk=args.chunk)
+        torch.cuda.synchronize()
+        start = torch.cuda.Event(enable_timing=True)
+        stop = torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        batch.advance(state, args.steps, chunk=args.chunk)
+        stop.record(stream)
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop) / args.steps
+    gcups = batch.cells / (ms * 1e-3) / 1e9
+    line = {
+        "metric": "GCUPS", "value": round(gcups, 3), "unit": "GCUPS", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "batched 2D TMxy sweep: 64 simulations, (ny, nx) ~ U{256..1024}^2 at "
+                               "20 nm, 2-4 waveguides, 10-cell CPML, Ez line sources; one persistent "
+                               "launch per step, hipGraph chunks",
+                   "cells": batch.cells, "simulations": len(batch.progs),
+                   "tile_rows": tr, "workgroups": wg},
+        "autotune_ms_per_step": {f"{k[0]}x{k[1]}": round(v, 4) for k, v in timings.items()},
+    }
+    print(json.dumps(line), flush=True)
+
+
+def build_adjoint_problem(steps=4000, checkpoint=200, seed=5):
+    """The 2D inverse-design bend: 600 x 600 cells at 20 nm, a 100 x 100
+    design box of density rho ~ U[0, 1] (eps = 2.085 + 10.165 rho) at the
+    centre, a 500 nm input guide along x from the left edge into the box and a
+    500 nm output guide along y from the box to the top edge; 10-cell CPML;
+    a soft Ez line source across the input guide at x = 40 with a cos^2
+    transverse profile (an approximation of the fundamental mode: the mode
+    solver is not part of this backend), Gaussian pulse at 1.55 um; the
+    objective is the overlap power of Ez on a line across the output guide
+    with the same profile, at three wavelengths around 1.55 um."""
+    import beamz_amd as bz
+    from beamz_amd.adjoint import AdjointProblem2D
+    n, dx, L = 600, 20e-9, 10
+    dt = 0.99 * dx / (bz.LIGHT_SPEED * np.sqrt(2.0))
+    base = np.full((n, n), 2.085)
+    w = 25
+    c0 = n // 2 - w // 2                      # guide rows / columns [c0, c0 + w)
+    design = (250, 350, 250, 350)
+    base[c0:c0 + w, :design[2]] = 12.25       # input guide: y rows, x from 0 to the box
+    base[design[1]:, c0:c0 + w] = 12.25       # output guide: x columns, y from the box up
+    prof = np.cos(np.pi * (np.arange(w) + 0.5 - w / 2) / w) ** 2
+    f0 = bz.LIGHT_SPEED / 1.55e-6
+    fw = bz.LIGHT_SPEED * 0.2e-6 / (1.55e-6) ** 2
+    src = bz.SoftSource("Ez", (c0, 40), (w, 1), bz.GaussianPulse(f0, fw), profile=prof[:, None])
+    freqs = bz.LIGHT_SPEED / np.array([1.5e-6, 1.55e-6, 1.6e-6])
+    mon = bz.FluxMonitor("y", 520, freqs, ranges={"x": (c0, c0 + w)}, components=("Ez",))
+    rho = np.random.default_rng(seed).uniform(0.0, 1.0, (100, 100))
+    prob = AdjointProblem2D(base, design, eps_min=2.085, eps_max=12.25, resolution=dx, dt=dt,
+                            steps=steps, sources=[src], monitor=mon, mode=prof, pml_cells=L,
+                            checkpoint=checkpoint)
+    return prob, rho
+
+
+def run_adjoint(args):
+    import torch
+    from beamz_amd.adjoint import objective
+    print("[bench] building the adjoint bend problem", file=sys.stderr, flush=True)
+    prob, rho0 = build_adjoint_problem()
+    prob.objective(rho0)                                  # warm: compile, capture graphs
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rho = torch.tensor(rho0, dtype=torch.float64, requires_grad=True)
+    J = objective(rho, prob)
+    t1 = time.perf_counter()
+    J.backward()
+    t2 = time.perf_counter()
+    g = rho.grad.numpy()
+    rng = np.random.default_rng(17)
+    pix = [tuple(p) for p in rng.integers(0, 100, size=(32, 2))]
+    h = 1e-3
+    fd = []
+    for a, b in pix:
+        rp, rm = rho0.copy(), rho0.copy()
+        rp[a, b] += h
+        rm[a, b] -= h
+        fd.append((prob.objective(rp) - prob.objective(rm)) / (2 * h))
+    fd = np.asarray(fd)
+    got = np.asarray([g[a, b] for a, b in pix])
+    rel = np.abs(got - fd) / np.max(np.abs(fd))
+    cells = prob.base_eps.size
+    line = {
+        "metric": "adjoint gradient vs central FD", "value": float(rel.max()), "unit": "max relative error",
+        "n_gpus": 1, "steps": prob.steps, "warmup": 1, "higher_is_better": False, "data": "synthetic",
+        "dtype": "f32",
+        "config": {"workload": "2D adjoint bend 600x600 at 20 nm, 100x100 design, 4000 steps, "
+                               "checkpoint every 200, FD h=1e-3 on 32 random pixels",
+                   "J": float(J), "forward_s": round(t1 - t0, 3), "backward_s": round(t2 - t1, 3),
+                   "forward_gcups": round(cells * prob.steps / (t1 - t0) / 1e9, 3),
+                   "fd_median_rel": float(np.median(rel))},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def algorithmic_bytes(prog, state):
     """Compulsory HBM bytes of one full step: every field, coefficient array and
     CPML memory element read once and written once (coefficients only read)."""
     fb = state.ex.element_size()
     fields = sum(t.numel() for t in state.fields())
-    coefs = sum(v.numel() for k, v in prog.coeffs.items() if v.dim() > 0) * 4
+    coefs = sum(v.numel() * v.element_size() for v in prog.coeffs.values() if v.dim() > 0)
     psi = sum(t.numel() * t.element_size() for t in state.psi_h + state.psi_e)
     return 2 * fields * fb + coefs + 2 * psi
 
@@ -97,6 +234,10 @@ def main():
     ap.add_argument("--no-monitors", action="store_true")
     ap.add_argument("--per-phase", action="store_true", help="use the per-phase kernels")
     ap.add_argument("--chunk", type=int, default=256)
+    ap.add_argument("--workload", default="3d", choices=("3d", "batched2d", "adjoint2d"),
+                    help="3d: the 1024x512x256 Yee+CPML step (default); batched2d: the 64-simulation "
+                         "2D sweep in one persistent launch per step; adjoint2d: the 600x600 bend "
+                         "gradient and its finite-difference check")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 1 GiB device-to-device copy (the byte-counter calibration "
                          "kernel for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)")
@@ -124,6 +265,13 @@ def main():
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(the z-slab sharding design is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "batched2d":
+        run_batched(args)
+        return
+    if args.workload == "adjoint2d":
+        run_adjoint(args)
+        return
 
     import torch
     from beamz_amd import runtime

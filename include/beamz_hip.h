@@ -204,6 +204,82 @@ int32_t bzh_streamed(void* stream, const bzh_buffer* args, int32_t nargs,
 int32_t bzh_program(void* stream, const bzh_buffer* args, int32_t nargs,
                     const bzh_program_attrs* attrs);
 
+/* ---- batched 2D simulations ------------------------------------------------
+ * No counterpart in the reference, which runs one simulation per call (its
+ * FFI is vmapped sequentially, cuda/runtime.py:85).  Many independent TMxy
+ * (Ez, Hx, Hy) simulations of uneven sizes advance together: each step is
+ * one persistent launch whose workgroups pop 64-wide tiles of every
+ * simulation from a device work queue; a call of nsteps steps is captured
+ * once as a hipGraph and replayed.  Two field banks per simulation: bank 0 is
+ * read by the first step, and after an odd step count the result is in bank
+ * 1 (the caller swaps the roles).  All buffers f32. */
+typedef struct bzh_sim2d {
+  bzh_buffer ez[2];        /* (ny+1, nx+1) */
+  bzh_buffer hx[2];        /* (ny, nx+1) */
+  bzh_buffer hy[2];        /* (ny+1, nx) */
+  bzh_buffer e_decay;      /* scalar */
+  bzh_buffer e_source;     /* scalar or (ny+1, nx+1) */
+  bzh_buffer h_decay;      /* scalar */
+  bzh_buffer h_source;     /* scalar */
+  bzh_buffer cpml_table;   /* [4, 3, 2L]: Hx_y, Hy_x, Ez_x, Ez_y x (a, b, 1/kappa); empty without CPML */
+  bzh_buffer psi[2][4];    /* per bank: Hx_y (2L, nx+1), Hy_x (ny+1, 2L), Ez_x (ny+1, 2L), Ez_y (2L, nx+1) */
+  bzh_buffer src_coeff[4]; /* pre-E Ez slab coefficients (sy, sx) */
+  bzh_buffer src_wave[4];  /* waveform [T], sampled at clip(step, 0, T-1) */
+  int32_t src_start[4][2]; /* slab origin (y, x) in the Ez support */
+  int32_t nsrc;            /* 0..4 */
+  int32_t cpml_cells;      /* CPML thickness on all four edges (0: none) */
+  int32_t metallic;        /* PEC edge bits: bottom 4, top 8, left 16, right 32 */
+  float resolution;        /* cell size (the inverse enters the curls) */
+} bzh_sim2d;
+
+typedef struct bzh_batch_attrs {
+  int32_t abi_version;
+  int32_t flags;           /* BZH_FLAG_GRAPH_CACHE */
+  int32_t nsteps;
+  int32_t tile_rows;       /* tile height: 4, 8, 16 or 32 */
+  int32_t workgroups;      /* persistent grid size; 0 picks 4 per compute unit */
+} bzh_batch_attrs;
+
+/* current_step: s32 device scalar, the step index of the first step. */
+int32_t bzh_batch2d(void* stream, const bzh_sim2d* sims, int32_t nsims,
+                    const bzh_buffer* current_step, const bzh_batch_attrs* attrs);
+
+/* ---- adjoint (reverse-time) TMxy steps ------------------------------------
+ * No counterpart in the reference, whose gradient is a host-side overlap of
+ * two forward runs (beamz/optimization/topology.py:314-342).  These entry
+ * points run the exact discrete adjoint of the TMxy step (CPML and PEC
+ * included) backward in time and accumulate d(objective)/d(E source
+ * coefficient) over an interior design box.  The forward problem is
+ * described by the same coefficients, CPML table, edge mask and resolution
+ * as the forward program.  All buffers f32 except grad (f64). */
+typedef struct bzh_adjoint2d {
+  bzh_buffer e_decay;        /* scalar */
+  bzh_buffer e_source;       /* scalar or (ny+1, nx+1) */
+  bzh_buffer h_decay;        /* scalar */
+  bzh_buffer h_source;       /* scalar */
+  bzh_buffer cpml_table;     /* [4, 3, 2L] or empty */
+  int32_t cpml_cells;
+  int32_t metallic;          /* PEC edge bits: bottom 4, top 8, left 16, right 32 */
+  float resolution;
+  bzh_buffer a_ez, a_hx, a_hy;  /* adjoint state: shapes of Ez, Hx, Hy */
+  bzh_buffer a_psi[4];          /* adjoint CPML memories: Hx_y, Hy_x, Ez_x, Ez_y */
+  bzh_buffer work[4];           /* scratch with the shapes of Ez, Ez, Hy, Hx */
+  bzh_buffer source_profile;    /* Ez-shaped adjoint source weights W, or empty */
+  bzh_buffer source_time;       /* [N]: the adjoint source of forward step n is c_n W */
+  int32_t design[4];            /* Ez-node box [y0, y1) x [x0, x1), inside CPML and PEC */
+  bzh_buffer record;            /* [K, y1-y0, x1-x0]: forward curl per step of a segment */
+  bzh_buffer grad;              /* f64 [y1-y0, x1-x0]: += sum_n g_n curl_n */
+} bzh_adjoint2d;
+
+/* Record the forward curl of the current (Hx, Hy) over the design box into
+ * record[slot]; call after each forward step of a recomputed segment. */
+int32_t bzh_adjoint2d_record(void* stream, const bzh_adjoint2d* adj, const bzh_buffer* hx,
+                             const bzh_buffer* hy, int32_t slot);
+/* Reverse forward steps n_last, n_last-1, ..., n_last-count+1, whose curls
+ * sit in record slots slot_last, slot_last-1, ...  flags: BZH_FLAG_GRAPH_CACHE. */
+int32_t bzh_adjoint2d_reverse(void* stream, const bzh_adjoint2d* adj, int32_t n_last,
+                              int32_t count, int32_t slot_last, int32_t flags);
+
 #ifdef __cplusplus
 }
 #endif
