@@ -221,7 +221,7 @@ def advance(state, prog, nsteps, *, chunk=GRAPH_MAX_STEPS):
 
 def step_kernel_name(prog):
     if uses_two_banks(prog):
-        return "fused_step"
+        return "fused_interior+fused_edge"
     return "h_phase_3d+e_phase_3d" if prog.is_3d else "h_phase_tm+e_phase_tm"
 
 
