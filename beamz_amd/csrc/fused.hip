@@ -271,34 +271,6 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
     lE[ks & 1][kR + 1][0][lane] = tz_;
     lE[ks & 1][kR + 1][1][lane] = tx_;
   }
-  // Streamed operands of plane k: E(k+1) of this row (and of the top halo
-  // row for wave 0), H_A(k) and the E coefficients of plane k.  They are
-  // issued one plane ahead, before the barrier of the previous plane, so the
-  // loads fly during the barrier and the E update.
-  struct Batch {
-    float ex, ey, ez, tx, tz, h0, h1, h2, s0, s1, s2;
-  };
-  auto issue = [&](int k, Batch& b) {
-    const uint32_t kk = min(k, nz - 1), kx = min(k, nz), kzc = min(k, nz - 1);
-    load_e(k + 1, b.ex, b.ey, b.ez);
-    b.tx = b.tz = 0.0f;
-    if (halo) load_top(k + 1, b.tx, b.tz);
-    b.h0 = ldu<T>(f.h_in[0], (kk * ny + yY) * (nx + 1), tY);
-    b.h1 = halo ? 0.0f : ldu<T>(f.h_in[1], (kk * (ny + 1) + yX) * nx, tX);
-    b.h2 = ldu<T>(f.h_in[2], (k * ny + yY) * nx, tX);
-    b.s0 = es0;
-    b.s1 = es1;
-    b.s2 = es2;
-    if constexpr (kEArr) {
-      if (!halo && k >= kz0) {
-        b.s0 = ldu<C>(f.esrc[0], (kx * (ny + 1) + yX) * nx, cX) * f.ratio_e;
-        b.s1 = ldu<C>(f.esrc[1], (kx * ny + yY) * (nx + 1), cY) * f.ratio_e;
-        b.s2 = ldu<C>(f.esrc[2], (kzc * (ny + 1) + yX) * (nx + 1), cY) * f.ratio_e;
-      }
-    }
-  };
-  Batch cur, nxt;
-  issue(ks, cur);
   __syncthreads();
 
   float phx = 0.0f, phy = 0.0f;  // H_new(k - 1) on this row
@@ -311,9 +283,13 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
     const int qzH = vxy ? slab_q(k, nz, L) : -1;
     const int qzE = slab_q(k, nz + 1, L);
 
-    // ---- this plane's batch (issued one plane ahead) and its CPML memories ----
-    const float exn = cur.ex, eyn = cur.ey, ezn = cur.ez, etx = cur.tx, etz = cur.tz;
-    const float ha0 = cur.h0, ha1 = cur.h1, ha2 = cur.h2;
+    // ---- one batch of loads: E(k+1), H_A(k), coefficients, CPML memories ----
+    float exn, eyn, ezn, etx = 0.0f, etz = 0.0f;
+    load_e(k + 1, exn, eyn, ezn);
+    if (halo) load_top(k + 1, etx, etz);
+    const float ha0 = ldu<T>(f.h_in[0], (kk * ny + yY) * (nx + 1), tY);
+    const float ha1 = halo ? 0.0f : ldu<T>(f.h_in[1], (kk * (ny + 1) + yX) * nx, tX);
+    const float ha2 = ldu<T>(f.h_in[2], (k * ny + yY) * nx, tX);
     float p0 = 0, p1 = 0, p2 = 0, p3 = 0, p4 = 0, p5 = 0;
     if (qyH >= 0) {
       p0 = ldu<P>(f.psi[0], (kk * L2 + qyH) * (nx + 1), pY);
@@ -331,6 +307,11 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
     }
     float p6 = 0, p7 = 0, p8 = 0, p9 = 0, p10 = 0, p11 = 0;
     if (!halo && st) {
+      if constexpr (kEArr) {
+        es0 = ldu<C>(f.esrc[0], (kx * (ny + 1) + yX) * nx, cX) * f.ratio_e;
+        es1 = ldu<C>(f.esrc[1], (kx * ny + yY) * (nx + 1), cY) * f.ratio_e;
+        es2 = ldu<C>(f.esrc[2], (kzc * (ny + 1) + yX) * (nx + 1), cY) * f.ratio_e;
+      }
       if (qyE >= 0) {
         p6 = ldu<P>(f.psi[6], (kx * L2 + qyE) * nx, pX);
         p11 = ldu<P>(f.psi[11], (kzc * L2 + qyE) * (nx + 1), pY);
@@ -405,7 +386,6 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
       if (okhy) stu<T>(f.h_out[1], (k * (ny + 1) + yX) * nx, tX, hy);
       if (okhz) stu<T>(f.h_out[2], (k * ny + yY) * nx, tX, hz);
     }
-    if (k + 1 < kz1) issue(k + 1, nxt);
     __syncthreads();
 
     // ---- E_new(k) on this row (owned waves) ----
@@ -449,9 +429,9 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
           if (okz) stu<P>(pout_base<P>(f, 10), (kzc * (ny + 1) + yX) * L2, qE, n);
         }
       }
-      float nex = ec[0] * ex + cur.s0 * (e0 - e1);
-      float ney = ec[1] * ey + cur.s1 * (e2 - e3);
-      float nez = ec[2] * ez + cur.s2 * (e4 - e5);
+      float nex = ec[0] * ex + es0 * (e0 - e1);
+      float ney = ec[1] * ey + es1 * (e2 - e3);
+      float nez = ec[2] * ez + es2 * (e4 - e5);
       const bool zf = (mf && k == 0) || (mk && k == nz);
       const bool yf = (mb && y == 0) || (mt && y == ny);
       if (zf || yf) nex = 0.0f;
@@ -466,17 +446,12 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
     ex = exn;
     ey = eyn;
     ez = ezn;
-    cur = nxt;
   }
 }
 
-// The x-interior variant is held to 8 waves per SIMD so that two 16-wave
-// workgroups share a compute unit (its scalar registers are the limit; the
-// compiler spills the excess to vector lanes).  The x-edge variant carries
-// more per-lane state and keeps the compiler's own choice.
+// x-interior tiles (no per-lane x checks) and x-edge tiles.
 template <typename T, typename P, typename C, int kR, bool kEArr>
-__global__ void __launch_bounds__(64 * (kR + 1)) __attribute__((amdgpu_waves_per_eu(8, 8)))
-fused_interior(FusedParams f) {
+__global__ void __launch_bounds__(64 * (kR + 1)) fused_interior(FusedParams f) {
   fused_body<T, P, C, kR, kEArr, false>(f);
 }
 template <typename T, typename P, typename C, int kR, bool kEArr>
