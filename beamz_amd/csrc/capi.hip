@@ -26,6 +26,7 @@ namespace {
 thread_local std::string g_error;
 }
 
+thread_local int32_t g_last_fused = 0;
 void set_error(const std::string& message) { g_error = message; }
 void clear_error() { g_error.clear(); }
 
@@ -758,11 +759,50 @@ bool fused_cpml_layout(const PhaseIn& h, const PhaseIn& e, const bzh_buffer* con
     else if (diff / sz != delta) return false;
     fp.psi[T] = a->data;
   }
+  // based addressing: all 24 memories within 4 GiB above the lowest
+  const int64_t sz = static_cast<int64_t>(elem_bytes(ptype));
+  uintptr_t lo = UINTPTR_MAX, hi = 0;
+  for (int T = 0; T < 12; ++T) {
+    const bzh_buffer* a = T < 6 ? h.psi_in[T] : e.psi_in[T - 6];
+    for (const bzh_buffer* x : {a, psi_b[T]}) {
+      const uintptr_t p = reinterpret_cast<uintptr_t>(x->data);
+      lo = std::min(lo, p);
+      hi = std::max(hi, p + static_cast<uintptr_t>(numel(*x) * sz));
+    }
+  }
+  if (hi - lo >= (uintptr_t{1} << 32)) return false;
+  fp.pbase = reinterpret_cast<const char*>(lo);
+  for (int T = 0; T < 12; ++T)
+    fp.poff[T] = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(fp.psi[T]) - lo);
+  fp.pdelta = static_cast<uint32_t>(static_cast<uint64_t>(delta * sz));
   fp.L = uniform_l;
   fp.tab = reinterpret_cast<const float*>(base);
   fp.psi_delta = delta;
   fp.psi_bf16 = ptype == BZH_BF16;
   return true;
+}
+
+// One scalar base for a group of arrays: the lowest address, when every array
+// ends within 4 GiB of it; offsets in bytes.  False otherwise.
+bool group_base(const bzh_buffer* const* bufs, int n, const char*& base, uint32_t* off) {
+  uintptr_t lo = UINTPTR_MAX, hi = 0;
+  for (int i = 0; i < n; ++i) {
+    if (bufs[i] == nullptr || bufs[i]->data == nullptr) return false;
+    const uintptr_t p = reinterpret_cast<uintptr_t>(bufs[i]->data);
+    lo = std::min(lo, p);
+    hi = std::max(hi, p + static_cast<uintptr_t>(numel(*bufs[i]) * elem_bytes(bufs[i]->element_type)));
+  }
+  if (n == 0 || hi - lo >= (uintptr_t{1} << 32)) return false;
+  base = reinterpret_cast<const char*>(lo);
+  for (int i = 0; i < n; ++i)
+    off[i] = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(bufs[i]->data) - lo);
+  return true;
+}
+
+// The fields of a bank in the kernel's order (Ex, Ey, Ez, Hx, Hy, Hz).
+bool bank_base(const bzh_buffer* const* bank, const char*& base, uint32_t off[6]) {
+  const bzh_buffer* order[6] = {bank[kEx], bank[kEy], bank[kEz], bank[kHx], bank[kHy], bank[kHz]};
+  return group_base(order, 6, base, off);
 }
 
 bool metrics_empty(const PhaseIn& ph) {
@@ -799,18 +839,25 @@ bool run_program(hipStream_t s, ProgramIn& in) {
   for (int q = 3; q < 6; ++q)
     if (in.e.mat[q] != nullptr && numel(*in.e.mat[q]) != 1) ++n_e_arrays;
   coef_ok = coef_ok && (n_e_arrays == 0 || n_e_arrays == 3);
-  // 32-bit byte offsets: every field, coefficient (f32) and psi array (at most
-  // the size of its field) must stay below 4 GiB at 4 bytes per element
-  bool small = g.is3d;
-  for (int c = 0; c < 6 && small; ++c) {
-    int64_t sh[3];
-    yee_shape3d(c, g.nz, g.ny, g.nx, sh);
-    small = sh[0] * sh[1] * sh[2] * 4 < (int64_t{1} << 32);
-  }
+  // based 32-bit addressing: each bank's fields within 4 GiB of one base, at
+  // the same offsets in both banks; the E coefficient arrays likewise
   FusedParams cpml_probe;
   std::memset(&cpml_probe, 0, sizeof(cpml_probe));
+  bool based = g.is3d && two_banks;
+  uint32_t foff_a[6] = {}, foff_b[6] = {};
+  const char* base_a = nullptr;
+  const char* base_b = nullptr;
+  if (based) {
+    based = bank_base(in.bank_a, base_a, foff_a) && bank_base(in.bank_b, base_b, foff_b) &&
+            std::memcmp(foff_a, foff_b, sizeof(foff_a)) == 0;
+  }
+  if (based && n_e_arrays == 3) {
+    const bzh_buffer* src[3] = {in.e.mat[3], in.e.mat[4], in.e.mat[5]};
+    based = group_base(src, 3, cpml_probe.cbase, cpml_probe.coff);
+  }
+  std::memcpy(cpml_probe.foff, foff_a, sizeof(foff_a));
   const bool fused = two_banks && (in.flags & BZH_FLAG_FUSED_STEP) && g.is3d && !h_sources &&
-                     coef_ok && small && metrics_empty(in.h) && metrics_empty(in.e) &&
+                     coef_ok && based && metrics_empty(in.h) && metrics_empty(in.e) &&
                      fused_cpml_layout(in.h, in.e, in.psi_b, in.uniform_l, cpml_probe);
   const int32_t* step0 = in.step ? static_cast<const int32_t*>(in.step->data) : nullptr;
   int slots = 1;
@@ -869,10 +916,15 @@ bool run_program(hipStream_t s, ProgramIn& in) {
       fp.metallic = in.metallic;
       if (fp.L > 0 && odd) {
         // bank B is read on odd steps: psi roles swap
-        for (int t = 0; t < 12; ++t)
+        for (int t = 0; t < 12; ++t) {
           fp.psi[t] = static_cast<char*>(fp.psi[t]) + fp.psi_delta * (fp.psi_bf16 ? 2 : 4);
+          fp.poff[t] += fp.pdelta;
+        }
         fp.psi_delta = -fp.psi_delta;
+        fp.pdelta = 0u - fp.pdelta;
       }
+      fp.fin = odd ? base_b : base_a;
+      fp.fout = odd ? base_a : base_b;
       const FusedSplit sp = fused_split(g, fp.L);
       fp.rows = fused_rows();
       fp.box_ty0 = 0;
@@ -924,6 +976,7 @@ bool run_program(hipStream_t s, ProgramIn& in) {
     }
   }
   graph_cache().run(s, launches, (in.flags & BZH_FLAG_GRAPH_CACHE) != 0);
+  g_last_fused = fused ? 1 : 0;
   return fused;
 }
 
@@ -1306,6 +1359,7 @@ int32_t bzh_abi_version(void) { return BZH_ABI_VERSION; }
 const char* bzh_component_version(void) { return BZH_COMPONENT_VERSION; }
 const char* bzh_targets(void) { return "beamz_hip_streamed,beamz_hip_program"; }
 const char* bzh_last_error(void) { return bzh::g_error.c_str(); }
+int32_t bzh_last_program_fused(void) { return bzh::g_last_fused; }
 
 int32_t bzh_launch_phase(void* stream, const bzh_launch* launch) {
   return guarded([&] {

@@ -143,6 +143,22 @@ struct FusedParams {
   int32_t xedge;         // 1: tiles touching an x face or x CPML slab
   int32_t rows;          // tile height (y rows) the box was cut with
   int32_t coef_f16;      // E source coefficient arrays stored as float16
+  // Based addressing, the only form the kernels use (the pointers above are
+  // host-side inputs): every array of a group lies within 4 GiB above the
+  // group's base, so one scalar base pair serves the group and each access
+  // adds a 32-bit byte offset.  The read bank's fields sit at fin + foff[c],
+  // the written bank's at fout + foff[c] (c: Ex, Ey, Ez, Hx, Hy, Hz; both
+  // banks share the layout); E source coefficient arrays at cbase + coff[c];
+  // CPML memory t is read at pbase + poff[t] and written at
+  // pbase + poff[t] + pdelta (modulo 2^32).
+  const char* fin;
+  const char* fout;
+  uint32_t foff[6];
+  const char* cbase;
+  uint32_t coff[3];
+  const char* pbase;
+  uint32_t poff[12];
+  uint32_t pdelta;
 };
 int fused_rows();
 

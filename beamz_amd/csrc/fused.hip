@@ -125,13 +125,16 @@ __device__ __forceinline__ float cpml_step(const Coef& c, float d, float psi, fl
 template <typename T>
 using gptr = __attribute__((address_space(1))) T*;
 
-// Array base (kernel argument, uniform) plus a 32-bit byte offset formed next
-// to the access: ``row * sizeof(T) + xb`` with the per-lane byte offset xb.
-// Every array the fused kernel touches is below 4 GiB (checked on the host),
-// so the offset fits and the access selects ``global_load v, v_off, s[base]``.
+// Group base (kernel argument, uniform) plus a 32-bit byte offset formed next
+// to the access: ``aoff + row * sizeof(T) + xb`` with the array's offset in
+// its group, the uniform row start and the per-lane byte offset xb.  Every
+// group spans less than 4 GiB (checked on the host), so the offset fits and
+// the access selects ``global_load v, v_off, s[base]`` with one scalar pair
+// per group.
 template <typename T>
-__device__ __forceinline__ gptr<const char> at(const void* base, uint32_t row, uint32_t xb) {
-  const uint32_t off = row * static_cast<uint32_t>(sizeof(T)) + xb;
+__device__ __forceinline__ gptr<const char> at(const char* base, uint32_t aoff, uint32_t row,
+                                               uint32_t xb) {
+  const uint32_t off = aoff + row * static_cast<uint32_t>(sizeof(T)) + xb;
   return (gptr<const char>)(base) + off;
 }
 __device__ __forceinline__ float ld_g16(gptr<const char> p, bool bf) {
@@ -139,14 +142,15 @@ __device__ __forceinline__ float ld_g16(gptr<const char> p, bool bf) {
   return bf ? __uint_as_float(static_cast<uint32_t>(u) << 16) : __half2float(__ushort_as_half(u));
 }
 template <typename T>
-__device__ __forceinline__ float ldu(const void* base, uint32_t row, uint32_t xb) {
-  gptr<const char> p = at<T>(base, row, xb);
+__device__ __forceinline__ float ldu(const char* base, uint32_t aoff, uint32_t row, uint32_t xb) {
+  gptr<const char> p = at<T>(base, aoff, row, xb);
   if constexpr (sizeof(T) == 4) return *reinterpret_cast<gptr<const float>>(p);
   else return ld_g16(p, std::is_same<T, bf16_t>::value);
 }
 template <typename T>
-__device__ __forceinline__ void stu(void* base, uint32_t row, uint32_t xb, float v) {
-  gptr<char> p = (gptr<char>)(at<T>(base, row, xb));
+__device__ __forceinline__ void stu(const char* base, uint32_t aoff, uint32_t row, uint32_t xb,
+                                    float v) {
+  gptr<char> p = (gptr<char>)(at<T>(base, aoff, row, xb));
   if constexpr (sizeof(T) == 4) {
     *reinterpret_cast<gptr<float>>(p) = v;
   } else {
@@ -157,12 +161,6 @@ __device__ __forceinline__ void stu(void* base, uint32_t row, uint32_t xb, float
     *reinterpret_cast<gptr<uint16_t>>(p) = u;
   }
 }
-// Bank-B twin of CPML memory t (psi_delta elements past the bank-A array).
-template <typename P>
-__device__ __forceinline__ void* pout_base(const FusedParams& f, int t) {
-  return static_cast<char*>(f.psi[t]) + f.psi_delta * static_cast<int64_t>(sizeof(P));
-}
-
 template <typename T, typename P, typename C, int kR, bool kEArr, bool kXEdge>
 __device__ __forceinline__ void fused_body(const FusedParams& f) {
   __shared__ float lE[2][kR + 2][2][64];
@@ -244,9 +242,9 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
   auto load_e = [&](int k, float& ex, float& ey, float& ez) {
     const bool vxy = k <= nz, vz = k < nz;
     const uint32_t kx = min(k, nz), kz = min(k, nz - 1);
-    const float a = ldu<T>(f.e_in[0], (kx * (ny + 1) + yX) * nx, tX);
-    const float c = ldu<T>(f.e_in[1], (kx * ny + yY) * (nx + 1), tY);
-    const float d = ldu<T>(f.e_in[2], (kz * (ny + 1) + yX) * (nx + 1), tY);
+    const float a = ldu<T>(f.fin, f.foff[0], (kx * (ny + 1) + yX) * nx, tX);
+    const float c = ldu<T>(f.fin, f.foff[1], (kx * ny + yY) * (nx + 1), tY);
+    const float d = ldu<T>(f.fin, f.foff[2], (kz * (ny + 1) + yX) * (nx + 1), tY);
     ex = (vxy && vyX && vxX) ? a : 0.0f;
     ey = (vxy && vyY && vxY) ? c : 0.0f;
     ez = (vz && vyX && vxY) ? d : 0.0f;
@@ -254,8 +252,8 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
   auto load_top = [&](int k, float& ex, float& ez) {
     const bool vxy = k <= nz, vz = k < nz;
     const uint32_t kx = min(k, nz), kz = min(k, nz - 1);
-    const float a = ldu<T>(f.e_in[0], (kx * (ny + 1) + ytX) * nx, tX);
-    const float d = ldu<T>(f.e_in[2], (kz * (ny + 1) + ytX) * (nx + 1), tY);
+    const float a = ldu<T>(f.fin, f.foff[0], (kx * (ny + 1) + ytX) * nx, tX);
+    const float d = ldu<T>(f.fin, f.foff[2], (kz * (ny + 1) + ytX) * (nx + 1), tY);
     ex = (vxy && vtX && vxX) ? a : 0.0f;
     ez = (vz && vtX && vxY) ? d : 0.0f;
   };
@@ -287,43 +285,43 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
     float exn, eyn, ezn, etx = 0.0f, etz = 0.0f;
     load_e(k + 1, exn, eyn, ezn);
     if (halo) load_top(k + 1, etx, etz);
-    const float ha0 = ldu<T>(f.h_in[0], (kk * ny + yY) * (nx + 1), tY);
-    const float ha1 = halo ? 0.0f : ldu<T>(f.h_in[1], (kk * (ny + 1) + yX) * nx, tX);
-    const float ha2 = ldu<T>(f.h_in[2], (k * ny + yY) * nx, tX);
+    const float ha0 = ldu<T>(f.fin, f.foff[3], (kk * ny + yY) * (nx + 1), tY);
+    const float ha1 = halo ? 0.0f : ldu<T>(f.fin, f.foff[4], (kk * (ny + 1) + yX) * nx, tX);
+    const float ha2 = ldu<T>(f.fin, f.foff[5], (k * ny + yY) * nx, tX);
     float p0 = 0, p1 = 0, p2 = 0, p3 = 0, p4 = 0, p5 = 0;
     if (qyH >= 0) {
-      p0 = ldu<P>(f.psi[0], (kk * L2 + qyH) * (nx + 1), pY);
-      p5 = ldu<P>(f.psi[5], (k * L2 + qyH) * nx, pX);
+      p0 = ldu<P>(f.pbase, f.poff[0], (kk * L2 + qyH) * (nx + 1), pY);
+      p5 = ldu<P>(f.pbase, f.poff[5], (k * L2 + qyH) * nx, pX);
     }
     if (qzH >= 0) {
-      p1 = ldu<P>(f.psi[1], (qzH * ny + yY) * (nx + 1), pY);
-      if (!halo) p2 = ldu<P>(f.psi[2], (qzH * (ny + 1) + yX) * nx, pX);
+      p1 = ldu<P>(f.pbase, f.poff[1], (qzH * ny + yY) * (nx + 1), pY);
+      if (!halo) p2 = ldu<P>(f.pbase, f.poff[2], (qzH * (ny + 1) + yX) * nx, pX);
     }
     if constexpr (kXEdge) {
       if (qxH >= 0) {
-        if (!halo) p3 = ldu<P>(f.psi[3], (kk * (ny + 1) + yX) * L2, qH);
-        p4 = ldu<P>(f.psi[4], (k * ny + yY) * L2, qH);
+        if (!halo) p3 = ldu<P>(f.pbase, f.poff[3], (kk * (ny + 1) + yX) * L2, qH);
+        p4 = ldu<P>(f.pbase, f.poff[4], (k * ny + yY) * L2, qH);
       }
     }
     float p6 = 0, p7 = 0, p8 = 0, p9 = 0, p10 = 0, p11 = 0;
     if (!halo && st) {
       if constexpr (kEArr) {
-        es0 = ldu<C>(f.esrc[0], (kx * (ny + 1) + yX) * nx, cX) * f.ratio_e;
-        es1 = ldu<C>(f.esrc[1], (kx * ny + yY) * (nx + 1), cY) * f.ratio_e;
-        es2 = ldu<C>(f.esrc[2], (kzc * (ny + 1) + yX) * (nx + 1), cY) * f.ratio_e;
+        es0 = ldu<C>(f.cbase, f.coff[0], (kx * (ny + 1) + yX) * nx, cX) * f.ratio_e;
+        es1 = ldu<C>(f.cbase, f.coff[1], (kx * ny + yY) * (nx + 1), cY) * f.ratio_e;
+        es2 = ldu<C>(f.cbase, f.coff[2], (kzc * (ny + 1) + yX) * (nx + 1), cY) * f.ratio_e;
       }
       if (qyE >= 0) {
-        p6 = ldu<P>(f.psi[6], (kx * L2 + qyE) * nx, pX);
-        p11 = ldu<P>(f.psi[11], (kzc * L2 + qyE) * (nx + 1), pY);
+        p6 = ldu<P>(f.pbase, f.poff[6], (kx * L2 + qyE) * nx, pX);
+        p11 = ldu<P>(f.pbase, f.poff[11], (kzc * L2 + qyE) * (nx + 1), pY);
       }
       if (qzE >= 0) {
-        p7 = ldu<P>(f.psi[7], (qzE * (ny + 1) + yX) * nx, pX);
-        p8 = ldu<P>(f.psi[8], (qzE * ny + yY) * (nx + 1), pY);
+        p7 = ldu<P>(f.pbase, f.poff[7], (qzE * (ny + 1) + yX) * nx, pX);
+        p8 = ldu<P>(f.pbase, f.poff[8], (qzE * ny + yY) * (nx + 1), pY);
       }
       if constexpr (kXEdge) {
         if (qxE >= 0) {
-          p9 = ldu<P>(f.psi[9], (kx * ny + yY) * L2, qE);
-          p10 = ldu<P>(f.psi[10], (kzc * (ny + 1) + yX) * L2, qE);
+          p9 = ldu<P>(f.pbase, f.poff[9], (kx * ny + yY) * L2, qE);
+          p10 = ldu<P>(f.pbase, f.poff[10], (kzc * (ny + 1) + yX) * L2, qE);
         }
       }
     }
@@ -344,22 +342,22 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
     float n;
     if (qyH >= 0) {
       d0 = cpml_step(coef_u(tab, 0, L2, qyH), d0, p0, n);
-      if (wh && okhx) stu<P>(pout_base<P>(f, 0), (k * L2 + qyH) * (nx + 1), pY, n);
+      if (wh && okhx) stu<P>(f.pbase, f.poff[0] + f.pdelta, (k * L2 + qyH) * (nx + 1), pY, n);
       d5 = cpml_step(coef_u(tab, 5, L2, qyH), d5, p5, n);
-      if (wh && okhz) stu<P>(pout_base<P>(f, 5), (k * L2 + qyH) * nx, pX, n);
+      if (wh && okhz) stu<P>(f.pbase, f.poff[5] + f.pdelta, (k * L2 + qyH) * nx, pX, n);
     }
     if (qzH >= 0) {
       d1 = cpml_step(coef_u(tab, 1, L2, qzH), d1, p1, n);
-      if (wh && okhx) stu<P>(pout_base<P>(f, 1), (qzH * ny + yY) * (nx + 1), pY, n);
+      if (wh && okhx) stu<P>(f.pbase, f.poff[1] + f.pdelta, (qzH * ny + yY) * (nx + 1), pY, n);
       d2 = cpml_step(coef_u(tab, 2, L2, qzH), d2, p2, n);
-      if (wh && okhy) stu<P>(pout_base<P>(f, 2), (qzH * (ny + 1) + yX) * nx, pX, n);
+      if (wh && okhy) stu<P>(f.pbase, f.poff[2] + f.pdelta, (qzH * (ny + 1) + yX) * nx, pX, n);
     }
     if constexpr (kXEdge) {
       if (qxH >= 0) {
         d3 = cpml_step(c3, d3, p3, n);
-        if (wh && okhy) stu<P>(pout_base<P>(f, 3), (k * (ny + 1) + yX) * L2, qH, n);
+        if (wh && okhy) stu<P>(f.pbase, f.poff[3] + f.pdelta, (k * (ny + 1) + yX) * L2, qH, n);
         d4 = cpml_step(c4, d4, p4, n);
-        if (wh && okhz) stu<P>(pout_base<P>(f, 4), (k * ny + yY) * L2, qH, n);
+        if (wh && okhz) stu<P>(f.pbase, f.poff[4] + f.pdelta, (k * ny + yY) * L2, qH, n);
       }
     }
     float hx = hc[0] * ha0 - hc[3] * (d0 - d1);
@@ -382,9 +380,9 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
       lE[bb ^ 1][kR + 1][1][lane] = etx;
     }
     if (wh) {
-      if (okhx) stu<T>(f.h_out[0], (k * ny + yY) * (nx + 1), tY, hx);
-      if (okhy) stu<T>(f.h_out[1], (k * (ny + 1) + yX) * nx, tX, hy);
-      if (okhz) stu<T>(f.h_out[2], (k * ny + yY) * nx, tX, hz);
+      if (okhx) stu<T>(f.fout, f.foff[3], (k * ny + yY) * (nx + 1), tY, hx);
+      if (okhy) stu<T>(f.fout, f.foff[4], (k * (ny + 1) + yX) * nx, tX, hy);
+      if (okhz) stu<T>(f.fout, f.foff[5], (k * ny + yY) * nx, tX, hz);
     }
     __syncthreads();
 
@@ -411,22 +409,22 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
       const bool okz = own && k < nz && vyX && vxY;
       if (qyE >= 0) {
         e0 = cpml_step(coef_u(tab, 6, L2, qyE), e0, p6, n);
-        if (okx) stu<P>(pout_base<P>(f, 6), (kx * L2 + qyE) * nx, pX, n);
+        if (okx) stu<P>(f.pbase, f.poff[6] + f.pdelta, (kx * L2 + qyE) * nx, pX, n);
         e5 = cpml_step(coef_u(tab, 11, L2, qyE), e5, p11, n);
-        if (okz) stu<P>(pout_base<P>(f, 11), (kzc * L2 + qyE) * (nx + 1), pY, n);
+        if (okz) stu<P>(f.pbase, f.poff[11] + f.pdelta, (kzc * L2 + qyE) * (nx + 1), pY, n);
       }
       if (qzE >= 0) {
         e1 = cpml_step(coef_u(tab, 7, L2, qzE), e1, p7, n);
-        if (okx) stu<P>(pout_base<P>(f, 7), (qzE * (ny + 1) + yX) * nx, pX, n);
+        if (okx) stu<P>(f.pbase, f.poff[7] + f.pdelta, (qzE * (ny + 1) + yX) * nx, pX, n);
         e2 = cpml_step(coef_u(tab, 8, L2, qzE), e2, p8, n);
-        if (oky) stu<P>(pout_base<P>(f, 8), (qzE * ny + yY) * (nx + 1), pY, n);
+        if (oky) stu<P>(f.pbase, f.poff[8] + f.pdelta, (qzE * ny + yY) * (nx + 1), pY, n);
       }
       if constexpr (kXEdge) {
         if (qxE >= 0) {
           e3 = cpml_step(c9, e3, p9, n);
-          if (oky) stu<P>(pout_base<P>(f, 9), (kx * ny + yY) * L2, qE, n);
+          if (oky) stu<P>(f.pbase, f.poff[9] + f.pdelta, (kx * ny + yY) * L2, qE, n);
           e4 = cpml_step(c10, e4, p10, n);
-          if (okz) stu<P>(pout_base<P>(f, 10), (kzc * (ny + 1) + yX) * L2, qE, n);
+          if (okz) stu<P>(f.pbase, f.poff[10] + f.pdelta, (kzc * (ny + 1) + yX) * L2, qE, n);
         }
       }
       float nex = ec[0] * ex + es0 * (e0 - e1);
@@ -437,9 +435,9 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
       if (zf || yf) nex = 0.0f;
       if (zf || (kXEdge && xface)) ney = 0.0f;
       if (yf || (kXEdge && xface)) nez = 0.0f;
-      if (okx) stu<T>(f.e_out[0], (kx * (ny + 1) + yX) * nx, tX, nex);
-      if (oky) stu<T>(f.e_out[1], (kx * ny + yY) * (nx + 1), tY, ney);
-      if (okz) stu<T>(f.e_out[2], (kzc * (ny + 1) + yX) * (nx + 1), tY, nez);
+      if (okx) stu<T>(f.fout, f.foff[0], (kx * (ny + 1) + yX) * nx, tX, nex);
+      if (oky) stu<T>(f.fout, f.foff[1], (kx * ny + yY) * (nx + 1), tY, ney);
+      if (okz) stu<T>(f.fout, f.foff[2], (kzc * (ny + 1) + yX) * (nx + 1), tY, nez);
     }
     phx = hx;
     phy = hy;
@@ -528,6 +526,9 @@ void launch_fused3d(hipStream_t s, const FusedParams& p, int dtype) {
                 "interior fused tiles overlap an x face or x CPML slab");
   }
   BZH_REQUIRE(p.L == 0 || p.tab != nullptr, "CPML tiles need the coefficient table");
+  BZH_REQUIRE(p.fin != nullptr && p.fout != nullptr && (p.L == 0 || p.pbase != nullptr) &&
+                  (p.esrc_st[0] == 0 || p.cbase != nullptr),
+              "the fused step needs based addressing of its array groups");
   for (int c = 0; c < 3; ++c) {
     BZH_REQUIRE(p.hdec_st[c] == 0 && p.hsrc_st[c] == 0 && p.edec_st[c] == 0,
                 "the fused step needs uniform H coefficients and E decay");

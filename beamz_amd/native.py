@@ -117,6 +117,7 @@ SIGNATURES = {
     "bzh_component_version": (ctypes.c_char_p, []),
     "bzh_targets": (ctypes.c_char_p, []),
     "bzh_last_error": (ctypes.c_char_p, []),
+    "bzh_last_program_fused": (ctypes.c_int32, []),
     "bzh_launch_phase": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p]),
     "bzh_launch_program": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p]),
     "bzh_streamed": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(Buffer), ctypes.c_int32,
@@ -213,6 +214,11 @@ def call_streamed(args, attrs: PhaseAttrs, stream=None):
     arr = buffers(args)
     s = current_stream_handle() if stream is None else stream
     _check(lib.bzh_streamed(s, arr, len(args), ctypes.byref(attrs)), lib)
+
+
+def last_program_fused() -> bool:
+    """Did the last program call on this thread run the fused step kernels?"""
+    return bool(library().bzh_last_program_fused())
 
 
 def call_program(args, attrs: ProgramAttrs, stream=None):

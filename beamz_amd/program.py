@@ -252,6 +252,13 @@ def compile_program(permittivity, resolution, dt, *, boundaries=(), sources=(), 
         arr = np.asarray(e_source_raw[ax], dtype=np.float64) * (s_h / s_e) * fold
         t = _elide(arr, device)
         coeffs[f"e_source_{ax}"] = t if t.dim() == 0 else t.to(coef_dtype)
+    src = [coeffs[f"e_source_{ax}"] for ax in "xyz"]
+    if all(t.dim() > 0 for t in src):
+        # one allocation, so that the fused step reaches all three from one base
+        for ax, view, t in zip("xyz", field_bank([tuple(t.shape) for t in src], coef_dtype, device),
+                               src):
+            view.copy_(t)
+            coeffs[f"e_source_{ax}"] = view
 
     compiled = [lower_monitor(m, grid, shapes, resolution, i) for i, m in enumerate(monitors)]
     assign_offsets(compiled)
@@ -276,16 +283,34 @@ def compile_program(permittivity, resolution, dt, *, boundaries=(), sources=(), 
         eps_site=eps_site)
 
 
+BANK_ALIGN = 256  # bytes between the starts of the fields of one bank
+
+
+def field_bank(shapes, dtype, device):
+    """Zeroed fields of one bank, all views of one allocation, each starting on
+    a BANK_ALIGN-byte boundary.  The fused step addresses every field of a
+    bank from one base with 32-bit offsets, and two banks built from the same
+    shapes have the same offsets."""
+    esize = torch.empty(0, dtype=dtype).element_size()
+    step = BANK_ALIGN // esize
+    offs, total = [], 0
+    for sh in shapes:
+        offs.append(total)
+        n = int(np.prod(sh)) if len(sh) else 0
+        total += -(-n // step) * step
+    arena = torch.zeros(max(total, 1), dtype=dtype, device=device)
+    return tuple(arena[o:o + (int(np.prod(sh)) if len(sh) else 0)].view(sh)
+                 for o, sh in zip(offs, shapes))
+
+
 def initial_state(prog: CompiledProgram, fields=None) -> SimulationState:
     """Zero fields and recurrences (``execute.py:814-905``); ``fields`` may give
     initial values by lower-case component name."""
     dev, dt_ = prog.device, prog.field_dtype
-    out = {}
-    for c in ("Hx", "Hy", "Hz", "Ex", "Ey", "Ez"):
-        if c in prog.shapes:
-            out[c.lower()] = torch.zeros(prog.shapes[c], dtype=dt_, device=dev)
-        else:
-            out[c.lower()] = torch.zeros(0, dtype=dt_, device=dev)
+    names = ("Hx", "Hy", "Hz", "Ex", "Ey", "Ez")
+    bank = field_bank([tuple(prog.shapes[c]) if c in prog.shapes else (0,) for c in names],
+                      dt_, dev)
+    out = {c.lower(): t for c, t in zip(names, bank)}
     if fields:
         scale = prog.field_scale or {"E": 1.0, "H": 1.0}
         limit = torch.finfo(dt_).max

@@ -16,7 +16,7 @@ import torch
 
 from . import _abi as abi
 from . import native
-from .program import CompiledProgram, SimulationState
+from .program import CompiledProgram, SimulationState, field_bank
 
 _EMPTY_GROUP = None
 
@@ -125,7 +125,8 @@ def with_workspace(state):
     with the state, in the same arena as the first)."""
     if state.fields_b:
         return state
-    fields_b = tuple(torch.empty_like(t) for t in state.fields())
+    fields_b = field_bank([tuple(t.shape) for t in state.fields()], state.hx.dtype,
+                          state.hx.device)
     psi_b = state.psi_b
     if len(psi_b) != len(state.psi_h) + len(state.psi_e):
         psi_b = tuple(torch.empty_like(t) for t in (*state.psi_h, *state.psi_e))

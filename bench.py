@@ -169,26 +169,37 @@ def run_adjoint(args):
     g = rho.grad.numpy()
     rng = np.random.default_rng(17)
     pix = [tuple(p) for p in rng.integers(0, 100, size=(32, 2))]
-    h = 1e-3
-    fd = []
-    for a, b in pix:
-        rp, rm = rho0.copy(), rho0.copy()
-        rp[a, b] += h
-        rm[a, b] -= h
-        fd.append((prob.objective(rp) - prob.objective(rm)) / (2 * h))
-    fd = np.asarray(fd)
     got = np.asarray([g[a, b] for a, b in pix])
-    rel = np.abs(got - fd) / np.max(np.abs(fd))
+
+    def central_fd(h):
+        fd = []
+        for a, b in pix:
+            rp, rm = rho0.copy(), rho0.copy()
+            rp[a, b] += h
+            rm[a, b] -= h
+            fd.append((prob.objective(rp) - prob.objective(rm)) / (2 * h))
+        fd = np.asarray(fd)
+        return np.abs(got - fd) / np.max(np.abs(fd))
+
+    # The forward runs in FP32: an h = 1e-3 difference of J carries its
+    # rounding (about 1e-3 of max |dJ/drho| at the median pixel here), while
+    # h = 1e-2 keeps both that and the O(h^2) truncation below 1e-3.  Both are
+    # reported; the value is the h = 1e-2 check.
+    rel_fine = central_fd(1e-3)
+    rel = central_fd(1e-2)
     cells = prob.base_eps.size
     line = {
         "metric": "adjoint gradient vs central FD", "value": float(rel.max()), "unit": "max relative error",
         "n_gpus": 1, "steps": prob.steps, "warmup": 1, "higher_is_better": False, "data": "synthetic",
         "dtype": "f32",
         "config": {"workload": "2D adjoint bend 600x600 at 20 nm, 100x100 design, 4000 steps, "
-                               "checkpoint every 200, FD h=1e-3 on 32 random pixels",
+                               "checkpoint every 200, central FD (FP32 forward) h=1e-2 on 32 "
+                               "random pixels; h=1e-3 reported beside it",
                    "J": float(J), "forward_s": round(t1 - t0, 3), "backward_s": round(t2 - t1, 3),
                    "forward_gcups": round(cells * prob.steps / (t1 - t0) / 1e9, 3),
-                   "fd_median_rel": float(np.median(rel))},
+                   "fd_median_rel": float(np.median(rel)),
+                   "fd_h1e-3_max_rel": float(rel_fine.max()),
+                   "fd_h1e-3_median_rel": float(np.median(rel_fine))},
     }
     print(json.dumps(line), flush=True)
 
@@ -302,7 +313,7 @@ def main():
         return
 
     import torch
-    from beamz_amd import runtime
+    from beamz_amd import native, runtime
 
     print(f"[bench] building workload dtype={args.dtype}", file=sys.stderr, flush=True)
     sim, eps = build_workload(args.dtype, monitors=not args.no_monitors, fused=not args.per_phase)
@@ -338,6 +349,8 @@ def main():
     ms = start.elapsed_time(stop) / args.steps
     gcups = prog.cells / (ms * 1e-3) / 1e9
     print(f"[bench] {ms:.3f} ms/step, {gcups:.2f} GCUPS", file=sys.stderr, flush=True)
+    if runtime.uses_two_banks(prog) and not native.last_program_fused():
+        raise RuntimeError("the timed steps did not run the fused step kernels")
 
     # dominant kernel: the full-step update launched alone (no sources/DFT), timed
     # with events on its own stream; algorithmic bytes per launch = one full step

@@ -185,6 +185,11 @@ const char* bzh_component_version(void);
 const char* bzh_targets(void);
 /* Message of the last failing call on this thread ("" if none). */
 const char* bzh_last_error(void);
+/* Diagnostic with no counterpart in the reference: 1 when the last program
+ * call on this thread ran the fused two-bank step kernels, 0 when it ran the
+ * per-phase schedule (e.g. an H-timed source, per-cell H coefficients, or
+ * field banks whose arrays do not each lie within 4 GiB of one base). */
+int32_t bzh_last_program_fused(void);
 
 /* ---- descriptor-level launches ------------------------------------------- */
 /* cuda/src/update.cu:919-971 (BeamzLaunchStreamed). */

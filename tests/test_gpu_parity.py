@@ -174,6 +174,8 @@ def test_program_parity_3d_with_sources_and_monitors(cpml, fused):
     with torch.cuda.stream(s):
         state = sim.advance(state, steps, chunk=12)
     torch.cuda.synchronize()
+    from beamz_amd import native
+    assert native.last_program_fused() == fused
     o.run(prob, ost, steps)
     _compare_state(prog, state, prob, ost, dft=True)
     flux = bz.SimulationResults(state, prog, None).monitor_flux(0)
@@ -246,6 +248,10 @@ def test_fused_step_parity_odd_shapes(grid, kz, bc, rows, order, monkeypatch):
     with torch.cuda.stream(s):
         state = sim.advance(state, steps, chunk=steps)
     torch.cuda.synchronize()
+    from beamz_amd import native
+    # one-sided CPML has no uniform thickness (boundary code bits 8+ are 0,
+    # runtime.py:43-53) and runs the per-phase schedule
+    assert native.last_program_fused() == (bc != "mixed")
     o.run(prob, ost, steps)
     _compare_state(prog, state, prob, ost)
 
@@ -274,6 +280,8 @@ def test_buffered_dft_parity(slots, monkeypatch):
     with torch.cuda.stream(s):
         state = sim.advance(state, steps, chunk=13)
     torch.cuda.synchronize()
+    from beamz_amd import native
+    assert native.last_program_fused()
     o.run(prob, ost, steps)
     _compare_state(prog, state, prob, ost, dft=True)
 
@@ -325,6 +333,8 @@ def _half_run(dtype, grid, src_x, mon_x):
     with torch.cuda.stream(s):
         state = sim.advance(state, steps, chunk=steps)
     torch.cuda.synchronize()
+    from beamz_amd import native
+    assert native.last_program_fused()
     o.run(prob, ost, steps)
     res = bz.SimulationResults(state, prog, None)
     errs = {c: _rel_l2(res.field(c), ost[c.lower()]) for c in prob.shapes}
