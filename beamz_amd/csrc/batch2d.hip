@@ -1,23 +1,29 @@
 // Batched TMxy steps: many independent 2D simulations of uneven sizes advance
 // together, one persistent launch per time step.
 //
-// Work: every simulation is cut into tiles of 64 (x) by kTY (y) Ez cells; the
-// tiles of all simulations form one queue (``tiles``, host-built, largest
-// simulations first).  A fixed grid of workgroups pops tiles from a per-step
-// counter with one atomic per tile until the queue is empty, so uneven grid
+// Work: every simulation is cut into items of 62 (x) by R (y) Ez cells; the
+// items of all simulations form one queue (``tiles``, host-built, largest
+// simulations first).  A fixed grid of waves pops items from a per-step
+// counter with one atomic per item until the queue is empty, so uneven grid
 // sizes load-balance without one launch per simulation.
 //
-// One tile, one full leapfrog step from bank A to bank B (kernels.py:731-811,
-// 1217-1290; ghost rule lattice.py:523-575; packed CPML kernels.py:184-264;
-// PEC faces _boundary_compile.py:118-160):
-//   1. Ez(A) rows j0-1 .. j0+kTY, columns i0-1 .. i0+64 into LDS, with the
-//      pre-E slab sources of the step added on the fly (execute.py:374-461;
-//      a cell gets the same sum in every tile that loads it);
-//   2. Hx_new on rows j0-1 .. j0+kTY-1 and Hy_new on columns i0-1 .. i0+63
-//      (the halo row/column is recomputed, not stored) into LDS;
-//   3. Ez_new on the owned cells from LDS.
-// CPML memories are read from bank A and written to bank B for owned cells
-// only.  Between steps the kernel boundary is the only synchronization.
+// One item is one wave marching up its R rows, one full leapfrog step from
+// bank A to bank B (kernels.py:731-811, 1217-1290; ghost rule
+// lattice.py:523-575; packed CPML kernels.py:184-264; PEC faces
+// _boundary_compile.py:118-160).  Lane l holds column x = 62 c - 1 + l
+// (lanes 1..62 own their column, lanes 0 and 63 are halos); x neighbours come
+// from ds_bpermute, and the y neighbours are the rows the wave carries in
+// registers, so no LDS and no barrier is involved.  At row j the wave loads
+// Ez(j+1) (with the pre-E slab sources of the step added as it is loaded, so
+// every item that loads a cell adds the same sum; execute.py:374-461), Hx(j),
+// Hy(j) and the E coefficient, then forms
+//   Hx_new(j) from Ez(j+1) - Ez(j),
+//   Hy_new(j) from Ez(j, x+1) - Ez(j, x),
+//   Ez_new(j) from Hy_new(x) - Hy_new(x-1) and Hx_new(j) - Hx_new(j-1),
+// carrying Ez(j+1) and Hx_new(j) to the next row.  The row below the item is
+// recomputed (Hx_new only).  CPML memories are read from bank A and written
+// to bank B for owned cells only.  Between steps the kernel boundary is the
+// only synchronization.
 
 #include "common.h"
 
@@ -34,124 +40,185 @@ __device__ __forceinline__ float gdiff2(float cur, float prev, int c, int N, boo
   return up - lo;
 }
 
-template <int kTY>
-__global__ void __launch_bounds__(256) batch2d_step(Batch2dParams p) {
-  constexpr int kTX = 64;
-  __shared__ float s_ez[kTY + 2][kTX + 2];
-  __shared__ float s_hx[kTY + 1][kTX];
-  __shared__ float s_hy[kTY][kTX + 1];
-  __shared__ int s_tile;
-  const int tid = threadIdx.x;
+__device__ __forceinline__ float shift2(float v, int addr) {
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(v)));
+}
+
+__global__ void __launch_bounds__(256) batch2d_march(Batch2dParams p) {
+  const int lane = threadIdx.x & 63;
   const int step = *p.step0 + p.k;
   const int a = p.bank, bnk = p.bank ^ 1;
+  const int R = p.tile_rows;
+  const int dn = ((lane + 1) & 63) << 2;  // permute address of lane + 1
+  const int up = ((lane - 1) & 63) << 2;  // ... of lane - 1
 
   for (;;) {
-    if (tid == 0) s_tile = atomicAdd(p.counter, 1);
-    __syncthreads();
-    const int t = s_tile;
-    __syncthreads();
+    int t = 0;
+    if (lane == 0) t = atomicAdd(p.counter, 1);
+    t = __builtin_amdgcn_readfirstlane(__shfl(t, 0));
     if (t >= p.ntiles) break;
     const int4 td = p.tiles[t];
+    // the descriptor's fields are read once into registers: the field stores
+    // below could otherwise alias it and force reloads on every row (the
+    // source slabs are unrolled over their fixed maximum, so that no local
+    // array is indexed at run time and nothing goes to scratch)
     const Sim2dDev& S = p.sims[td.x];
+    const float* __restrict__ ez_a = S.ez[a];
+    const float* __restrict__ hx_a = S.hx[a];
+    const float* __restrict__ hy_a = S.hy[a];
+    float* __restrict__ ez_b = S.ez[bnk];
+    float* __restrict__ hx_b = S.hx[bnk];
+    float* __restrict__ hy_b = S.hy[bnk];
+    const float* __restrict__ pa0 = S.psi[a][0];
+    const float* __restrict__ pa1 = S.psi[a][1];
+    const float* __restrict__ pa2 = S.psi[a][2];
+    const float* __restrict__ pa3 = S.psi[a][3];
+    float* __restrict__ pb0 = S.psi[bnk][0];
+    float* __restrict__ pb1 = S.psi[bnk][1];
+    float* __restrict__ pb2 = S.psi[bnk][2];
+    float* __restrict__ pb3 = S.psi[bnk][3];
+    const float* __restrict__ tab = S.tab;
+    const float* __restrict__ e_src = S.e_src;
     const int ny = S.ny, nx = S.nx, L = S.L, L2 = 2 * S.L, m = S.metallic;
     const bool mb = m & kBottom, mt = m & kTop, ml = m & kLeft, mr = m & kRight;
     const float idx = S.inv_dx;
     const float h_dec = *S.h_dec, h_src = *S.h_src, e_dec = *S.e_dec;
-    const int j0 = td.y * kTY, i0 = td.z * kTX;
     const int W = nx + 1;  // row length of Ez and Hx
+    const int nsrc = S.nsrc;
+    int sy0[4], sx0[4], ssy[4], ssx[4], sT[4];
+    const float* sco[4];
+    const float* swv[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      sy0[q] = S.src_y0[q];
+      sx0[q] = S.src_x0[q];
+      ssy[q] = S.src_sy[q];
+      ssx[q] = S.src_sx[q];
+      sT[q] = S.src_T[q];
+      sco[q] = S.src_coeff[q];
+      swv[q] = S.src_wave[q];
+    }
+    const int y0 = td.y * R, y1 = min(y0 + R, ny + 1);
+    const int x = 62 * td.z - 1 + lane;
+    const bool vez = x >= 0 && x <= nx;   // Ez / Hx column
+    const bool vhy = x >= 0 && x < nx;    // Hy column
+    const bool own = lane >= 1 && lane <= 62 && vez;
+    const bool xpec = (ml && x == 0) || (mr && x == nx);
+    // x-slab terms of this lane: Hy_x (term 1, nx samples), Ez_x (term 2, nx+1)
+    const int q1 = vhy ? slab_q2(x, nx, L) : -1;
+    const int q2 = vez ? slab_q2(x, nx + 1, L) : -1;
+    float a1 = 0, b1 = 0, k1 = 0, a2 = 0, b2 = 0, k2 = 0;
+    if (q1 >= 0) {
+      a1 = tab[(1 * 3 + 0) * L2 + q1];
+      b1 = tab[(1 * 3 + 1) * L2 + q1];
+      k1 = tab[(1 * 3 + 2) * L2 + q1];
+    }
+    if (q2 >= 0) {
+      a2 = tab[(2 * 3 + 0) * L2 + q2];
+      b2 = tab[(2 * 3 + 1) * L2 + q2];
+      k2 = tab[(2 * 3 + 2) * L2 + q2];
+    }
 
-    // 1. Ez(A) with this step's pre-E sources
-    for (int e = tid; e < (kTY + 2) * (kTX + 2); e += 256) {
-      const int r = e / (kTX + 2), c = e - r * (kTX + 2);
-      const int y = j0 - 1 + r, x = i0 - 1 + c;
+    // Ez(A) at (j, x) with this step's pre-E sources; 0 off the grid
+    auto ez_row = [&](int j) -> float {
       float v = 0.0f;
-      if (y >= 0 && y <= ny && x >= 0 && x <= nx) {
-        v = S.ez[a][y * W + x];
-        for (int q = 0; q < S.nsrc; ++q) {
-          const int sy = y - S.src_y0[q], sx = x - S.src_x0[q];
-          if (sy >= 0 && sy < S.src_sy[q] && sx >= 0 && sx < S.src_sx[q]) {
+      if (j <= ny && vez) {
+        v = ez_a[j * W + x];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int sy = j - sy0[q], sx = x - sx0[q];
+          if (q < nsrc && sy >= 0 && sy < ssy[q] && sx >= 0 && sx < ssx[q]) {
             int wi = step;
-            wi = wi < 0 ? 0 : (wi > S.src_T[q] - 1 ? S.src_T[q] - 1 : wi);
-            v += S.src_coeff[q][sy * S.src_sx[q] + sx] * S.src_wave[q][wi];
+            wi = wi < 0 ? 0 : (wi > sT[q] - 1 ? sT[q] - 1 : wi);
+            v += sco[q][sy * ssx[q] + sx] * swv[q][wi];
           }
         }
       }
-      s_ez[r][c] = v;
-    }
-    __syncthreads();
+      return v;
+    };
 
-    // 2a. Hx (ny, nx+1): dH Hx - sH * corrected(dEz/dy); rows j0-1 .. j0+kTY-1
-    for (int e = tid; e < (kTY + 1) * kTX; e += 256) {
-      const int r = e / kTX, c = e - r * kTX;
-      const int y = j0 - 1 + r, x = i0 + c;
-      float h = 0.0f;
-      if (y >= 0 && y < ny && x <= nx) {
-        float d = (s_ez[r + 1][c + 1] - s_ez[r][c + 1]) * idx;
-        const int q = slab_q2(y, ny, L);
+    const bool es_arr = S.e_src_st != 0;
+    const float es_uni = es_arr ? 0.0f : e_src[0];
+    auto hx_row = [&](int j) { return (j < ny && vez) ? hx_a[j * W + x] : 0.0f; };
+    auto hy_row = [&](int j) { return (j <= ny && vhy) ? hy_a[j * nx + x] : 0.0f; };
+    auto es_row = [&](int j) { return (es_arr && j <= ny && vez) ? e_src[j * W + x] : es_uni; };
+
+    // Each row's operands are loaded one row ahead, so the loads of row j+1
+    // fly while row j is computed.
+    const int js = y0 > 0 ? y0 - 1 : 0;
+    float ez = ez_row(js), ezn = ez_row(js + 1);
+    float hxa = hx_row(js), hya = hy_row(js), esv = es_row(js);
+    float hx_prev = 0.0f;  // Hx_new(j - 1)
+    for (int j = js; j < y1; ++j) {
+      const bool st = j >= y0;  // this item owns row j
+      float ezn2 = 0.0f, hxa2 = 0.0f, hya2 = 0.0f, esv2 = es_uni;
+      if (j + 1 < y1) {
+        ezn2 = ez_row(j + 2);
+        hxa2 = hx_row(j + 1);
+        hya2 = hy_row(j + 1);
+        esv2 = es_row(j + 1);
+      }
+      // Hx_new(j) (ny rows, nx+1 columns): dH Hx - sH * corrected(dEz/dy)
+      float hx = 0.0f;
+      if (j < ny && vez) {
+        float d = (ezn - ez) * idx;
+        const int q = slab_q2(j, ny, L);
         if (q >= 0) {
           const int o = q * W + x;
-          const float psi = S.tab[(0 * 3 + 1) * L2 + q] * S.psi[a][0][o] + S.tab[(0 * 3 + 0) * L2 + q] * d;
-          if (r >= 1) S.psi[bnk][0][o] = psi;
-          d = d * S.tab[(0 * 3 + 2) * L2 + q] + psi;
+          const float psi = tab[(0 * 3 + 1) * L2 + q] * pa0[o] + tab[(0 * 3 + 0) * L2 + q] * d;
+          if (st && own) pb0[o] = psi;
+          d = d * tab[(0 * 3 + 2) * L2 + q] + psi;
         }
-        const int o = y * W + x;
-        h = h_dec * S.hx[a][o] - h_src * d;
-        if ((ml && x == 0) || (mr && x == nx)) h = 0.0f;
-        if (r >= 1) S.hx[bnk][o] = h;
+        hx = h_dec * hxa - h_src * d;
+        if (xpec) hx = 0.0f;
+        if (st && own) hx_b[j * W + x] = hx;
       }
-      s_hx[r][c] = h;
-    }
-    // 2b. Hy (ny+1, nx): dH Hy + sH * corrected(dEz/dx); columns i0-1 .. i0+63
-    for (int e = tid; e < kTY * (kTX + 1); e += 256) {
-      const int r = e / (kTX + 1), c = e - r * (kTX + 1);
-      const int y = j0 + r, x = i0 - 1 + c;
-      float h = 0.0f;
-      if (y <= ny && x >= 0 && x < nx) {
-        float d = (s_ez[r + 1][c + 1] - s_ez[r + 1][c]) * idx;
-        const int q = slab_q2(x, nx, L);
-        if (q >= 0) {
-          const int o = y * L2 + q;
-          const float psi = S.tab[(1 * 3 + 1) * L2 + q] * S.psi[a][1][o] + S.tab[(1 * 3 + 0) * L2 + q] * d;
-          if (c >= 1) S.psi[bnk][1][o] = psi;
-          d = d * S.tab[(1 * 3 + 2) * L2 + q] + psi;
+      if (st) {
+        // Hy_new(j) (ny+1 rows, nx columns): dH Hy + sH * corrected(dEz/dx)
+        const float ez_r = shift2(ez, dn);
+        float hy = 0.0f;
+        if (vhy) {
+          float d = (ez_r - ez) * idx;
+          if (q1 >= 0) {
+            const int o = j * L2 + q1;
+            const float psi = b1 * pa1[o] + a1 * d;
+            if (own) pb1[o] = psi;
+            d = d * k1 + psi;
+          }
+          hy = h_dec * hya + h_src * d;
+          if ((mb && j == 0) || (mt && j == ny)) hy = 0.0f;
+          if (own) hy_b[j * nx + x] = hy;
         }
-        const int o = y * nx + x;
-        h = h_dec * S.hy[a][o] + h_src * d;
-        if ((mb && y == 0) || (mt && y == ny)) h = 0.0f;
-        if (c >= 1) S.hy[bnk][o] = h;
+        // Ez_new(j): dE Ez + sE * (corrected dHy/dx - corrected dHx/dy)
+        const float hy_l = shift2(hy, up);
+        if (own) {
+          float d0 = gdiff2(hy, hy_l, x, nx, ml, mr) * idx;
+          float d1 = gdiff2(hx, hx_prev, j, ny, mb, mt) * idx;
+          if (q2 >= 0) {
+            const int o = j * L2 + q2;
+            const float psi = b2 * pa2[o] + a2 * d0;
+            pb2[o] = psi;
+            d0 = d0 * k2 + psi;
+          }
+          const int qy = slab_q2(j, ny + 1, L);
+          if (qy >= 0) {
+            const int o = qy * W + x;
+            const float psi = tab[(3 * 3 + 1) * L2 + qy] * pa3[o] + tab[(3 * 3 + 0) * L2 + qy] * d1;
+            pb3[o] = psi;
+            d1 = d1 * tab[(3 * 3 + 2) * L2 + qy] + psi;
+          }
+          float e = e_dec * ez + esv * (d0 - d1);
+          if ((mb && j == 0) || (mt && j == ny) || xpec) e = 0.0f;
+          ez_b[j * W + x] = e;
+        }
       }
-      s_hy[r][c] = h;
+      hx_prev = hx;
+      ez = ezn;
+      ezn = ezn2;
+      hxa = hxa2;
+      hya = hya2;
+      esv = esv2;
     }
-    __syncthreads();
-
-    // 3. Ez (ny+1, nx+1): dE Ez + sE * (corrected dHy/dx - corrected dHx/dy)
-    for (int e = tid; e < kTY * kTX; e += 256) {
-      const int r = e / kTX, c = e - r * kTX;
-      const int y = j0 + r, x = i0 + c;
-      if (y > ny || x > nx) continue;
-      float d0 = gdiff2(s_hy[r][c + 1], s_hy[r][c], x, nx, ml, mr) * idx;
-      float d1 = gdiff2(s_hx[r + 1][c], s_hx[r][c], y, ny, mb, mt) * idx;
-      const int qx = slab_q2(x, nx + 1, L);
-      if (qx >= 0) {
-        const int o = y * L2 + qx;
-        const float psi = S.tab[(2 * 3 + 1) * L2 + qx] * S.psi[a][2][o] + S.tab[(2 * 3 + 0) * L2 + qx] * d0;
-        S.psi[bnk][2][o] = psi;
-        d0 = d0 * S.tab[(2 * 3 + 2) * L2 + qx] + psi;
-      }
-      const int qy = slab_q2(y, ny + 1, L);
-      if (qy >= 0) {
-        const int o = qy * W + x;
-        const float psi = S.tab[(3 * 3 + 1) * L2 + qy] * S.psi[a][3][o] + S.tab[(3 * 3 + 0) * L2 + qy] * d1;
-        S.psi[bnk][3][o] = psi;
-        d1 = d1 * S.tab[(3 * 3 + 2) * L2 + qy] + psi;
-      }
-      const int o = y * W + x;
-      const float es = S.e_src[S.e_src_st ? o : 0];
-      float ez = e_dec * s_ez[r + 1][c + 1] + es * (d0 - d1);
-      if ((mb && y == 0) || (mt && y == ny) || (ml && x == 0) || (mr && x == nx)) ez = 0.0f;
-      S.ez[bnk][o] = ez;
-    }
-    __syncthreads();
   }
 }
 
@@ -166,14 +233,10 @@ void launch_batch2d(hipStream_t s, const Batch2dParams& p) {
   BZH_REQUIRE(p.workgroups >= 1, "batched launch needs at least one workgroup");
   BZH_REQUIRE(p.sims != nullptr && p.tiles != nullptr && p.counter != nullptr && p.step0 != nullptr,
               "batched launch is missing its descriptors");
+  BZH_REQUIRE(p.tile_rows == 8 || p.tile_rows == 16 || p.tile_rows == 32 || p.tile_rows == 64,
+              "batched item height must be 8, 16, 32 or 64 rows");
   const dim3 block(256, 1, 1), grid(p.workgroups, 1, 1);
-  switch (p.tile_rows) {
-    case 4: hipLaunchKernelGGL((batch2d_step<4>), grid, block, 0, s, p); break;
-    case 8: hipLaunchKernelGGL((batch2d_step<8>), grid, block, 0, s, p); break;
-    case 16: hipLaunchKernelGGL((batch2d_step<16>), grid, block, 0, s, p); break;
-    case 32: hipLaunchKernelGGL((batch2d_step<32>), grid, block, 0, s, p); break;
-    default: throw Failure{"batched tile height must be 4, 8, 16 or 32"};
-  }
+  hipLaunchKernelGGL(batch2d_march, grid, block, 0, s, p);
   BZH_HIP(hipGetLastError());
 }
 

@@ -1224,13 +1224,14 @@ void run_batch2d(hipStream_t s, const bzh_sim2d* sims, int nsims, const bzh_buff
                  const bzh_batch_attrs& a) {
   BZH_REQUIRE(sims != nullptr && nsims >= 1, "empty batch");
   BZH_REQUIRE(a.nsteps >= 1, "step count must be positive");
-  BZH_REQUIRE(a.tile_rows == 4 || a.tile_rows == 8 || a.tile_rows == 16 || a.tile_rows == 32,
-              "tile_rows must be 4, 8, 16 or 32");
+  BZH_REQUIRE(a.tile_rows == 8 || a.tile_rows == 16 || a.tile_rows == 32 || a.tile_rows == 64,
+              "tile_rows must be 8, 16, 32 or 64");
   BZH_REQUIRE(step != nullptr && step->element_type == BZH_S32 && step->data && numel(*step) == 1,
               "current_step must be an s32 scalar");
   std::vector<Sim2dDev> desc(nsims);
   for (int i = 0; i < nsims; ++i) desc[i] = describe_sim2d(sims[i], i);
-  // queue: largest simulations first, row-major tiles within one
+  // queue: largest simulations first, row-major items (62 columns by
+  // tile_rows rows) within one
   std::vector<int> order(nsims);
   for (int i = 0; i < nsims; ++i) order[i] = i;
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
@@ -1238,7 +1239,7 @@ void run_batch2d(hipStream_t s, const bzh_sim2d* sims, int nsims, const bzh_buff
   });
   std::vector<int4> tiles;
   for (int i : order) {
-    const int ty = (desc[i].ny + 1 + a.tile_rows - 1) / a.tile_rows, tx = (desc[i].nx + 1 + 63) / 64;
+    const int ty = (desc[i].ny + 1 + a.tile_rows - 1) / a.tile_rows, tx = (desc[i].nx + 1 + 61) / 62;
     for (int r = 0; r < ty; ++r)
       for (int c = 0; c < tx; ++c) tiles.push_back(make_int4(i, r, c, 0));
   }

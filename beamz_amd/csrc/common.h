@@ -250,13 +250,13 @@ struct Sim2dDev {
 
 struct Batch2dParams {
   const Sim2dDev* sims;
-  const int4* tiles;     // (simulation, tile row, tile column, 0), queue order
+  const int4* tiles;     // (simulation, item row, item column, 0), queue order
   int32_t ntiles;
   int* counter;          // queue head of this step (zeroed before the call)
   int32_t bank;          // bank read by this step
   const int32_t* step0;  // device scalar: step index of step 0 of this call
   int32_t k;             // step within this call
-  int32_t tile_rows;     // tile height (64-wide tiles)
+  int32_t tile_rows;     // rows per item (items are 62 columns wide)
   int32_t workgroups;    // persistent grid size
 };
 void launch_batch2d(hipStream_t s, const Batch2dParams& p);

@@ -213,8 +213,9 @@ int32_t bzh_program(void* stream, const bzh_buffer* args, int32_t nargs,
  * No counterpart in the reference, which runs one simulation per call (its
  * FFI is vmapped sequentially, cuda/runtime.py:85).  Many independent TMxy
  * (Ez, Hx, Hy) simulations of uneven sizes advance together: each step is
- * one persistent launch whose workgroups pop 64-wide tiles of every
- * simulation from a device work queue; a call of nsteps steps is captured
+ * one persistent launch whose waves pop work items (62 columns by tile_rows
+ * rows) of every simulation from a device work queue and march up their
+ * rows; a call of nsteps steps is captured
  * once as a hipGraph and replayed.  Two field banks per simulation: bank 0 is
  * read by the first step, and after an odd step count the result is in bank
  * 1 (the caller swaps the roles).  All buffers f32. */
@@ -241,7 +242,7 @@ typedef struct bzh_batch_attrs {
   int32_t abi_version;
   int32_t flags;           /* BZH_FLAG_GRAPH_CACHE */
   int32_t nsteps;
-  int32_t tile_rows;       /* tile height: 4, 8, 16 or 32 */
+  int32_t tile_rows;       /* rows per work item (62 columns wide): 8, 16, 32 or 64 */
   int32_t workgroups;      /* persistent grid size; 0 picks 4 per compute unit */
 } bzh_batch_attrs;
 
