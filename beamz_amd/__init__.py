@@ -9,6 +9,7 @@ from . import _abi
 from .api import Simulation, SimulationResults, courant_dt, resolve_backend
 from .boundaries import PEC, PML
 from .constants import EPS_0, ETA_0, LIGHT_SPEED, MU_0, nm, um
+from .modes import ModeSource, ModeSpec
 from .monitors import FluxMonitor
 from .native import HipBackendUnavailable, HipLaunchError
 from .program import SimulationState, compile_program, initial_state
@@ -17,7 +18,8 @@ from .sources import CustomSource, GaussianPulse, SoftSource
 __all__ = [
     "Simulation", "SimulationResults", "SimulationState", "compile_program", "initial_state",
     "courant_dt", "resolve_backend", "PEC", "PML", "FluxMonitor", "CustomSource",
-    "GaussianPulse", "SoftSource", "HipBackendUnavailable", "HipLaunchError",
+    "GaussianPulse", "SoftSource", "ModeSource", "ModeSpec", "HipBackendUnavailable",
+    "HipLaunchError",
     "EPS_0", "ETA_0", "LIGHT_SPEED", "MU_0", "nm", "um",
 ]
 __version__ = _abi.COMPONENT_VERSION
