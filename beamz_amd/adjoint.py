@@ -78,7 +78,8 @@ class AdjointProblem2D:
     eps_min, eps_max : float
         rho = 0 and rho = 1 permittivities.
     resolution, dt, steps : grid step, time step and run length.
-    sources : soft or custom Ez sources (pre-E), outside the design box.
+    sources : sources outside the design box (soft, custom or mode sources;
+        their coefficients then do not depend on rho).
     monitor : ``FluxMonitor`` with ``components=("Ez",)`` over the output line.
     mode : (P,) array of overlap weights m_p over the monitor's points.
     pml_cells : CPML thickness on every edge.

@@ -132,7 +132,7 @@ def solve_line_modes(eps_line, omega, dx, count, target_neff):
     k0 = float(omega) / LIGHT_SPEED
     main = -2.0 / dx ** 2 + k0 ** 2 * eps
     off = np.full(n - 1, 1.0 / dx ** 2)
-    w, v = eigh_tridiagonal(main, off)
+    w, v = eigh_tridiagonal(main, off, lapack_driver="stev")
     n2 = w / k0 ** 2
     count = max(1, min(int(count), n))
     near = np.argsort(np.abs(n2 - float(target_neff) ** 2))[:count]

@@ -122,15 +122,15 @@ def build_adjoint_problem(steps=4000, checkpoint=200, seed=5):
     design box of density rho ~ U[0, 1] (eps = 2.085 + 10.165 rho) at the
     centre, a 500 nm input guide along x from the left edge into the box and a
     500 nm output guide along y from the box to the top edge; 10-cell CPML;
-    a soft Ez line source across the input guide at x = 15 with a cos^2
-    transverse profile (an approximation of the fundamental mode: the mode
-    solver is not part of this backend), Gaussian pulse at 1.55 um with a
-    0.3 um bandwidth peaking at 2 / fwidth, so that the pulse has crossed the
-    output monitor (a line across the output guide at y = 400) well within
-    the 4000 steps; the objective is the overlap power of Ez on that line with
-    the same profile, at three wavelengths around 1.55 um."""
+    the fundamental mode of the input guide launched along +x at x = 15 over
+    1.2 um (a mode source), Gaussian pulse at 1.55 um with a 0.3 um bandwidth
+    peaking at 2 / fwidth, so that the pulse has crossed the output monitor (a
+    line across the output guide at y = 400) well within the 4000 steps; the
+    objective is the overlap power of Ez on that line with the output guide's
+    fundamental mode, at three wavelengths around 1.55 um."""
     import beamz_amd as bz
     from beamz_amd.adjoint import AdjointProblem2D
+    from beamz_amd.modes import solve_line_modes
     n, dx, L = 600, 20e-9, 10
     dt = 0.99 * dx / (bz.LIGHT_SPEED * np.sqrt(2.0))
     base = np.full((n, n), 2.085)
@@ -139,16 +139,19 @@ def build_adjoint_problem(steps=4000, checkpoint=200, seed=5):
     design = (250, 350, 250, 350)
     base[c0:c0 + w, :design[2]] = 12.25       # input guide: y rows, x from 0 to the box
     base[design[1]:, c0:c0 + w] = 12.25       # output guide: x columns, y from the box up
-    prof = np.cos(np.pi * (np.arange(w) + 0.5 - w / 2) / w) ** 2
     f0 = bz.LIGHT_SPEED / 1.55e-6
     fw = bz.LIGHT_SPEED * 0.3e-6 / (1.55e-6) ** 2
-    src = bz.SoftSource("Ez", (c0, 15), (w, 1), bz.GaussianPulse(f0, fw, offset=2.0),
-                        profile=prof[:, None])
+    src = bz.ModeSource((15.5 * dx, (c0 + w / 2) * dx, 0.0), (0.0, 60 * dx, 0.0),
+                        bz.GaussianPulse(f0, fw, offset=2.0), "+")
     freqs = bz.LIGHT_SPEED / np.array([1.5e-6, 1.55e-6, 1.6e-6])
     mon = bz.FluxMonitor("y", 400, freqs, ranges={"x": (c0, c0 + w)}, components=("Ez",))
+    # overlap weights: the output guide's fundamental Ez profile across the line
+    line = np.concatenate([base[400, :], base[400, -1:]])
+    _, ez = solve_line_modes(line, 2 * np.pi * f0, dx, 3, 0.98 * np.sqrt(12.25))
+    mode = ez[0][c0:c0 + w] * np.sign(ez[0][c0 + w // 2])
     rho = np.random.default_rng(seed).uniform(0.0, 1.0, (100, 100))
     prob = AdjointProblem2D(base, design, eps_min=2.085, eps_max=12.25, resolution=dx, dt=dt,
-                            steps=steps, sources=[src], monitor=mon, mode=prof, pml_cells=L,
+                            steps=steps, sources=[src], monitor=mon, mode=mode, pml_cells=L,
                             checkpoint=checkpoint)
     return prob, rho
 
@@ -192,9 +195,9 @@ def run_adjoint(args):
         "metric": "adjoint gradient vs central FD", "value": float(rel.max()), "unit": "max relative error",
         "n_gpus": 1, "steps": prob.steps, "warmup": 1, "higher_is_better": False, "data": "synthetic",
         "dtype": "f32",
-        "config": {"workload": "2D adjoint bend 600x600 at 20 nm, 100x100 design, 4000 steps, "
-                               "checkpoint every 200, central FD (FP32 forward) h=1e-2 on 32 "
-                               "random pixels; h=1e-3 reported beside it",
+        "config": {"workload": "2D adjoint bend 600x600 at 20 nm, 100x100 design, mode source "
+                               "at 1.55 um, 4000 steps, checkpoint every 200, central FD (FP32 "
+                               "forward) h=1e-2 on 32 random pixels; h=1e-3 reported beside it",
                    "J": float(J), "forward_s": round(t1 - t0, 3), "backward_s": round(t2 - t1, 3),
                    "forward_gcups": round(cells * prob.steps / (t1 - t0) / 1e9, 3),
                    "fd_median_rel": float(np.median(rel)),
