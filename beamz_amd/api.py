@@ -59,6 +59,7 @@ class SimulationResults:
     state: object
     program: object
     performance: SimulationPerformance
+    termination: object = None   # RunTermination of an automatically terminated run
 
     def monitor_flux(self, index=0):
         m = self.program.monitors[index]
@@ -137,8 +138,16 @@ class Simulation:
         state = self.initial_state() if state is None else state
         return runtime.advance(state, prog, self.steps if steps is None else int(steps), chunk=chunk)
 
-    def run(self, state=None, *, chunk=runtime.GRAPH_MAX_STEPS):
+    def run(self, state=None, *, chunk=runtime.GRAPH_MAX_STEPS, termination=None):
+        """Run the configured steps; with an ``AutoTermination`` policy, run in
+        its chunks and stop early once converged (``execute.py:1312-1484``)."""
         prog = self.compile()
+        if termination is not None:
+            from .termination import run_until_terminated
+            state, report, runtime_s = run_until_terminated(self, termination, state)
+            steps = report.steps
+            perf = SimulationPerformance(prog.cells, steps, runtime_s)
+            return SimulationResults(state, prog, perf, termination=report)
         state = self.initial_state() if state is None else state
         torch.cuda.synchronize()
         t0 = _time.perf_counter()
