@@ -61,6 +61,8 @@ class CompiledMonitor:
     window_code: int
     value_offset: int = 0
     weight_offset: int = 0
+    index: int = 0             # the normal (E-node) index of the plane or line
+    cell_range: tuple = ()     # 2D: the tangential cell interval of the line samples
 
 
 def lower_monitor(mon: FluxMonitor, grid, shapes, dx, index):
@@ -88,12 +90,14 @@ def lower_monitor(mon: FluxMonitor, grid, shapes, dx, index):
             active = c in ("Ez", "Hx", "Hy") and c in wanted
             plans[c] = line_gather_plan(c, shapes[c], dx, mon.normal, mon.index, rng) if active else None
         measure = dx  # _line_integral_scale_2d (monitors.py:61-68)
+    cell_range = rng if not is_3d else ()
     points = next(p[0].shape[0] for p in plans.values() if p is not None)
     return CompiledMonitor(
         name=mon.name or f"monitor_{index}", freqs=freqs, plans=plans, points=points,
         normal=mon.normal, normal_sign=float(mon.normal_sign), measure=measure,
         interval=max(1, int(mon.interval)), t_start=float(mon.t_start), t_end=float(mon.t_end),
-        window_code=1 if mon.window == "hann" else 0)
+        window_code=1 if mon.window == "hann" else 0, index=int(mon.index),
+        cell_range=tuple(int(v) for v in cell_range))
 
 
 def assign_offsets(compiled):

@@ -1,0 +1,194 @@
+"""Modal S-parameters of 2D (TMxy) simulations from DFT line monitors.
+
+Restates the reference's extraction for the TM branch of a 2D simulation:
+``beamz/analysis/sparameters.py:164-423`` (port waves), ``424-566``
+(``s_parameters``: incident/scattered selection, incident floor, power
+diagnostics), ``92-108`` (safe ratio, wave selectors),
+``beamz/analysis/mode_projection.py:433-546`` (port projection: unit-power
+mode, forward/backward mode matrix, pseudo-inverse) and
+``beamz/analysis/modal_projection/geometry.py:120-152`` (port components and
+flux signs).
+
+Each port names a ``FluxMonitor`` line normal to its axis.  At every
+frequency the port's mode is solved on the monitor line (the solver of
+``modes.py``), sampled at the monitor points, scaled to unit modal power, and
+the monitor's E/H phasors are projected on the forward and backward mode:
+``[E; H] = a_plus [e; h] + a_minus [e; -h]``.  This build's line monitors
+interpolate E and H onto the same points, so no E-to-H plane delay enters.
+"plus" is the wave travelling towards +axis on both axes (the mode's H takes
+the physical sign from Faraday's law; the reference's solver flips H for
+y-normal planes, ``plane.py:196-197``, which this extraction does not copy).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .constants import LIGHT_SPEED, MU_0
+from .modes import solve_line_modes
+
+# (E component, H component, signed flux sign) of a TM port by axis
+_TM_PARTS = {"x": ("Ez", "Hy", -1.0), "y": ("Ez", "Hx", 1.0)}
+
+
+@dataclass(frozen=True)
+class Port:
+    """A modal port on a monitor line: ``direction`` "+" when the incident
+    wave travels along +axis into the device."""
+
+    name: str
+    monitor: str
+    axis: str
+    direction: str = "+"
+    mode_index: int = 0
+    polarization: str = "tm"
+
+    def __post_init__(self):
+        if self.axis not in _TM_PARTS:
+            raise ValueError("a 2D port is normal to x or y")
+        if self.direction not in ("+", "-"):
+            raise ValueError("port direction must be '+' or '-'")
+        if self.polarization != "tm":
+            raise ValueError("2D TMxy simulations carry tm ports only")
+
+
+@dataclass(frozen=True)
+class SParameterResult:
+    s_matrix: dict
+    frequencies: np.ndarray
+    diagnostics: dict = field(default_factory=dict)
+
+
+def wave_selectors(port: Port):
+    """(incident, scattered) branch of a 2D port (``sparameters.py:99-107``)."""
+    incident = "plus" if port.direction == "+" else "minus"
+    return incident, "minus" if incident == "plus" else "plus"
+
+
+def safe_ratio(num, den, eps=1e-18):
+    out = np.zeros_like(np.asarray(num), dtype=np.complex128)
+    valid = np.abs(den) > eps
+    out[valid] = np.asarray(num)[valid] / np.asarray(den)[valid]
+    return out
+
+
+def port_projection(e_line, frequency, dx, points, axis, mode_index=0):
+    """Unit-power forward mode (e, h) at the monitor points and the
+    pseudo-inverse of [[e, e], [h, -h]] (``mode_projection.py:433-546``).
+
+    ``e_line`` is the Ez-site permittivity along the monitor line (one value
+    per Ez node); ``points`` are the monitor's cell indices along the line
+    (sample j sits between nodes j and j + 1)."""
+    omega = 2.0 * np.pi * float(frequency)
+    eps = np.asarray(e_line, dtype=np.float64)
+    target = 0.98 * np.sqrt(max(float(eps.max()), 1e-12))
+    m = int(mode_index) + 1
+    neffs, ezs = solve_line_modes(eps, omega, dx, 2 * m + 5, target)
+    if neffs.size <= mode_index:
+        raise ValueError(f"the mode solver found {neffs.size} modes for mode_index={mode_index}")
+    neff = float(neffs[mode_index])
+    node = ezs[mode_index]
+    pts = np.asarray(points, dtype=np.int64)
+    e = 0.5 * (node[pts] + node[pts + 1]).astype(np.complex128)
+    # transverse H of the mode travelling towards +axis: Faraday's law gives
+    # Hy = -beta Ez / (omega mu0) along x and Hx = +beta Ez / (omega mu0)
+    # along y, so "plus" is always the wave travelling towards +axis
+    sign = _TM_PARTS[axis][2]
+    h = sign * neff * omega / LIGHT_SPEED / (omega * MU_0) * e
+    power = 0.5 * np.real(np.sum(dx * sign * e * np.conjugate(h)))
+    norm = np.sqrt(max(abs(power), 1e-30))
+    e, h = e / norm, h / norm
+    mat = np.column_stack([np.concatenate([e, h]), np.concatenate([e, -h])])
+    return {"e": e, "h": h, "mode_matrix": mat, "pinv": np.linalg.pinv(mat),
+            "condition_number": float(np.linalg.cond(mat)), "mode_neff": neff}
+
+
+def port_waves(results, port: Port, frequencies):
+    """(a_plus, a_minus) of ``port`` at ``frequencies`` from the monitor's
+    phasors (``sparameters.py:164-423``)."""
+    prog = results.program
+    idx = next((i for i, m in enumerate(prog.monitors) if m.name == port.monitor), None)
+    if idx is None:
+        raise ValueError(f"Missing monitor '{port.monitor}' for port '{port.name}'.")
+    mon = prog.monitors[idx]
+    if mon.normal != port.axis:
+        raise ValueError(f"port '{port.name}' axis {port.axis} does not match its monitor")
+    e_comp, h_comp, _ = _TM_PARTS[port.axis]
+    ph = results.monitor_phasors(idx)
+    if ph.get(e_comp) is None or ph.get(h_comp) is None:
+        raise ValueError(f"monitor '{port.monitor}' must record {e_comp} and {h_comp}")
+    # the monitor's line: Ez-site permittivity along its tangential axis
+    eps_ez = np.asarray(prog.eps_site["Ez"])
+    line_index = int(mon_index(prog, idx))
+    e_line = eps_ez[:, line_index] if port.axis == "x" else eps_ez[line_index, :]
+    points = monitor_points(prog, idx)
+    out = {"a_plus": np.zeros(len(frequencies), complex), "a_minus": np.zeros(len(frequencies), complex),
+           "condition_number": np.zeros(len(frequencies)), "mode_neff": np.zeros(len(frequencies))}
+    mfreqs = np.asarray(mon.freqs)
+    for k, f in enumerate(frequencies):
+        j = int(np.argmin(np.abs(mfreqs - f)))
+        if not np.isclose(mfreqs[j], f, rtol=1e-12, atol=0.0):
+            raise ValueError(f"monitor '{port.monitor}' holds no DFT at {f} Hz")
+        proj = port_projection(e_line, f, prog.resolution, points, port.axis, port.mode_index)
+        vec = np.concatenate([ph[e_comp][j], ph[h_comp][j]])
+        a = proj["pinv"] @ vec
+        out["a_plus"][k], out["a_minus"][k] = a[0], a[1]
+        out["condition_number"][k] = proj["condition_number"]
+        out["mode_neff"][k] = proj["mode_neff"]
+    out["P_plus"] = np.abs(out["a_plus"]) ** 2
+    out["P_minus"] = np.abs(out["a_minus"]) ** 2
+    return out
+
+
+def mon_index(prog, i):
+    """The normal index of monitor ``i`` (kept on the compiled monitor)."""
+    return prog.monitors[i].index
+
+
+def monitor_points(prog, i):
+    """Cell indices of monitor ``i``'s samples along its line."""
+    return np.arange(*prog.monitors[i].cell_range)
+
+
+def s_parameters(results, source_port, ports, output_ports=None, frequencies=None,
+                 min_incident_db=-40.0):
+    """Modal S-parameters ``S[(out, src)] = b_out / a_src`` with power
+    diagnostics (``sparameters.py:424-566``)."""
+    port_map = {p.name: p for p in ports}
+    if len(port_map) != len(ports):
+        raise ValueError("port names must be unique")
+    src_name = source_port.name if isinstance(source_port, Port) else str(source_port)
+    if src_name not in port_map:
+        raise ValueError(f"source_port '{src_name}' not found in ports.")
+    if frequencies is None:
+        mon = next(m for m in results.program.monitors if m.name == port_map[src_name].monitor)
+        frequencies = np.asarray(mon.freqs, dtype=float)
+    freqs = np.atleast_1d(np.asarray(frequencies, dtype=float))
+    if freqs.size == 0 or np.any(freqs <= 0):
+        raise ValueError("frequencies must be strictly positive.")
+    outs = list(port_map) if output_ports is None else [
+        p.name if isinstance(p, Port) else str(p) for p in output_ports]
+    waves = {name: port_waves(results, p, freqs) for name, p in port_map.items()}
+    inc, _ = wave_selectors(port_map[src_name])
+    a_in = waves[src_name][f"a_{inc}"]
+    peak = float(np.max(np.abs(a_in))) if a_in.size else 0.0
+    floor = max(1e-18, peak * 10.0 ** (float(min_incident_db) / 20.0))
+    valid = np.abs(a_in) >= floor
+    s = {}
+    p_out = np.zeros(freqs.size)
+    for name in outs:
+        _, scat = wave_selectors(port_map[name])
+        b = waves[name][f"a_{scat}"]
+        s[(name, src_name)] = np.where(valid, safe_ratio(b, a_in), 0.0 + 0.0j)
+        p_out += np.abs(b) ** 2
+    p_in = np.abs(a_in) ** 2
+    power_sum = p_out / np.maximum(p_in, 1e-18)
+    diagnostics = {
+        "frequencies": freqs, "source_port": src_name, "output_ports": outs, "waves": waves,
+        "P_in": p_in, "P_guided_out": p_out,
+        "power_sum": np.where(valid, power_sum, np.nan),
+        "loss_est": np.where(valid, 1.0 - power_sum, np.nan), "valid_mask": valid,
+    }
+    return SParameterResult(s_matrix=s, frequencies=freqs, diagnostics=diagnostics)

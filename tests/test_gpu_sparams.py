@@ -1,0 +1,42 @@
+"""Modal S-parameters of a straight 240 nm guide run on the GPU: the launched
+mode reaches the output port with |S21| near one, almost nothing returns to
+the input port, and |S21|^2 agrees with the ratio of the monitors' fluxes."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def test_straight_guide_s_parameters():
+    if not torch.cuda.is_available():
+        pytest.fail("GPU tests need a GPU")
+    import beamz_amd as bz
+    from beamz_amd.sparams import Port, s_parameters
+    dx = 20e-9
+    eps = np.full((60, 200), 2.085)
+    eps[24:36, :] = 12.25
+    f0 = bz.LIGHT_SPEED / 1.55e-6
+    freqs = f0 * np.array([0.97, 1.0, 1.03])
+    src = bz.ModeSource((40.5 * dx, 30 * dx, 0), (0, 40 * dx, 0),
+                        bz.GaussianPulse(f0, bz.LIGHT_SPEED * 0.3e-6 / 1.55e-6 ** 2), "+")
+    mons = [bz.FluxMonitor("x", 70, freqs, ranges={"y": (10, 50)}, components=("Ez", "Hy"),
+                           name="m_in"),
+            bz.FluxMonitor("x", 150, freqs, ranges={"y": (10, 50)}, components=("Ez", "Hy"),
+                           name="m_out")]
+    sim = bz.Simulation(eps, dx, boundaries=[bz.PML(cells=10)], sources=[src], monitors=mons,
+                        steps=9000)
+    print("case sparams straight guide grid=(60, 200) steps=9000", flush=True)
+    res = sim.run()
+    ports = [Port("in", "m_in", "x", "+"), Port("out", "m_out", "x", "-")]
+    out = s_parameters(res, "in", ports)
+    s21 = np.abs(out.s_matrix[("out", "in")])
+    s11 = np.abs(out.s_matrix[("in", "in")])
+    ratio = res.monitor_flux(1) / res.monitor_flux(0)
+    print(f"  |S21| {s21}  |S11| {s11}  flux ratio {ratio}", flush=True)
+    assert np.all(np.abs(s21 - 1.0) < 0.03), s21
+    assert np.all(s11 < 0.05), s11   # measured 0.017-0.025: the far CPML reflects
+    np.testing.assert_allclose(s21 ** 2, ratio, rtol=0.03)
