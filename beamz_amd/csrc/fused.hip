@@ -161,8 +161,42 @@ __device__ __forceinline__ void stu(const char* base, uint32_t aoff, uint32_t ro
     *reinterpret_cast<gptr<uint16_t>>(p) = u;
   }
 }
+// The kernel argument block, re-read at a use: with 2-byte fields (8 waves
+// per SIMD, fewer scalar registers) the CPML memory offsets and the
+// coefficient table, needed only on slab rows and planes, are reloaded
+// through the scalar cache there instead of holding scalar registers across
+// the z-march (1.80 -> 1.75 ms at 1024x512x256 f16; FP32, at 7 waves, keeps
+// them resident: 2.53 against 2.56 ms).
+__device__ __forceinline__ __attribute__((address_space(4))) const FusedParams* kargs() {
+  auto p = (__attribute__((address_space(4))) const FusedParams*)(
+      __builtin_amdgcn_kernarg_segment_ptr());
+  asm volatile("" : "+s"(p));
+  return p;
+}
+template <typename T>
+struct SlabArgs {
+  const FusedParams& f;
+  __device__ __forceinline__ const char* base() const {
+    if constexpr (sizeof(T) == 2) return kargs()->pbase;
+    else return f.pbase;
+  }
+  __device__ __forceinline__ uint32_t in(int t) const {
+    if constexpr (sizeof(T) == 2) return kargs()->poff[t];
+    else return f.poff[t];
+  }
+  __device__ __forceinline__ uint32_t out(int t) const {
+    if constexpr (sizeof(T) == 2) return kargs()->poff[t] + kargs()->pdelta;
+    else return f.poff[t] + f.pdelta;
+  }
+  __device__ __forceinline__ cfloat* tab() const {
+    if constexpr (sizeof(T) == 2) return as_const(kargs()->tab);
+    else return as_const(f.tab);
+  }
+};
+
 template <typename T, typename P, typename C, int kR, bool kEArr, bool kXEdge>
 __device__ __forceinline__ void fused_body(const FusedParams& f) {
+  const SlabArgs<T> sa{f};
   __shared__ float lE[2][kR + 2][2][64];
   __shared__ float lH[2][kR + 1][2][64];
   const int lane = threadIdx.x & 63;
@@ -181,7 +215,6 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
   const int x = bx * 62 - 1 + lane;
   const int dn = ((lane + 1) & 63) << 2;  // permute address of lane + 1
   const int up = ((lane - 1) & 63) << 2;  // ... of lane - 1
-  cfloat* tab = as_const(f.tab);
 
   // ---- x: per lane ----
   bool vxX = true, vxY = true, xface = false;
@@ -290,17 +323,17 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
     const float ha2 = ldu<T>(f.fin, f.foff[5], (k * ny + yY) * nx, tX);
     float p0 = 0, p1 = 0, p2 = 0, p3 = 0, p4 = 0, p5 = 0;
     if (qyH >= 0) {
-      p0 = ldu<P>(f.pbase, f.poff[0], (kk * L2 + qyH) * (nx + 1), pY);
-      p5 = ldu<P>(f.pbase, f.poff[5], (k * L2 + qyH) * nx, pX);
+      p0 = ldu<P>(sa.base(), sa.in(0), (kk * L2 + qyH) * (nx + 1), pY);
+      p5 = ldu<P>(sa.base(), sa.in(5), (k * L2 + qyH) * nx, pX);
     }
     if (qzH >= 0) {
-      p1 = ldu<P>(f.pbase, f.poff[1], (qzH * ny + yY) * (nx + 1), pY);
-      if (!halo) p2 = ldu<P>(f.pbase, f.poff[2], (qzH * (ny + 1) + yX) * nx, pX);
+      p1 = ldu<P>(sa.base(), sa.in(1), (qzH * ny + yY) * (nx + 1), pY);
+      if (!halo) p2 = ldu<P>(sa.base(), sa.in(2), (qzH * (ny + 1) + yX) * nx, pX);
     }
     if constexpr (kXEdge) {
       if (qxH >= 0) {
-        if (!halo) p3 = ldu<P>(f.pbase, f.poff[3], (kk * (ny + 1) + yX) * L2, qH);
-        p4 = ldu<P>(f.pbase, f.poff[4], (k * ny + yY) * L2, qH);
+        if (!halo) p3 = ldu<P>(sa.base(), sa.in(3), (kk * (ny + 1) + yX) * L2, qH);
+        p4 = ldu<P>(sa.base(), sa.in(4), (k * ny + yY) * L2, qH);
       }
     }
     float p6 = 0, p7 = 0, p8 = 0, p9 = 0, p10 = 0, p11 = 0;
@@ -311,17 +344,17 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
         es2 = ldu<C>(f.cbase, f.coff[2], (kzc * (ny + 1) + yX) * (nx + 1), cY) * f.ratio_e;
       }
       if (qyE >= 0) {
-        p6 = ldu<P>(f.pbase, f.poff[6], (kx * L2 + qyE) * nx, pX);
-        p11 = ldu<P>(f.pbase, f.poff[11], (kzc * L2 + qyE) * (nx + 1), pY);
+        p6 = ldu<P>(sa.base(), sa.in(6), (kx * L2 + qyE) * nx, pX);
+        p11 = ldu<P>(sa.base(), sa.in(11), (kzc * L2 + qyE) * (nx + 1), pY);
       }
       if (qzE >= 0) {
-        p7 = ldu<P>(f.pbase, f.poff[7], (qzE * (ny + 1) + yX) * nx, pX);
-        p8 = ldu<P>(f.pbase, f.poff[8], (qzE * ny + yY) * (nx + 1), pY);
+        p7 = ldu<P>(sa.base(), sa.in(7), (qzE * (ny + 1) + yX) * nx, pX);
+        p8 = ldu<P>(sa.base(), sa.in(8), (qzE * ny + yY) * (nx + 1), pY);
       }
       if constexpr (kXEdge) {
         if (qxE >= 0) {
-          p9 = ldu<P>(f.pbase, f.poff[9], (kx * ny + yY) * L2, qE);
-          p10 = ldu<P>(f.pbase, f.poff[10], (kzc * (ny + 1) + yX) * L2, qE);
+          p9 = ldu<P>(sa.base(), sa.in(9), (kx * ny + yY) * L2, qE);
+          p10 = ldu<P>(sa.base(), sa.in(10), (kzc * (ny + 1) + yX) * L2, qE);
         }
       }
     }
@@ -341,23 +374,23 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
     const bool wh = own && st;
     float n;
     if (qyH >= 0) {
-      d0 = cpml_step(coef_u(tab, 0, L2, qyH), d0, p0, n);
-      if (wh && okhx) stu<P>(f.pbase, f.poff[0] + f.pdelta, (k * L2 + qyH) * (nx + 1), pY, n);
-      d5 = cpml_step(coef_u(tab, 5, L2, qyH), d5, p5, n);
-      if (wh && okhz) stu<P>(f.pbase, f.poff[5] + f.pdelta, (k * L2 + qyH) * nx, pX, n);
+      d0 = cpml_step(coef_u(sa.tab(), 0, L2, qyH), d0, p0, n);
+      if (wh && okhx) stu<P>(sa.base(), sa.out(0), (k * L2 + qyH) * (nx + 1), pY, n);
+      d5 = cpml_step(coef_u(sa.tab(), 5, L2, qyH), d5, p5, n);
+      if (wh && okhz) stu<P>(sa.base(), sa.out(5), (k * L2 + qyH) * nx, pX, n);
     }
     if (qzH >= 0) {
-      d1 = cpml_step(coef_u(tab, 1, L2, qzH), d1, p1, n);
-      if (wh && okhx) stu<P>(f.pbase, f.poff[1] + f.pdelta, (qzH * ny + yY) * (nx + 1), pY, n);
-      d2 = cpml_step(coef_u(tab, 2, L2, qzH), d2, p2, n);
-      if (wh && okhy) stu<P>(f.pbase, f.poff[2] + f.pdelta, (qzH * (ny + 1) + yX) * nx, pX, n);
+      d1 = cpml_step(coef_u(sa.tab(), 1, L2, qzH), d1, p1, n);
+      if (wh && okhx) stu<P>(sa.base(), sa.out(1), (qzH * ny + yY) * (nx + 1), pY, n);
+      d2 = cpml_step(coef_u(sa.tab(), 2, L2, qzH), d2, p2, n);
+      if (wh && okhy) stu<P>(sa.base(), sa.out(2), (qzH * (ny + 1) + yX) * nx, pX, n);
     }
     if constexpr (kXEdge) {
       if (qxH >= 0) {
         d3 = cpml_step(c3, d3, p3, n);
-        if (wh && okhy) stu<P>(f.pbase, f.poff[3] + f.pdelta, (k * (ny + 1) + yX) * L2, qH, n);
+        if (wh && okhy) stu<P>(sa.base(), sa.out(3), (k * (ny + 1) + yX) * L2, qH, n);
         d4 = cpml_step(c4, d4, p4, n);
-        if (wh && okhz) stu<P>(f.pbase, f.poff[4] + f.pdelta, (k * ny + yY) * L2, qH, n);
+        if (wh && okhz) stu<P>(sa.base(), sa.out(4), (k * ny + yY) * L2, qH, n);
       }
     }
     float hx = hc[0] * ha0 - hc[3] * (d0 - d1);
@@ -408,23 +441,23 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
       const bool oky = own && k <= nz && vyY && vxY;
       const bool okz = own && k < nz && vyX && vxY;
       if (qyE >= 0) {
-        e0 = cpml_step(coef_u(tab, 6, L2, qyE), e0, p6, n);
-        if (okx) stu<P>(f.pbase, f.poff[6] + f.pdelta, (kx * L2 + qyE) * nx, pX, n);
-        e5 = cpml_step(coef_u(tab, 11, L2, qyE), e5, p11, n);
-        if (okz) stu<P>(f.pbase, f.poff[11] + f.pdelta, (kzc * L2 + qyE) * (nx + 1), pY, n);
+        e0 = cpml_step(coef_u(sa.tab(), 6, L2, qyE), e0, p6, n);
+        if (okx) stu<P>(sa.base(), sa.out(6), (kx * L2 + qyE) * nx, pX, n);
+        e5 = cpml_step(coef_u(sa.tab(), 11, L2, qyE), e5, p11, n);
+        if (okz) stu<P>(sa.base(), sa.out(11), (kzc * L2 + qyE) * (nx + 1), pY, n);
       }
       if (qzE >= 0) {
-        e1 = cpml_step(coef_u(tab, 7, L2, qzE), e1, p7, n);
-        if (okx) stu<P>(f.pbase, f.poff[7] + f.pdelta, (qzE * (ny + 1) + yX) * nx, pX, n);
-        e2 = cpml_step(coef_u(tab, 8, L2, qzE), e2, p8, n);
-        if (oky) stu<P>(f.pbase, f.poff[8] + f.pdelta, (qzE * ny + yY) * (nx + 1), pY, n);
+        e1 = cpml_step(coef_u(sa.tab(), 7, L2, qzE), e1, p7, n);
+        if (okx) stu<P>(sa.base(), sa.out(7), (qzE * (ny + 1) + yX) * nx, pX, n);
+        e2 = cpml_step(coef_u(sa.tab(), 8, L2, qzE), e2, p8, n);
+        if (oky) stu<P>(sa.base(), sa.out(8), (qzE * ny + yY) * (nx + 1), pY, n);
       }
       if constexpr (kXEdge) {
         if (qxE >= 0) {
           e3 = cpml_step(c9, e3, p9, n);
-          if (oky) stu<P>(f.pbase, f.poff[9] + f.pdelta, (kx * ny + yY) * L2, qE, n);
+          if (oky) stu<P>(sa.base(), sa.out(9), (kx * ny + yY) * L2, qE, n);
           e4 = cpml_step(c10, e4, p10, n);
-          if (okz) stu<P>(f.pbase, f.poff[10] + f.pdelta, (kzc * (ny + 1) + yX) * L2, qE, n);
+          if (okz) stu<P>(sa.base(), sa.out(10), (kzc * (ny + 1) + yX) * L2, qE, n);
         }
       }
       float nex = ec[0] * ex + es0 * (e0 - e1);
