@@ -161,42 +161,43 @@ __device__ __forceinline__ void stu(const char* base, uint32_t aoff, uint32_t ro
     *reinterpret_cast<gptr<uint16_t>>(p) = u;
   }
 }
-// The kernel argument block, re-read at a use: with 2-byte fields (8 waves
-// per SIMD, fewer scalar registers) the CPML memory offsets and the
-// coefficient table, needed only on slab rows and planes, are reloaded
-// through the scalar cache there instead of holding scalar registers across
-// the z-march (1.80 -> 1.75 ms at 1024x512x256 f16; FP32, at 7 waves, keeps
-// them resident: 2.53 against 2.56 ms).
+// The kernel argument block, re-read at a use: in the x-interior kernel for
+// 2-byte fields (8 waves per SIMD, fewer scalar registers) the CPML memory
+// offsets and the coefficient table, needed only on slab rows and planes, are
+// reloaded through the scalar cache there instead of holding scalar registers
+// across the z-march (interior 1.47 -> 1.39 ms at 1024x512x256 f16).  FP32
+// and the x-edge kernels keep them resident (reloading measured 2.53 -> 2.56
+// ms in FP32 and 0.33 -> 0.36 ms for the f16 x-edge box).
 __device__ __forceinline__ __attribute__((address_space(4))) const FusedParams* kargs() {
   auto p = (__attribute__((address_space(4))) const FusedParams*)(
       __builtin_amdgcn_kernarg_segment_ptr());
   asm volatile("" : "+s"(p));
   return p;
 }
-template <typename T>
+template <bool kReload>
 struct SlabArgs {
   const FusedParams& f;
   __device__ __forceinline__ const char* base() const {
-    if constexpr (sizeof(T) == 2) return kargs()->pbase;
+    if constexpr (kReload) return kargs()->pbase;
     else return f.pbase;
   }
   __device__ __forceinline__ uint32_t in(int t) const {
-    if constexpr (sizeof(T) == 2) return kargs()->poff[t];
+    if constexpr (kReload) return kargs()->poff[t];
     else return f.poff[t];
   }
   __device__ __forceinline__ uint32_t out(int t) const {
-    if constexpr (sizeof(T) == 2) return kargs()->poff[t] + kargs()->pdelta;
+    if constexpr (kReload) return kargs()->poff[t] + kargs()->pdelta;
     else return f.poff[t] + f.pdelta;
   }
   __device__ __forceinline__ cfloat* tab() const {
-    if constexpr (sizeof(T) == 2) return as_const(kargs()->tab);
+    if constexpr (kReload) return as_const(kargs()->tab);
     else return as_const(f.tab);
   }
 };
 
 template <typename T, typename P, typename C, int kR, bool kEArr, bool kXEdge>
 __device__ __forceinline__ void fused_body(const FusedParams& f) {
-  const SlabArgs<T> sa{f};
+  const SlabArgs<sizeof(T) == 2 && !kXEdge> sa{f};
   __shared__ float lE[2][kR + 2][2][64];
   __shared__ float lH[2][kR + 1][2][64];
   const int lane = threadIdx.x & 63;
