@@ -27,6 +27,7 @@ thread_local std::string g_error;
 }
 
 thread_local int32_t g_last_fused = 0;
+thread_local int32_t g_last_pairs = 0;
 void set_error(const std::string& message) { g_error = message; }
 void clear_error() { g_error.clear(); }
 
@@ -319,7 +320,9 @@ void run_phase(hipStream_t s, const PhaseParams& p, int phase, const Grid& g) {
 }
 
 // ---- schedule ------------------------------------------------------------------
-enum class Kind : int32_t { kPhase3d, kPhaseTm, kSource, kDft, kFused, kBatch2d, kZero, kAdjoint };
+enum class Kind : int32_t {
+  kPhase3d, kPhaseTm, kSource, kDft, kFused, kBatch2d, kZero, kAdjoint, kPair, kMonBox
+};
 
 struct Launch {
   Kind kind;
@@ -387,6 +390,18 @@ void enqueue(hipStream_t s, const Launch& l) {
       FusedParams p;
       std::memcpy(&p, l.bytes.data(), sizeof(p));
       launch_fused3d(s, p, l.dtype);
+      break;
+    }
+    case Kind::kPair: {
+      PairParams p;
+      std::memcpy(&p, l.bytes.data(), sizeof(p));
+      launch_pair3d(s, p, l.dtype);
+      break;
+    }
+    case Kind::kMonBox: {
+      MonBoxParams p;
+      std::memcpy(&p, l.bytes.data(), sizeof(p));
+      launch_monbox(s, p);
       break;
     }
   }
@@ -724,6 +739,118 @@ FusedSplit fused_split(const Grid& g, int L) {
   return sp;
 }
 
+// Work split of the pair step (two steps per launch): tiles of 60 x 13
+// owned cells whose lanes cover x = 60 t - 2 .. 60 t + 61; interior tiles
+// have all of them in [M, nx - M - 1].  z chunks of 32 planes
+// (BZH_PAIR_KZ overrides it), shortened to 16 while the launch is small
+// (fewer than 2048 workgroups); each chunk recomputes two planes of the first
+// step, so shorter chunks cost more than they gain in occupancy.
+FusedSplit pair_split(const Grid& g, int L) {
+  const int W = kPairCols, TY = kPairRows;
+  const int Tx = (g.nx + 1 + W - 1) / W, Ty = (g.ny + 1 + TY - 1) / TY, planes = g.nz + 1;
+  const int M = std::max(L, 1);
+  int tx0 = (M + 2 + W - 1) / W;                           // W tx0 - 2 >= M
+  int tx1 = g.nx - M - 2 >= 0 ? std::min((g.nx - M - 2) / W, Tx) : 0;  // W tx1 <= nx - M - 2
+  if (tx1 <= tx0) tx0 = tx1 = 0;
+  FusedSplit sp{};
+  sp.interior = {tx0, tx1 - tx0, tx1 - tx0, 0, Ty, 1, 0};
+  sp.edge = {0, tx0 + (Tx - tx1), tx0, tx1 - tx0, Ty, 1, 0};
+  int kz_env = 0;
+  if (const char* env = std::getenv("BZH_PAIR_KZ")) kz_env = std::max(1, std::atoi(env));
+  for (FusedBox* b : {&sp.interior, &sp.edge}) {
+    const int64_t tiles = static_cast<int64_t>(b->ntx) * b->nty;
+    if (tiles == 0) continue;
+    int kz = kz_env > 0 ? kz_env : 32;
+    if (kz_env == 0)
+      while (kz > 16 && tiles * ((planes + kz - 1) / kz) < 2048) kz = std::max(16, kz / 2);
+    b->kz = kz;
+    b->nzc = (planes + kz - 1) / kz;
+  }
+  return sp;
+}
+
+// Two steps per launch: by default for float32 fields (BZH_PAIR=1 forces it
+// for 2-byte fields too, BZH_PAIR=0 disables it).
+bool pairs_enabled(int dtype) {
+  const char* env = std::getenv("BZH_PAIR");
+  if (env && std::string(env) == "0") return false;
+  if (env && std::string(env) == "1") return true;
+  return dtype == BZH_F32;
+}
+
+// Intermediate bank and monitor boxes of the pair step, one per (device,
+// stream) like the DFT ring: calls on one stream are ordered by it.  The bank
+// grows on demand; a replaced bank is freed once the stream has drained.
+struct PairWorkspace {
+  char* bank = nullptr;
+  size_t bytes = 0;
+  int32_t* boxes = nullptr;
+};
+PairWorkspace pair_workspace(hipStream_t s, size_t bank_bytes) {
+  static std::mutex mu;
+  static std::map<std::pair<int, hipStream_t>, PairWorkspace> spaces;
+  int device = 0;
+  BZH_HIP(hipGetDevice(&device));
+  std::lock_guard<std::mutex> lock(mu);
+  PairWorkspace& w = spaces[{device, s}];
+  if (w.boxes == nullptr) {
+    void* d = nullptr;
+    BZH_HIP(hipMalloc(&d, kMaxMonBoxes * 6 * sizeof(int32_t)));
+    w.boxes = static_cast<int32_t*>(d);
+  }
+  if (w.bytes < bank_bytes) {
+    if (w.bank != nullptr) {
+      BZH_HIP(hipStreamSynchronize(s));
+      BZH_HIP(hipFree(w.bank));
+      w.bank = nullptr;
+      w.bytes = 0;
+    }
+    void* d = nullptr;
+    BZH_HIP(hipMalloc(&d, bank_bytes));
+    w.bank = static_cast<char*>(d);
+    w.bytes = bank_bytes;
+  }
+  return w;
+}
+
+// Slab count of a source group (0 when absent).
+int group_slabs(const SourceGroupIn& grp) {
+  const bzh_buffer* c = grp.coeffs;
+  return (c && c->data && c->rank >= 1) ? static_cast<int>(c->dims[0]) : 0;
+}
+
+// Append the slabs of an E source group to the pair's mid-step list.
+void add_mid_slabs(PairParams& pp, const SourceGroupIn& grp, int comp, int kstep, bool post,
+                   int metallic) {
+  const int n = group_slabs(grp);
+  if (n == 0) return;
+  const bzh_buffer& c = *grp.coeffs;
+  BZH_REQUIRE(c.element_type == BZH_F32 && c.rank == 4 && grp.waves->element_type == BZH_F32 &&
+                  grp.waves->rank == 2 && grp.waves->dims[0] == n && grp.waves->dims[1] >= 1 &&
+                  grp.starts->element_type == BZH_S32 && grp.starts->rank == 2 &&
+                  grp.starts->dims[0] == n && grp.starts->dims[1] == 3,
+              "source groups are f32 [n, sz, sy, sx] coefficients, f32 [n, T] waveforms and s32 [n, 3] starts");
+  BZH_REQUIRE(pp.msrc_n + n <= kMaxMidSlabs, "too many source slabs between pair steps");
+  const int64_t cells = c.dims[1] * c.dims[2] * c.dims[3];
+  const int T = static_cast<int>(grp.waves->dims[1]);
+  for (int s = 0; s < n; ++s) {
+    MidSlab m;
+    std::memset(&m, 0, sizeof(m));
+    m.coeff = static_cast<const float*>(c.data) + s * cells;
+    m.wave = static_cast<const float*>(grp.waves->data) + static_cast<int64_t>(s) * T;
+    m.start = static_cast<const int32_t*>(grp.starts->data) + 3 * s;
+    m.sz = static_cast<int>(c.dims[1]);
+    m.sy = static_cast<int>(c.dims[2]);
+    m.sx = static_cast<int>(c.dims[3]);
+    m.T = T;
+    m.comp = comp;
+    m.kstep = kstep;
+    m.post = post ? 1 : 0;
+    if (post) mask_ends(kEx + comp, metallic, true, m.mlo, m.mhi);
+    pp.msrc[pp.msrc_n++] = m;
+  }
+}
+
 size_t elem_bytes(int t) { return t == BZH_BF16 || t == BZH_F16 ? 2 : (t == BZH_F64 ? 8 : 4); }
 
 // Can the fused kernel take this phase pair?  Fills the CPML part of ``fp``.
@@ -867,14 +994,45 @@ bool run_program(hipStream_t s, ProgramIn& in) {
     slots = dft_slots(in.nsteps);
     if (slots > 1) stage = dft_stage(s, static_cast<size_t>(slots) * probe.M * 6 * probe.Pmax);
   }
-  std::vector<Launch> launches;
-  launches.reserve(static_cast<size_t>(in.nsteps) * 8);
-  const Kind pk = g.is3d ? Kind::kPhase3d : Kind::kPhaseTm;
-  for (int k = 0; k < in.nsteps; ++k) {
-    // bank roles: in-place uses A for everything; two banks ping-pong A->B->A
-    const bool odd = two_banks && (k % 2 == 1);
-    const bzh_buffer* const* cur = odd ? in.bank_b : in.bank_a;
-    const bzh_buffer* const* nxt = two_banks ? (odd ? in.bank_a : in.bank_b) : cur;
+  // Two steps per launch (the pair kernel) where the fused step runs, the
+  // source slabs between the steps fit the kernel and the monitors its boxes.
+  // Each pair swaps the banks once, so an even number of pairs per call keeps
+  // the single-step bank parity (result in the workspace after an odd count).
+  int mid_slabs = 0;
+  for (int c = 0; c < 3; ++c) {
+    if (c < in.ngroups) mid_slabs += group_slabs(in.groups[c]);
+    if (6 + c < in.ngroups) mid_slabs += group_slabs(in.groups[6 + c]);
+  }
+  int npairs = 0;
+  int mon_m = 0;
+  if (in.has_monitors) mon_m = make_dft(in.mon, in.bank_a, g, in.dt, in.monitor_count).M;
+  if (fused && pairs_enabled(g.dtype) && mid_slabs <= kMaxMidSlabs && mon_m <= kMaxMonBoxes &&
+      (mid_slabs == 0 || step0 != nullptr)) {
+    npairs = in.nsteps / 2;
+    if (npairs % 2 == 1) --npairs;
+  }
+  // the intermediate bank (same layout as bank A) and the monitor boxes
+  const bzh_buffer* cbank[6] = {};
+  bzh_buffer cbufs[6];
+  PairWorkspace pws;
+  if (npairs > 0 && mon_m > 0) {
+    size_t span = 0;
+    const int order[6] = {kEx, kEy, kEz, kHx, kHy, kHz};
+    for (int q = 0; q < 6; ++q) {
+      const bzh_buffer& b = *in.bank_a[order[q]];
+      span = std::max(span, static_cast<size_t>(cpml_probe.foff[q]) +
+                                static_cast<size_t>(numel(b)) * elem_bytes(b.element_type));
+    }
+    pws = pair_workspace(s, span);
+    for (int q = 0; q < 6; ++q) {
+      cbufs[order[q]] = *in.bank_a[order[q]];
+      cbufs[order[q]].data = pws.bank + cpml_probe.foff[q];
+    }
+    for (int q = 0; q < 6; ++q) cbank[q] = &cbufs[q];
+  }
+
+  // The fused-step parameters for a launch that reads bank B when cur_b.
+  auto fused_params = [&](const bzh_buffer* const* cur, const bzh_buffer* const* nxt, bool cur_b) {
     PhaseIn h = in.h, e = in.e;
     for (int c = 0; c < 3; ++c) {
       h.tin[c] = cur[kHx + c];
@@ -884,67 +1042,161 @@ bool run_program(hipStream_t s, ProgramIn& in) {
       e.tout[c] = nxt[kEx + c];
       e.src[c] = nxt[kHx + c];
     }
+    // validate both phases through the generic decoder, then repack
+    const PhaseParams hp = make_phase(h, BZH_PHASE_H, g, in.dt, in.resolution, in.metallic, in.uniform_l);
+    const PhaseParams ep = make_phase(e, BZH_PHASE_E, g, in.dt, in.resolution, in.metallic, in.uniform_l);
+    FusedParams fp = cpml_probe;
+    for (int c = 0; c < 3; ++c) {
+      fp.h_in[c] = hp.tin[c];
+      fp.h_out[c] = hp.tout[c];
+      fp.e_in[c] = ep.tin[c];
+      fp.e_out[c] = ep.tout[c];
+      fp.hdec[c] = hp.dec[c];
+      fp.hsrc[c] = hp.scf[c];
+      fp.edec[c] = ep.dec[c];
+      fp.esrc[c] = ep.scf[c];
+      fp.hdec_st[c] = hp.dec_st[c];
+      fp.hsrc_st[c] = hp.scf_st[c];
+      fp.edec_st[c] = ep.dec_st[c];
+      fp.esrc_st[c] = ep.scf_st[c];
+    }
+    fp.coef_f16 = ep.scf_f16;
+    fp.inv_dx = hp.inv_dx;
+    fp.ratio_h = 1.0f;
+    fp.ratio_e = 1.0f;
+    fp.nz = g.nz;
+    fp.ny = g.ny;
+    fp.nx = g.nx;
+    fp.metallic = in.metallic;
+    if (fp.L > 0 && cur_b) {
+      // bank B is read: psi roles swap
+      for (int t = 0; t < 12; ++t) {
+        fp.psi[t] = static_cast<char*>(fp.psi[t]) + fp.psi_delta * (fp.psi_bf16 ? 2 : 4);
+        fp.poff[t] += fp.pdelta;
+      }
+      fp.psi_delta = -fp.psi_delta;
+      fp.pdelta = 0u - fp.pdelta;
+    }
+    fp.fin = cur_b ? base_b : base_a;
+    fp.fout = cur_b ? base_a : base_b;
+    fp.box_ty0 = 0;
+    fp.box_z0 = 0;
+    fp.box_z1 = g.nz + 1;
+    return fp;
+  };
+  auto set_box = [&](FusedParams& fp, const FusedBox& b, int v) {
+    fp.xedge = v;
+    fp.box_tx0 = b.tx0;
+    fp.box_ntx = b.ntx;
+    fp.box_tsplit = b.tsplit;
+    fp.box_tskip = b.tskip;
+    fp.box_nty = b.nty;
+    fp.box_kz = b.kz;
+    fp.box_nzc = b.nzc;
+    fp.box_band = fused_banded() ? (b.ntx * b.nty + 7) / 8 : 0;
+  };
+
+  std::vector<Launch> launches;
+  launches.reserve(static_cast<size_t>(in.nsteps) * 8);
+  // the DFT of step k, read from ``bank`` (buffered: stage, and fold the ring
+  // every ``slots`` steps and at the end of the call)
+  auto add_dft = [&](int k, const bzh_buffer* const* bank) {
+    DftParams dp = make_dft(in.mon, bank, g, in.dt, in.monitor_count);
+    dp.k = k;
+    if (slots > 1) {
+      dp.mode = 1;
+      dp.slots = slots;
+      dp.stage = stage;
+      launches.push_back(make_launch(Kind::kDft, 0, g.dtype, dp));
+      if ((k + 1) % slots == 0 || k == in.nsteps - 1) {
+        dp.mode = 2;
+        dp.nk = (k % slots) + 1;
+        dp.k0 = k + 1 - dp.nk;
+        launches.push_back(make_launch(Kind::kDft, 0, g.dtype, dp));
+      }
+    } else {
+      launches.push_back(make_launch(Kind::kDft, 0, g.dtype, dp));
+    }
+  };
+  if (npairs > 0 && mon_m > 0) {
+    const DftParams probe = make_dft(in.mon, in.bank_a, g, in.dt, in.monitor_count);
+    MonBoxParams mb;
+    std::memset(&mb, 0, sizeof(mb));
+    mb.indices = probe.indices;
+    mb.weights = probe.weights;
+    mb.M = probe.M;
+    mb.Pmax = probe.Pmax;
+    mb.K = probe.K;
+    const int order[6] = {kEx, kEy, kEz, kHx, kHy, kHz};
+    for (int q = 0; q < 6; ++q)
+      for (int d = 0; d < 3; ++d) mb.dims[q][d] = static_cast<int>(in.bank_a[order[q]]->dims[d]);
+    mb.box = pws.boxes;
+    launches.push_back(make_launch(Kind::kMonBox, 0, g.dtype, mb));
+  }
+
+  const Kind pk = g.is3d ? Kind::kPhase3d : Kind::kPhaseTm;
+  bool cur_b = false;  // the current fields live in bank B
+  int pairs_left = npairs;
+  for (int k = 0; k < in.nsteps;) {
+    const bzh_buffer* const* cur = cur_b ? in.bank_b : in.bank_a;
+    const bzh_buffer* const* nxt = two_banks ? (cur_b ? in.bank_a : in.bank_b) : cur;
     // pre_e sources act on the E bank read by this step
     for (int c = 0; c < 3 && c < in.ngroups; ++c)
       add_source_launches(launches, in.groups[c], 0, c, cur, g, in.metallic, step0, k);
-    if (fused) {
-      // validate both phases through the generic decoder, then repack
-      const PhaseParams hp = make_phase(h, BZH_PHASE_H, g, in.dt, in.resolution, in.metallic, in.uniform_l);
-      const PhaseParams ep = make_phase(e, BZH_PHASE_E, g, in.dt, in.resolution, in.metallic, in.uniform_l);
-      FusedParams fp = cpml_probe;
-      for (int c = 0; c < 3; ++c) {
-        fp.h_in[c] = hp.tin[c];
-        fp.h_out[c] = hp.tout[c];
-        fp.e_in[c] = ep.tin[c];
-        fp.e_out[c] = ep.tout[c];
-        fp.hdec[c] = hp.dec[c];
-        fp.hsrc[c] = hp.scf[c];
-        fp.edec[c] = ep.dec[c];
-        fp.esrc[c] = ep.scf[c];
-        fp.hdec_st[c] = hp.dec_st[c];
-        fp.hsrc_st[c] = hp.scf_st[c];
-        fp.edec_st[c] = ep.dec_st[c];
-        fp.esrc_st[c] = ep.scf_st[c];
+    if (pairs_left > 0) {
+      // steps k and k + 1 in one launch: E sources of step k and pre-E
+      // sources of step k + 1 are applied inside, between the steps
+      PairParams pp;
+      std::memset(&pp, 0, sizeof(pp));
+      pp.f = fused_params(cur, nxt, cur_b);
+      pp.f.rows = kPairRows;
+      pp.step0 = step0;
+      for (int c = 0; c < 3 && 6 + c < in.ngroups; ++c)
+        add_mid_slabs(pp, in.groups[6 + c], c, k, true, in.metallic);
+      pp.msrc_post = pp.msrc_n;
+      for (int c = 0; c < 3 && c < in.ngroups; ++c)
+        add_mid_slabs(pp, in.groups[c], c, k + 1, false, in.metallic);
+      if (mon_m > 0) {
+        pp.cmid = pws.bank;
+        pp.mbox = pws.boxes;
+        pp.mbox_n = mon_m;
       }
-      fp.coef_f16 = ep.scf_f16;
-      fp.inv_dx = hp.inv_dx;
-      fp.ratio_h = 1.0f;
-      fp.ratio_e = 1.0f;
-      fp.nz = g.nz;
-      fp.ny = g.ny;
-      fp.nx = g.nx;
-      fp.metallic = in.metallic;
-      if (fp.L > 0 && odd) {
-        // bank B is read on odd steps: psi roles swap
-        for (int t = 0; t < 12; ++t) {
-          fp.psi[t] = static_cast<char*>(fp.psi[t]) + fp.psi_delta * (fp.psi_bf16 ? 2 : 4);
-          fp.poff[t] += fp.pdelta;
-        }
-        fp.psi_delta = -fp.psi_delta;
-        fp.pdelta = 0u - fp.pdelta;
-      }
-      fp.fin = odd ? base_b : base_a;
-      fp.fout = odd ? base_a : base_b;
-      const FusedSplit sp = fused_split(g, fp.L);
-      fp.rows = fused_rows();
-      fp.box_ty0 = 0;
-      fp.box_z0 = 0;
-      fp.box_z1 = g.nz + 1;
+      const FusedSplit sp = pair_split(g, pp.f.L);
       for (int v = 0; v < 2; ++v) {
         const FusedBox& b = v == 0 ? sp.interior : sp.edge;
         if (b.ntx == 0 || b.nty == 0 || b.nzc == 0) continue;
-        fp.xedge = v;
-        fp.box_tx0 = b.tx0;
-        fp.box_ntx = b.ntx;
-        fp.box_tsplit = b.tsplit;
-        fp.box_tskip = b.tskip;
-        fp.box_nty = b.nty;
-        fp.box_kz = b.kz;
-        fp.box_nzc = b.nzc;
-        fp.box_band = fused_banded() ? (b.ntx * b.nty + 7) / 8 : 0;
+        set_box(pp.f, b, v);
+        launches.push_back(make_launch(Kind::kPair, 0, g.dtype, pp));
+      }
+      if (in.has_monitors) add_dft(k, cbank);
+      for (int c = 0; c < 3 && 6 + c < in.ngroups; ++c)
+        add_source_launches(launches, in.groups[6 + c], 2, c, nxt, g, in.metallic, step0, k + 1);
+      if (in.has_monitors) add_dft(k + 1, nxt);
+      --pairs_left;
+      cur_b = two_banks ? !cur_b : cur_b;
+      k += 2;
+      continue;
+    }
+    if (fused) {
+      FusedParams fp = fused_params(cur, nxt, cur_b);
+      const FusedSplit sp = fused_split(g, fp.L);
+      fp.rows = fused_rows();
+      for (int v = 0; v < 2; ++v) {
+        const FusedBox& b = v == 0 ? sp.interior : sp.edge;
+        if (b.ntx == 0 || b.nty == 0 || b.nzc == 0) continue;
+        set_box(fp, b, v);
         launches.push_back(make_launch(Kind::kFused, 0, g.dtype, fp));
       }
     } else {
+      PhaseIn h = in.h, e = in.e;
+      for (int c = 0; c < 3; ++c) {
+        h.tin[c] = cur[kHx + c];
+        h.tout[c] = nxt[kHx + c];
+        h.src[c] = cur[kEx + c];
+        e.tin[c] = cur[kEx + c];
+        e.tout[c] = nxt[kEx + c];
+        e.src[c] = nxt[kHx + c];
+      }
       PhaseParams hp = make_phase(h, BZH_PHASE_H, g, in.dt, in.resolution, in.metallic, in.uniform_l);
       launches.push_back(make_launch(pk, BZH_PHASE_H, g.dtype, hp));
       for (int c = 0; c < 3 && 3 + c < in.ngroups; ++c)
@@ -954,28 +1206,12 @@ bool run_program(hipStream_t s, ProgramIn& in) {
     }
     for (int c = 0; c < 3 && 6 + c < in.ngroups; ++c)
       add_source_launches(launches, in.groups[6 + c], 2, c, nxt, g, in.metallic, step0, k);
-    if (in.has_monitors) {
-      DftParams dp = make_dft(in.mon, nxt, g, in.dt, in.monitor_count);
-      dp.k = k;
-      if (slots > 1) {
-        // buffered: stage this step, fold the ring every ``slots`` steps and
-        // at the end of the call
-        dp.mode = 1;
-        dp.slots = slots;
-        dp.stage = stage;
-        launches.push_back(make_launch(Kind::kDft, 0, g.dtype, dp));
-        if ((k + 1) % slots == 0 || k == in.nsteps - 1) {
-          dp.mode = 2;
-          dp.nk = (k % slots) + 1;
-          dp.k0 = k + 1 - dp.nk;
-          launches.push_back(make_launch(Kind::kDft, 0, g.dtype, dp));
-        }
-      } else {
-        launches.push_back(make_launch(Kind::kDft, 0, g.dtype, dp));
-      }
-    }
+    if (in.has_monitors) add_dft(k, nxt);
+    cur_b = two_banks ? !cur_b : cur_b;
+    ++k;
   }
   graph_cache().run(s, launches, (in.flags & BZH_FLAG_GRAPH_CACHE) != 0);
+  g_last_pairs = npairs;
   g_last_fused = fused ? 1 : 0;
   return fused;
 }
@@ -1361,6 +1597,8 @@ const char* bzh_component_version(void) { return BZH_COMPONENT_VERSION; }
 const char* bzh_targets(void) { return "beamz_hip_streamed,beamz_hip_program"; }
 const char* bzh_last_error(void) { return bzh::g_error.c_str(); }
 int32_t bzh_last_program_fused(void) { return bzh::g_last_fused; }
+
+int32_t bzh_last_program_pairs(void) { return bzh::g_last_pairs; }
 
 int32_t bzh_launch_phase(void* stream, const bzh_launch* launch) {
   return guarded([&] {

@@ -162,8 +162,52 @@ struct FusedParams {
 };
 int fused_rows();
 
+// A source slab applied between the two steps of a pair launch: an E-timing
+// slab of the first step (``post``: before the intermediate monitor sample,
+// skipping PEC-constrained cells) or a pre-E slab of the second.
+struct MidSlab {
+  const float* coeff;     // this slab's [sz, sy, sx] coefficients (storage units)
+  const float* wave;      // this slab's waveform [T]
+  const int32_t* start;   // device int32[3]: (z, y, x) of the slab origin
+  int32_t sz, sy, sx, T;
+  int32_t comp;           // 0 Ex, 1 Ey, 2 Ez
+  int32_t kstep;          // waveform index = *step0 + kstep, clipped to [0, T-1]
+  int32_t post;
+  int32_t mlo[3], mhi[3]; // PEC masks of post slabs (z, y, x)
+};
+constexpr int kMaxMidSlabs = 4;
+constexpr int kMaxMonBoxes = 8;
+
+// Two leapfrog steps in one launch (bank A -> bank B).  ``f`` carries the
+// single-step layout and work box (tiles of 60 x 13 owned cells); the first
+// step's fields are written to the bank at ``cmid`` (same layout, foff) only
+// where a monitor box [z0, z1] x [y0, y1] x [x0, x1] (inclusive Yee indices,
+// device int32[mbox_n][6]) holds the cell, for the intermediate DFT sample.
+struct PairParams {
+  FusedParams f;
+  MidSlab msrc[kMaxMidSlabs];  // post slabs first, then pre-E slabs
+  int32_t msrc_n, msrc_post;
+  const int32_t* step0;
+  char* cmid;
+  const int32_t* mbox;
+  int32_t mbox_n;
+};
+constexpr int kPairCols = 60;   // owned columns of a pair tile
+constexpr int kPairRows = 13;   // owned rows of a pair tile (16 waves)
+
+// Bounding boxes of the monitors' gather points (one workgroup per monitor).
+struct MonBoxParams {
+  const int32_t* indices;  // [M, 6, Pmax, K]
+  const float* weights;
+  int32_t M, Pmax, K;
+  int32_t dims[6][3];      // component shapes (z, y, x)
+  int32_t* box;            // out: int32[M][6]
+};
+
 // Launchers implemented in the .hip files.
 void launch_fused3d(hipStream_t s, const FusedParams& p, int dtype);
+void launch_pair3d(hipStream_t s, const PairParams& p, int dtype);
+void launch_monbox(hipStream_t s, const MonBoxParams& p);
 void launch_phase3d(hipStream_t s, const PhaseParams& p, int phase, int dtype);
 void launch_phase_tm(hipStream_t s, const PhaseParams& p, int phase, int dtype);
 

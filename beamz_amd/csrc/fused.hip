@@ -195,6 +195,17 @@ struct SlabArgs {
   }
 };
 
+// The pair kernel's argument block, re-read at a use (see kargs): its CPML
+// offsets, coefficient table, source slabs and monitor boxes are needed only
+// on some planes and rows, and holding them across the z-march spills scalar
+// registers to vector lanes.
+__device__ __forceinline__ __attribute__((address_space(4))) const PairParams* pair_kargs() {
+  auto p = (__attribute__((address_space(4))) const PairParams*)(
+      __builtin_amdgcn_kernarg_segment_ptr());
+  asm volatile("" : "+s"(p));
+  return p;
+}
+
 template <typename T, typename P, typename C, int kR, bool kEArr, bool kXEdge>
 __device__ __forceinline__ void fused_body(const FusedParams& f) {
   const SlabArgs<sizeof(T) == 2 && !kXEdge> sa{f};
@@ -549,6 +560,537 @@ void launch_field(hipStream_t s, const FusedParams& p, int nblocks) {
   else launch_coef<T, float>(s, p, nblocks);
 }
 
+
+// ---- two leapfrog steps per launch -------------------------------------------
+//
+// A workgroup of 16 waves owns 60 (x) by 13 (y) cells of the second step's
+// output and marches z over a chunk of planes.  Lane l sits at
+// x = i0 - 2 + l, wave w on row y = j0 - 2 + w.  The first step (H', E') is
+// computed on the whole 64 x 16 block: H' is complete on lanes 0..62, E' on
+// lanes 1..62 and waves 1..15.  The second (H'', E'') runs where its inputs
+// are complete: H'' on lanes 1..61 and waves 1..14, E'' on lanes 2..61 and
+// waves 2..14, the owned cells.  Along z the second step lags, so rows still
+// cross through LDS with one barrier per plane.  Iteration k computes
+//   before the barrier: H'(k)   from E(k), E(k+1), E(k) of the row above
+//                                (published at k-1);
+//   after it:           E'(k)   from H'(k), H'(k-1), H'(k) of the row below
+//                                (published before this barrier),
+//                       H''(k-1) from E'(k-1), E'(k), E'(k-1) of the row above
+//                                (published at k-1),
+//                       E''(k-2) from H''(k-2), H''(k-3), H''(k-2) of the
+//                                row below (published at k-1).
+// A chunk owning planes [kz0, kz1) starts at plane kz0 - 2 and ends at
+// kz1 + 1.  The first step's fields stay in registers and LDS; they reach
+// HBM only inside the monitor boxes (the intermediate bank read by the DFT of
+// the first step).  Between the steps the kernel applies the first step's
+// E-timing source slabs (before that sample) and the second step's pre-E
+// slabs (after it), in the order the single-step schedule applies them.
+// Arithmetic per update is the single-step kernel's; in half precision the
+// intermediate fields are not rounded to the storage type.
+template <typename T, typename P, typename C, bool kEArr, bool kXEdge>
+__device__ __forceinline__ void pair_body(const PairParams& pp) {
+  const FusedParams& f = pp.f;
+  const SlabArgs<kXEdge> sa{f};  // x-edge tiles: reloaded (x-slab terms raise the pressure)
+  constexpr int kW = kPairRows + 3;
+  __shared__ float lE1[2][kW + 1][2][64];  // E rows (Ez, Ex) for H'
+  __shared__ float lH1[2][kW][2][64];      // H' rows (Hz, Hx) for E'
+  __shared__ float lE2[2][kW][2][64];      // E' rows for H''
+  __shared__ float lH2[2][kW][2][64];      // H'' rows for E''
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const bool halo = w == 0;  // computes H' (Hx, Hz) only and loads the top E row
+
+  int bx, by, kz0, kz1;
+  if (!decode_tile(f, bx, by, kz0, kz1)) return;
+
+  const int nz = f.nz, ny = f.ny, nx = f.nx, L = f.L, L2 = 2 * f.L;
+  const int m = f.metallic;
+  const bool mf = m & kFront, mk = m & kBack, mb = m & kBottom, mt = m & kTop;
+  const bool ml = m & kLeft, mr = m & kRight;
+  const float idx = f.inv_dx;
+  const int j0 = by * kPairRows;
+  const int x = bx * kPairCols - 2 + lane;
+  const int dn = ((lane + 1) & 63) << 2;
+  const int up = ((lane - 1) & 63) << 2;
+
+  bool vxX = true, vxY = true, xface = false;
+  int xX = x, xY = x, qxH = -1, qxE = -1;
+  if constexpr (kXEdge) {
+    vxX = x >= 0 && x < nx;
+    vxY = x >= 0 && x <= nx;
+    xX = min(max(x, 0), nx - 1);
+    xY = min(max(x, 0), nx);
+    qxH = vxX ? slab_q(x, nx, L) : -1;
+    qxE = vxY ? slab_q(x, nx + 1, L) : -1;
+    xface = (ml && x == 0) || (mr && x == nx);
+  }
+  const bool own = lane >= 2 && lane <= 61 && w >= 2 && w <= kPairRows + 1;
+  const uint32_t tX = xX * sizeof(T), tY = xY * sizeof(T);
+  const uint32_t pX = xX * sizeof(P), pY = xY * sizeof(P);
+  const uint32_t cX = xX * sizeof(C), cY = xY * sizeof(C);
+  const uint32_t qH = max(qxH, 0) * sizeof(P), qE = max(qxE, 0) * sizeof(P);
+  Coef c3{}, c4{}, c9{}, c10{};
+  if constexpr (kXEdge) {
+    if (qxH >= 0) {
+      c3 = coef_v(f.tab, 3, L2, qxH);
+      c4 = coef_v(f.tab, 4, L2, qxH);
+    }
+    if (qxE >= 0) {
+      c9 = coef_v(f.tab, 9, L2, qxE);
+      c10 = coef_v(f.tab, 10, L2, qxE);
+    }
+  }
+
+  const int y = j0 - 2 + w;
+  const bool vyX = y >= 0 && y <= ny, vyY = y >= 0 && y < ny;
+  const int yX = min(max(y, 0), ny), yY = min(max(y, 0), ny - 1);
+  const int qyH = vyY ? slab_q(y, ny, L) : -1;
+  const int qyE = vyX ? slab_q(y, ny + 1, L) : -1;
+  const bool y_pec_h = (mb && y == 0) || (mt && y == ny);
+  const int yt = j0 - 2 + kW;
+  const bool vtX = yt <= ny;
+  const int ytX = min(yt, ny);
+
+  float hc[6], ec[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    hc[q] = as_const(f.hdec[q])[0];
+    hc[3 + q] = as_const(f.hsrc[q])[0] * f.ratio_h;
+    ec[q] = as_const(f.edec[q])[0];
+  }
+  float es0 = 0.0f, es1 = 0.0f, es2 = 0.0f;
+  if constexpr (!kEArr) {
+    es0 = as_const(f.esrc[0])[0] * f.ratio_e;
+    es1 = as_const(f.esrc[1])[0] * f.ratio_e;
+    es2 = as_const(f.esrc[2])[0] * f.ratio_e;
+  }
+
+  // monitor boxes: bit b set on lanes whose (y, x) lies in box b; planes are
+  // tested per k.  Only owned cells write the intermediate bank.
+  using cint = const __attribute__((address_space(4))) int32_t;
+  uint32_t mbits = 0;
+  for (int b = 0; b < pp.mbox_n; ++b) {
+    cint* bx6 = (cint*)(pair_kargs()->mbox) + 6 * b;
+    if (y >= bx6[2] && y <= bx6[3] && x >= bx6[4] && x <= bx6[5]) mbits |= 1u << b;
+  }
+  mbits = own ? mbits : 0u;
+  const bool mwave = __builtin_amdgcn_readfirstlane(__any(mbits != 0u)) != 0;
+  auto sample_plane = [&](int k) {
+    uint32_t zb = 0;
+    for (int b = 0; b < pair_kargs()->mbox_n; ++b) {
+      cint* bx6 = (cint*)(pair_kargs()->mbox) + 6 * b;
+      if (k >= bx6[0] && k <= bx6[1]) zb |= 1u << b;
+    }
+    return (mbits & zb) != 0u;
+  };
+
+  auto load_e = [&](int k, float& ex, float& ey, float& ez) {
+    const bool vxy = k <= nz, vz = k < nz;
+    const uint32_t kx = min(k, nz), kz = min(k, nz - 1);
+    const float a = ldu<T>(f.fin, f.foff[0], (kx * (ny + 1) + yX) * nx, tX);
+    const float c = ldu<T>(f.fin, f.foff[1], (kx * ny + yY) * (nx + 1), tY);
+    const float d = ldu<T>(f.fin, f.foff[2], (kz * (ny + 1) + yX) * (nx + 1), tY);
+    ex = (vxy && vyX && vxX) ? a : 0.0f;
+    ey = (vxy && vyY && vxY) ? c : 0.0f;
+    ez = (vz && vyX && vxY) ? d : 0.0f;
+  };
+  auto load_top = [&](int k, float& ex, float& ez) {
+    const bool vxy = k <= nz, vz = k < nz;
+    const uint32_t kx = min(k, nz), kz = min(k, nz - 1);
+    const float a = ldu<T>(f.fin, f.foff[0], (kx * (ny + 1) + ytX) * nx, tX);
+    const float d = ldu<T>(f.fin, f.foff[2], (kz * (ny + 1) + ytX) * (nx + 1), tY);
+    ex = (vxy && vtX && vxX) ? a : 0.0f;
+    ez = (vz && vtX && vxY) ? d : 0.0f;
+  };
+
+  // CPML memories of plane k: H terms 0..5 at [t], E terms 6..11 at [t - 6]
+  auto load_psi_h = [&](int k, float (&p)[6]) {
+    const uint32_t kk = min(k, nz - 1), kh = min(k, nz);
+    const int qzH = k < nz ? slab_q(k, nz, L) : -1;
+    if (qyH >= 0) {
+      p[0] = ldu<P>(sa.base(), sa.in(0), (kk * L2 + qyH) * (nx + 1), pY);
+      p[5] = ldu<P>(sa.base(), sa.in(5), (kh * L2 + qyH) * nx, pX);
+    }
+    if (qzH >= 0) {
+      p[1] = ldu<P>(sa.base(), sa.in(1), (qzH * ny + yY) * (nx + 1), pY);
+      if (!halo) p[2] = ldu<P>(sa.base(), sa.in(2), (qzH * (ny + 1) + yX) * nx, pX);
+    }
+    if constexpr (kXEdge) {
+      if (qxH >= 0) {
+        if (!halo) p[3] = ldu<P>(sa.base(), sa.in(3), (kk * (ny + 1) + yX) * L2, qH);
+        p[4] = ldu<P>(sa.base(), sa.in(4), (kh * ny + yY) * L2, qH);
+      }
+    }
+  };
+  auto load_psi_e = [&](int k, float (&p)[6]) {
+    const uint32_t kx = min(k, nz), kzc = min(k, nz - 1);
+    const int qzE = k <= nz ? slab_q(k, nz + 1, L) : -1;
+    if (qyE >= 0) {
+      p[0] = ldu<P>(sa.base(), sa.in(6), (kx * L2 + qyE) * nx, pX);
+      p[5] = ldu<P>(sa.base(), sa.in(11), (kzc * L2 + qyE) * (nx + 1), pY);
+    }
+    if (qzE >= 0) {
+      p[1] = ldu<P>(sa.base(), sa.in(7), (qzE * (ny + 1) + yX) * nx, pX);
+      p[2] = ldu<P>(sa.base(), sa.in(8), (qzE * ny + yY) * (nx + 1), pY);
+    }
+    if constexpr (kXEdge) {
+      if (qxE >= 0) {
+        p[3] = ldu<P>(sa.base(), sa.in(9), (kx * ny + yY) * L2, qE);
+        p[4] = ldu<P>(sa.base(), sa.in(10), (kzc * (ny + 1) + yX) * L2, qE);
+      }
+    }
+  };
+
+  // H update of plane k on this row (kernels.py:458-503), psi terms 0..5
+  // from pin to pout; invalid positions come out zero.
+  auto h_update = [&](int k, float ez, float ey, float ex, float eyn, float exn, float ez_up,
+                      float ex_up, float ha0, float ha1, float ha2, const float (&pin)[6],
+                      float (&pout)[6], float& hx, float& hy, float& hz) {
+    const bool vxy = k < nz;
+    const int qzH = vxy ? slab_q(k, nz, L) : -1;
+    float d0 = (ez_up - ez) * idx;               // dEz/dy
+    float d1 = (eyn - ey) * idx;                 // dEy/dz
+    float d2 = (exn - ex) * idx;                 // dEx/dz
+    float d3 = (lane_shift(ez, dn) - ez) * idx;  // dEz/dx
+    float d4 = (lane_shift(ey, dn) - ey) * idx;  // dEy/dx
+    float d5 = (ex_up - ex) * idx;               // dEx/dy
+    if (qyH >= 0) {
+      d0 = cpml_step(coef_u(sa.tab(), 0, L2, qyH), d0, pin[0], pout[0]);
+      d5 = cpml_step(coef_u(sa.tab(), 5, L2, qyH), d5, pin[5], pout[5]);
+    }
+    if (qzH >= 0) {
+      d1 = cpml_step(coef_u(sa.tab(), 1, L2, qzH), d1, pin[1], pout[1]);
+      d2 = cpml_step(coef_u(sa.tab(), 2, L2, qzH), d2, pin[2], pout[2]);
+    }
+    if constexpr (kXEdge) {
+      if (qxH >= 0) {
+        d3 = cpml_step(c3, d3, pin[3], pout[3]);
+        d4 = cpml_step(c4, d4, pin[4], pout[4]);
+      }
+    }
+    hx = hc[0] * ha0 - hc[3] * (d0 - d1);
+    hy = hc[1] * ha1 - hc[4] * (d2 - d3);
+    hz = hc[2] * ha2 - hc[5] * (d4 - d5);
+    if (kXEdge && xface) hx = 0.0f;
+    if (y_pec_h) hy = 0.0f;
+    if ((mf && k == 0) || (mk && k == nz)) hz = 0.0f;
+    hx = (vxy && vyY && vxY) ? hx : 0.0f;
+    hy = (vxy && vyX && vxX) ? hy : 0.0f;
+    hz = (k <= nz && vyY && vxX) ? hz : 0.0f;
+  };
+
+  // E update of plane k on this row (kernels.py:554-626, ghost rule
+  // lattice.py:523-575), psi terms 6..11 from pin[t - 6] to pout[t - 6];
+  // invalid positions come out zero.
+  auto e_update = [&](int k, float hx, float hy, float hz, float hz_dn, float hx_dn, float phx,
+                      float phy, float ex, float ey, float ez, float s0, float s1, float s2,
+                      const float (&pin)[6], float (&pout)[6], float& nex, float& ney,
+                      float& nez) {
+    const int qzE = k <= nz ? slab_q(k, nz + 1, L) : -1;
+    const float hz_l = lane_shift(hz, up);
+    const float hy_l = lane_shift(hy, up);
+    float e0 = gdiff(hz, hz_dn, y, ny, mb, mt) * idx;  // dHz/dy
+    float e1 = gdiff(hy, phy, k, nz, mf, mk) * idx;    // dHy/dz
+    float e2 = gdiff(hx, phx, k, nz, mf, mk) * idx;    // dHx/dz
+    float e3, e4;                                      // dHz/dx, dHy/dx
+    if constexpr (kXEdge) {
+      e3 = gdiff(hz, hz_l, x, nx, ml, mr) * idx;
+      e4 = gdiff(hy, hy_l, x, nx, ml, mr) * idx;
+    } else {
+      e3 = (hz - hz_l) * idx;
+      e4 = (hy - hy_l) * idx;
+    }
+    float e5 = gdiff(hx, hx_dn, y, ny, mb, mt) * idx;  // dHx/dy
+    if (qyE >= 0) {
+      e0 = cpml_step(coef_u(sa.tab(), 6, L2, qyE), e0, pin[0], pout[0]);
+      e5 = cpml_step(coef_u(sa.tab(), 11, L2, qyE), e5, pin[5], pout[5]);
+    }
+    if (qzE >= 0) {
+      e1 = cpml_step(coef_u(sa.tab(), 7, L2, qzE), e1, pin[1], pout[1]);
+      e2 = cpml_step(coef_u(sa.tab(), 8, L2, qzE), e2, pin[2], pout[2]);
+    }
+    if constexpr (kXEdge) {
+      if (qxE >= 0) {
+        e3 = cpml_step(c9, e3, pin[3], pout[3]);
+        e4 = cpml_step(c10, e4, pin[4], pout[4]);
+      }
+    }
+    nex = ec[0] * ex + s0 * (e0 - e1);
+    ney = ec[1] * ey + s1 * (e2 - e3);
+    nez = ec[2] * ez + s2 * (e4 - e5);
+    const bool zf = (mf && k == 0) || (mk && k == nz);
+    const bool yf = (mb && y == 0) || (mt && y == ny);
+    if (zf || yf) nex = 0.0f;
+    if (zf || (kXEdge && xface)) ney = 0.0f;
+    if (yf || (kXEdge && xface)) nez = 0.0f;
+    nex = (k <= nz && vyX && vxX) ? nex : 0.0f;
+    ney = (k <= nz && vyY && vxY) ? ney : 0.0f;
+    nez = (k < nz && vyX && vxY) ? nez : 0.0f;
+  };
+
+  // stores of plane k (fields to ``out``, CPML memories to the written bank)
+  auto store_h = [&](const char* out, int k, float hx, float hy, float hz) {
+    if (k < nz && vyY && vxY) stu<T>(out, f.foff[3], (k * ny + yY) * (nx + 1), tY, hx);
+    if (k < nz && vyX && vxX) stu<T>(out, f.foff[4], (k * (ny + 1) + yX) * nx, tX, hy);
+    if (k <= nz && vyY && vxX) stu<T>(out, f.foff[5], (k * ny + yY) * nx, tX, hz);
+  };
+  auto store_e = [&](const char* out, int k, float ex, float ey, float ez) {
+    if (k <= nz && vyX && vxX) stu<T>(out, f.foff[0], (k * (ny + 1) + yX) * nx, tX, ex);
+    if (k <= nz && vyY && vxY) stu<T>(out, f.foff[1], (k * ny + yY) * (nx + 1), tY, ey);
+    if (k < nz && vyX && vxY) stu<T>(out, f.foff[2], (k * (ny + 1) + yX) * (nx + 1), tY, ez);
+  };
+  auto store_psi_h = [&](int k, const float (&p)[6]) {
+    const bool okx = k < nz && vyY && vxY, oky = k < nz && vyX && vxX, okz = k <= nz && vyY && vxX;
+    const int qzH = k < nz ? slab_q(k, nz, L) : -1;
+    if (qyH >= 0) {
+      if (okx) stu<P>(sa.base(), sa.out(0), (k * L2 + qyH) * (nx + 1), pY, p[0]);
+      if (okz) stu<P>(sa.base(), sa.out(5), (k * L2 + qyH) * nx, pX, p[5]);
+    }
+    if (qzH >= 0) {
+      if (okx) stu<P>(sa.base(), sa.out(1), (qzH * ny + yY) * (nx + 1), pY, p[1]);
+      if (oky) stu<P>(sa.base(), sa.out(2), (qzH * (ny + 1) + yX) * nx, pX, p[2]);
+    }
+    if constexpr (kXEdge) {
+      if (qxH >= 0) {
+        if (oky) stu<P>(sa.base(), sa.out(3), (k * (ny + 1) + yX) * L2, qH, p[3]);
+        if (okz) stu<P>(sa.base(), sa.out(4), (k * ny + yY) * L2, qH, p[4]);
+      }
+    }
+  };
+  auto store_psi_e = [&](int k, const float (&p)[6]) {
+    const bool okx = k <= nz && vyX && vxX, oky = k <= nz && vyY && vxY, okz = k < nz && vyX && vxY;
+    const int qzE = k <= nz ? slab_q(k, nz + 1, L) : -1;
+    if (qyE >= 0) {
+      if (okx) stu<P>(sa.base(), sa.out(6), (k * L2 + qyE) * nx, pX, p[0]);
+      if (okz) stu<P>(sa.base(), sa.out(11), (k * L2 + qyE) * (nx + 1), pY, p[5]);
+    }
+    if (qzE >= 0) {
+      if (okx) stu<P>(sa.base(), sa.out(7), (qzE * (ny + 1) + yX) * nx, pX, p[1]);
+      if (oky) stu<P>(sa.base(), sa.out(8), (qzE * ny + yY) * (nx + 1), pY, p[2]);
+    }
+    if constexpr (kXEdge) {
+      if (qxE >= 0) {
+        if (oky) stu<P>(sa.base(), sa.out(9), (k * ny + yY) * L2, qE, p[3]);
+        if (okz) stu<P>(sa.base(), sa.out(10), (k * (ny + 1) + yX) * L2, qE, p[4]);
+      }
+    }
+  };
+
+  // source slabs [s0, s1) on E'(k) of this row (execute.py:374-461): the
+  // slab's plane and row are uniform, its columns per lane
+  auto mid_sources = [&](int s0, int s1, int k, float& vx, float& vy, float& vz) {
+    for (int s = s0; s < s1; ++s) {
+      const auto& ms = pair_kargs()->msrc[s];
+      cint* st = (cint*)(ms.start);
+      const int cz = k - st[0], cy = y - st[1], cx = x - st[2];
+      const int comp = ms.comp;
+      const int Dz = comp == 2 ? nz : nz + 1, Dy = comp == 1 ? ny : ny + 1;
+      const int Dx = comp == 0 ? nx : nx + 1;
+      if (cz < 0 || cz >= ms.sz || cy < 0 || cy >= ms.sy || k >= Dz || y < 0 || y >= Dy) continue;
+      if (ms.post && ((ms.mlo[0] && k == 0) || (ms.mhi[0] && k == Dz - 1) ||
+                      (ms.mlo[1] && y == 0) || (ms.mhi[1] && y == Dy - 1)))
+        continue;
+      int widx = *(cint*)(pair_kargs()->step0) + ms.kstep;
+      widx = widx < 0 ? 0 : (widx > ms.T - 1 ? ms.T - 1 : widx);
+      const float wv = as_const(ms.wave)[widx];
+      const bool in = cx >= 0 && cx < ms.sx && x >= 0 && x < Dx &&
+                      !(ms.post && ((ms.mlo[2] && x == 0) || (ms.mhi[2] && x == Dx - 1)));
+      float add = 0.0f;
+      if (in) add = ((const float*)ms.coeff)[(cz * ms.sy + cy) * ms.sx + cx] * wv;
+      if (comp == 0) vx += add;
+      else if (comp == 1) vy += add;
+      else vz += add;
+    }
+  };
+
+  const int ks = max(kz0 - 2, 0);
+  const int kE = max(kz0 - 1, 0);  // first plane of E' the chunk needs
+  float ex, ey, ez;                // E(k) on this row
+  load_e(ks, ex, ey, ez);
+  lE1[ks & 1][w][0][lane] = ez;
+  lE1[ks & 1][w][1][lane] = ex;
+  if (halo) {
+    float tx_, tz_;
+    load_top(ks, tx_, tz_);
+    lE1[ks & 1][kW][0][lane] = tz_;
+    lE1[ks & 1][kW][1][lane] = tx_;
+  }
+  __syncthreads();
+
+  float h1x = 0.0f, h1y = 0.0f, h1z = 0.0f;  // H'(k-1)
+  float e1x = 0.0f, e1y = 0.0f, e1z = 0.0f;  // E'(k-1)
+  float e2x = 0.0f, e2y = 0.0f, e2z = 0.0f;  // E'(k-2)
+  float g2x = 0.0f, g2y = 0.0f, g2z = 0.0f;  // H''(k-2)
+  float g3x = 0.0f, g3y = 0.0f;              // H''(k-3)
+  float sa0 = es0, sa1 = es1, sa2 = es2;     // E source coefficients of plane k-1
+  float sb0 = es0, sb1 = es1, sb2 = es2;     // ... of plane k-2
+  float mh[6] = {}, me1[6] = {}, me2[6] = {};  // psi': H terms at k-1, E terms at k-1, k-2
+  const int npost = pair_kargs()->msrc_post, nsrc = pair_kargs()->msrc_n;
+
+  for (int k = ks; k <= kz1 + 1; ++k) {
+    const int bb = k & 1;
+    const bool g1 = k <= kz1;
+    const bool gE1 = g1 && k >= kE && w >= 1;
+    const bool gH2 = k - 1 >= kE && k - 1 < kz1 && w >= 1 && w <= kW - 2;
+    const bool gE2 = k - 2 >= kz0 && k - 2 < kz1 && w >= 2 && w <= kW - 2;
+    const bool sampled = mwave && k >= kz0 && k < kz1 && sample_plane(k);
+
+    // ---- first step: H'(k) ----
+    float exn = 0.0f, eyn = 0.0f, ezn = 0.0f;
+    float hx = 0.0f, hy = 0.0f, hz = 0.0f;
+    float q1[6] = {}, pe[6] = {};
+    float s0 = es0, s1 = es1, s2 = es2;
+    if (g1) {
+      float etx = 0.0f, etz = 0.0f;
+      load_e(k + 1, exn, eyn, ezn);
+      if (halo) load_top(k + 1, etx, etz);
+      const uint32_t kk = min(k, nz - 1), kh = min(k, nz);
+      const float ha0 = ldu<T>(f.fin, f.foff[3], (kk * ny + yY) * (nx + 1), tY);
+      const float ha1 = halo ? 0.0f : ldu<T>(f.fin, f.foff[4], (kk * (ny + 1) + yX) * nx, tX);
+      const float ha2 = ldu<T>(f.fin, f.foff[5], (kh * ny + yY) * nx, tX);
+      float ph[6] = {};
+      load_psi_h(k, ph);
+      if (gE1) {
+        if constexpr (kEArr) {
+          const uint32_t kx = min(k, nz), kzc = min(k, nz - 1);
+          s0 = ldu<C>(f.cbase, f.coff[0], (kx * (ny + 1) + yX) * nx, cX) * f.ratio_e;
+          s1 = ldu<C>(f.cbase, f.coff[1], (kx * ny + yY) * (nx + 1), cY) * f.ratio_e;
+          s2 = ldu<C>(f.cbase, f.coff[2], (kzc * (ny + 1) + yX) * (nx + 1), cY) * f.ratio_e;
+        }
+        load_psi_e(k, pe);
+      }
+      const float ez_up = lE1[bb][w + 1][0][lane];
+      const float ex_up = lE1[bb][w + 1][1][lane];
+      h_update(k, ez, ey, ex, eyn, exn, ez_up, ex_up, ha0, ha1, ha2, ph, q1, hx, hy, hz);
+      lH1[bb][w][0][lane] = hz;
+      lH1[bb][w][1][lane] = hx;
+      lE1[bb ^ 1][w][0][lane] = ezn;
+      lE1[bb ^ 1][w][1][lane] = exn;
+      if (halo) {
+        lE1[bb ^ 1][kW][0][lane] = etz;
+        lE1[bb ^ 1][kW][1][lane] = etx;
+      }
+      if (sampled) store_h(pair_kargs()->cmid, k, hx, hy, hz);
+    }
+    __syncthreads();
+
+    // ---- first step: E'(k), then the sources between the steps ----
+    float nex = 0.0f, ney = 0.0f, nez = 0.0f;
+    float q2[6] = {};
+    if (gE1) {
+      const float hz_dn = lH1[bb][w - 1][0][lane];
+      const float hx_dn = lH1[bb][w - 1][1][lane];
+      e_update(k, hx, hy, hz, hz_dn, hx_dn, h1x, h1y, ex, ey, ez, s0, s1, s2, pe, q2, nex, ney,
+               nez);
+      if (npost > 0) mid_sources(0, npost, k, nex, ney, nez);
+      if (sampled) store_e(pair_kargs()->cmid, k, nex, ney, nez);
+      if (nsrc > npost) mid_sources(npost, nsrc, k, nex, ney, nez);
+      lE2[bb][w][0][lane] = nez;
+      lE2[bb][w][1][lane] = nex;
+    }
+
+    // ---- second step: H''(k-1) ----
+    float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+    if (gH2) {
+      const int k1 = k - 1;
+      const float ez_up = lE2[bb ^ 1][w + 1][0][lane];
+      const float ex_up = lE2[bb ^ 1][w + 1][1][lane];
+      float q3[6] = {};
+      h_update(k1, e1z, e1y, e1x, ney, nex, ez_up, ex_up, h1x, h1y, h1z, mh, q3, gx, gy, gz);
+      lH2[bb][w][0][lane] = gz;
+      lH2[bb][w][1][lane] = gx;
+      if (own && k1 >= kz0) {
+        store_h(f.fout, k1, gx, gy, gz);
+        if (L > 0) store_psi_h(k1, q3);
+      }
+    }
+
+    // ---- second step: E''(k-2) ----
+    if (gE2) {
+      const int k2 = k - 2;
+      const float hz_dn = lH2[bb ^ 1][w - 1][0][lane];
+      const float hx_dn = lH2[bb ^ 1][w - 1][1][lane];
+      float q4[6] = {};
+      float fx, fy, fz;
+      e_update(k2, g2x, g2y, g2z, hz_dn, hx_dn, g3x, g3y, e2x, e2y, e2z, sb0, sb1, sb2, me2, q4,
+               fx, fy, fz);
+      if (own) {
+        store_e(f.fout, k2, fx, fy, fz);
+        if (L > 0) store_psi_e(k2, q4);
+      }
+    }
+
+    g3x = g2x;
+    g3y = g2y;
+    g2x = gx;
+    g2y = gy;
+    g2z = gz;
+    e2x = e1x;
+    e2y = e1y;
+    e2z = e1z;
+    e1x = nex;
+    e1y = ney;
+    e1z = nez;
+    h1x = hx;
+    h1y = hy;
+    h1z = hz;
+    sb0 = sa0;
+    sb1 = sa1;
+    sb2 = sa2;
+    sa0 = s0;
+    sa1 = s1;
+    sa2 = s2;
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {
+      me2[t] = me1[t];
+      me1[t] = q2[t];
+      mh[t] = q1[t];
+    }
+    ex = exn;
+    ey = eyn;
+    ez = ezn;
+  }
+}
+
+template <typename T, typename P, typename C, bool kEArr>
+__global__ void __launch_bounds__(64 * (kPairRows + 3)) pair_interior(PairParams p) {
+  pair_body<T, P, C, kEArr, false>(p);
+}
+template <typename T, typename P, typename C, bool kEArr>
+__global__ void __launch_bounds__(64 * (kPairRows + 3)) pair_edge(PairParams p) {
+  pair_body<T, P, C, kEArr, true>(p);
+}
+
+template <typename T, typename P, typename C>
+void launch_pair_wg(hipStream_t s, const PairParams& p, int nblocks) {
+  const dim3 block(64 * (kPairRows + 3), 1, 1), grid(nblocks, 1, 1);
+  const bool earr = p.f.esrc_st[0] != 0;
+  if (p.f.xedge) {
+    if (earr) hipLaunchKernelGGL((pair_edge<T, P, C, true>), grid, block, 0, s, p);
+    else hipLaunchKernelGGL((pair_edge<T, P, float, false>), grid, block, 0, s, p);
+  } else {
+    if (earr) hipLaunchKernelGGL((pair_interior<T, P, C, true>), grid, block, 0, s, p);
+    else hipLaunchKernelGGL((pair_interior<T, P, float, false>), grid, block, 0, s, p);
+  }
+}
+
+template <typename T, typename P>
+void launch_pair_coef(hipStream_t s, const PairParams& p, int nblocks) {
+  if constexpr (std::is_same<T, float>::value) {
+    launch_pair_wg<T, P, float>(s, p, nblocks);
+  } else {
+    if (p.f.coef_f16) launch_pair_wg<T, P, __half>(s, p, nblocks);
+    else launch_pair_wg<T, P, float>(s, p, nblocks);
+  }
+}
+
+template <typename T>
+void launch_pair_field(hipStream_t s, const PairParams& p, int nblocks) {
+  if (p.f.psi_bf16) launch_pair_coef<T, bf16_t>(s, p, nblocks);
+  else launch_pair_coef<T, float>(s, p, nblocks);
+}
+
+
 }  // namespace
 
 int fused_rows() { return rows_setting(); }
@@ -592,6 +1134,64 @@ void launch_fused3d(hipStream_t s, const FusedParams& p, int dtype) {
     case BZH_F32: launch_field<float>(s, p, static_cast<int>(nblocks)); break;
     case BZH_BF16: launch_field<bf16_t>(s, p, static_cast<int>(nblocks)); break;
     case BZH_F16: launch_field<__half>(s, p, static_cast<int>(nblocks)); break;
+    default: throw Failure{"unsupported field dtype"};
+  }
+  BZH_HIP(hipGetLastError());
+}
+
+void launch_pair3d(hipStream_t s, const PairParams& pp, int dtype) {
+  const FusedParams& p = pp.f;
+  BZH_REQUIRE(p.box_ntx >= 0 && p.box_nty >= 0 && p.box_kz >= 1 && p.box_nzc >= 0,
+              "pair box is malformed");
+  BZH_REQUIRE(p.rows == kPairRows, "pair tiles are cut with the pair tile height");
+  BZH_REQUIRE(!p.coef_f16 || dtype != BZH_F32, "float16 coefficients need half-precision fields");
+  const int ntx_total = (p.nx + 1 + kPairCols - 1) / kPairCols;
+  const int nty_total = (p.ny + 1 + kPairRows - 1) / kPairRows;
+  BZH_REQUIRE(p.box_tx0 >= 0 && p.box_tsplit >= 0 && p.box_tskip >= 0 &&
+                  p.box_tx0 + p.box_ntx + p.box_tskip <= ntx_total && p.box_ty0 >= 0 &&
+                  p.box_ty0 + p.box_nty <= nty_total && p.box_z0 >= 0 &&
+                  p.box_z1 <= p.nz + 1 && p.box_z0 + (p.box_nzc - 1) * p.box_kz < p.box_z1,
+              "pair box exceeds the grid");
+  if (!p.xedge) {
+    // the interior variant reads lanes x = i0-2 .. i0+61 unchecked and assumes
+    // no x face or x CPML slab under them
+    const int M = p.L > 0 ? p.L : 1;
+    const int lo = p.box_tx0 * kPairCols - 2;
+    const int hi = (p.box_tx0 + p.box_ntx + p.box_tskip - 1) * kPairCols + kPairCols + 1;
+    BZH_REQUIRE(p.box_ntx == 0 || (lo >= M && hi <= p.nx - M - 1),
+                "interior pair tiles overlap an x face or x CPML slab");
+  }
+  BZH_REQUIRE(p.L == 0 || p.tab != nullptr, "CPML tiles need the coefficient table");
+  BZH_REQUIRE(p.fin != nullptr && p.fout != nullptr && (p.L == 0 || p.pbase != nullptr) &&
+                  (p.esrc_st[0] == 0 || p.cbase != nullptr),
+              "the pair step needs based addressing of its array groups");
+  for (int c = 0; c < 3; ++c) {
+    BZH_REQUIRE(p.hdec_st[c] == 0 && p.hsrc_st[c] == 0 && p.edec_st[c] == 0,
+                "the pair step needs uniform H coefficients and E decay");
+    BZH_REQUIRE(p.esrc_st[c] == p.esrc_st[0], "E source coefficients must be all arrays or all scalars");
+  }
+  BZH_REQUIRE(pp.msrc_n >= 0 && pp.msrc_n <= kMaxMidSlabs && pp.msrc_post >= 0 &&
+                  pp.msrc_post <= pp.msrc_n && (pp.msrc_n == 0 || pp.step0 != nullptr),
+              "pair source slabs are malformed");
+  for (int i = 0; i < pp.msrc_n; ++i) {
+    const MidSlab& m = pp.msrc[i];
+    BZH_REQUIRE(m.coeff && m.wave && m.start && m.sz >= 0 && m.sy >= 0 && m.sx >= 0 && m.T >= 1 &&
+                    m.comp >= 0 && m.comp < 3,
+                "pair source slab is malformed");
+  }
+  BZH_REQUIRE(pp.mbox_n >= 0 && pp.mbox_n <= kMaxMonBoxes &&
+                  (pp.mbox_n == 0 || (pp.mbox != nullptr && pp.cmid != nullptr)),
+              "pair monitor boxes are malformed");
+  const int64_t tiles = static_cast<int64_t>(p.box_ntx) * p.box_nty;
+  BZH_REQUIRE(p.box_band == 0 || (p.box_band >= 1 && 8 * int64_t{p.box_band} >= tiles),
+              "pair bands do not cover the tiles");
+  const int64_t nblocks = (p.box_band > 0 ? 8 * int64_t{p.box_band} : tiles) * p.box_nzc;
+  BZH_REQUIRE(nblocks < (int64_t{1} << 31), "pair grid too large");
+  if (nblocks <= 0) return;
+  switch (dtype) {
+    case BZH_F32: launch_pair_field<float>(s, pp, static_cast<int>(nblocks)); break;
+    case BZH_BF16: launch_pair_field<bf16_t>(s, pp, static_cast<int>(nblocks)); break;
+    case BZH_F16: launch_pair_field<__half>(s, pp, static_cast<int>(nblocks)); break;
     default: throw Failure{"unsupported field dtype"};
   }
   BZH_HIP(hipGetLastError());

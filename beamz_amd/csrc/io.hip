@@ -10,6 +10,8 @@
 // The DFT accumulators and their phase factors are float64: the phase
 // 2*pi*f*t_phys is evaluated in float64 from t_phys = t + (k + 1) dt.
 
+#include <climits>
+
 #include "common.h"
 
 namespace bzh {
@@ -260,6 +262,65 @@ void dft_dispatch(hipStream_t s, const DftParams& p) {
   }
 }
 
+
+// Bounding boxes of the monitors' gather points, for the pair step's
+// intermediate bank: box m = [zmin, zmax, ymin, ymax, xmin, xmax] over every
+// point the gather reads (weight != 0, index inside the component).  The
+// init kernel empties the boxes; blocks (b, m) fold slices of monitor m's
+// points in with atomics.
+__global__ void __launch_bounds__(64) monbox_init_kernel(MonBoxParams p) {
+  const int i = threadIdx.x;
+  if (i < p.M * 6) p.box[i] = (i % 2 == 0) ? INT_MAX : INT_MIN;
+}
+
+constexpr int kBoxBlock = 256;
+constexpr int kBoxSlices = 64;
+
+__global__ void __launch_bounds__(kBoxBlock) monbox_kernel(MonBoxParams p) {
+  __shared__ int red[6][kBoxBlock];
+  const int m = blockIdx.y;
+  int v[6] = {INT_MAX, INT_MIN, INT_MAX, INT_MIN, INT_MAX, INT_MIN};
+  const int64_t per = static_cast<int64_t>(p.Pmax) * p.K;
+  for (int c = 0; c < 6; ++c) {
+    const int Sz = p.dims[c][0], Sy = p.dims[c][1], Sx = p.dims[c][2];
+    const int64_t n = static_cast<int64_t>(Sz) * Sy * Sx;
+    if (n <= 0) continue;
+    const int64_t base = (static_cast<int64_t>(m) * 6 + c) * per;
+    for (int64_t i = blockIdx.x * int64_t{kBoxBlock} + threadIdx.x; i < per;
+         i += int64_t{kBoxBlock} * gridDim.x) {
+      const int idx = p.indices[base + i];
+      if (p.weights[base + i] == 0.0f || idx < 0 || idx >= n) continue;
+      const int z = idx / (Sy * Sx);
+      const int r = idx - z * (Sy * Sx);
+      const int y = r / Sx;
+      const int x = r - y * Sx;
+      v[0] = min(v[0], z);
+      v[1] = max(v[1], z);
+      v[2] = min(v[2], y);
+      v[3] = max(v[3], y);
+      v[4] = min(v[4], x);
+      v[5] = max(v[5], x);
+    }
+  }
+  for (int q = 0; q < 6; ++q) red[q][threadIdx.x] = v[q];
+  __syncthreads();
+  for (int h = kBoxBlock / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < h) {
+      for (int q = 0; q < 6; ++q) {
+        const int a = red[q][threadIdx.x], b = red[q][threadIdx.x + h];
+        red[q][threadIdx.x] = (q % 2 == 0) ? min(a, b) : max(a, b);
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    for (int q = 0; q < 6; q += 2) {
+      if (red[q][0] != INT_MAX) atomicMin(&p.box[m * 6 + q], red[q][0]);
+      if (red[q + 1][0] != INT_MIN) atomicMax(&p.box[m * 6 + q + 1], red[q + 1][0]);
+    }
+  }
+}
+
 }  // namespace
 
 void launch_source(hipStream_t s, const SourceParams& p) {
@@ -289,6 +350,15 @@ void launch_dft(hipStream_t s, const DftParams& p) {
     case BZH_F16: dft_dispatch<__half>(s, p); break;
     default: throw Failure{"unsupported DFT field dtype"};
   }
+  BZH_HIP(hipGetLastError());
+}
+
+
+void launch_monbox(hipStream_t s, const MonBoxParams& p) {
+  BZH_REQUIRE(p.M >= 1 && p.M <= kMaxMonBoxes && p.box != nullptr, "monitor boxes are malformed");
+  if (p.Pmax <= 0 || p.K <= 0) return;
+  hipLaunchKernelGGL(monbox_init_kernel, dim3(1), dim3(64), 0, s, p);
+  hipLaunchKernelGGL(monbox_kernel, dim3(kBoxSlices, p.M), dim3(kBoxBlock), 0, s, p);
   BZH_HIP(hipGetLastError());
 }
 

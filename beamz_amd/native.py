@@ -118,6 +118,7 @@ SIGNATURES = {
     "bzh_targets": (ctypes.c_char_p, []),
     "bzh_last_error": (ctypes.c_char_p, []),
     "bzh_last_program_fused": (ctypes.c_int32, []),
+    "bzh_last_program_pairs": (ctypes.c_int32, []),
     "bzh_launch_phase": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p]),
     "bzh_launch_program": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p]),
     "bzh_streamed": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(Buffer), ctypes.c_int32,
@@ -219,6 +220,11 @@ def call_streamed(args, attrs: PhaseAttrs, stream=None):
 def last_program_fused() -> bool:
     """Did the last program call on this thread run the fused step kernels?"""
     return bool(library().bzh_last_program_fused())
+
+
+def last_program_pairs() -> int:
+    """Two-step (pair) launches in the last program call on this thread."""
+    return int(library().bzh_last_program_pairs())
 
 
 def call_program(args, attrs: ProgramAttrs, stream=None):

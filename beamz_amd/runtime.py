@@ -220,16 +220,18 @@ def advance(state, prog, nsteps, *, chunk=GRAPH_MAX_STEPS):
     return state
 
 
-def step_kernel_name(prog):
+def step_kernel_name(prog, pairs=False):
+    """The step kernels as rocprofv3 names them; ``pairs`` when the last
+    program call ran two steps per launch."""
     if uses_two_banks(prog):
-        return "fused_interior+fused_edge"
+        return "pair_interior+pair_edge" if pairs else "fused_interior+fused_edge"
     return "h_phase_3d+e_phase_3d" if prog.is_3d else "h_phase_tm+e_phase_tm"
 
 
 def time_step_kernel(state, prog, *, launches=20, stream=None):
     """Average device time of one bare field step (no sources or monitors),
-    measured with events on ``stream``: two steps per call so the banks return
-    to their starting roles."""
+    measured with events on ``stream``: four steps per call (two pair launches
+    where the pair kernel runs) so the banks return to their starting roles."""
     stream = stream or torch.cuda.current_stream()
     with torch.cuda.stream(stream):
         layout_state = state
@@ -239,19 +241,19 @@ def time_step_kernel(state, prog, *, launches=20, stream=None):
             layout_state = with_workspace(layout_state)
             args = (*_temporal_io(prog, layout_state), *_source_group_arguments_empty(prog),
                     layout_state.current_step)
-            attrs = _program_attrs(prog, 2, abi.PROGRAM_LAYOUT_SOURCE_TEMPORAL_CPML)
+            attrs = _program_attrs(prog, 4, abi.PROGRAM_LAYOUT_SOURCE_TEMPORAL_CPML)
             attrs.coincident_source_group_mask = 0
         else:
             layout = abi.PROGRAM_LAYOUT_CPML_IN_PLACE if prog.cpml_enabled else abi.PROGRAM_LAYOUT_YEE_IN_PLACE
             args = _graph_io(prog, layout_state)
-            attrs = _program_attrs(prog, 2, layout)
+            attrs = _program_attrs(prog, 4, layout)
         native.call_program(args, attrs)  # warm (captures the graph)
         start.record(stream)
         for _ in range(launches):
             native.call_program(args, attrs)
         stop.record(stream)
     torch.cuda.synchronize()
-    return start.elapsed_time(stop) / (2 * launches)
+    return start.elapsed_time(stop) / (4 * launches)
 
 
 def _source_group_arguments_empty(prog):

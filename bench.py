@@ -221,11 +221,12 @@ PROFILES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
 
 
 def measured_traffic(kernel, dtype):
-    """HBM bytes per step of ``kernel`` from the committed rocprofv3 PMC
+    """HBM bytes per launch of ``kernel`` from the committed rocprofv3 PMC
     summary (two passes, FETCH_SIZE and WRITE_SIZE, scaled by the counters'
     readings of a plain copy of known size), or None when no summary for this
     kernel and storage type exists.  The summary carries the line
-    ``traffic: kernel=<name> dtype=<dtype> bytes_per_step=<int>``."""
+    ``traffic: kernel=<name> dtype=<dtype> bytes_per_launch=<int>``
+    (``bytes_per_step`` for a kernel that advances one step per launch)."""
     if not os.path.isdir(PROFILES):
         return None
     for name in sorted(os.listdir(PROFILES)):
@@ -238,7 +239,7 @@ def measured_traffic(kernel, dtype):
                     continue
                 fields = dict(kv.split("=", 1) for kv in line.split()[1:])
                 if fields.get("kernel") == kernel and fields.get("dtype") == dtype:
-                    return int(fields["bytes_per_step"])
+                    return int(fields.get("bytes_per_launch", fields.get("bytes_per_step")))
     return None
 
 
@@ -355,10 +356,14 @@ def main():
     if runtime.uses_two_banks(prog) and not native.last_program_fused():
         raise RuntimeError("the timed steps did not run the fused step kernels")
 
-    # dominant kernel: the full-step update launched alone (no sources/DFT), timed
-    # with events on its own stream; algorithmic bytes per launch = one full step
-    nb = algorithmic_bytes(prog, state)
-    k_ms = runtime.time_step_kernel(state, prog, launches=20, stream=stream)
+    pairs = native.last_program_pairs() > 0
+    # dominant kernel: the step update launched alone (no sources/DFT), timed
+    # with events on its own stream.  One launch advances one full step, or two
+    # where the pair kernel runs; algorithmic bytes per launch = that many full
+    # steps, and the launch time is the x-interior plus x-edge kernel
+    per_launch = 2 if pairs else 1
+    nb = algorithmic_bytes(prog, state) * per_launch
+    k_ms = runtime.time_step_kernel(state, prog, launches=20, stream=stream) * per_launch
     achieved = nb / (k_ms * 1e-3) / 1e9
     peak = 8000.0
     line = {
@@ -379,7 +384,8 @@ def main():
                         "random-eps 512x256x64 slab, Ez plane source, 2 x 21-wavelength FP64 DFT flux planes",
             "cells": prog.cells,
             "field_storage": args.dtype,
-            "schedule": "per-phase" if args.per_phase else "fused two-bank",
+            "schedule": "per-phase" if args.per_phase else (
+                "fused two-bank, two steps per launch" if pairs else "fused two-bank"),
         },
         "roofline": {
             "bound": "hbm",
@@ -387,8 +393,8 @@ def main():
             "peak": peak,
             "unit": "GB/s",
             "frac": round(achieved / peak, 4),
-            "traffic": measured_traffic(runtime.step_kernel_name(prog), args.dtype),
-            "kernel": runtime.step_kernel_name(prog),
+            "traffic": measured_traffic(runtime.step_kernel_name(prog, pairs), args.dtype),
+            "kernel": runtime.step_kernel_name(prog, pairs),
             "kernel_ms": round(k_ms, 4),
             "algorithmic_bytes_per_launch": int(nb),
         },

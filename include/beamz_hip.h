@@ -190,6 +190,12 @@ const char* bzh_last_error(void);
  * per-phase schedule (e.g. an H-timed source, per-cell H coefficients, or
  * field banks whose arrays do not each lie within 4 GiB of one base). */
 int32_t bzh_last_program_fused(void);
+/* Diagnostic with no counterpart in the reference: the number of two-step
+ * (pair) launches in the last program call on this thread.  The pair kernel
+ * advances the fused step two steps per pass; a call runs an even number of
+ * pairs (so the bank parity of the single-step schedule holds) and single
+ * steps for the rest.  BZH_PAIR=0 in the environment disables it. */
+int32_t bzh_last_program_pairs(void);
 
 /* ---- descriptor-level launches ------------------------------------------- */
 /* cuda/src/update.cu:919-971 (BeamzLaunchStreamed). */

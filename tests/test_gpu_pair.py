@@ -1,0 +1,117 @@
+"""Two leapfrog steps per launch (the pair kernel) against the FP64 oracle
+and against the single-step schedule.
+
+The pair kernel runs wherever the fused step runs; these cases cut its
+60 x 13 tiles and z chunks in every place (x-interior and x-edge tiles,
+partial y tiles, one-plane chunks), put source slabs of both kinds between
+the two steps of every pair (a pre-E Ez plane and an E-timed Ex slab) and
+sample x-, y- and z-normal monitors on the intermediate step.  Calls of 5
+steps run two pairs and one single step each, so the bank parity across
+calls is exercised as well.  Tolerances are the single-step parity tests'
+(relative L2 2e-5 per field, 5e-5 for CPML memories and DFT arenas).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import fdtd64 as o
+from tests.test_gpu_parity import _close, _compare_state, _plane_source, _require_gpu, _setup
+
+pytestmark = pytest.mark.gpu
+
+DX = 40e-9
+
+
+def _dt(grid):
+    return 0.9 * DX / (o.LIGHT_SPEED * np.sqrt(len(grid)))
+
+
+def _sources(grid, steps):
+    """A pre-E Ez plane and an E-timed Ex slab near the bottom y face."""
+    dt = _dt(grid)
+    pre = _plane_source(grid, dt, steps, 5)
+    nz, ny, nx = grid
+    rng = np.random.default_rng(9)
+    coeff = (rng.random((3, 4, nx - 6)) * 1e-1).astype(np.float32).astype(np.float64)
+    t = o.sample_times(0.0, dt, steps)
+    f0 = 2.2e14
+    wave = o.gaussian_pulse(t, f0, 0.4 * f0, offset=1.0).astype(np.float32).astype(np.float64)
+    post = o.Source("Ex", "e", (nz // 2 - 1, 0, 3), coeff, wave)
+    return [pre, post]
+
+
+def _monitors(grid):
+    import beamz_amd as bz
+    dt = _dt(grid)
+    nz, ny, nx = grid
+    freqs = np.array([1.7e14, 1.9e14, 2.1e14])
+    return [bz.FluxMonitor("x", nx // 2 + 1, freqs),
+            bz.FluxMonitor("y", ny - 4, freqs[1:], interval=2),
+            bz.FluxMonitor("z", 3, freqs, window="hann", t_start=2.5 * dt, t_end=9.5 * dt)]
+
+
+@pytest.mark.parametrize("chunk", [12, 5])
+@pytest.mark.parametrize("kz", ["1", "3", "16"])
+@pytest.mark.parametrize("grid,bc", [((13, 17, 130), "cpml"), ((9, 28, 64), "pec"),
+                                     ((11, 14, 197), "cpml")])
+def test_pair_parity(grid, bc, kz, chunk, monkeypatch):
+    _require_gpu()
+    from beamz_amd import native
+    monkeypatch.setenv("BZH_PAIR_KZ", kz)
+    steps = 12
+    kw = {"cpml": dict(cpml_cells=3), "pec": {}}[bc]
+    sim, prog, state, prob, ost = _setup(grid, seed=71, steps=steps, sources=_sources(grid, steps),
+                                         monitors=_monitors(grid), **kw)
+    print(f"case pair_parity grid={grid} bc={bc} kz={kz} chunk={chunk}", flush=True)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        state = sim.advance(state, steps, chunk=chunk)
+    torch.cuda.synchronize()
+    assert native.last_program_fused()
+    assert native.last_program_pairs() == (steps % chunk or chunk) // 2 // 2 * 2
+    o.run(prob, ost, steps)
+    _compare_state(prog, state, prob, ost, dft=True)
+
+
+def test_pair_parity_uniform_medium():
+    """Scalar E coefficients (uniform permittivity): the kernel variant
+    without coefficient arrays."""
+    _require_gpu()
+    from beamz_amd import native
+    grid = (10, 15, 140)
+    steps = 8
+    sim, prog, state, prob, ost = _setup(grid, seed=73, steps=steps, cpml_cells=4,
+                                         sources=_sources(grid, steps), monitors=_monitors(grid),
+                                         eps=np.full(grid, 2.25))
+    print(f"case pair_parity_uniform grid={grid}", flush=True)
+    state = sim.advance(state, steps, chunk=steps)
+    torch.cuda.synchronize()
+    assert native.last_program_pairs() == 4
+    o.run(prob, ost, steps)
+    _compare_state(prog, state, prob, ost, dft=True)
+
+
+def test_pair_matches_single_steps(monkeypatch):
+    """The same 16 steps with and without pairs: FP32 results agree to
+    rounding (the arithmetic per update is the same; only contraction and
+    summation order of sources may differ)."""
+    _require_gpu()
+    from beamz_amd import native
+    grid = (24, 40, 250)
+    steps = 16
+    runs = []
+    for pair in ("1", "0"):
+        monkeypatch.setenv("BZH_PAIR", pair)
+        sim, prog, state, prob, ost = _setup(grid, seed=75, steps=steps, cpml_cells=5,
+                                             sources=_sources(grid, steps), monitors=_monitors(grid))
+        print(f"case pair_vs_single grid={grid} pair={pair}", flush=True)
+        state = sim.advance(state, steps, chunk=steps)
+        torch.cuda.synchronize()
+        assert native.last_program_pairs() == (8 if pair == "1" else 0)
+        runs.append([t.cpu().numpy().copy() for t in state.fields()] +
+                    [state.dft_re.cpu().numpy().copy(), state.dft_im.cpu().numpy().copy()])
+    for i, (a, b) in enumerate(zip(*runs)):
+        _close(f"leaf {i}", a, b, rel=1e-6, mabs=1e-5)

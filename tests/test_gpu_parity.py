@@ -43,9 +43,9 @@ def _require_gpu():
 
 def _setup(grid, *, seed, cpml_cells=None, faces=None, metallic=None, kappa_max=2.0,
            sources=(), monitors=(), steps=1, dt_scale=0.9, fused=True,
-           field_dtype=torch.float32, field_amplitude=1e-5):
+           field_dtype=torch.float32, field_amplitude=1e-5, eps=None):
     import beamz_amd as bz
-    eps = eps_raster(grid, seed)
+    eps = eps_raster(grid, seed) if eps is None else np.asarray(eps, dtype=np.float64)
     dx = 40e-9
     dt = dt_scale * dx / (o.LIGHT_SPEED * np.sqrt(len(grid)))
     bounds = []
