@@ -742,9 +742,10 @@ FusedSplit fused_split(const Grid& g, int L) {
 // Work split of the pair step (two steps per launch): tiles of 60 x 13
 // owned cells whose lanes cover x = 60 t - 2 .. 60 t + 61; interior tiles
 // have all of them in [M, nx - M - 1].  z chunks of 32 planes
-// (BZH_PAIR_KZ overrides it), shortened to 16 while the launch is small
-// (fewer than 2048 workgroups); each chunk recomputes two planes of the first
-// step, so shorter chunks cost more than they gain in occupancy.
+// (BZH_PAIR_KZ overrides it), shortened to 16 while a box has fewer than 1024
+// workgroups (the x-edge box shares its launch with the x-interior one); each
+// chunk recomputes two planes of the first step, so shorter chunks cost more
+// than they gain in balance.
 FusedSplit pair_split(const Grid& g, int L) {
   const int W = kPairCols, TY = kPairRows;
   const int Tx = (g.nx + 1 + W - 1) / W, Ty = (g.ny + 1 + TY - 1) / TY, planes = g.nz + 1;
@@ -762,7 +763,7 @@ FusedSplit pair_split(const Grid& g, int L) {
     if (tiles == 0) continue;
     int kz = kz_env > 0 ? kz_env : 32;
     if (kz_env == 0)
-      while (kz > 16 && tiles * ((planes + kz - 1) / kz) < 2048) kz = std::max(16, kz / 2);
+      while (kz > 16 && tiles * ((planes + kz - 1) / kz) < 1024) kz = std::max(16, kz / 2);
     b->kz = kz;
     b->nzc = (planes + kz - 1) / kz;
   }
@@ -1161,13 +1162,32 @@ bool run_program(hipStream_t s, ProgramIn& in) {
         pp.mbox = pws.boxes;
         pp.mbox_n = mon_m;
       }
+      // one launch: the x-interior box's blocks, then the x-edge box's
       const FusedSplit sp = pair_split(g, pp.f.L);
-      for (int v = 0; v < 2; ++v) {
-        const FusedBox& b = v == 0 ? sp.interior : sp.edge;
-        if (b.ntx == 0 || b.nty == 0 || b.nzc == 0) continue;
-        set_box(pp.f, b, v);
-        launches.push_back(make_launch(Kind::kPair, 0, g.dtype, pp));
+      auto blocks = [](const FusedBox& b, int band) {
+        const int64_t tiles = static_cast<int64_t>(b.ntx) * b.nty;
+        return static_cast<int32_t>((band > 0 ? 8 * int64_t{band} : tiles) * b.nzc);
+      };
+      set_box(pp.f, sp.interior, 0);
+      const FusedBox& eb = sp.edge;
+      pp.e_tx0 = eb.tx0;
+      pp.e_ntx = eb.ntx;
+      pp.e_tsplit = eb.tsplit;
+      pp.e_tskip = eb.tskip;
+      pp.e_kz = eb.kz;
+      pp.e_nzc = eb.nzc;
+      pp.e_band = fused_banded() ? (eb.ntx * eb.nty + 7) / 8 : 0;
+      pp.nblk_int = (sp.interior.ntx > 0 && sp.interior.nty > 0) ? blocks(sp.interior, pp.f.box_band) : 0;
+      pp.nblk_edge = (eb.ntx > 0 && eb.nty > 0) ? blocks(eb, pp.e_band) : 0;
+      if (pp.nblk_int == 0) {
+        pp.f.box_ntx = 0;
+        pp.f.box_kz = std::max(pp.f.box_kz, 1);
       }
+      if (pp.nblk_edge == 0) {
+        pp.e_ntx = 0;
+        pp.e_kz = std::max(pp.e_kz, 1);
+      }
+      launches.push_back(make_launch(Kind::kPair, 0, g.dtype, pp));
       if (in.has_monitors) add_dft(k, cbank);
       for (int c = 0; c < 3 && 6 + c < in.ngroups; ++c)
         add_source_launches(launches, in.groups[6 + c], 2, c, nxt, g, in.metallic, step0, k + 1);

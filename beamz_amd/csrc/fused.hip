@@ -104,6 +104,37 @@ __device__ __forceinline__ bool decode_tile(const FusedParams& f, int& bx, int& 
   return true;
 }
 
+// Work item of a pair workgroup: block ``blk`` of the part (x-interior box
+// of f, or the x-edge box of the pair parameters) with ``nblk`` blocks.
+__device__ __forceinline__ bool decode_pair_tile(const PairParams& pp, bool edge, int blk,
+                                                 int nblk, int& bx, int& by, int& kz0,
+                                                 int& kz1) {
+  const FusedParams& f = pp.f;
+  const int tx0 = edge ? pp.e_tx0 : f.box_tx0, ntx = edge ? pp.e_ntx : f.box_ntx;
+  const int tsplit = edge ? pp.e_tsplit : f.box_tsplit, tskip = edge ? pp.e_tskip : f.box_tskip;
+  const int kz = edge ? pp.e_kz : f.box_kz, nzc = edge ? pp.e_nzc : f.box_nzc;
+  const int band = edge ? pp.e_band : f.box_band;
+  const int lam = xcd_remap(blk, nblk);
+  int t, zc;
+  if (band > 0) {
+    const int per = band * nzc;
+    const int c = lam / per, rem = lam - c * per;
+    zc = rem / band;
+    t = c * band + (rem - zc * band);
+    if (t >= ntx * f.box_nty) return false;
+  } else {
+    t = lam % (ntx * f.box_nty);
+    zc = lam / (ntx * f.box_nty);
+  }
+  bx = t % ntx;
+  by = f.box_ty0 + t / ntx;
+  if (bx >= tsplit) bx += tskip;
+  bx += tx0;
+  kz0 = f.box_z0 + zc * kz;
+  kz1 = min(kz0 + kz, f.box_z1);
+  return true;
+}
+
 struct Coef {
   float a, b, ik;
 };
@@ -587,21 +618,30 @@ void launch_field(hipStream_t s, const FusedParams& p, int nblocks) {
 // slabs (after it), in the order the single-step schedule applies them.
 // Arithmetic per update is the single-step kernel's; in half precision the
 // intermediate fields are not rounded to the storage type.
+constexpr int kPairWaves = kPairRows + 3;
+// LDS rows of the pair kernel, shared by both of its variants.
+struct PairLds {
+  float e1[2][kPairWaves + 1][2][64];  // E rows (Ez, Ex) for H'
+  float h1[2][kPairWaves][2][64];      // H' rows (Hz, Hx) for E'
+  float e2[2][kPairWaves][2][64];      // E' rows for H''
+  float h2[2][kPairWaves][2][64];      // H'' rows for E''
+};
+
 template <typename T, typename P, typename C, bool kEArr, bool kXEdge>
-__device__ __forceinline__ void pair_body(const PairParams& pp) {
+__device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, int blk, int nblk) {
   const FusedParams& f = pp.f;
   const SlabArgs<kXEdge> sa{f};  // x-edge tiles: reloaded (x-slab terms raise the pressure)
-  constexpr int kW = kPairRows + 3;
-  __shared__ float lE1[2][kW + 1][2][64];  // E rows (Ez, Ex) for H'
-  __shared__ float lH1[2][kW][2][64];      // H' rows (Hz, Hx) for E'
-  __shared__ float lE2[2][kW][2][64];      // E' rows for H''
-  __shared__ float lH2[2][kW][2][64];      // H'' rows for E''
+  constexpr int kW = kPairWaves;
+  auto& lE1 = lds.e1;
+  auto& lH1 = lds.h1;
+  auto& lE2 = lds.e2;
+  auto& lH2 = lds.h2;
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const bool halo = w == 0;  // computes H' (Hx, Hz) only and loads the top E row
 
   int bx, by, kz0, kz1;
-  if (!decode_tile(f, bx, by, kz0, kz1)) return;
+  if (!decode_pair_tile(pp, kXEdge, blk, nblk, bx, by, kz0, kz1)) return;
 
   const int nz = f.nz, ny = f.ny, nx = f.nx, L = f.L, L2 = 2 * f.L;
   const int m = f.metallic;
@@ -1052,26 +1092,24 @@ __device__ __forceinline__ void pair_body(const PairParams& pp) {
   }
 }
 
+// One launch for both boxes: the x-interior workgroups first, then the
+// x-edge ones (both variants run one 16-wave workgroup per CU, so merging
+// costs no occupancy and the two boxes overlap instead of running back to
+// back).
 template <typename T, typename P, typename C, bool kEArr>
-__global__ void __launch_bounds__(64 * (kPairRows + 3)) pair_interior(PairParams p) {
-  pair_body<T, P, C, kEArr, false>(p);
-}
-template <typename T, typename P, typename C, bool kEArr>
-__global__ void __launch_bounds__(64 * (kPairRows + 3)) pair_edge(PairParams p) {
-  pair_body<T, P, C, kEArr, true>(p);
+__global__ void __launch_bounds__(64 * kPairWaves) pair_step(PairParams p) {
+  __shared__ PairLds lds;
+  const int b = blockIdx.x;
+  if (b < p.nblk_int) pair_body<T, P, C, kEArr, false>(p, lds, b, p.nblk_int);
+  else pair_body<T, P, C, kEArr, true>(p, lds, b - p.nblk_int, p.nblk_edge);
 }
 
 template <typename T, typename P, typename C>
 void launch_pair_wg(hipStream_t s, const PairParams& p, int nblocks) {
-  const dim3 block(64 * (kPairRows + 3), 1, 1), grid(nblocks, 1, 1);
+  const dim3 block(64 * kPairWaves, 1, 1), grid(nblocks, 1, 1);
   const bool earr = p.f.esrc_st[0] != 0;
-  if (p.f.xedge) {
-    if (earr) hipLaunchKernelGGL((pair_edge<T, P, C, true>), grid, block, 0, s, p);
-    else hipLaunchKernelGGL((pair_edge<T, P, float, false>), grid, block, 0, s, p);
-  } else {
-    if (earr) hipLaunchKernelGGL((pair_interior<T, P, C, true>), grid, block, 0, s, p);
-    else hipLaunchKernelGGL((pair_interior<T, P, float, false>), grid, block, 0, s, p);
-  }
+  if (earr) hipLaunchKernelGGL((pair_step<T, P, C, true>), grid, block, 0, s, p);
+  else hipLaunchKernelGGL((pair_step<T, P, float, false>), grid, block, 0, s, p);
 }
 
 template <typename T, typename P>
@@ -1141,18 +1179,33 @@ void launch_fused3d(hipStream_t s, const FusedParams& p, int dtype) {
 
 void launch_pair3d(hipStream_t s, const PairParams& pp, int dtype) {
   const FusedParams& p = pp.f;
-  BZH_REQUIRE(p.box_ntx >= 0 && p.box_nty >= 0 && p.box_kz >= 1 && p.box_nzc >= 0,
-              "pair box is malformed");
   BZH_REQUIRE(p.rows == kPairRows, "pair tiles are cut with the pair tile height");
   BZH_REQUIRE(!p.coef_f16 || dtype != BZH_F32, "float16 coefficients need half-precision fields");
   const int ntx_total = (p.nx + 1 + kPairCols - 1) / kPairCols;
   const int nty_total = (p.ny + 1 + kPairRows - 1) / kPairRows;
-  BZH_REQUIRE(p.box_tx0 >= 0 && p.box_tsplit >= 0 && p.box_tskip >= 0 &&
-                  p.box_tx0 + p.box_ntx + p.box_tskip <= ntx_total && p.box_ty0 >= 0 &&
-                  p.box_ty0 + p.box_nty <= nty_total && p.box_z0 >= 0 &&
-                  p.box_z1 <= p.nz + 1 && p.box_z0 + (p.box_nzc - 1) * p.box_kz < p.box_z1,
+  BZH_REQUIRE(p.box_ty0 >= 0 && p.box_nty >= 0 && p.box_ty0 + p.box_nty <= nty_total &&
+                  p.box_z0 >= 0 && p.box_z1 <= p.nz + 1,
               "pair box exceeds the grid");
-  if (!p.xedge) {
+  // each part: tiles inside the grid, z chunks inside [z0, z1), enough blocks
+  struct Part {
+    int tx0, ntx, tsplit, tskip, kz, nzc, band, nblk;
+  };
+  const Part parts[2] = {
+      {p.box_tx0, p.box_ntx, p.box_tsplit, p.box_tskip, p.box_kz, p.box_nzc, p.box_band, pp.nblk_int},
+      {pp.e_tx0, pp.e_ntx, pp.e_tsplit, pp.e_tskip, pp.e_kz, pp.e_nzc, pp.e_band, pp.nblk_edge}};
+  for (const Part& q : parts) {
+    BZH_REQUIRE(q.ntx >= 0 && q.kz >= 1 && q.nzc >= 0 && q.tx0 >= 0 && q.tsplit >= 0 &&
+                    q.tskip >= 0 && q.tx0 + q.ntx + q.tskip <= ntx_total &&
+                    p.box_z0 + (q.nzc - 1) * q.kz < p.box_z1,
+                "pair box exceeds the grid");
+    const int64_t tiles = static_cast<int64_t>(q.ntx) * p.box_nty;
+    BZH_REQUIRE(q.band == 0 || (q.band >= 1 && 8 * int64_t{q.band} >= tiles),
+                "pair bands do not cover the tiles");
+    const int64_t need = (q.band > 0 ? 8 * int64_t{q.band} : tiles) * q.nzc;
+    BZH_REQUIRE(q.nblk == need, "pair block count does not match its box");
+  }
+  BZH_REQUIRE(pp.nblk_int % 8 == 0, "the x-interior part must be a multiple of 8 blocks");
+  if (pp.nblk_int > 0) {
     // the interior variant reads lanes x = i0-2 .. i0+61 unchecked and assumes
     // no x face or x CPML slab under them
     const int M = p.L > 0 ? p.L : 1;
@@ -1182,10 +1235,7 @@ void launch_pair3d(hipStream_t s, const PairParams& pp, int dtype) {
   BZH_REQUIRE(pp.mbox_n >= 0 && pp.mbox_n <= kMaxMonBoxes &&
                   (pp.mbox_n == 0 || (pp.mbox != nullptr && pp.cmid != nullptr)),
               "pair monitor boxes are malformed");
-  const int64_t tiles = static_cast<int64_t>(p.box_ntx) * p.box_nty;
-  BZH_REQUIRE(p.box_band == 0 || (p.box_band >= 1 && 8 * int64_t{p.box_band} >= tiles),
-              "pair bands do not cover the tiles");
-  const int64_t nblocks = (p.box_band > 0 ? 8 * int64_t{p.box_band} : tiles) * p.box_nzc;
+  const int64_t nblocks = int64_t{pp.nblk_int} + pp.nblk_edge;
   BZH_REQUIRE(nblocks < (int64_t{1} << 31), "pair grid too large");
   if (nblocks <= 0) return;
   switch (dtype) {
@@ -1196,5 +1246,4 @@ void launch_pair3d(hipStream_t s, const PairParams& pp, int dtype) {
   }
   BZH_HIP(hipGetLastError());
 }
-
 }  // namespace bzh

@@ -358,9 +358,9 @@ def main():
 
     pairs = native.last_program_pairs() > 0
     # dominant kernel: the step update launched alone (no sources/DFT), timed
-    # with events on its own stream.  One launch advances one full step, or two
-    # where the pair kernel runs; algorithmic bytes per launch = that many full
-    # steps, and the launch time is the x-interior plus x-edge kernel
+    # with events on its own stream.  One launch advances one full step (the
+    # x-interior plus x-edge kernels), or two where the pair kernel runs;
+    # algorithmic bytes per launch = that many full steps
     per_launch = 2 if pairs else 1
     nb = algorithmic_bytes(prog, state) * per_launch
     k_ms = runtime.time_step_kernel(state, prog, launches=20, stream=stream) * per_launch
