@@ -60,6 +60,35 @@ class SimulationResults:
     program: object
     performance: SimulationPerformance
     termination: object = None   # RunTermination of an automatically terminated run
+    sources: tuple = ()          # the simulation's sources, for source normalization
+    t0: float = 0.0              # start of the simulation time grid t0 + i dt, i < steps
+    steps: int = 0
+
+    def normalization(self, index=0, source=None):
+        """Source normalization at monitor ``index``'s frequencies
+        (``observe.py:237-247``), or None when no source has a spectrum."""
+        from .normalization import source_normalization
+        freqs = np.asarray(self.program.monitors[index].freqs, dtype=float)
+        return source_normalization(self.sources, freqs, self.t0, self.program.dt,
+                                    self.steps, source=source)
+
+    def normalized_flux(self, index=0, source=None):
+        """Flux divided by the source's power norm |S(f)|^2
+        (``observe.py:271-312``); the raw flux when there is no normalization."""
+        from .normalization import scale_frequency_axis
+        flux = self.monitor_flux(index)
+        norm = self.normalization(index, source)
+        return flux if norm is None else scale_frequency_axis(flux, norm.power_norm)
+
+    def normalized_phasors(self, index=0, source=None):
+        """Monitor phasors divided by the source's field-amplitude norm S(f)."""
+        from .normalization import scale_frequency_axis
+        ph = self.monitor_phasors(index)
+        norm = self.normalization(index, source)
+        if norm is None:
+            return ph
+        return {c: (None if v is None else scale_frequency_axis(v, norm.field_amplitude_norm))
+                for c, v in ph.items()}
 
     def monitor_flux(self, index=0):
         m = self.program.monitors[index]
@@ -147,11 +176,13 @@ class Simulation:
             state, report, runtime_s = run_until_terminated(self, termination, state)
             steps = report.steps
             perf = SimulationPerformance(prog.cells, steps, runtime_s)
-            return SimulationResults(state, prog, perf, termination=report)
+            return SimulationResults(state, prog, perf, termination=report,
+                                     sources=tuple(self.sources), t0=self.t0, steps=self.steps)
         state = self.initial_state() if state is None else state
         torch.cuda.synchronize()
         t0 = _time.perf_counter()
         state = runtime.advance(state, prog, self.steps, chunk=chunk)
         torch.cuda.synchronize()
         perf = SimulationPerformance(prog.cells, self.steps, _time.perf_counter() - t0)
-        return SimulationResults(state, prog, perf)
+        return SimulationResults(state, prog, perf, sources=tuple(self.sources), t0=self.t0,
+                                 steps=self.steps)

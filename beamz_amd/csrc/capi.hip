@@ -1164,9 +1164,12 @@ bool run_program(hipStream_t s, ProgramIn& in) {
       }
       // one launch: the x-interior box's blocks, then the x-edge box's
       const FusedSplit sp = pair_split(g, pp.f.L);
+      // (without bands the block count is padded to a multiple of 8; the
+      // kernel drops the padding blocks)
       auto blocks = [](const FusedBox& b, int band) {
         const int64_t tiles = static_cast<int64_t>(b.ntx) * b.nty;
-        return static_cast<int32_t>((band > 0 ? 8 * int64_t{band} : tiles) * b.nzc);
+        const int64_t n = (band > 0 ? 8 * int64_t{band} : tiles) * b.nzc;
+        return static_cast<int32_t>((n + 7) / 8 * 8);
       };
       set_box(pp.f, sp.interior, 0);
       const FusedBox& eb = sp.edge;

@@ -125,6 +125,7 @@ __device__ __forceinline__ bool decode_pair_tile(const PairParams& pp, bool edge
   } else {
     t = lam % (ntx * f.box_nty);
     zc = lam / (ntx * f.box_nty);
+    if (zc >= nzc) return false;  // padding up to a multiple of 8 blocks
   }
   bx = t % ntx;
   by = f.box_ty0 + t / ntx;
@@ -1202,7 +1203,8 @@ void launch_pair3d(hipStream_t s, const PairParams& pp, int dtype) {
     BZH_REQUIRE(q.band == 0 || (q.band >= 1 && 8 * int64_t{q.band} >= tiles),
                 "pair bands do not cover the tiles");
     const int64_t need = (q.band > 0 ? 8 * int64_t{q.band} : tiles) * q.nzc;
-    BZH_REQUIRE(q.nblk == need, "pair block count does not match its box");
+    BZH_REQUIRE(q.nblk == need || (q.band == 0 && q.nblk == (need + 7) / 8 * 8),
+                "pair block count does not match its box");
   }
   BZH_REQUIRE(pp.nblk_int % 8 == 0, "the x-interior part must be a multiple of 8 blocks");
   if (pp.nblk_int > 0) {
