@@ -75,7 +75,18 @@ class ModeSource:
         f0 = getattr(self.source_time, "freq0", None)
         if f0 is None or not np.isfinite(float(f0)) or float(f0) <= 0.0:
             raise ValueError("ModeSource source_time must define a positive freq0.")
-        self.axis  # validates the size
+        if any(not np.isfinite(float(s)) or float(s) < 0.0 for s in self.size):
+            raise ValueError("ModeSource size must be non-negative and finite.")
+        if not any(float(s) == 0.0 for s in self.size):
+            raise ValueError("ModeSource size needs a zero extent (the injection axis).")
+
+    @property
+    def axis3d(self) -> str:
+        """The injection axis of a 3D source: its single zero extent."""
+        zero = [a for a, s in zip("xyz", self.size) if float(s) == 0.0]
+        if len(zero) != 1:
+            raise ValueError("a 3D ModeSource needs exactly one zero extent")
+        return zero[0]
 
     @property
     def axis(self) -> str:

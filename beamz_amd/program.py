@@ -217,8 +217,11 @@ def compile_program(permittivity, resolution, dt, *, boundaries=(), sources=(), 
             customs.append(s)
         elif isinstance(s, ModeSource):
             if is_3d:
-                raise ValueError("mode sources are lowered for 2D (TMxy) simulations")
-            customs.extend(lower_mode_source_2d(s, eps_site["Ez"], resolution, dt, t0, num_steps))
+                from .modes3d import lower_mode_source_3d
+                customs.extend(lower_mode_source_3d(s, eps_site, tuple(eps.shape), resolution, dt,
+                                                    t0, num_steps))
+            else:
+                customs.extend(lower_mode_source_2d(s, eps_site["Ez"], resolution, dt, t0, num_steps))
         else:
             raise TypeError(f"unsupported source {type(s).__name__}")
 
