@@ -63,6 +63,7 @@ class CompiledMonitor:
     weight_offset: int = 0
     index: int = 0             # the normal (E-node) index of the plane or line
     cell_range: tuple = ()     # 2D: the tangential cell interval of the line samples
+    ranges: tuple = ()         # 3D: ((axis, (start, stop)), ...) of the plane's tangential cells
 
 
 def lower_monitor(mon: FluxMonitor, grid, shapes, dx, index):
@@ -91,13 +92,14 @@ def lower_monitor(mon: FluxMonitor, grid, shapes, dx, index):
             plans[c] = line_gather_plan(c, shapes[c], dx, mon.normal, mon.index, rng) if active else None
         measure = dx  # _line_integral_scale_2d (monitors.py:61-68)
     cell_range = rng if not is_3d else ()
+    plane_ranges = tuple((a, tuple(int(v) for v in ranges[a])) for a in tangential) if is_3d else ()
     points = next(p[0].shape[0] for p in plans.values() if p is not None)
     return CompiledMonitor(
         name=mon.name or f"monitor_{index}", freqs=freqs, plans=plans, points=points,
         normal=mon.normal, normal_sign=float(mon.normal_sign), measure=measure,
         interval=max(1, int(mon.interval)), t_start=float(mon.t_start), t_end=float(mon.t_end),
         window_code=1 if mon.window == "hann" else 0, index=int(mon.index),
-        cell_range=tuple(int(v) for v in cell_range))
+        cell_range=tuple(int(v) for v in cell_range), ranges=plane_ranges)
 
 
 def assign_offsets(compiled):

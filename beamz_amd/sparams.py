@@ -1,4 +1,4 @@
-"""Modal S-parameters of 2D (TMxy) simulations from DFT line monitors.
+"""Modal S-parameters from DFT monitors: 2D (TMxy) lines and 3D planes.
 
 Restates the reference's extraction for the TM branch of a 2D simulation:
 ``beamz/analysis/sparameters.py:164-423`` (port waves), ``424-566``
@@ -28,9 +28,13 @@ import numpy as np
 
 from .constants import LIGHT_SPEED, MU_0
 from .modes import solve_line_modes
+from .modes3d import _AXIS_POS, _cyclic, _plane_index, _to_bc, solve_plane_modes
 
 # (E component, H component, signed flux sign) of a TM port by axis
 _TM_PARTS = {"x": ("Ez", "Hy", -1.0), "y": ("Ez", "Hx", 1.0)}
+# (E_b, E_c, H_b, H_c) of a 3D port, (a, b, c) cyclic
+_PARTS_3D = {"x": ("Ey", "Ez", "Hy", "Hz"), "y": ("Ez", "Ex", "Hz", "Hx"),
+             "z": ("Ex", "Ey", "Hx", "Hy")}
 
 
 @dataclass(frozen=True)
@@ -46,12 +50,12 @@ class Port:
     polarization: str = "tm"
 
     def __post_init__(self):
-        if self.axis not in _TM_PARTS:
-            raise ValueError("a 2D port is normal to x or y")
+        if self.axis not in _PARTS_3D:
+            raise ValueError("a port is normal to x, y or z")
         if self.direction not in ("+", "-"):
             raise ValueError("port direction must be '+' or '-'")
-        if self.polarization != "tm":
-            raise ValueError("2D TMxy simulations carry tm ports only")
+        if self.polarization not in ("tm", "vector"):
+            raise ValueError("ports are 'tm' (2D TMxy) or 'vector' (3D)")
 
 
 @dataclass(frozen=True)
@@ -115,6 +119,10 @@ def port_waves(results, port: Port, frequencies):
     mon = prog.monitors[idx]
     if mon.normal != port.axis:
         raise ValueError(f"port '{port.name}' axis {port.axis} does not match its monitor")
+    if np.ndim(prog.eps_site["Ez"]) == 3:
+        return _port_waves_3d(results, port, frequencies, idx, mon)
+    if port.axis not in _TM_PARTS:
+        raise ValueError("a 2D port is normal to x or y")
     e_comp, h_comp, _ = _TM_PARTS[port.axis]
     ph = results.monitor_phasors(idx)
     if ph.get(e_comp) is None or ph.get(h_comp) is None:
@@ -133,6 +141,80 @@ def port_waves(results, port: Port, frequencies):
             raise ValueError(f"monitor '{port.monitor}' holds no DFT at {f} Hz")
         proj = port_projection(e_line, f, prog.resolution, points, port.axis, port.mode_index)
         vec = np.concatenate([ph[e_comp][j], ph[h_comp][j]])
+        a = proj["pinv"] @ vec
+        out["a_plus"][k], out["a_minus"][k] = a[0], a[1]
+        out["condition_number"][k] = proj["condition_number"]
+        out["mode_neff"][k] = proj["mode_neff"]
+    out["P_plus"] = np.abs(out["a_plus"]) ** 2
+    out["P_minus"] = np.abs(out["a_minus"]) ** 2
+    return out
+
+
+def _centres(axis, eb, ec, hb, hc):
+    """Mode fields of a Nb x Nc window averaged onto its cell centres, in the
+    3D plane monitor's point order (tangential axes in storage order)."""
+    eb_q = 0.5 * (eb[:, :-1] + eb[:, 1:])
+    hc_q = 0.5 * (hc[:, :-1] + hc[:, 1:])
+    ec_q = 0.5 * (ec[:-1, :] + ec[1:, :])
+    hb_q = 0.5 * (hb[:-1, :] + hb[1:, :])
+    _, b, c = _cyclic(axis)
+    flip = _AXIS_POS[b] > _AXIS_POS[c]   # (b, c) not in storage order
+    return [v.T.ravel() if flip else v.ravel() for v in (eb_q, ec_q, hb_q, hc_q)]
+
+
+def port_projection_3d(eps_site, grid, dx, frequency, axis, index, ranges, mode_index=0):
+    """Unit-power forward plane mode at a 3D monitor's points and the
+    pseudo-inverse of [[e, e], [h, -h]] (``mode_projection.py:162-432``).
+
+    The mode is solved on the monitor's window (``ranges``: tangential cell
+    intervals), bounded by PEC, and averaged onto the cell centres where the
+    monitor samples E and H; it is renormalized to unit power there."""
+    a, b, c = _cyclic(axis)
+    rng = dict(ranges)
+    (b0, b1), (c0, c1) = rng[b], rng[c]
+
+    def sl(comp_axis_b, comp_axis_c):
+        return _plane_index(axis, index, slice(b0, b1 + comp_axis_b), slice(c0, c1 + comp_axis_c))
+
+    eps_b = _to_bc(axis, np.asarray(eps_site["E" + b])[sl(0, 1)])
+    eps_c = _to_bc(axis, np.asarray(eps_site["E" + c])[sl(1, 0)])
+    ea = np.asarray(eps_site["E" + a])
+    lo, hi = max(index - 1, 0), min(index, ea.shape[_AXIS_POS[a]] - 1)
+    eps_a = 0.5 * (_to_bc(axis, ea[_plane_index(axis, lo, slice(b0, b1 + 1), slice(c0, c1 + 1))]) +
+                   _to_bc(axis, ea[_plane_index(axis, hi, slice(b0, b1 + 1), slice(c0, c1 + 1))]))
+    omega = 2.0 * np.pi * float(frequency)
+    target = 0.98 * np.sqrt(max(float(eps_b.max()), float(eps_c.max())))
+    m = int(mode_index) + 1
+    modes = solve_plane_modes(eps_b, eps_c, eps_a, omega, dx, 2 * m + 5, target)
+    if modes.neff.size <= mode_index:
+        raise ValueError(f"the plane mode solver found {modes.neff.size} modes for mode_index={mode_index}")
+    k = int(mode_index)
+    eb, ec, hb, hc = _centres(axis, modes.eb[k], modes.ec[k], modes.hb[k], modes.hc[k])
+    power = 0.5 * float(np.sum(eb * hc) - np.sum(ec * hb)) * dx * dx
+    s = 1.0 / np.sqrt(max(abs(power), 1e-300))
+    e = np.concatenate([eb, ec]).astype(np.complex128) * s
+    h = np.concatenate([hb, hc]).astype(np.complex128) * s
+    mat = np.column_stack([np.concatenate([e, h]), np.concatenate([e, -h])])
+    return {"e": e, "h": h, "mode_matrix": mat, "pinv": np.linalg.pinv(mat),
+            "condition_number": float(np.linalg.cond(mat)), "mode_neff": float(modes.neff[k])}
+
+
+def _port_waves_3d(results, port, frequencies, idx, mon):
+    prog = results.program
+    parts = _PARTS_3D[port.axis]
+    ph = results.monitor_phasors(idx)
+    if any(ph.get(c) is None for c in parts):
+        raise ValueError(f"monitor '{port.monitor}' must record {', '.join(parts)}")
+    out = {"a_plus": np.zeros(len(frequencies), complex), "a_minus": np.zeros(len(frequencies), complex),
+           "condition_number": np.zeros(len(frequencies)), "mode_neff": np.zeros(len(frequencies))}
+    mfreqs = np.asarray(mon.freqs)
+    for k, f in enumerate(frequencies):
+        j = int(np.argmin(np.abs(mfreqs - f)))
+        if not np.isclose(mfreqs[j], f, rtol=1e-12, atol=0.0):
+            raise ValueError(f"monitor '{port.monitor}' holds no DFT at {f} Hz")
+        proj = port_projection_3d(prog.eps_site, prog.grid, prog.resolution, f, port.axis, mon.index,
+                                  mon.ranges, port.mode_index)
+        vec = np.concatenate([ph[c][j] for c in parts])
         a = proj["pinv"] @ vec
         out["a_plus"][k], out["a_minus"][k] = a[0], a[1]
         out["condition_number"][k] = proj["condition_number"]

@@ -40,3 +40,40 @@ def test_straight_guide_s_parameters():
     assert np.all(np.abs(s21 - 1.0) < 0.03), s21
     assert np.all(s11 < 0.05), s11   # measured 0.017-0.025: the far CPML reflects
     np.testing.assert_allclose(s21 ** 2, ratio, rtol=0.03)
+
+
+def test_straight_strip_s_parameters_3d():
+    """A 450 x 220 nm silicon strip along x (25 nm cells, 10-cell CPML), its
+    TE-like mode launched by a 3D mode source: vector plane-mode projection at
+    two ports gives |S21| near one and |S11| small, and |S21|^2 agrees with
+    the ratio of the monitors' fluxes."""
+    if not torch.cuda.is_available():
+        pytest.fail("GPU tests need a GPU")
+    import beamz_amd as bz
+    from beamz_amd.sparams import Port, s_parameters
+    dx = 25e-9
+    nz, ny, nx = 56, 64, 170
+    eps = np.full((nz, ny, nx), 2.085)
+    eps[24:33, 23:41, :] = 12.25
+    f0 = bz.LIGHT_SPEED / 1.55e-6
+    freqs = f0 * np.array([0.97, 1.0, 1.03])
+    src = bz.ModeSource((40.5 * dx, 32 * dx, 28.5 * dx), (0, 44 * dx, 36 * dx),
+                        bz.GaussianPulse(f0, bz.LIGHT_SPEED * 0.4e-6 / 1.55e-6 ** 2), "+")
+    rng = {"z": (10, 46), "y": (10, 54)}
+    mons = [bz.FluxMonitor("x", 70, freqs, ranges=rng, name="m_in"),
+            bz.FluxMonitor("x", 130, freqs, ranges=rng, name="m_out")]
+    sim = bz.Simulation(eps, dx, boundaries=[bz.PML(cells=10)], sources=[src], monitors=mons,
+                        steps=4000)
+    print(f"case sparams 3D strip grid={(nz, ny, nx)} steps=4000", flush=True)
+    res = sim.run()
+    ports = [Port("in", "m_in", "x", "+", polarization="vector"),
+             Port("out", "m_out", "x", "-", polarization="vector")]
+    out = s_parameters(res, "in", ports)
+    s21 = np.abs(out.s_matrix[("out", "in")])
+    s11 = np.abs(out.s_matrix[("in", "in")])
+    ratio = res.monitor_flux(1) / res.monitor_flux(0)
+    print(f"  |S21| {s21}  |S11| {s11}  flux ratio {ratio}  neff "
+          f"{out.diagnostics['waves']['in']['mode_neff']}", flush=True)
+    assert np.all(np.abs(s21 - 1.0) < 0.03), s21   # measured 0.9986-0.9989
+    assert np.all(s11 < 0.05), s11                  # measured 0.007-0.013
+    np.testing.assert_allclose(s21 ** 2, ratio, rtol=0.03)

@@ -68,3 +68,34 @@ def test_s_parameters_from_synthetic_port_fields():
     np.testing.assert_allclose(out.diagnostics["power_sum"], 0.65, atol=1e-9)
     with pytest.raises(ValueError):
         s_parameters(res, "missing", ports)
+
+
+@pytest.mark.parametrize("axis", ["x", "y", "z"])
+def test_3d_projection_recovers_planted_amplitudes(axis):
+    """A 3D port on a strip guide along ``axis``: the unit-power plane mode
+    averaged onto the monitor's cell centres, and planted forward/backward
+    amplitudes recovered from its fields."""
+    from beamz_amd.lattice import sample_e_site_3d
+    from beamz_amd.sparams import port_projection_3d
+    dx = 40e-9
+    n = {"z": 20, "y": 24, "x": 28}
+    grid = (n["z"], n["y"], n["x"])
+    eps = np.full(grid, 2.085)
+    core = {"x": slice(8, 20), "y": slice(6, 18), "z": slice(6, 14)}
+    sl = [core["z"], core["y"], core["x"]]
+    sl[{"z": 0, "y": 1, "x": 2}[axis]] = slice(None)
+    eps[tuple(sl)] = 12.25
+    site = {c: sample_e_site_3d(eps, c) for c in ("Ex", "Ey", "Ez")}
+    tang = {"x": ("z", "y"), "y": ("z", "x"), "z": ("y", "x")}[axis]
+    ranges = tuple((a, (2, n[a] - 2)) for a in tang)
+    proj = port_projection_3d(site, grid, dx, F0, axis, n[axis] // 2, ranges)
+    e, h = proj["e"], proj["h"]
+    P = (n[tang[0]] - 4) * (n[tang[1]] - 4)
+    assert e.size == 2 * P and h.size == 2 * P
+    # unit power towards +axis: 1/2 sum(E_b H_c - E_c H_b) dA
+    assert 0.5 * np.real(np.sum(e[:P] * np.conj(h[P:])) - np.sum(e[P:] * np.conj(h[:P]))) * dx * dx \
+        == pytest.approx(1.0, rel=1e-12)
+    fwd, bwd = 0.6 + 0.3j, -0.1 + 0.05j
+    vec = np.concatenate([fwd * e + bwd * e, fwd * h - bwd * h])
+    np.testing.assert_allclose(proj["pinv"] @ vec, [fwd, bwd], rtol=1e-10, atol=1e-12)
+    assert np.sqrt(2.085) < proj["mode_neff"] < 3.5   # guided
