@@ -741,7 +741,8 @@ FusedSplit fused_split(const Grid& g, int L) {
 
 // Work split of the pair step (two steps per launch): tiles of 60 x 13
 // owned cells whose lanes cover x = 60 t - 2 .. 60 t + 61; interior tiles
-// have all of them in [M, nx - M - 1].  z chunks of 32 planes
+// have all of them in [M, nx - M - 1].  z chunks of 48 planes (32: 3.35 ms,
+// 64: 3.32 ms against 3.29 ms per launch at 1024x512x256)
 // (BZH_PAIR_KZ overrides it), shortened to 16 while a box has fewer than 1024
 // workgroups (the x-edge box shares its launch with the x-interior one); each
 // chunk recomputes two planes of the first step, so shorter chunks cost more
@@ -761,7 +762,7 @@ FusedSplit pair_split(const Grid& g, int L) {
   for (FusedBox* b : {&sp.interior, &sp.edge}) {
     const int64_t tiles = static_cast<int64_t>(b->ntx) * b->nty;
     if (tiles == 0) continue;
-    int kz = kz_env > 0 ? kz_env : 32;
+    int kz = kz_env > 0 ? kz_env : 48;
     if (kz_env == 0)
       while (kz > 16 && tiles * ((planes + kz - 1) / kz) < 1024) kz = std::max(16, kz / 2);
     b->kz = kz;

@@ -191,10 +191,10 @@ struct PairParams {
   char* cmid;
   const int32_t* mbox;
   int32_t mbox_n;
-  // One launch covers both boxes: workgroups [0, nblk_int) run the x-interior
-  // box of ``f`` (box_* fields), the next nblk_edge the x-edge box below (same
-  // rows and z range).  nblk_int is a multiple of 8, so both parts deal their
-  // tiles to the XCDs from XCD 0.
+  // One launch covers both boxes: workgroups [0, nblk_edge) run the x-edge
+  // box below (same rows and z range), the next nblk_int the x-interior box
+  // of ``f`` (box_* fields).  nblk_edge is a multiple of 8, so both parts deal
+  // their tiles to the XCDs from XCD 0.
   int32_t nblk_int, nblk_edge;
   int32_t e_tx0, e_ntx, e_tsplit, e_tskip, e_kz, e_nzc, e_band;
 };

@@ -944,6 +944,17 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, in
     }
   };
 
+  // does any slab between the steps cross this wave's row and lanes?  (its
+  // planes are tested per k inside mid_sources)
+  bool swave = false;
+  for (int s = 0; s < pair_kargs()->msrc_n; ++s) {
+    const auto& ms = pair_kargs()->msrc[s];
+    cint* st = (cint*)(ms.start);
+    const int cy = y - st[1], cx = x - st[2];
+    const bool hit = cy >= 0 && cy < ms.sy && cx >= 0 && cx < ms.sx;
+    swave = swave || __builtin_amdgcn_readfirstlane(__any(hit)) != 0;
+  }
+
   const int ks = max(kz0 - 2, 0);
   const int kE = max(kz0 - 1, 0);  // first plane of E' the chunk needs
   float ex, ey, ez;                // E(k) on this row
@@ -1023,9 +1034,9 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, in
       const float hx_dn = lH1[bb][w - 1][1][lane];
       e_update(k, hx, hy, hz, hz_dn, hx_dn, h1x, h1y, ex, ey, ez, s0, s1, s2, pe, q2, nex, ney,
                nez);
-      if (npost > 0) mid_sources(0, npost, k, nex, ney, nez);
+      if (swave && npost > 0) mid_sources(0, npost, k, nex, ney, nez);
       if (sampled) store_e(pair_kargs()->cmid, k, nex, ney, nez);
-      if (nsrc > npost) mid_sources(npost, nsrc, k, nex, ney, nez);
+      if (swave && nsrc > npost) mid_sources(npost, nsrc, k, nex, ney, nez);
       lE2[bb][w][0][lane] = nez;
       lE2[bb][w][1][lane] = nex;
     }
@@ -1093,16 +1104,16 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, in
   }
 }
 
-// One launch for both boxes: the x-interior workgroups first, then the
-// x-edge ones (both variants run one 16-wave workgroup per CU, so merging
-// costs no occupancy and the two boxes overlap instead of running back to
-// back).
+// One launch for both boxes: the x-edge workgroups first (the slower ones,
+// so that they do not form the tail), then the x-interior ones.  Both
+// variants run one 16-wave workgroup per CU, so merging costs no occupancy and
+// the two boxes overlap instead of running back to back.
 template <typename T, typename P, typename C, bool kEArr>
 __global__ void __launch_bounds__(64 * kPairWaves) pair_step(PairParams p) {
   __shared__ PairLds lds;
   const int b = blockIdx.x;
-  if (b < p.nblk_int) pair_body<T, P, C, kEArr, false>(p, lds, b, p.nblk_int);
-  else pair_body<T, P, C, kEArr, true>(p, lds, b - p.nblk_int, p.nblk_edge);
+  if (b < p.nblk_edge) pair_body<T, P, C, kEArr, true>(p, lds, b, p.nblk_edge);
+  else pair_body<T, P, C, kEArr, false>(p, lds, b - p.nblk_edge, p.nblk_int);
 }
 
 template <typename T, typename P, typename C>
@@ -1206,7 +1217,7 @@ void launch_pair3d(hipStream_t s, const PairParams& pp, int dtype) {
     BZH_REQUIRE(q.nblk == need || (q.band == 0 && q.nblk == (need + 7) / 8 * 8),
                 "pair block count does not match its box");
   }
-  BZH_REQUIRE(pp.nblk_int % 8 == 0, "the x-interior part must be a multiple of 8 blocks");
+  BZH_REQUIRE(pp.nblk_edge % 8 == 0, "the x-edge part must be a multiple of 8 blocks");
   if (pp.nblk_int > 0) {
     // the interior variant reads lanes x = i0-2 .. i0+61 unchecked and assumes
     // no x face or x CPML slab under them
