@@ -741,12 +741,11 @@ FusedSplit fused_split(const Grid& g, int L) {
 
 // Work split of the pair step (two steps per launch): tiles of 60 x 13
 // owned cells whose lanes cover x = 60 t - 2 .. 60 t + 61; interior tiles
-// have all of them in [M, nx - M - 1].  z chunks of 48 planes (32: 3.35 ms,
-// 64: 3.32 ms against 3.29 ms per launch at 1024x512x256)
-// (BZH_PAIR_KZ overrides it), shortened to 16 while a box has fewer than 1024
-// workgroups (the x-edge box shares its launch with the x-interior one); each
-// chunk recomputes two planes of the first step, so shorter chunks cost more
-// than they gain in balance.
+// have all of them in [M, nx - M - 1].  z chunks of 48 planes for both boxes
+// (32: 3.35 ms, 64: 3.32 ms against 3.29 ms per launch at 1024x512x256;
+// BZH_PAIR_KZ overrides it).  Each chunk recomputes two planes of the first
+// step, and the x-edge box shares its launch with the x-interior one, so its
+// chunks are not shortened to fill the GPU on their own.
 FusedSplit pair_split(const Grid& g, int L) {
   const int W = kPairCols, TY = kPairRows;
   const int Tx = (g.nx + 1 + W - 1) / W, Ty = (g.ny + 1 + TY - 1) / TY, planes = g.nz + 1;
@@ -757,14 +756,10 @@ FusedSplit pair_split(const Grid& g, int L) {
   FusedSplit sp{};
   sp.interior = {tx0, tx1 - tx0, tx1 - tx0, 0, Ty, 1, 0};
   sp.edge = {0, tx0 + (Tx - tx1), tx0, tx1 - tx0, Ty, 1, 0};
-  int kz_env = 0;
-  if (const char* env = std::getenv("BZH_PAIR_KZ")) kz_env = std::max(1, std::atoi(env));
+  int kz = 48;
+  if (const char* env = std::getenv("BZH_PAIR_KZ")) kz = std::max(1, std::atoi(env));
   for (FusedBox* b : {&sp.interior, &sp.edge}) {
-    const int64_t tiles = static_cast<int64_t>(b->ntx) * b->nty;
-    if (tiles == 0) continue;
-    int kz = kz_env > 0 ? kz_env : 48;
-    if (kz_env == 0)
-      while (kz > 16 && tiles * ((planes + kz - 1) / kz) < 1024) kz = std::max(16, kz / 2);
+    if (static_cast<int64_t>(b->ntx) * b->nty == 0) continue;
     b->kz = kz;
     b->nzc = (planes + kz - 1) / kz;
   }
