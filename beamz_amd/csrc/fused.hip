@@ -688,6 +688,7 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, in
   const int qyH = vyY ? slab_q(y, ny, L) : -1;
   const int qyE = vyX ? slab_q(y, ny + 1, L) : -1;
   const bool y_pec_h = (mb && y == 0) || (mt && y == ny);
+  const bool row_plain = y >= 1 && y < ny;  // no y face, ghost or validity cut on this row
   const int yt = j0 - 2 + kW;
   const bool vtX = yt <= ny;
   const int ytX = min(yt, ny);
@@ -783,7 +784,9 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, in
   };
 
   // H update of plane k on this row (kernels.py:458-503), psi terms 0..5
-  // from pin to pout; invalid positions come out zero.
+  // from pin to pout; invalid positions come out zero.  On planes and rows
+  // away from every face (``plain``, uniform) the face masks and the validity
+  // zeroing are no-ops and are skipped.
   auto h_update = [&](int k, float ez, float ey, float ex, float eyn, float exn, float ez_up,
                       float ex_up, float ha0, float ha1, float ha2, const float (&pin)[6],
                       float (&pout)[6], float& hx, float& hy, float& hz) {
@@ -812,28 +815,42 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, in
     hx = hc[0] * ha0 - hc[3] * (d0 - d1);
     hy = hc[1] * ha1 - hc[4] * (d2 - d3);
     hz = hc[2] * ha2 - hc[5] * (d4 - d5);
-    if (kXEdge && xface) hx = 0.0f;
-    if (y_pec_h) hy = 0.0f;
-    if ((mf && k == 0) || (mk && k == nz)) hz = 0.0f;
-    hx = (vxy && vyY && vxY) ? hx : 0.0f;
-    hy = (vxy && vyX && vxX) ? hy : 0.0f;
-    hz = (k <= nz && vyY && vxX) ? hz : 0.0f;
+    const bool plain = !kXEdge && k >= 1 && k < nz && row_plain;
+    if (!plain) {
+      if (kXEdge && xface) hx = 0.0f;
+      if (y_pec_h) hy = 0.0f;
+      if ((mf && k == 0) || (mk && k == nz)) hz = 0.0f;
+      hx = (vxy && vyY && vxY) ? hx : 0.0f;
+      hy = (vxy && vyX && vxX) ? hy : 0.0f;
+      hz = (k <= nz && vyY && vxX) ? hz : 0.0f;
+    }
   };
 
   // E update of plane k on this row (kernels.py:554-626, ghost rule
   // lattice.py:523-575), psi terms 6..11 from pin[t - 6] to pout[t - 6];
-  // invalid positions come out zero.
+  // invalid positions come out zero.  On ``plain`` planes and rows the ghost
+  // rule reduces to plain differences and the masks are no-ops.
   auto e_update = [&](int k, float hx, float hy, float hz, float hz_dn, float hx_dn, float phx,
                       float phy, float ex, float ey, float ez, float s0, float s1, float s2,
                       const float (&pin)[6], float (&pout)[6], float& nex, float& ney,
                       float& nez) {
     const int qzE = k <= nz ? slab_q(k, nz + 1, L) : -1;
+    const bool plain = !kXEdge && k >= 1 && k < nz && row_plain;
     const float hz_l = lane_shift(hz, up);
     const float hy_l = lane_shift(hy, up);
-    float e0 = gdiff(hz, hz_dn, y, ny, mb, mt) * idx;  // dHz/dy
-    float e1 = gdiff(hy, phy, k, nz, mf, mk) * idx;    // dHy/dz
-    float e2 = gdiff(hx, phx, k, nz, mf, mk) * idx;    // dHx/dz
-    float e3, e4;                                      // dHz/dx, dHy/dx
+    float e0, e1, e2, e5;
+    if (plain) {
+      e0 = (hz - hz_dn) * idx;
+      e1 = (hy - phy) * idx;
+      e2 = (hx - phx) * idx;
+      e5 = (hx - hx_dn) * idx;
+    } else {
+      e0 = gdiff(hz, hz_dn, y, ny, mb, mt) * idx;  // dHz/dy
+      e1 = gdiff(hy, phy, k, nz, mf, mk) * idx;    // dHy/dz
+      e2 = gdiff(hx, phx, k, nz, mf, mk) * idx;    // dHx/dz
+      e5 = gdiff(hx, hx_dn, y, ny, mb, mt) * idx;  // dHx/dy
+    }
+    float e3, e4;                                  // dHz/dx, dHy/dx
     if constexpr (kXEdge) {
       e3 = gdiff(hz, hz_l, x, nx, ml, mr) * idx;
       e4 = gdiff(hy, hy_l, x, nx, ml, mr) * idx;
@@ -841,7 +858,6 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, in
       e3 = (hz - hz_l) * idx;
       e4 = (hy - hy_l) * idx;
     }
-    float e5 = gdiff(hx, hx_dn, y, ny, mb, mt) * idx;  // dHx/dy
     if (qyE >= 0) {
       e0 = cpml_step(coef_u(sa.tab(), 6, L2, qyE), e0, pin[0], pout[0]);
       e5 = cpml_step(coef_u(sa.tab(), 11, L2, qyE), e5, pin[5], pout[5]);
@@ -859,14 +875,16 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, in
     nex = ec[0] * ex + s0 * (e0 - e1);
     ney = ec[1] * ey + s1 * (e2 - e3);
     nez = ec[2] * ez + s2 * (e4 - e5);
-    const bool zf = (mf && k == 0) || (mk && k == nz);
-    const bool yf = (mb && y == 0) || (mt && y == ny);
-    if (zf || yf) nex = 0.0f;
-    if (zf || (kXEdge && xface)) ney = 0.0f;
-    if (yf || (kXEdge && xface)) nez = 0.0f;
-    nex = (k <= nz && vyX && vxX) ? nex : 0.0f;
-    ney = (k <= nz && vyY && vxY) ? ney : 0.0f;
-    nez = (k < nz && vyX && vxY) ? nez : 0.0f;
+    if (!plain) {
+      const bool zf = (mf && k == 0) || (mk && k == nz);
+      const bool yf = (mb && y == 0) || (mt && y == ny);
+      if (zf || yf) nex = 0.0f;
+      if (zf || (kXEdge && xface)) ney = 0.0f;
+      if (yf || (kXEdge && xface)) nez = 0.0f;
+      nex = (k <= nz && vyX && vxX) ? nex : 0.0f;
+      ney = (k <= nz && vyY && vxY) ? ney : 0.0f;
+      nez = (k < nz && vyX && vxY) ? nez : 0.0f;
+    }
   };
 
   // stores of plane k (fields to ``out``, CPML memories to the written bank)
