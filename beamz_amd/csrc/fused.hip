@@ -700,6 +700,7 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, in
     hc[3 + q] = as_const(f.hsrc[q])[0] * f.ratio_h;
     ec[q] = as_const(f.edec[q])[0];
   }
+  const float hci[3] = {hc[3] * idx, hc[4] * idx, hc[5] * idx};
   float es0 = 0.0f, es1 = 0.0f, es2 = 0.0f;
   if constexpr (!kEArr) {
     es0 = as_const(f.esrc[0])[0] * f.ratio_e;
@@ -792,11 +793,19 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, in
                       float (&pout)[6], float& hx, float& hy, float& hz) {
     const bool vxy = k < nz;
     const int qzH = vxy ? slab_q(k, nz, L) : -1;
+    const float ez_r = lane_shift(ez, dn), ey_r = lane_shift(ey, dn);
+    if (!kXEdge && k >= 1 && k < nz && row_plain && qyH < 0 && qzH < 0) {
+      // no face and no CPML term: 1/dx folded into the coefficients
+      hx = hc[0] * ha0 - hci[0] * ((ez_up - ez) - (eyn - ey));
+      hy = hc[1] * ha1 - hci[1] * ((exn - ex) - (ez_r - ez));
+      hz = hc[2] * ha2 - hci[2] * ((ey_r - ey) - (ex_up - ex));
+      return;
+    }
     float d0 = (ez_up - ez) * idx;               // dEz/dy
     float d1 = (eyn - ey) * idx;                 // dEy/dz
     float d2 = (exn - ex) * idx;                 // dEx/dz
-    float d3 = (lane_shift(ez, dn) - ez) * idx;  // dEz/dx
-    float d4 = (lane_shift(ey, dn) - ey) * idx;  // dEy/dx
+    float d3 = (ez_r - ez) * idx;                // dEz/dx
+    float d4 = (ey_r - ey) * idx;                // dEy/dx
     float d5 = (ex_up - ex) * idx;               // dEx/dy
     if (qyH >= 0) {
       d0 = cpml_step(coef_u(sa.tab(), 0, L2, qyH), d0, pin[0], pout[0]);
@@ -838,6 +847,13 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, in
     const bool plain = !kXEdge && k >= 1 && k < nz && row_plain;
     const float hz_l = lane_shift(hz, up);
     const float hy_l = lane_shift(hy, up);
+    if (plain && qyE < 0 && qzE < 0) {
+      // no face and no CPML term: 1/dx folded into the coefficients
+      nex = ec[0] * ex + (s0 * idx) * ((hz - hz_dn) - (hy - phy));
+      ney = ec[1] * ey + (s1 * idx) * ((hx - phx) - (hz - hz_l));
+      nez = ec[2] * ez + (s2 * idx) * ((hy - hy_l) - (hx - hx_dn));
+      return;
+    }
     float e0, e1, e2, e5;
     if (plain) {
       e0 = (hz - hz_dn) * idx;
