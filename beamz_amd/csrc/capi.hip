@@ -816,9 +816,10 @@ int group_slabs(const SourceGroupIn& grp) {
   return (c && c->data && c->rank >= 1) ? static_cast<int>(c->dims[0]) : 0;
 }
 
-// Append the slabs of an E source group to the pair's mid-step list.
-void add_mid_slabs(PairParams& pp, const SourceGroupIn& grp, int comp, int kstep, bool post,
-                   int metallic) {
+// Append the slabs of a source group (E components, or H with ``hfield``) to
+// an in-kernel slab list of at most ``cap`` entries.
+void add_mid_slabs(MidSlab* list, int32_t& count, int cap, const SourceGroupIn& grp, int comp,
+                   int kstep, bool post, bool hfield, int metallic) {
   const int n = group_slabs(grp);
   if (n == 0) return;
   const bzh_buffer& c = *grp.coeffs;
@@ -827,7 +828,7 @@ void add_mid_slabs(PairParams& pp, const SourceGroupIn& grp, int comp, int kstep
                   grp.starts->element_type == BZH_S32 && grp.starts->rank == 2 &&
                   grp.starts->dims[0] == n && grp.starts->dims[1] == 3,
               "source groups are f32 [n, sz, sy, sx] coefficients, f32 [n, T] waveforms and s32 [n, 3] starts");
-  BZH_REQUIRE(pp.msrc_n + n <= kMaxMidSlabs, "too many source slabs between pair steps");
+  BZH_REQUIRE(count + n <= cap, "too many source slabs for the fused step");
   const int64_t cells = c.dims[1] * c.dims[2] * c.dims[3];
   const int T = static_cast<int>(grp.waves->dims[1]);
   for (int s = 0; s < n; ++s) {
@@ -840,11 +841,12 @@ void add_mid_slabs(PairParams& pp, const SourceGroupIn& grp, int comp, int kstep
     m.sy = static_cast<int>(c.dims[2]);
     m.sx = static_cast<int>(c.dims[3]);
     m.T = T;
+    m.hfield = hfield ? 1 : 0;
     m.comp = comp;
     m.kstep = kstep;
     m.post = post ? 1 : 0;
-    if (post) mask_ends(kEx + comp, metallic, true, m.mlo, m.mhi);
-    pp.msrc[pp.msrc_n++] = m;
+    if (post) mask_ends((hfield ? kHx : kEx) + comp, metallic, true, m.mlo, m.mhi);
+    list[count++] = m;
   }
 }
 
@@ -946,11 +948,9 @@ bool run_program(hipStream_t s, ProgramIn& in) {
     BZH_REQUIRE(gb.nz == g.nz && gb.ny == g.ny && gb.nx == g.nx && gb.dtype == g.dtype,
                 "workspace bank does not match the field bank");
   }
-  bool h_sources = false;
-  for (int c = 0; c < 3 && 3 + c < in.ngroups; ++c) {
-    const bzh_buffer* cb = in.groups[3 + c].coeffs;
-    if (cb && cb->data && cb->rank >= 1 && cb->dims[0] > 0) h_sources = true;
-  }
+  // H-timing slabs run inside the fused kernels (up to kMaxHSlabs per step)
+  int h_slabs = 0;
+  for (int c = 0; c < 3 && 3 + c < in.ngroups; ++c) h_slabs += group_slabs(in.groups[3 + c]);
   // the fused kernel takes scalar H coefficients and E decay, E source
   // coefficients that are all arrays or all scalars, and arrays addressable
   // with 32-bit element offsets; anything else runs the per-phase schedule
@@ -980,7 +980,8 @@ bool run_program(hipStream_t s, ProgramIn& in) {
     based = group_base(src, 3, cpml_probe.cbase, cpml_probe.coff);
   }
   std::memcpy(cpml_probe.foff, foff_a, sizeof(foff_a));
-  const bool fused = two_banks && (in.flags & BZH_FLAG_FUSED_STEP) && g.is3d && !h_sources &&
+  const bool fused = two_banks && (in.flags & BZH_FLAG_FUSED_STEP) && g.is3d &&
+                     h_slabs <= kMaxHSlabs && (h_slabs == 0 || in.step != nullptr) &&
                      coef_ok && based && metrics_empty(in.h) && metrics_empty(in.e) &&
                      fused_cpml_layout(in.h, in.e, in.psi_b, in.uniform_l, cpml_probe);
   const int32_t* step0 = in.step ? static_cast<const int32_t*>(in.step->data) : nullptr;
@@ -995,7 +996,7 @@ bool run_program(hipStream_t s, ProgramIn& in) {
   // source slabs between the steps fit the kernel and the monitors its boxes.
   // Each pair swaps the banks once, so an even number of pairs per call keeps
   // the single-step bank parity (result in the workspace after an odd count).
-  int mid_slabs = 0;
+  int mid_slabs = 2 * h_slabs;  // the H slabs of both steps
   for (int c = 0; c < 3; ++c) {
     if (c < in.ngroups) mid_slabs += group_slabs(in.groups[c]);
     if (6 + c < in.ngroups) mid_slabs += group_slabs(in.groups[6 + c]);
@@ -1149,10 +1150,16 @@ bool run_program(hipStream_t s, ProgramIn& in) {
       pp.f.rows = kPairRows;
       pp.step0 = step0;
       for (int c = 0; c < 3 && 6 + c < in.ngroups; ++c)
-        add_mid_slabs(pp, in.groups[6 + c], c, k, true, in.metallic);
+        add_mid_slabs(pp.msrc, pp.msrc_n, kMaxMidSlabs, in.groups[6 + c], c, k, true, false, in.metallic);
       pp.msrc_post = pp.msrc_n;
       for (int c = 0; c < 3 && c < in.ngroups; ++c)
-        add_mid_slabs(pp, in.groups[c], c, k + 1, false, in.metallic);
+        add_mid_slabs(pp.msrc, pp.msrc_n, kMaxMidSlabs, in.groups[c], c, k + 1, false, false, in.metallic);
+      pp.msrc_e = pp.msrc_n;
+      for (int c = 0; c < 3 && 3 + c < in.ngroups; ++c)
+        add_mid_slabs(pp.msrc, pp.msrc_n, kMaxMidSlabs, in.groups[3 + c], c, k, true, true, in.metallic);
+      pp.msrc_h1 = pp.msrc_n;
+      for (int c = 0; c < 3 && 3 + c < in.ngroups; ++c)
+        add_mid_slabs(pp.msrc, pp.msrc_n, kMaxMidSlabs, in.groups[3 + c], c, k + 1, true, true, in.metallic);
       if (mon_m > 0) {
         pp.cmid = pws.bank;
         pp.mbox = pws.boxes;
@@ -1198,6 +1205,9 @@ bool run_program(hipStream_t s, ProgramIn& in) {
     }
     if (fused) {
       FusedParams fp = fused_params(cur, nxt, cur_b);
+      fp.step0 = step0;
+      for (int c = 0; c < 3 && 3 + c < in.ngroups; ++c)
+        add_mid_slabs(fp.hslab, fp.hslab_n, kMaxHSlabs, in.groups[3 + c], c, k, true, true, in.metallic);
       const FusedSplit sp = fused_split(g, fp.L);
       fp.rows = fused_rows();
       for (int v = 0; v < 2; ++v) {

@@ -108,6 +108,26 @@ enum : int32_t {
   kRight = 32
 };
 
+// A source slab applied inside a fused launch (``sources`` of
+// execute.py:374-461 on one E or H component): in the pair kernel between
+// its two steps (an E-timing slab of the first step, ``post``: before the
+// intermediate monitor sample, skipping PEC-constrained cells; a pre-E slab
+// of the second) or after an H update (H-timing slabs, ``post``); in the
+// single-step kernel after its H update.
+struct MidSlab {
+  const float* coeff;     // this slab's [sz, sy, sx] coefficients (storage units)
+  const float* wave;      // this slab's waveform [T]
+  const int32_t* start;   // device int32[3]: (z, y, x) of the slab origin
+  int32_t sz, sy, sx, T;
+  int32_t hfield;         // 0: E component, 1: H component
+  int32_t comp;           // 0 x, 1 y, 2 z
+  int32_t kstep;          // waveform index = *step0 + kstep, clipped to [0, T-1]
+  int32_t post;
+  int32_t mlo[3], mhi[3]; // PEC masks of post slabs (z, y, x)
+};
+constexpr int kMaxMidSlabs = 8;
+constexpr int kMaxHSlabs = 4;
+
 // One fused leapfrog step reading bank A and writing bank B (fields and CPML
 // memories).  Requires uniform H coefficients, isotropic metrics and, with
 // CPML, one slab thickness L on every term, the 36 term arrays packed as
@@ -159,23 +179,14 @@ struct FusedParams {
   const char* pbase;
   uint32_t poff[12];
   uint32_t pdelta;
+  // H-timing source slabs of this step (single-step launches only), applied
+  // after the H update; the waveform index reads *step0
+  MidSlab hslab[kMaxHSlabs];
+  int32_t hslab_n;
+  const int32_t* step0;
 };
 int fused_rows();
 
-// A source slab applied between the two steps of a pair launch: an E-timing
-// slab of the first step (``post``: before the intermediate monitor sample,
-// skipping PEC-constrained cells) or a pre-E slab of the second.
-struct MidSlab {
-  const float* coeff;     // this slab's [sz, sy, sx] coefficients (storage units)
-  const float* wave;      // this slab's waveform [T]
-  const int32_t* start;   // device int32[3]: (z, y, x) of the slab origin
-  int32_t sz, sy, sx, T;
-  int32_t comp;           // 0 Ex, 1 Ey, 2 Ez
-  int32_t kstep;          // waveform index = *step0 + kstep, clipped to [0, T-1]
-  int32_t post;
-  int32_t mlo[3], mhi[3]; // PEC masks of post slabs (z, y, x)
-};
-constexpr int kMaxMidSlabs = 4;
 constexpr int kMaxMonBoxes = 8;
 
 // Two leapfrog steps in one launch (bank A -> bank B).  ``f`` carries the
@@ -185,8 +196,11 @@ constexpr int kMaxMonBoxes = 8;
 // device int32[mbox_n][6]) holds the cell, for the intermediate DFT sample.
 struct PairParams {
   FusedParams f;
-  MidSlab msrc[kMaxMidSlabs];  // post slabs first, then pre-E slabs
-  int32_t msrc_n, msrc_post;
+  // source slabs, in order: [0, msrc_post) E-timing of the first step,
+  // [msrc_post, msrc_e) pre-E of the second, [msrc_e, msrc_h1) H-timing of the
+  // first, [msrc_h1, msrc_n) H-timing of the second
+  MidSlab msrc[kMaxMidSlabs];
+  int32_t msrc_n, msrc_post, msrc_e, msrc_h1;
   const int32_t* step0;
   char* cmid;
   const int32_t* mbox;

@@ -238,6 +238,51 @@ __device__ __forceinline__ __attribute__((address_space(4))) const PairParams* p
   return p;
 }
 
+using cint = const __attribute__((address_space(4))) int32_t;
+using kslab = const __attribute__((address_space(4))) MidSlab;
+
+// Add one source slab to plane k of a row (execute.py:374-461): the slab's
+// plane and row are uniform, its columns per lane (lane at x).  vx, vy, vz are
+// the E or H components of the row, as the slab's ``hfield`` says.
+__device__ __forceinline__ void slab_add(kslab& ms, int step, int k, int y, int x, int nz, int ny,
+                                         int nx, float& vx, float& vy, float& vz) {
+  cint* st = (cint*)(ms.start);
+  const int cz = k - st[0], cy = y - st[1], cx = x - st[2];
+  const int comp = ms.comp;
+  int Dz, Dy, Dx;  // the component's extent
+  if (ms.hfield) {
+    Dz = comp == 2 ? nz + 1 : nz;
+    Dy = comp == 1 ? ny + 1 : ny;
+    Dx = comp == 0 ? nx + 1 : nx;
+  } else {
+    Dz = comp == 2 ? nz : nz + 1;
+    Dy = comp == 1 ? ny : ny + 1;
+    Dx = comp == 0 ? nx : nx + 1;
+  }
+  if (cz < 0 || cz >= ms.sz || cy < 0 || cy >= ms.sy || k >= Dz || y < 0 || y >= Dy) return;
+  if (ms.post && ((ms.mlo[0] && k == 0) || (ms.mhi[0] && k == Dz - 1) || (ms.mlo[1] && y == 0) ||
+                  (ms.mhi[1] && y == Dy - 1)))
+    return;
+  int widx = step + ms.kstep;
+  widx = widx < 0 ? 0 : (widx > ms.T - 1 ? ms.T - 1 : widx);
+  const float wv = as_const(ms.wave)[widx];
+  const bool in = cx >= 0 && cx < ms.sx && x >= 0 && x < Dx &&
+                  !(ms.post && ((ms.mlo[2] && x == 0) || (ms.mhi[2] && x == Dx - 1)));
+  float add = 0.0f;
+  if (in) add = ((const float*)ms.coeff)[(cz * ms.sy + cy) * ms.sx + cx] * wv;
+  if (comp == 0) vx += add;
+  else if (comp == 1) vy += add;
+  else vz += add;
+}
+
+// Does slab ``ms`` cross row y on any lane of this wave (lane at x)?
+__device__ __forceinline__ bool slab_hits_wave(kslab& ms, int y, int x) {
+  cint* st = (cint*)(ms.start);
+  const int cy = y - st[1], cx = x - st[2];
+  const bool hit = cy >= 0 && cy < ms.sy && cx >= 0 && cx < ms.sx;
+  return __builtin_amdgcn_readfirstlane(__any(hit)) != 0;
+}
+
 template <typename T, typename P, typename C, int kR, bool kEArr, bool kXEdge>
 __device__ __forceinline__ void fused_body(const FusedParams& f) {
   const SlabArgs<sizeof(T) == 2 && !kXEdge> sa{f};
@@ -334,6 +379,9 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
     ex = (vxy && vtX && vxX) ? a : 0.0f;
     ez = (vz && vtX && vxY) ? d : 0.0f;
   };
+
+  bool hwave = false;  // an H-timing slab crosses this wave's row
+  for (int s = 0; s < f.hslab_n; ++s) hwave = hwave || slab_hits_wave(kargs()->hslab[s], y, x);
 
   const int ks = kz0 > 0 ? kz0 - 1 : 0;  // plane kz0-1 only rebuilds H_new(kz0-1)
   float ex, ey, ez;
@@ -446,6 +494,13 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
     hx = okhx ? hx : 0.0f;
     hy = okhy ? hy : 0.0f;
     hz = okhz ? hz : 0.0f;
+    // H-timing source slabs of this step (execute.py:467-541: after the H
+    // update, before E reads H)
+    if (hwave) {
+      const int step = *(cint*)(kargs()->step0);
+      for (int s = 0; s < kargs()->hslab_n; ++s)
+        slab_add(kargs()->hslab[s], step, k, y, x, nz, ny, nx, hx, hy, hz);
+    }
 
     // publish H_new(k) and E(k+1) rows, store owned H
     lH[bb][w][0][lane] = hz;
@@ -710,7 +765,6 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, in
 
   // monitor boxes: bit b set on lanes whose (y, x) lies in box b; planes are
   // tested per k.  Only owned cells write the intermediate bank.
-  using cint = const __attribute__((address_space(4))) int32_t;
   uint32_t mbits = 0;
   for (int b = 0; b < pp.mbox_n; ++b) {
     cint* bx6 = (cint*)(pair_kargs()->mbox) + 6 * b;
@@ -951,43 +1005,18 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, in
     }
   };
 
-  // source slabs [s0, s1) on E'(k) of this row (execute.py:374-461): the
-  // slab's plane and row are uniform, its columns per lane
+  // source slabs [s0, s1) of the pair's list on plane k of this row
   auto mid_sources = [&](int s0, int s1, int k, float& vx, float& vy, float& vz) {
-    for (int s = s0; s < s1; ++s) {
-      const auto& ms = pair_kargs()->msrc[s];
-      cint* st = (cint*)(ms.start);
-      const int cz = k - st[0], cy = y - st[1], cx = x - st[2];
-      const int comp = ms.comp;
-      const int Dz = comp == 2 ? nz : nz + 1, Dy = comp == 1 ? ny : ny + 1;
-      const int Dx = comp == 0 ? nx : nx + 1;
-      if (cz < 0 || cz >= ms.sz || cy < 0 || cy >= ms.sy || k >= Dz || y < 0 || y >= Dy) continue;
-      if (ms.post && ((ms.mlo[0] && k == 0) || (ms.mhi[0] && k == Dz - 1) ||
-                      (ms.mlo[1] && y == 0) || (ms.mhi[1] && y == Dy - 1)))
-        continue;
-      int widx = *(cint*)(pair_kargs()->step0) + ms.kstep;
-      widx = widx < 0 ? 0 : (widx > ms.T - 1 ? ms.T - 1 : widx);
-      const float wv = as_const(ms.wave)[widx];
-      const bool in = cx >= 0 && cx < ms.sx && x >= 0 && x < Dx &&
-                      !(ms.post && ((ms.mlo[2] && x == 0) || (ms.mhi[2] && x == Dx - 1)));
-      float add = 0.0f;
-      if (in) add = ((const float*)ms.coeff)[(cz * ms.sy + cy) * ms.sx + cx] * wv;
-      if (comp == 0) vx += add;
-      else if (comp == 1) vy += add;
-      else vz += add;
-    }
+    const int step = *(cint*)(pair_kargs()->step0);
+    for (int s = s0; s < s1; ++s)
+      slab_add(pair_kargs()->msrc[s], step, k, y, x, nz, ny, nx, vx, vy, vz);
   };
 
   // does any slab between the steps cross this wave's row and lanes?  (its
   // planes are tested per k inside mid_sources)
   bool swave = false;
-  for (int s = 0; s < pair_kargs()->msrc_n; ++s) {
-    const auto& ms = pair_kargs()->msrc[s];
-    cint* st = (cint*)(ms.start);
-    const int cy = y - st[1], cx = x - st[2];
-    const bool hit = cy >= 0 && cy < ms.sy && cx >= 0 && cx < ms.sx;
-    swave = swave || __builtin_amdgcn_readfirstlane(__any(hit)) != 0;
-  }
+  for (int s = 0; s < pair_kargs()->msrc_n; ++s)
+    swave = swave || slab_hits_wave(pair_kargs()->msrc[s], y, x);
 
   const int ks = max(kz0 - 2, 0);
   const int kE = max(kz0 - 1, 0);  // first plane of E' the chunk needs
@@ -1011,7 +1040,8 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, in
   float sa0 = es0, sa1 = es1, sa2 = es2;     // E source coefficients of plane k-1
   float sb0 = es0, sb1 = es1, sb2 = es2;     // ... of plane k-2
   float mh[6] = {}, me1[6] = {}, me2[6] = {};  // psi': H terms at k-1, E terms at k-1, k-2
-  const int npost = pair_kargs()->msrc_post, nsrc = pair_kargs()->msrc_n;
+  const int npost = pair_kargs()->msrc_post, ne = pair_kargs()->msrc_e;
+  const int nh1 = pair_kargs()->msrc_h1, nsrc = pair_kargs()->msrc_n;
 
   for (int k = ks; k <= kz1 + 1; ++k) {
     const int bb = k & 1;
@@ -1048,6 +1078,7 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, in
       const float ez_up = lE1[bb][w + 1][0][lane];
       const float ex_up = lE1[bb][w + 1][1][lane];
       h_update(k, ez, ey, ex, eyn, exn, ez_up, ex_up, ha0, ha1, ha2, ph, q1, hx, hy, hz);
+      if (swave && nh1 > ne) mid_sources(ne, nh1, k, hx, hy, hz);
       lH1[bb][w][0][lane] = hz;
       lH1[bb][w][1][lane] = hx;
       lE1[bb ^ 1][w][0][lane] = ezn;
@@ -1070,7 +1101,7 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, in
                nez);
       if (swave && npost > 0) mid_sources(0, npost, k, nex, ney, nez);
       if (sampled) store_e(pair_kargs()->cmid, k, nex, ney, nez);
-      if (swave && nsrc > npost) mid_sources(npost, nsrc, k, nex, ney, nez);
+      if (swave && ne > npost) mid_sources(npost, ne, k, nex, ney, nez);
       lE2[bb][w][0][lane] = nez;
       lE2[bb][w][1][lane] = nex;
     }
@@ -1083,6 +1114,7 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, in
       const float ex_up = lE2[bb ^ 1][w + 1][1][lane];
       float q3[6] = {};
       h_update(k1, e1z, e1y, e1x, ney, nex, ez_up, ex_up, h1x, h1y, h1z, mh, q3, gx, gy, gz);
+      if (swave && nsrc > nh1) mid_sources(nh1, nsrc, k1, gx, gy, gz);
       lH2[bb][w][0][lane] = gz;
       lH2[bb][w][1][lane] = gx;
       if (own && k1 >= kz0) {
@@ -1203,6 +1235,14 @@ void launch_fused3d(hipStream_t s, const FusedParams& p, int dtype) {
   BZH_REQUIRE(p.fin != nullptr && p.fout != nullptr && (p.L == 0 || p.pbase != nullptr) &&
                   (p.esrc_st[0] == 0 || p.cbase != nullptr),
               "the fused step needs based addressing of its array groups");
+  BZH_REQUIRE(p.hslab_n >= 0 && p.hslab_n <= kMaxHSlabs && (p.hslab_n == 0 || p.step0 != nullptr),
+              "fused H source slabs are malformed");
+  for (int i = 0; i < p.hslab_n; ++i) {
+    const MidSlab& m = p.hslab[i];
+    BZH_REQUIRE(m.coeff && m.wave && m.start && m.sz >= 0 && m.sy >= 0 && m.sx >= 0 && m.T >= 1 &&
+                    m.comp >= 0 && m.comp < 3 && m.hfield == 1,
+                "fused H source slab is malformed");
+  }
   for (int c = 0; c < 3; ++c) {
     BZH_REQUIRE(p.hdec_st[c] == 0 && p.hsrc_st[c] == 0 && p.edec_st[c] == 0,
                 "the fused step needs uniform H coefficients and E decay");
@@ -1270,15 +1310,17 @@ void launch_pair3d(hipStream_t s, const PairParams& pp, int dtype) {
                 "the pair step needs uniform H coefficients and E decay");
     BZH_REQUIRE(p.esrc_st[c] == p.esrc_st[0], "E source coefficients must be all arrays or all scalars");
   }
-  BZH_REQUIRE(pp.msrc_n >= 0 && pp.msrc_n <= kMaxMidSlabs && pp.msrc_post >= 0 &&
-                  pp.msrc_post <= pp.msrc_n && (pp.msrc_n == 0 || pp.step0 != nullptr),
+  BZH_REQUIRE(pp.msrc_n >= 0 && pp.msrc_n <= kMaxMidSlabs && 0 <= pp.msrc_post &&
+                  pp.msrc_post <= pp.msrc_e && pp.msrc_e <= pp.msrc_h1 && pp.msrc_h1 <= pp.msrc_n &&
+                  (pp.msrc_n == 0 || pp.step0 != nullptr),
               "pair source slabs are malformed");
   for (int i = 0; i < pp.msrc_n; ++i) {
     const MidSlab& m = pp.msrc[i];
     BZH_REQUIRE(m.coeff && m.wave && m.start && m.sz >= 0 && m.sy >= 0 && m.sx >= 0 && m.T >= 1 &&
-                    m.comp >= 0 && m.comp < 3,
+                    m.comp >= 0 && m.comp < 3 && m.hfield == (i >= pp.msrc_e ? 1 : 0),
                 "pair source slab is malformed");
   }
+  BZH_REQUIRE(p.hslab_n == 0, "pair launches carry their H slabs in the pair list");
   BZH_REQUIRE(pp.mbox_n >= 0 && pp.mbox_n <= kMaxMonBoxes &&
                   (pp.mbox_n == 0 || (pp.mbox != nullptr && pp.cmid != nullptr)),
               "pair monitor boxes are malformed");

@@ -134,6 +134,16 @@ def _elide(arr, device):
     return _dev(a32, device)
 
 
+# H-timing slabs the fused kernels apply in-kernel per step (kMaxHSlabs of
+# the library); programs with more run the per-phase schedule
+MAX_H_SLABS = 4
+
+
+def h_source_slabs(groups):
+    """Slab count of the three H-timing source groups."""
+    return sum(int(g[0].shape[0]) for g in groups[3:6] if g is not None)
+
+
 def compile_program(permittivity, resolution, dt, *, boundaries=(), sources=(), monitors=(),
                     num_steps=1, t0=0.0, conductivity=None, device=None,
                     field_dtype=torch.float32, flags=abi.DEFAULT_FLAGS, field_scale=None):
@@ -254,7 +264,7 @@ def compile_program(permittivity, resolution, dt, *, boundaries=(), sources=(), 
 
     # E source coefficients: float32, or float16 when half-precision fields
     # run the fused step (the folded values c dt / (eps dx) lie in [0.02, 1])
-    fused = bool(is_3d and (flags & abi.FLAG_FUSED_STEP) and all(g is None for g in groups[3:6]))
+    fused = bool(is_3d and (flags & abi.FLAG_FUSED_STEP) and h_source_slabs(groups) <= MAX_H_SLABS)
     coef_dtype = torch.float16 if (half and fused) else torch.float32
     for ax in "xyz":
         arr = np.asarray(e_source_raw[ax], dtype=np.float64) * (s_h / s_e) * fold

@@ -114,10 +114,12 @@ def update_e(state: SimulationState, prog: CompiledProgram) -> SimulationState:
 
 
 def uses_two_banks(prog) -> bool:
-    """The fused two-bank step applies to 3D programs without H-timed sources
-    when the fused flag is set (``abi.FLAG_FUSED_STEP``)."""
+    """The fused two-bank step applies to 3D programs with at most
+    ``MAX_H_SLABS`` H-timed source slabs when the fused flag is set
+    (``abi.FLAG_FUSED_STEP``)."""
+    from .program import MAX_H_SLABS, h_source_slabs
     return (prog.is_3d and bool(prog.flags & abi.FLAG_FUSED_STEP)
-            and all(g is None for g in prog.source_groups[3:6]))
+            and h_source_slabs(prog.source_groups) <= MAX_H_SLABS)
 
 
 def with_workspace(state):

@@ -115,3 +115,39 @@ def test_pair_matches_single_steps(monkeypatch):
                     [state.dft_re.cpu().numpy().copy(), state.dft_im.cpu().numpy().copy()])
     for i, (a, b) in enumerate(zip(*runs)):
         _close(f"leaf {i}", a, b, rel=1e-6, mabs=1e-5)
+
+
+def _h_sources(grid, steps):
+    """H-timing slabs on Hy (touching the bottom y face, where a PEC face
+    masks it) and Hz, applied after the H update of both steps of a pair."""
+    dt = _dt(grid)
+    nz, ny, nx = grid
+    rng = np.random.default_rng(13)
+    t = o.sample_times(0.0, dt, steps)
+    f0 = 1.9e14
+    wave = o.gaussian_pulse(t, f0, 0.5 * f0, offset=1.0).astype(np.float32).astype(np.float64)
+    hy = (rng.random((4, 3, nx - 10)) * 2e-3).astype(np.float32).astype(np.float64)
+    hz = (rng.random((2, 5, 7)) * 2e-3).astype(np.float32).astype(np.float64)
+    return [o.Source("Hy", "h", (nz // 2 - 2, 0, 5), hy, wave),
+            o.Source("Hz", "h", (2, ny // 2, nx // 2 - 3), hz, wave)]
+
+
+@pytest.mark.parametrize("chunk", [12, 5])
+@pytest.mark.parametrize("grid,bc", [((13, 17, 130), "cpml"), ((9, 28, 64), "pec")])
+def test_pair_parity_with_h_sources(grid, bc, chunk):
+    """H-timing slabs run inside the pair and single-step kernels (the
+    mode-source launch uses them): parity with the oracle as above."""
+    _require_gpu()
+    from beamz_amd import native
+    steps = 12
+    kw = {"cpml": dict(cpml_cells=3), "pec": {}}[bc]
+    srcs = _sources(grid, steps) + _h_sources(grid, steps)
+    sim, prog, state, prob, ost = _setup(grid, seed=77, steps=steps, sources=srcs,
+                                         monitors=_monitors(grid), **kw)
+    print(f"case pair_parity_h grid={grid} bc={bc} chunk={chunk}", flush=True)
+    state = sim.advance(state, steps, chunk=chunk)
+    torch.cuda.synchronize()
+    assert native.last_program_fused()
+    assert native.last_program_pairs() == (steps % chunk or chunk) // 2 // 2 * 2
+    o.run(prob, ost, steps)
+    _compare_state(prog, state, prob, ost, dft=True)
