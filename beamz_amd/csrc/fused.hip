@@ -270,9 +270,11 @@ __device__ __forceinline__ void slab_add(kslab& ms, int step, int k, int y, int 
                   !(ms.post && ((ms.mlo[2] && x == 0) || (ms.mhi[2] && x == Dx - 1)));
   float add = 0.0f;
   if (in) add = ((const float*)ms.coeff)[(cz * ms.sy + cy) * ms.sx + cx] * wv;
-  if (comp == 0) vx += add;
-  else if (comp == 1) vy += add;
-  else vz += add;
+  // by value (a component picked through the references is indexed in
+  // scratch memory); adding 0 leaves the other two unchanged
+  vx += comp == 0 ? add : 0.0f;
+  vy += comp == 1 ? add : 0.0f;
+  vz += comp == 2 ? add : 0.0f;
 }
 
 // Does slab ``ms`` cross row y on any lane of this wave (lane at x)?
@@ -283,7 +285,7 @@ __device__ __forceinline__ bool slab_hits_wave(kslab& ms, int y, int x) {
   return __builtin_amdgcn_readfirstlane(__any(hit)) != 0;
 }
 
-template <typename T, typename P, typename C, int kR, bool kEArr, bool kXEdge>
+template <typename T, typename P, typename C, int kR, bool kEArr, bool kXEdge, bool kHSrc>
 __device__ __forceinline__ void fused_body(const FusedParams& f) {
   const SlabArgs<sizeof(T) == 2 && !kXEdge> sa{f};
   __shared__ float lE[2][kR + 2][2][64];
@@ -381,7 +383,8 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
   };
 
   bool hwave = false;  // an H-timing slab crosses this wave's row
-  for (int s = 0; s < f.hslab_n; ++s) hwave = hwave || slab_hits_wave(kargs()->hslab[s], y, x);
+  if constexpr (kHSrc)
+    for (int s = 0; s < f.hslab_n; ++s) hwave = hwave || slab_hits_wave(kargs()->hslab[s], y, x);
 
   const int ks = kz0 > 0 ? kz0 - 1 : 0;  // plane kz0-1 only rebuilds H_new(kz0-1)
   float ex, ey, ez;
@@ -496,7 +499,7 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
     hz = okhz ? hz : 0.0f;
     // H-timing source slabs of this step (execute.py:467-541: after the H
     // update, before E reads H)
-    if (hwave) {
+    if (kHSrc && hwave) {
       const int step = *(cint*)(kargs()->step0);
       for (int s = 0; s < kargs()->hslab_n; ++s)
         slab_add(kargs()->hslab[s], step, k, y, x, nz, ny, nx, hx, hy, hz);
@@ -585,34 +588,42 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
 // instruction, and the second workgroup's loads cover the latency (1.80 ms
 // against 2.11 ms at 1024x512x256 f16).  FP32 keeps the compiler's 7 waves:
 // forced to 8 it spills to scratch and runs slower (2.68 ms against 2.53 ms).
-template <typename T, typename P, typename C, int kR, bool kEArr>
+// kHSrc: the variant that applies H-timing source slabs (mode-source runs);
+// the others carry none of its code or registers.
+template <typename T, typename P, typename C, int kR, bool kEArr, bool kHSrc>
 __global__ void __launch_bounds__(64 * (kR + 1)) fused_interior(FusedParams f) {
-  fused_body<T, P, C, kR, kEArr, false>(f);
+  fused_body<T, P, C, kR, kEArr, false, kHSrc>(f);
 }
-template <typename T, typename P, typename C, int kR, bool kEArr>
+template <typename T, typename P, typename C, int kR, bool kEArr, bool kHSrc>
 __global__ void __launch_bounds__(64 * (kR + 1)) __attribute__((amdgpu_waves_per_eu(8, 8)))
 fused_interior_half(FusedParams f) {
-  fused_body<T, P, C, kR, kEArr, false>(f);
+  fused_body<T, P, C, kR, kEArr, false, kHSrc>(f);
 }
-template <typename T, typename P, typename C, int kR, bool kEArr>
+template <typename T, typename P, typename C, int kR, bool kEArr, bool kHSrc>
 __global__ void __launch_bounds__(64 * (kR + 1)) fused_edge(FusedParams f) {
-  fused_body<T, P, C, kR, kEArr, true>(f);
+  fused_body<T, P, C, kR, kEArr, true, kHSrc>(f);
+}
+
+template <typename T, typename P, typename C, int kR, bool kHSrc>
+void launch_wg_h(hipStream_t s, const FusedParams& p, int nblocks) {
+  const dim3 block(64 * (kR + 1), 1, 1), grid(nblocks, 1, 1);
+  const bool earr = p.esrc_st[0] != 0;
+  if (p.xedge) {
+    if (earr) hipLaunchKernelGGL((fused_edge<T, P, C, kR, true, kHSrc>), grid, block, 0, s, p);
+    else hipLaunchKernelGGL((fused_edge<T, P, float, kR, false, kHSrc>), grid, block, 0, s, p);
+  } else if constexpr (sizeof(T) == 2) {
+    if (earr) hipLaunchKernelGGL((fused_interior_half<T, P, C, kR, true, kHSrc>), grid, block, 0, s, p);
+    else hipLaunchKernelGGL((fused_interior_half<T, P, float, kR, false, kHSrc>), grid, block, 0, s, p);
+  } else {
+    if (earr) hipLaunchKernelGGL((fused_interior<T, P, C, kR, true, kHSrc>), grid, block, 0, s, p);
+    else hipLaunchKernelGGL((fused_interior<T, P, float, kR, false, kHSrc>), grid, block, 0, s, p);
+  }
 }
 
 template <typename T, typename P, typename C, int kR>
 void launch_wg(hipStream_t s, const FusedParams& p, int nblocks) {
-  const dim3 block(64 * (kR + 1), 1, 1), grid(nblocks, 1, 1);
-  const bool earr = p.esrc_st[0] != 0;
-  if (p.xedge) {
-    if (earr) hipLaunchKernelGGL((fused_edge<T, P, C, kR, true>), grid, block, 0, s, p);
-    else hipLaunchKernelGGL((fused_edge<T, P, float, kR, false>), grid, block, 0, s, p);
-  } else if constexpr (sizeof(T) == 2) {
-    if (earr) hipLaunchKernelGGL((fused_interior_half<T, P, C, kR, true>), grid, block, 0, s, p);
-    else hipLaunchKernelGGL((fused_interior_half<T, P, float, kR, false>), grid, block, 0, s, p);
-  } else {
-    if (earr) hipLaunchKernelGGL((fused_interior<T, P, C, kR, true>), grid, block, 0, s, p);
-    else hipLaunchKernelGGL((fused_interior<T, P, float, kR, false>), grid, block, 0, s, p);
-  }
+  if (p.hslab_n > 0) launch_wg_h<T, P, C, kR, true>(s, p, nblocks);
+  else launch_wg_h<T, P, C, kR, false>(s, p, nblocks);
 }
 
 // Rows per workgroup: 15 by default, 8 selectable (BZH_FUSED_ROWS) for
@@ -683,7 +694,7 @@ struct PairLds {
   float h2[2][kPairWaves][2][64];      // H'' rows for E''
 };
 
-template <typename T, typename P, typename C, bool kEArr, bool kXEdge>
+template <typename T, typename P, typename C, bool kEArr, bool kXEdge, bool kHSrc>
 __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, int blk, int nblk) {
   const FusedParams& f = pp.f;
   const SlabArgs<kXEdge> sa{f};  // x-edge tiles: reloaded (x-slab terms raise the pressure)
@@ -1005,11 +1016,17 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, in
     }
   };
 
-  // source slabs [s0, s1) of the pair's list on plane k of this row
-  auto mid_sources = [&](int s0, int s1, int k, float& vx, float& vy, float& vz) {
-    const int step = *(cint*)(pair_kargs()->step0);
-    for (int s = s0; s < s1; ++s)
-      slab_add(pair_kargs()->msrc[s], step, k, y, x, nz, ny, nx, vx, vy, vz);
+  // source slabs of the pair's list on plane k of this row: part 0 E-timing of
+  // the first step, 1 pre-E of the second, 2 H-timing of the first, 3
+  // H-timing of the second (bounds re-read from the argument block, so that
+  // they hold no scalar registers across the z-march)
+  auto mid_sources = [&](int part, int k, float& vx, float& vy, float& vz) {
+    auto* a = pair_kargs();
+    const int s0 = part == 0 ? 0 : (part == 1 ? a->msrc_post : (part == 2 ? a->msrc_e : a->msrc_h1));
+    const int s1 = part == 0 ? a->msrc_post : (part == 1 ? a->msrc_e : (part == 2 ? a->msrc_h1 : a->msrc_n));
+    if (s1 <= s0) return;
+    const int step = *(cint*)(a->step0);
+    for (int s = s0; s < s1; ++s) slab_add(a->msrc[s], step, k, y, x, nz, ny, nx, vx, vy, vz);
   };
 
   // does any slab between the steps cross this wave's row and lanes?  (its
@@ -1040,8 +1057,6 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, in
   float sa0 = es0, sa1 = es1, sa2 = es2;     // E source coefficients of plane k-1
   float sb0 = es0, sb1 = es1, sb2 = es2;     // ... of plane k-2
   float mh[6] = {}, me1[6] = {}, me2[6] = {};  // psi': H terms at k-1, E terms at k-1, k-2
-  const int npost = pair_kargs()->msrc_post, ne = pair_kargs()->msrc_e;
-  const int nh1 = pair_kargs()->msrc_h1, nsrc = pair_kargs()->msrc_n;
 
   for (int k = ks; k <= kz1 + 1; ++k) {
     const int bb = k & 1;
@@ -1078,7 +1093,7 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, in
       const float ez_up = lE1[bb][w + 1][0][lane];
       const float ex_up = lE1[bb][w + 1][1][lane];
       h_update(k, ez, ey, ex, eyn, exn, ez_up, ex_up, ha0, ha1, ha2, ph, q1, hx, hy, hz);
-      if (swave && nh1 > ne) mid_sources(ne, nh1, k, hx, hy, hz);
+      if (kHSrc && swave) mid_sources(2, k, hx, hy, hz);
       lH1[bb][w][0][lane] = hz;
       lH1[bb][w][1][lane] = hx;
       lE1[bb ^ 1][w][0][lane] = ezn;
@@ -1099,9 +1114,9 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, in
       const float hx_dn = lH1[bb][w - 1][1][lane];
       e_update(k, hx, hy, hz, hz_dn, hx_dn, h1x, h1y, ex, ey, ez, s0, s1, s2, pe, q2, nex, ney,
                nez);
-      if (swave && npost > 0) mid_sources(0, npost, k, nex, ney, nez);
+      if (swave) mid_sources(0, k, nex, ney, nez);
       if (sampled) store_e(pair_kargs()->cmid, k, nex, ney, nez);
-      if (swave && ne > npost) mid_sources(npost, ne, k, nex, ney, nez);
+      if (swave) mid_sources(1, k, nex, ney, nez);
       lE2[bb][w][0][lane] = nez;
       lE2[bb][w][1][lane] = nex;
     }
@@ -1114,7 +1129,7 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, in
       const float ex_up = lE2[bb ^ 1][w + 1][1][lane];
       float q3[6] = {};
       h_update(k1, e1z, e1y, e1x, ney, nex, ez_up, ex_up, h1x, h1y, h1z, mh, q3, gx, gy, gz);
-      if (swave && nsrc > nh1) mid_sources(nh1, nsrc, k1, gx, gy, gz);
+      if (kHSrc && swave) mid_sources(3, k1, gx, gy, gz);
       lH2[bb][w][0][lane] = gz;
       lH2[bb][w][1][lane] = gx;
       if (own && k1 >= kz0) {
@@ -1174,20 +1189,26 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, in
 // so that they do not form the tail), then the x-interior ones.  Both
 // variants run one 16-wave workgroup per CU, so merging costs no occupancy and
 // the two boxes overlap instead of running back to back.
-template <typename T, typename P, typename C, bool kEArr>
+template <typename T, typename P, typename C, bool kEArr, bool kHSrc>
 __global__ void __launch_bounds__(64 * kPairWaves) pair_step(PairParams p) {
   __shared__ PairLds lds;
   const int b = blockIdx.x;
-  if (b < p.nblk_edge) pair_body<T, P, C, kEArr, true>(p, lds, b, p.nblk_edge);
-  else pair_body<T, P, C, kEArr, false>(p, lds, b - p.nblk_edge, p.nblk_int);
+  if (b < p.nblk_edge) pair_body<T, P, C, kEArr, true, kHSrc>(p, lds, b, p.nblk_edge);
+  else pair_body<T, P, C, kEArr, false, kHSrc>(p, lds, b - p.nblk_edge, p.nblk_int);
 }
 
 template <typename T, typename P, typename C>
 void launch_pair_wg(hipStream_t s, const PairParams& p, int nblocks) {
   const dim3 block(64 * kPairWaves, 1, 1), grid(nblocks, 1, 1);
   const bool earr = p.f.esrc_st[0] != 0;
-  if (earr) hipLaunchKernelGGL((pair_step<T, P, C, true>), grid, block, 0, s, p);
-  else hipLaunchKernelGGL((pair_step<T, P, float, false>), grid, block, 0, s, p);
+  const bool hsrc = p.msrc_n > p.msrc_e;  // H-timing slabs in the list
+  if (hsrc) {
+    if (earr) hipLaunchKernelGGL((pair_step<T, P, C, true, true>), grid, block, 0, s, p);
+    else hipLaunchKernelGGL((pair_step<T, P, float, false, true>), grid, block, 0, s, p);
+  } else {
+    if (earr) hipLaunchKernelGGL((pair_step<T, P, C, true, false>), grid, block, 0, s, p);
+    else hipLaunchKernelGGL((pair_step<T, P, float, false, false>), grid, block, 0, s, p);
+  }
 }
 
 template <typename T, typename P>
