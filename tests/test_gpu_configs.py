@@ -8,7 +8,8 @@
 - the 3D silicon waveguide (128 x 256 x 512 cells at 25 nm, 450 x 220 nm Si
   core along x, 12-cell CPML, Gaussian plane source at x = 30, three flux
   planes at 11 wavelengths), 4 fused steps from random fields (the oracle
-  takes seconds per step at 16.8 M cells)."""
+  takes seconds per step at 16.8 M cells), and the full 4000-step run with
+  two steps per launch against single steps."""
 
 from __future__ import annotations
 
@@ -120,3 +121,40 @@ def test_3d_silicon_waveguide_fused_steps():
             sl = slice(m.value_offset + c * F * P, m.value_offset + (c + 1) * F * P)
             _close(f"dft_re {name}", re[sl].reshape(F, P), ost["dft_re"][mi][c], rel=5e-5)
             _close(f"dft_im {name}", im[sl].reshape(F, P), ost["dft_im"][mi][c], rel=5e-5)
+
+
+def test_3d_silicon_waveguide_full_run_pairs_match_single_steps(monkeypatch):
+    """The 3D silicon waveguide (128 x 256 x 512, 4000 steps, the plane source
+    and three flux planes) run to the end with two steps per launch and with
+    single steps: the flux spectra agree to FP32 rounding accumulated over the
+    run (relative 1e-5 of the largest)."""
+    _require_gpu()
+    import beamz_amd as bz
+    from beamz_amd import native
+    nz, ny, nx, dx, L, steps = 128, 256, 512, 25e-9, 12, 4000
+    dt = 0.99 * dx / (bz.LIGHT_SPEED * np.sqrt(3))
+    eps = np.full((nz, ny, nx), 2.085)
+    eps[59:68, 119:137, :] = 12.25
+    f0 = bz.LIGHT_SPEED / 1.55e-6
+    fw = bz.LIGHT_SPEED * 0.2e-6 / 1.55e-6 ** 2
+    freqs = bz.LIGHT_SPEED / np.linspace(1.5e-6, 1.6e-6, 11)
+    rngs = {"z": (L, nz - L), "y": (L, ny - L)}
+    fluxes = []
+    for pair in ("1", "0"):
+        monkeypatch.setenv("BZH_PAIR", pair)
+        src = bz.SoftSource("Ez", (L, L, 30), (nz - 2 * L, ny + 1 - 2 * L, 1), bz.GaussianPulse(f0, fw))
+        mons = [bz.FluxMonitor("x", xi, freqs, ranges=rngs) for xi in (60, 256, 450)]
+        sim = bz.Simulation(eps, dx, dt=dt, boundaries=[bz.PML(cells=L)], sources=[src],
+                            monitors=mons, steps=steps)
+        print(f"case config 3D silicon waveguide full run pair={pair} steps={steps}", flush=True)
+        res = sim.run()
+        torch.cuda.synchronize()
+        assert native.last_program_fused()
+        assert (native.last_program_pairs() > 0) == (pair == "1")
+        fluxes.append([res.monitor_flux(i) for i in range(3)])
+        del res, sim
+        torch.cuda.empty_cache()
+    for i, (a, b) in enumerate(zip(*fluxes)):
+        err = np.max(np.abs(a - b)) / np.max(np.abs(b))
+        print(f"  monitor {i}: max relative flux difference {err:.2e}", flush=True)
+        assert np.max(np.abs(b)) > 0.0 and err <= 1e-5, (i, err)
