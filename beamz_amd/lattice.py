@@ -101,16 +101,26 @@ def plane_gather_plan(component, shape, dx, normal, plane_index, ranges):
     (``lattice.py:252-333``, uniform grid).  Returns ``(index[P, K], weight[P, K])``
     with zero-weight corners compacted away."""
     tangential = {"x": ("z", "y"), "y": ("z", "x"), "z": ("y", "x")}[normal]
+    u = (np.arange(*ranges[tangential[0]]) + 0.5) * dx
+    v = (np.arange(*ranges[tangential[1]]) + 0.5) * dx
+    return plane_gather_plan_at(component, shape, dx, normal, plane_index * dx, u, v)
+
+
+def plane_gather_plan_at(component, shape, dx, normal, position, u, v):
+    """The plane plan at normal coordinate ``position`` (metres) with samples
+    on the tangential coordinate grid ``u x v`` (``lattice.py:252-354``): the
+    stored plane order is ``(z, y)`` for x, ``(z, x)`` for y, ``(y, x)`` for z,
+    and targets outside the component's coordinates take its edge value."""
+    tangential = {"x": ("z", "y"), "y": ("z", "x"), "z": ("y", "x")}[normal]
     pos = {"z": 0, "y": 1, "x": 2}
     coords = {a: (np.arange(shape[pos[a]]) + OFFSETS_3D[component][pos[a]]) * dx
               for a in pos}
-    u = (np.arange(*ranges[tangential[0]]) + 0.5) * dx
-    v = (np.arange(*ranges[tangential[1]]) + 0.5) * dx
-    gu, gv = np.meshgrid(u, v, indexing="ij")
+    gu, gv = np.meshgrid(np.asarray(u, dtype=np.float64), np.asarray(v, dtype=np.float64),
+                         indexing="ij")
     plans = (
         interpolation_plan(coords[tangential[0]], gu.ravel()),
         interpolation_plan(coords[tangential[1]], gv.ravel()),
-        interpolation_plan(coords[normal], np.full(gu.size, plane_index * dx)),
+        interpolation_plan(coords[normal], np.full(gu.size, float(position))),
     )
     corners = np.array([(a, b, c) for a in (0, 1) for b in (0, 1) for c in (0, 1)])
     index = {}

@@ -117,6 +117,9 @@ def port_waves(results, port: Port, frequencies):
     if idx is None:
         raise ValueError(f"Missing monitor '{port.monitor}' for port '{port.name}'.")
     mon = prog.monitors[idx]
+    if getattr(mon, "placed", False):
+        raise ValueError(f"port '{port.name}': its monitor '{port.monitor}' is placed by centre and "
+                         "size; ports project onto index-space monitors (normal, index, ranges)")
     if mon.normal != port.axis:
         raise ValueError(f"port '{port.name}' axis {port.axis} does not match its monitor")
     if np.ndim(prog.eps_site["Ez"]) == 3:

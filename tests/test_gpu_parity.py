@@ -59,8 +59,16 @@ def _setup(grid, *, seed, cpml_cells=None, faces=None, metallic=None, kappa_max=
         bounds.append(bz.PEC(edges=tuple(metallic)))
     customs = [bz.CustomSource(s.component, s.start, s.coeff, s.waveform, s.timing) for s in sources]
     omons = []
-    for m in monitors:
+    for i, m in enumerate(monitors):
         prog_shapes = o.shapes_3d(grid) if len(grid) == 3 else o.shapes_tm(grid)
+        if m.placed:
+            # a monitor placed by centre and size: the host lowering (pinned on
+            # the CPU by test_monitor_placement) gives the plans, sign and measure
+            cm = bz.monitors.lower_monitor(m, grid, prog_shapes, dx, i)
+            omons.append(o.Monitor(cm.freqs, cm.plans, cm.normal, cm.normal_sign, cm.measure,
+                                   interval=m.interval, t_start=m.t_start, t_end=m.t_end,
+                                   hann=m.window == "hann"))
+            continue
         plans = {}
         for c in o.COMPONENTS:
             if c not in (m.components or bz.monitors.FLUX_COMPONENTS[m.normal]) or c not in prog_shapes:

@@ -1,0 +1,82 @@
+"""Flux monitors placed by physical centre and size, on the GPU, against the
+FP64 oracle: the kernels accumulate the DFT of whatever gather plan the host
+lowering gives (pinned on the CPU by ``test_monitor_placement``), here
+planes between E-node planes with extents off the cell edges (eight-corner
+trilinear plans) in the pair and single-step kernels, and 2D lines read at
+lattice nodes (one-point plans, weight 0 outside a component's array).
+The flux uses the lowered sign and area; tolerances as the other parity
+tests (relative L2 2e-5 per field, 5e-5 for DFT arenas, flux 1e-4)."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import fdtd64 as o
+from tests.test_gpu_parity import _compare_state, _plane_source, _require_gpu, _setup
+
+pytestmark = pytest.mark.gpu
+
+DX = 40e-9
+
+
+def _flux_close(sim_flux, want):
+    np.testing.assert_allclose(sim_flux, want, rtol=1e-4, atol=1e-6 * np.abs(want).max())
+
+
+@pytest.mark.parametrize("chunk", [12, 5])
+def test_placed_planes_3d(chunk):
+    _require_gpu()
+    import beamz_amd as bz
+    grid = (13, 17, 40)
+    steps = 12
+    dt = 0.9 * DX / (o.LIGHT_SPEED * np.sqrt(3))
+    freqs = np.array([1.8e14, 2.0e14, 2.2e14])
+    mons = [bz.FluxMonitor(center=(23.3 * DX, 8.6 * DX, 6.2 * DX), size=(0.0, 9.7 * DX, 7.4 * DX),
+                           freqs=freqs, name="x_plane"),
+            bz.FluxMonitor(center=(20.0 * DX, 8.5 * DX, 9.55 * DX), size=(30.2 * DX, 11.0 * DX, 0.0),
+                           freqs=freqs[1:], name="z_plane", interval=2)]
+    sim, prog, state, prob, ost = _setup(grid, seed=81, cpml_cells=3, steps=steps,
+                                         sources=[_plane_source(grid, dt, steps, 7)], monitors=mons)
+    print(f"case placed_planes_3d grid={grid} chunk={chunk}", flush=True)
+    assert all(m.placed for m in prog.monitors)
+    assert max(p[0].shape[1] for m in prog.monitors for p in m.plans.values() if p is not None) == 8
+    state = sim.advance(state, steps, chunk=chunk)
+    torch.cuda.synchronize()
+    o.run(prob, ost, steps)
+    _compare_state(prog, state, prob, ost, dft=True)
+    res = bz.api.SimulationResults(state=state, program=prog, performance=None)
+    for i in range(2):
+        _flux_close(res.monitor_flux(i), o.monitor_flux(prob, ost, i))
+
+
+def test_placed_lines_2d():
+    _require_gpu()
+    import beamz_amd as bz
+    grid = (40, 150)
+    steps = 100
+    dt = 0.99 * DX / (o.LIGHT_SPEED * np.sqrt(2))
+    t = o.sample_times(0.0, dt, steps)
+    wave = o.gaussian_pulse(t, 2.0e14, 6e13, offset=2.0, carrier="sin").astype(np.float32).astype(np.float64)
+    src = o.Source("Ez", "pre_e", (15, 20), np.full((10, 1), -0.05), wave)
+    freqs = np.array([1.9e14, 2.0e14, 2.1e14])
+    mons = [bz.FluxMonitor(center=(120.3 * DX, 20.0 * DX), size=(0.0, 30.0 * DX, 1e-6), freqs=freqs,
+                           name="x_line"),
+            # along the top edge: Hx has no row there, so its samples weigh 0
+            bz.FluxMonitor(center=(60.0 * DX, 40.0 * DX), size=(80.0 * DX, 0.0, 1e-6), freqs=freqs,
+                           name="top_line"),
+            bz.FluxMonitor(center=(70.0 * DX, 31.0 * DX), size=(100.0 * DX, 0.0, 1e-6), freqs=freqs,
+                           name="y_line")]
+    sim, prog, state, prob, ost = _setup(grid, seed=83, cpml_cells=6, kappa_max=5.0,
+                                         sources=[src], monitors=mons, steps=steps)
+    print(f"case placed_lines_2d grid={grid} steps={steps}", flush=True)
+    assert [m.normal_sign for m in prog.monitors] == [1.0, -1.0, -1.0]
+    state = sim.advance(state, steps, chunk=64)
+    torch.cuda.synchronize()
+    o.run(prob, ost, steps)
+    _compare_state(prog, state, prob, ost, dft=True)
+    res = bz.api.SimulationResults(state=state, program=prog, performance=None)
+    for i in (0, 2):
+        _flux_close(res.monitor_flux(i), o.monitor_flux(prob, ost, i))
+    assert np.all(res.monitor_flux(1) == 0.0) and np.all(o.monitor_flux(prob, ost, 1) == 0.0)
