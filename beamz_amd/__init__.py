@@ -13,13 +13,13 @@ from .modes import ModeSource, ModeSpec
 from .monitors import FluxMonitor
 from .native import HipBackendUnavailable, HipLaunchError
 from .program import SimulationState, compile_program, initial_state
-from .sources import CustomSource, GaussianPulse, SoftSource
+from .sources import CustomSource, GaussianPulse, GaussianSource, SoftSource
 from .termination import AutoTermination, RunTermination
 
 __all__ = [
     "Simulation", "SimulationResults", "SimulationState", "compile_program", "initial_state",
     "courant_dt", "resolve_backend", "PEC", "PML", "FluxMonitor", "CustomSource",
-    "GaussianPulse", "SoftSource", "ModeSource", "ModeSpec", "AutoTermination",
+    "GaussianPulse", "GaussianSource", "SoftSource", "ModeSource", "ModeSpec", "AutoTermination",
     "RunTermination", "HipBackendUnavailable",
     "HipLaunchError",
     "EPS_0", "ETA_0", "LIGHT_SPEED", "MU_0", "nm", "um",

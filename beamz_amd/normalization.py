@@ -25,7 +25,7 @@ from typing import Optional
 import numpy as np
 
 from .modes import ModeSource
-from .sources import CustomSource, SoftSource
+from .sources import CustomSource, GaussianSource, SoftSource
 
 
 @dataclass(frozen=True)
@@ -82,7 +82,7 @@ def source_signal(source, t0, dt, n):
     if isinstance(source, CustomSource):
         return np.asarray(source.waveform, dtype=np.float64).reshape(-1)[:n]
     signal = source.source_time if isinstance(source, ModeSource) else (
-        source.signal if isinstance(source, SoftSource) else None)
+        source.signal if isinstance(source, (SoftSource, GaussianSource)) else None)
     if signal is None:
         return None
     if hasattr(signal, "sample"):

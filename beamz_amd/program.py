@@ -23,7 +23,8 @@ from .constants import EPS_0, ETA_0, MU_0
 from .lattice import component_shapes, sample_e_site_3d, sample_site_tm
 from .monitors import assign_offsets, lower_monitor, pack_monitors
 from .modes import ModeSource, lower_mode_source_2d
-from .sources import CustomSource, SoftSource, group_sources, lower_soft_source
+from .sources import (CustomSource, GaussianSource, SoftSource, group_sources, lower_gaussian_source,
+                      lower_soft_source)
 
 _COMPONENT_CODE = {"Hx": 0, "Hy": 1, "Hz": 2, "Ex": 0, "Ey": 1, "Ez": 2}
 
@@ -223,6 +224,8 @@ def compile_program(permittivity, resolution, dt, *, boundaries=(), sources=(), 
     for s in sources:
         if isinstance(s, SoftSource):
             customs.append(lower_soft_source(s, eps_site[s.component], dt, t0, num_steps))
+        elif isinstance(s, GaussianSource):
+            customs.append(lower_gaussian_source(s, eps_site["Ez"], resolution, dt, t0, num_steps))
         elif isinstance(s, CustomSource):
             customs.append(s)
         elif isinstance(s, ModeSource):

@@ -89,6 +89,90 @@ class SoftSource:
     profile: Optional[np.ndarray] = None
 
 
+@dataclass(frozen=True)
+class GaussianSource:
+    """A soft Ez source with a Gaussian spatial profile placed in physical
+    coordinates (``sources/specs.py:207-252``): ``position`` (x, y) or
+    (x, y, z) in metres, standard width ``width`` in metres, and ``signal``,
+    an array of samples on the time grid (the reference's form; a waveform
+    object with ``sample`` or a callable of time is also taken)."""
+
+    position: tuple
+    width: float
+    signal: object
+
+    def __post_init__(self):
+        position = tuple(float(v) for v in self.position)
+        if len(position) not in (2, 3) or not all(np.isfinite(position)):
+            raise ValueError("GaussianSource position must be a finite 2D or 3D coordinate.")
+        width = float(self.width)
+        if not np.isfinite(width) or width <= 0.0:
+            raise ValueError("GaussianSource width must be a positive finite value.")
+        object.__setattr__(self, "position", position)
+        object.__setattr__(self, "width", width)
+
+
+def interpolate_signal(signal, time, dt):
+    """A sample array read at ``time`` (``time.py:17-32``): linear between
+    samples ``floor(time / dt)`` and the next, the last sample at the last
+    index, 0 before the first and after the last."""
+    arr = np.asarray(signal, dtype=np.float64).reshape(-1)
+    if arr.size == 0:
+        return 0.0
+    pos = float(time / dt)
+    low = int(np.floor(pos))
+    frac = pos - low
+    if 0 <= low < arr.size - 1:
+        return float((1.0 - frac) * arr[low] + frac * arr[low + 1])
+    if low == arr.size - 1:
+        return float(arr[low])
+    return 0.0
+
+
+def gaussian_profile(src: GaussianSource, shape, dx, is_3d):
+    """The Ez-site Gaussian of the reference's grid path
+    (``sources/compiler.py:649-700``): on each axis the Ez coordinates within
+    ``4 width`` of the centre (the nearest one if none is), and
+    ``exp(-r^2 / (2 width^2))`` on their product.  Ez sits on nodes in x and
+    y, and on cell centres in z.  Returns ``(start, profile)``."""
+    centre = dict(zip(("x", "y", "z"), src.position))
+    axes = ("z", "y", "x") if is_3d else ("y", "x")
+    starts, coords = [], []
+    for axis, n in zip(axes, shape):
+        values = (np.arange(n) + (0.5 if axis == "z" else 0.0)) * dx
+        active = np.flatnonzero(np.abs(values - centre[axis]) <= 4.0 * src.width)
+        if active.size:
+            lo, hi = int(active[0]), int(active[-1]) + 1
+        else:
+            lo = int(np.argmin(np.abs(values - centre[axis])))
+            hi = lo + 1
+        starts.append(lo)
+        coords.append(values[lo:hi] - centre[axis])
+    mesh = np.meshgrid(*coords, indexing="ij")
+    r2 = sum(m * m for m in mesh)
+    return tuple(starts), np.exp(-r2 / (2.0 * src.width ** 2))
+
+
+def lower_gaussian_source(src: GaussianSource, eps_ez, dx, dt, t0, total_steps):
+    """``sources/compiler.py:418-460`` for TMxy and 3D: coefficient
+    ``-profile dt / (eps0 eps_Ez)``, waveform at ``t0 + (i + 1/2) dt``,
+    timing ``pre_e``."""
+    eps_ez = np.asarray(eps_ez, dtype=np.float64)
+    is_3d = eps_ez.ndim == 3
+    if len(src.position) == 3 and not is_3d:
+        raise ValueError("a 3D GaussianSource position needs a 3D simulation")
+    start, prof = gaussian_profile(src, eps_ez.shape, dx, is_3d)
+    region = tuple(slice(a, a + n) for a, n in zip(start, prof.shape))
+    coeff = -prof * dt / (EPS_0 * eps_ez[region])
+    sig = src.signal
+    if hasattr(sig, "sample") or callable(sig):
+        wave = sample_waveform(sig, t0, dt, total_steps, half_step=True)
+    else:
+        times = float(t0) + (np.arange(int(total_steps)) + 0.5) * dt
+        wave = np.array([interpolate_signal(sig, t, dt) for t in times])
+    return CustomSource("Ez", start, coeff, wave, "pre_e")
+
+
 def lower_soft_source(src: SoftSource, eps_site, dt, t0, total_steps):
     region = tuple(slice(a, a + n) for a, n in zip(src.start, src.shape))
     eps = np.asarray(eps_site)[region]

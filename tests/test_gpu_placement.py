@@ -80,3 +80,49 @@ def test_placed_lines_2d():
     for i in (0, 2):
         _flux_close(res.monitor_flux(i), o.monitor_flux(prob, ost, i))
     assert np.all(res.monitor_flux(1) == 0.0) and np.all(o.monitor_flux(prob, ost, 1) == 0.0)
+
+
+def test_gaussian_source_and_placed_monitor_2d():
+    """A physically placed Gaussian source (the reference's GaussianSource)
+    driving placed monitors through ``Simulation.run``: against the oracle run
+    on the same lowered slab and plans."""
+    _require_gpu()
+    import beamz_amd as bz
+    from beamz_amd.sources import lower_gaussian_source
+    grid = (60, 120)
+    steps = 1200
+    eps = np.full(grid, 2.085)
+    eps[26:34, :] = 12.25
+    dt = 0.99 * DX / (o.LIGHT_SPEED * np.sqrt(2))
+    f0 = 1.93e14
+    # a sample array (the reference's form of the signal): the pulse peaks at
+    # 2e-14 s and has passed the x-normal monitor well before 1200 steps (5.6e-14 s)
+    signal = bz.GaussianPulse(f0, 1.5e14, offset=3.0, carrier="sin").sample(np.arange(steps) * dt)[0]
+    src = bz.GaussianSource((20.3 * DX, 30.0 * DX), 1.7 * DX, signal)
+    freqs = np.array([0.95 * f0, f0, 1.05 * f0])
+    mons = [bz.FluxMonitor(center=(95.2 * DX, 30.0 * DX), size=(0.0, 40.0 * DX, 1e-6), freqs=freqs),
+            bz.FluxMonitor(center=(60.0 * DX, 50.5 * DX), size=(90.0 * DX, 0.0, 1e-6), freqs=freqs)]
+    sim = bz.Simulation(eps, DX, dt=dt, boundaries=[bz.PML(cells=8, kappa_max=5.0)], sources=[src],
+                        monitors=mons, steps=steps)
+    prog = sim.compile()
+    cs = lower_gaussian_source(src, prog.eps_site["Ez"], DX, dt, 0.0, steps)
+    osrc = o.Source(cs.component, cs.timing, cs.start, np.asarray(cs.coeff, np.float32).astype(np.float64),
+                    np.asarray(cs.waveform, np.float32).astype(np.float64))
+    shapes = o.shapes_tm(grid)
+    omons = []
+    for i, m in enumerate(mons):
+        cm = bz.monitors.lower_monitor(m, grid, shapes, DX, i)
+        omons.append(o.Monitor(cm.freqs, cm.plans, cm.normal, cm.normal_sign, cm.measure))
+    prob = o.build_problem(eps, DX, dt, cpml=o.CpmlSpec(cells=8, kappa_max=5.0), sources=[osrc],
+                           monitors=omons)
+    ost = o.init_state(prob)
+    print(f"case gaussian_source_2d grid={grid} steps={steps}", flush=True)
+    res = sim.run()
+    torch.cuda.synchronize()
+    o.run(prob, ost, steps)
+    for c in prob.shapes:
+        from tests.test_gpu_parity import _close
+        _close(c, getattr(res.state, c.lower()).cpu().numpy(), ost[c.lower()])
+    for i in range(2):
+        _flux_close(res.monitor_flux(i), o.monitor_flux(prob, ost, i))
+    assert np.all(res.monitor_flux(0) > 0)
