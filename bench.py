@@ -305,8 +305,8 @@ def main():
                           "copy_ms": round(ms, 4), "copy_GBps": round(8 * n / ms / 1e6, 1)}))
         return
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(the z-slab sharding design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py runs on one GPU only: the library has no multi-GPU path; the value "
+              "is one GPU's throughput (the path would shard as z-slabs, one per GPU)", file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "batched2d":

@@ -5,7 +5,7 @@ and ``bench.py``'s CPU-baseline leg, never by the ``beamz_amd`` package.
 
 Every function cites the reference file:line whose behaviour it restates.
 The reference runs this arithmetic in JAX/XLA (float32 fields); here every
-array is float64 so the GPU result can be judged against an exact-arithmetic
+array is float64 so the GPU result can be compared with an exact-arithmetic
 stand-in.  Differences from the reference that are deliberate:
 
 * time is ``t0 + (step + 1) * dt`` evaluated in float64 instead of a running
