@@ -216,6 +216,7 @@ class CompiledMonitor:
     ranges: tuple = ()         # 3D: ((axis, (start, stop)), ...) of the plane's tangential cells
     placed: bool = False       # lowered from a physical centre and size
     position: float = None     # the normal coordinate of the samples (metres)
+    samples: tuple = ()        # placed 3D plane: the two tangential sample coordinate arrays
 
 
 def _frequencies(mon):
@@ -302,6 +303,7 @@ def lower_placed(mon: FluxMonitor, grid, shapes, dx, index):
         measure = plane_sample_area(coords[0], coords[1], dx)
         sign, cell_range = 1.0, ()
         plane_ranges = tuple((a, tuple(int(v) for v in ranges[a])) for a in tangential)
+        samples = tuple(coords)
     else:
         if mon.normal not in ("x", "y"):
             raise ValueError("a 2D line monitor is normal to x or y")
@@ -317,14 +319,15 @@ def lower_placed(mon: FluxMonitor, grid, shapes, dx, index):
         plane_index = int(xs[0]) if mon.normal == "x" else int(ys[0])
         tang = ys if mon.normal == "x" else xs
         cell_range = (int(tang[0]), int(tang[-1]) + 1)
-        plane_ranges = ()
+        plane_ranges, samples = (), ()
     points = next(p[0].shape[0] for p in plans.values() if p is not None)
     return CompiledMonitor(
         name=mon.name or f"monitor_{index}", freqs=freqs, plans=plans, points=points,
         normal=mon.normal, normal_sign=sign * float(mon.normal_sign), measure=measure,
         interval=max(1, int(mon.interval)), t_start=float(mon.t_start), t_end=float(mon.t_end),
         window_code=1 if mon.window == "hann" else 0, index=int(plane_index),
-        cell_range=cell_range, ranges=plane_ranges, placed=True, position=float(position))
+        cell_range=cell_range, ranges=plane_ranges, placed=True, position=float(position),
+        samples=samples)
 
 
 def assign_offsets(compiled):

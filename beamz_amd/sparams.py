@@ -117,12 +117,18 @@ def port_waves(results, port: Port, frequencies):
     if idx is None:
         raise ValueError(f"Missing monitor '{port.monitor}' for port '{port.name}'.")
     mon = prog.monitors[idx]
-    if getattr(mon, "placed", False):
-        raise ValueError(f"port '{port.name}': its monitor '{port.monitor}' is placed by centre and "
-                         "size; ports project onto index-space monitors (normal, index, ranges)")
     if mon.normal != port.axis:
         raise ValueError(f"port '{port.name}' axis {port.axis} does not match its monitor")
-    if np.ndim(prog.eps_site["Ez"]) == 3:
+    is_3d = np.ndim(prog.eps_site["Ez"]) == 3
+    if getattr(mon, "placed", False):
+        node = placed_port_plane(mon, prog.resolution) if is_3d else None
+        if node is None:
+            raise ValueError(
+                f"port '{port.name}': its monitor '{port.monitor}' is placed by centre and size off "
+                "the lattice; a port needs a 3D plane on an E-node plane with its edges on cell "
+                "faces, or an index-space monitor (normal, index, ranges)")
+        return _port_waves_3d(results, port, frequencies, idx, mon, node)
+    if is_3d:
         return _port_waves_3d(results, port, frequencies, idx, mon)
     if port.axis not in _TM_PARTS:
         raise ValueError("a 2D port is normal to x or y")
@@ -202,7 +208,21 @@ def port_projection_3d(eps_site, grid, dx, frequency, axis, index, ranges, mode_
             "condition_number": float(np.linalg.cond(mat)), "mode_neff": float(modes.neff[k])}
 
 
-def _port_waves_3d(results, port, frequencies, idx, mon):
+def placed_port_plane(mon, dx, tol=1e-6):
+    """The E-node plane index of a placed 3D monitor whose samples are those
+    of an index-space monitor (normal coordinate on an E-node plane, samples
+    on the cell centres of its snapped ranges), or None."""
+    node = float(mon.position) / float(dx)
+    if abs(node - round(node)) > tol or len(mon.samples) != 2:
+        return None
+    for (axis, (r0, r1)), coords in zip(mon.ranges, mon.samples):
+        want = (np.arange(r0, r1) + 0.5) * dx
+        if coords.shape != want.shape or np.max(np.abs(coords - want), initial=0.0) > tol * dx:
+            return None
+    return int(round(node))
+
+
+def _port_waves_3d(results, port, frequencies, idx, mon, index=None):
     prog = results.program
     parts = _PARTS_3D[port.axis]
     ph = results.monitor_phasors(idx)
@@ -215,8 +235,8 @@ def _port_waves_3d(results, port, frequencies, idx, mon):
         j = int(np.argmin(np.abs(mfreqs - f)))
         if not np.isclose(mfreqs[j], f, rtol=1e-12, atol=0.0):
             raise ValueError(f"monitor '{port.monitor}' holds no DFT at {f} Hz")
-        proj = port_projection_3d(prog.eps_site, prog.grid, prog.resolution, f, port.axis, mon.index,
-                                  mon.ranges, port.mode_index)
+        proj = port_projection_3d(prog.eps_site, prog.grid, prog.resolution, f, port.axis,
+                                  mon.index if index is None else index, mon.ranges, port.mode_index)
         vec = np.concatenate([ph[c][j] for c in parts])
         a = proj["pinv"] @ vec
         out["a_plus"][k], out["a_minus"][k] = a[0], a[1]

@@ -61,7 +61,12 @@ def test_straight_strip_s_parameters_3d():
                         bz.GaussianPulse(f0, bz.LIGHT_SPEED * 0.4e-6 / 1.55e-6 ** 2), "+")
     rng = {"z": (10, 46), "y": (10, 54)}
     mons = [bz.FluxMonitor("x", 70, freqs, ranges=rng, name="m_in"),
-            bz.FluxMonitor("x", 130, freqs, ranges=rng, name="m_out")]
+            bz.FluxMonitor("x", 130, freqs, ranges=rng, name="m_out"),
+            # the same planes placed by centre and size (the reference's form)
+            bz.FluxMonitor(center=(70 * dx, 32 * dx, 28 * dx), size=(0, 44 * dx, 36 * dx), freqs=freqs,
+                           name="p_in"),
+            bz.FluxMonitor(center=(130 * dx, 32 * dx, 28 * dx), size=(0, 44 * dx, 36 * dx), freqs=freqs,
+                           name="p_out")]
     sim = bz.Simulation(eps, dx, boundaries=[bz.PML(cells=10)], sources=[src], monitors=mons,
                         steps=4000)
     print(f"case sparams 3D strip grid={(nz, ny, nx)} steps=4000", flush=True)
@@ -77,3 +82,8 @@ def test_straight_strip_s_parameters_3d():
     assert np.all(np.abs(s21 - 1.0) < 0.03), s21   # measured 0.9986-0.9989
     assert np.all(s11 < 0.05), s11                  # measured 0.007-0.013
     np.testing.assert_allclose(s21 ** 2, ratio, rtol=0.03)
+    # ports on the placed planes: the same samples, so the same S-matrix
+    placed = s_parameters(res, "in", [Port("in", "p_in", "x", "+", polarization="vector"),
+                                      Port("out", "p_out", "x", "-", polarization="vector")])
+    for key in (("out", "in"), ("in", "in")):
+        np.testing.assert_allclose(placed.s_matrix[key], out.s_matrix[key], rtol=1e-5, atol=1e-7)

@@ -167,6 +167,26 @@ def test_2d_line_direct_sampling_and_sign():
     np.testing.assert_array_equal(cs.plans["Hy"][1][:, 0], 1.0)
 
 
+def test_placed_port_plane_alignment():
+    from beamz_amd.sparams import placed_port_plane
+    dx = 25e-9
+    grid = (56, 64, 170)
+    shapes = component_shapes(grid)
+    aligned = lower_monitor(FluxMonitor(center=(70 * dx, 32 * dx, 28 * dx), size=(0.0, 44 * dx, 36 * dx),
+                                        freqs=[2e14]), grid, shapes, dx, 0)
+    assert dict(aligned.ranges) == {"z": (10, 46), "y": (10, 54)}
+    assert placed_port_plane(aligned, dx) == 70
+    between = lower_monitor(FluxMonitor(center=(70.5 * dx, 32 * dx, 28 * dx),
+                                        size=(0.0, 44 * dx, 36 * dx), freqs=[2e14]), grid, shapes, dx, 0)
+    off_edges = lower_monitor(FluxMonitor(center=(70 * dx, 32.3 * dx, 28 * dx),
+                                          size=(0.0, 44 * dx, 36 * dx), freqs=[2e14]), grid, shapes, dx, 0)
+    assert placed_port_plane(between, dx) is None and placed_port_plane(off_edges, dx) is None
+    # an extent clamped at the domain edge keeps its physical sample spacing: not a port plane
+    clamped = lower_monitor(FluxMonitor(center=(70 * dx, 32 * dx, 28 * dx), size=(0.0, 70 * dx, 36 * dx),
+                                        freqs=[2e14]), grid, shapes, dx, 0)
+    assert dict(clamped.ranges)["y"] == (0, 64) and placed_port_plane(clamped, dx) is None
+
+
 def test_placed_monitors_are_refused_as_ports():
     from types import SimpleNamespace
     from beamz_amd.sparams import Port, port_waves
@@ -175,5 +195,5 @@ def test_placed_monitors_are_refused_as_ports():
     cm = lower_monitor(FluxMonitor(center=(12 * dx, 15 * dx), size=(0.0, 24 * dx, 1e-6), freqs=[2e14],
                                    name="p"), grid, component_shapes(grid), dx, 0)
     prog = SimpleNamespace(monitors=[cm], eps_site={"Ez": np.ones((31, 41))})
-    with pytest.raises(ValueError, match="centre and size"):
+    with pytest.raises(ValueError, match="off the lattice"):
         port_waves(SimpleNamespace(program=prog), Port("in", "p", "x"), [2e14])
