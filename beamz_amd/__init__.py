@@ -10,7 +10,7 @@ from .api import Simulation, SimulationResults, courant_dt, resolve_backend
 from .boundaries import PEC, PML
 from .constants import EPS_0, ETA_0, LIGHT_SPEED, MU_0, nm, um
 from .modes import ModeSource, ModeSpec
-from .monitors import FluxMonitor
+from .monitors import FieldMonitor, FluxMonitor
 from .native import HipBackendUnavailable, HipLaunchError
 from .program import SimulationState, compile_program, initial_state
 from .sources import CustomSource, GaussianPulse, GaussianSource, SoftSource
@@ -18,7 +18,7 @@ from .termination import AutoTermination, RunTermination
 
 __all__ = [
     "Simulation", "SimulationResults", "SimulationState", "compile_program", "initial_state",
-    "courant_dt", "resolve_backend", "PEC", "PML", "FluxMonitor", "CustomSource",
+    "courant_dt", "resolve_backend", "PEC", "PML", "FieldMonitor", "FluxMonitor", "CustomSource",
     "GaussianPulse", "GaussianSource", "SoftSource", "ModeSource", "ModeSpec", "AutoTermination",
     "RunTermination", "HipBackendUnavailable",
     "HipLaunchError",

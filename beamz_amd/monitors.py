@@ -78,6 +78,35 @@ class FluxMonitor:
         return self.center is not None
 
 
+@dataclass(frozen=True)
+class FieldMonitor(FluxMonitor):
+    """DFT fields of chosen components on a placed plane or line (the
+    reference's ``FieldMonitor``, ``monitors/monitors.py:341-367``): at least
+    one zero extent (the first names the normal), ``fields`` any of the six
+    components (in 2D the TMxy ones are recorded).  Lowered as a placed
+    ``FluxMonitor``; ``SimulationResults.monitor_phasors`` returns the fields."""
+
+    fields: tuple = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
+
+    def __post_init__(self):
+        if self.freqs is None or np.asarray(self.freqs).size == 0:
+            raise ValueError("FieldMonitor requires at least one frequency.")
+        if self.center is None or self.size is None:
+            raise ValueError("FieldMonitor placement requires both center and size.")
+        center, size = normalize_center_size(self.center, self.size)
+        if not np.any(np.isclose(np.asarray(size), 0.0, rtol=0.0, atol=1e-15)):
+            raise ValueError("FieldMonitor size must contain at least one zero extent.")
+        fields = tuple(str(c) for c in self.fields)
+        invalid = set(fields) - set(COMPONENTS)
+        if not fields or invalid:
+            raise ValueError(f"Unsupported FieldMonitor fields: {sorted(invalid)}")
+        object.__setattr__(self, "center", center)
+        object.__setattr__(self, "size", size)
+        object.__setattr__(self, "normal", "xyz"[int(np.argmin(np.abs(np.asarray(size))))])
+        object.__setattr__(self, "fields", fields)
+        object.__setattr__(self, "components", fields)
+
+
 def normalize_center_size(center, size):
     """3-tuples of a monitor's centre and size (``monitors.py:20-29``): finite
     centre, finite non-negative size, a 2-value tuple gets a zero z entry."""

@@ -36,11 +36,15 @@ def test_placed_planes_3d(chunk):
     mons = [bz.FluxMonitor(center=(23.3 * DX, 8.6 * DX, 6.2 * DX), size=(0.0, 9.7 * DX, 7.4 * DX),
                            freqs=freqs, name="x_plane"),
             bz.FluxMonitor(center=(20.0 * DX, 8.5 * DX, 9.55 * DX), size=(30.2 * DX, 11.0 * DX, 0.0),
-                           freqs=freqs[1:], name="z_plane", interval=2)]
+                           freqs=freqs[1:], name="z_plane", interval=2),
+            # all six components on a line along y (the reference's FieldMonitor)
+            bz.FieldMonitor(center=(17.3 * DX, 8.2 * DX, 5.7 * DX), size=(0.0, 9.4 * DX, 0.0),
+                            freqs=freqs, name="y_line")]
     sim, prog, state, prob, ost = _setup(grid, seed=81, cpml_cells=3, steps=steps,
                                          sources=[_plane_source(grid, dt, steps, 7)], monitors=mons)
     print(f"case placed_planes_3d grid={grid} chunk={chunk}", flush=True)
     assert all(m.placed for m in prog.monitors)
+    assert all(p is not None for p in prog.monitors[2].plans.values())
     assert max(p[0].shape[1] for m in prog.monitors for p in m.plans.values() if p is not None) == 8
     state = sim.advance(state, steps, chunk=chunk)
     torch.cuda.synchronize()

@@ -197,3 +197,32 @@ def test_placed_monitors_are_refused_as_ports():
     prog = SimpleNamespace(monitors=[cm], eps_site={"Ez": np.ones((31, 41))})
     with pytest.raises(ValueError, match="off the lattice"):
         port_waves(SimpleNamespace(program=prog), Port("in", "p", "x"), [2e14])
+
+
+def test_field_monitors_and_ragged_arenas():
+    """The reference's ragged-arena case (``tests/integration/test_compiled_engine.py:1226-1262``):
+    two Ez field monitors on a 1 x 1 domain at resolution 0.1, with 4 and 1
+    frequencies, packed back to back."""
+    from beamz_amd.monitors import FieldMonitor, assign_offsets
+    grid = (10, 10)
+    shapes = component_shapes(grid)
+    many = FieldMonitor(center=(0.3, 0.5, 0.0), size=(0.0, 0.8, 0.0), freqs=(1e9, 2e9, 3e9, 4e9),
+                        fields=("Ez",), name="many_points")
+    few = FieldMonitor(center=(0.7, 0.5, 0.0), size=(0.0, 0.2, 0.0), freqs=(2e9,), fields=("Ez",),
+                       name="few_points")
+    cm = [lower_monitor(m, grid, shapes, 0.1, i) for i, m in enumerate((many, few))]
+    assert [m.points for m in cm] == [9, 3]          # nodes y = 1..9 and 4..6
+    assert [k for k, v in cm[0].plans.items() if v is not None] == ["Ez"]
+    values, weights = assign_offsets(cm)
+    assert (cm[0].value_offset, cm[1].value_offset) == (0, 6 * 4 * 9)
+    assert (cm[0].weight_offset, cm[1].weight_offset) == (0, 4)
+    assert values == 6 * 4 * 9 + 6 * 1 * 3 and weights == 5
+    # field monitors need one zero extent at least and known components
+    with pytest.raises(ValueError, match="at least one zero"):
+        FieldMonitor(center=(0.0, 0.0, 0.0), size=(1.0, 1.0, 1.0), freqs=(1e9,))
+    with pytest.raises(ValueError, match="Unsupported"):
+        FieldMonitor(center=(0.0, 0.0, 0.0), size=(0.0, 1.0, 1.0), freqs=(1e9,), fields=("Bz",))
+    line = lower_monitor(FieldMonitor(center=(5.0, 4.0, 3.0), size=(0.0, 4.0, 0.0), freqs=(1e9,)),
+                         (8, 9, 10), component_shapes((8, 9, 10)), 1.0, 0)
+    assert dict(line.ranges) == {"z": (3, 4), "y": (2, 6)} and line.points == 4
+    assert all(p is not None for p in line.plans.values())
