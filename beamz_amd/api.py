@@ -54,6 +54,29 @@ class SimulationPerformance:
         return self.cells * self.steps / max(self.runtime_s, 1e-30) / 1e9
 
 
+@dataclass(frozen=True)
+class MonitorResult:
+    """One monitor's frequency-domain acquisition, keyed by name on
+    ``SimulationResults`` (the DFT part of ``results.py:537-620``, ``830-880``).
+
+    ``dft_fields``: the raw accumulations sum_t w s m cos/sin(2 pi f t) v as
+    complex [F, P] arrays per recorded component (what the reference stores);
+    ``phasors``: 2 DFT / wsum with the H half-step phase (``observe.py:166-187``);
+    ``flux``: the frequency-domain flux (``observe.py:198-234``)."""
+
+    name: str
+    dft_fields: dict
+    dft_frequencies: np.ndarray
+    dft_weight_sum: np.ndarray
+    dft_base_dt: float
+    resolution: float
+    normal_axis: int
+    normal_sign: float
+    power_scale: float
+    phasors: dict
+    flux: np.ndarray
+
+
 @dataclass
 class SimulationResults:
     state: object
@@ -101,6 +124,36 @@ class SimulationResults:
         s = self.state
         return _phasors(m, s.dft_re.cpu().numpy(), s.dft_im.cpu().numpy(), s.dft_w.cpu().numpy(),
                         self.program.dt)
+
+    @property
+    def monitors(self):
+        """``{name: MonitorResult}`` of every monitor, in program order."""
+        s = self.state
+        re, im, w = (s.dft_re.cpu().numpy(), s.dft_im.cpu().numpy(), s.dft_w.cpu().numpy())
+        out = {}
+        for m in self.program.monitors:
+            F, P = m.freqs.size, m.points
+            raw = {}
+            for c, name in enumerate(("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")):
+                if m.plans.get(name) is None:
+                    continue
+                sl = slice(m.value_offset + c * F * P, m.value_offset + (c + 1) * F * P)
+                raw[name] = (np.asarray(re[sl], np.float64) + 1j * np.asarray(im[sl], np.float64)).reshape(F, P)
+            out[m.name] = MonitorResult(
+                name=m.name, dft_fields=raw, dft_frequencies=np.asarray(m.freqs, float),
+                dft_weight_sum=np.asarray(w[m.weight_offset:m.weight_offset + F], np.float64),
+                dft_base_dt=float(self.program.dt), resolution=float(self.program.resolution),
+                normal_axis={"x": 0, "y": 1, "z": 2}[m.normal], normal_sign=float(m.normal_sign),
+                power_scale=float(m.measure), phasors=_phasors(m, re, im, w, self.program.dt),
+                flux=_flux(m, re, im, w, self.program.dt))
+        return out
+
+    def __getitem__(self, name):
+        """The ``MonitorResult`` of the monitor called ``name``."""
+        mons = self.monitors
+        if name not in mons:
+            raise KeyError(f"no monitor named {name!r}; monitors: {sorted(mons)}")
+        return mons[name]
 
     def field(self, name):
         """Field in physical units (half-precision storage is rescaled)."""
