@@ -176,6 +176,11 @@ class Simulation:
         ``SoftSource``/``CustomSource`` and ``FluxMonitor``.
     steps : int
         Number of time steps of ``run``.
+    time : array, optional
+        The reference's time grid instead of ``dt``/``steps``/``t0``: finite,
+        strictly increasing, uniformly spaced (relative 1e-7), at least two
+        entries; then dt = time[1] - time[0], steps = len(time), t0 = time[0]
+        (``simulation/api.py:474-489``, ``799-816``).
     field_dtype : torch.float32, torch.float16 or torch.bfloat16
         Field storage.  Half-precision fields are stored scaled (E / s_e,
         H / s_h with s_h = s_e / eta0); ``field_scale`` sets s_e, by default
@@ -185,7 +190,19 @@ class Simulation:
     def __init__(self, permittivity, resolution, *, dt=None, courant=0.99, boundaries=(),
                  sources=(), monitors=(), steps=1, t0=0.0, conductivity=None,
                  backend="auto", field_dtype=torch.float32, device=None,
-                 flags=abi.DEFAULT_FLAGS | abi.FLAG_FUSED_STEP, field_scale=None):
+                 flags=abi.DEFAULT_FLAGS | abi.FLAG_FUSED_STEP, field_scale=None, time=None):
+        if time is not None:
+            values = np.array(time, dtype=float)
+            if values.ndim != 1 or values.size < 2:
+                raise ValueError("FDTD requires a 1D time array with at least two entries")
+            gaps = np.diff(values)
+            if not np.all(np.isfinite(values)) or np.any(gaps <= 0.0):
+                raise ValueError("Simulation time values must be finite and strictly increasing")
+            if not np.allclose(gaps, gaps[0], rtol=1e-7, atol=0.0):
+                raise ValueError("Simulation time values must be uniformly spaced")
+            if dt is not None and not np.isclose(float(dt), gaps[0], rtol=1e-7, atol=0.0):
+                raise ValueError("dt disagrees with the time grid")
+            dt, steps, t0 = float(gaps[0]), int(values.size), float(values[0])
         self.permittivity = np.asarray(permittivity, dtype=np.float64)
         self.resolution = float(resolution)
         self.dt = float(dt) if dt is not None else courant_dt(resolution, self.permittivity.ndim, courant)

@@ -227,34 +227,3 @@ def test_field_monitors_and_ragged_arenas():
     assert dict(line.ranges) == {"z": (3, 4), "y": (2, 6)} and line.points == 4
     assert all(p is not None for p in line.plans.values())
 
-
-def test_results_by_monitor_name():
-    """``results["name"]`` views the arenas as the reference's monitor results
-    do: raw DFT sums per component, phasors 2 DFT / wsum, and the flux."""
-    from types import SimpleNamespace
-    import torch
-    from beamz_amd.api import SimulationResults
-    from beamz_amd.monitors import assign_offsets
-    grid = (10, 10)
-    shapes = component_shapes(grid)
-    mons = [lower_monitor(FluxMonitor(center=(0.3, 0.5), size=(0.0, 0.8, 1.0), freqs=(1e9, 2e9),
-                                      name="a"), grid, shapes, 0.1, 0),
-            lower_monitor(FluxMonitor(center=(0.5, 0.7), size=(0.6, 0.0, 1.0), freqs=(3e9,),
-                                      name="b"), grid, shapes, 0.1, 1)]
-    nv, nw = assign_offsets(mons)
-    rng = np.random.default_rng(3)
-    state = SimpleNamespace(dft_re=torch.tensor(rng.normal(size=nv)), dft_im=torch.tensor(rng.normal(size=nv)),
-                            dft_w=torch.tensor(rng.uniform(1.0, 2.0, size=nw)))
-    prog = SimpleNamespace(monitors=mons, dt=1e-17, resolution=0.1)
-    res = SimulationResults(state=state, program=prog, performance=None)
-    b = res["b"]
-    assert set(b.dft_fields) == {"Ez", "Hx"} and b.dft_fields["Ez"].shape == (1, 7)
-    m = mons[1]
-    ez = slice(m.value_offset + 2 * 7, m.value_offset + 3 * 7)
-    np.testing.assert_array_equal(b.dft_fields["Ez"][0], state.dft_re.numpy()[ez] + 1j * state.dft_im.numpy()[ez])
-    np.testing.assert_allclose(b.phasors["Ez"], 2.0 * b.dft_fields["Ez"] / b.dft_weight_sum[:, None])
-    np.testing.assert_array_equal(b.flux, res.monitor_flux(1))
-    assert (b.normal_axis, b.normal_sign, b.power_scale) == (1, -1.0, pytest.approx(0.1))
-    assert list(res.monitors) == ["a", "b"]
-    with pytest.raises(KeyError, match="no monitor"):
-        res["c"]
