@@ -22,6 +22,40 @@ import time
 import numpy as np
 
 
+WORKLOADS = {
+    "3d": "3D Yee+CPML 1024x512x256 (x,y,z) at 25 nm, 16-cell CPML, seeded random-eps 512x256x64 "
+          "slab, Ez plane source, 2 x 21-wavelength FP64 DFT flux planes",
+    "waveguide3d": "3D silicon waveguide 512x256x128 (x,y,z) at 25 nm, 450x220 nm Si core in SiO2 "
+                   "along x, 12-cell CPML, 1.55 um Gaussian-pulse Ez plane source at x=30, "
+                   "3 x 11-wavelength FP64 DFT flux planes",
+}
+
+
+def build_waveguide_workload(dtype_name, monitors=True, fused=True):
+    """The 512x256x128 silicon strip: eps 12.25 core of 9 x 18 cells
+    (220 x 450 nm) centred in eps 2.085, running along x."""
+    import torch
+    import beamz_amd as bz
+    from beamz_amd import _abi
+
+    nz, ny, nx, dx, L = 128, 256, 512, 25e-9, 12
+    dt = 0.99 * dx / (bz.LIGHT_SPEED * np.sqrt(3.0))
+    eps = np.full((nz, ny, nx), 2.085)
+    eps[59:68, 119:137, :] = 12.25
+    f0 = bz.LIGHT_SPEED / 1.55e-6
+    fw = bz.LIGHT_SPEED * 0.2e-6 / 1.55e-6 ** 2
+    src = bz.SoftSource("Ez", (L, L, 30), (nz - 2 * L, ny + 1 - 2 * L, 1), bz.GaussianPulse(f0, fw))
+    freqs = bz.LIGHT_SPEED / np.linspace(1.5e-6, 1.6e-6, 11)
+    rngs = {"z": (L, nz - L), "y": (L, ny - L)}
+    mons = [bz.FluxMonitor("x", xi, freqs, ranges=rngs) for xi in (60, 256, 450)]
+    field_dtype = {"f32": torch.float32, "bf16": torch.bfloat16, "f16": torch.float16}[dtype_name]
+    flags = _abi.DEFAULT_FLAGS | (_abi.FLAG_FUSED_STEP if fused else 0)
+    sim = bz.Simulation(eps, dx, dt=dt, boundaries=[bz.PML(cells=L)], sources=[src],
+                        monitors=mons if monitors else [], steps=4000, field_dtype=field_dtype,
+                        flags=flags)
+    return sim, eps
+
+
 def build_workload(dtype_name, monitors=True, fused=True):
     import torch
     import beamz_amd as bz
@@ -263,8 +297,8 @@ def cpu_baseline(eps, prog, budget_s=12.0):
     steps = max(2, int(budget_s / max(t1, 1e-3)))
     gcups, tt = port.time_gcups(pg, prog.cells, steps)
     return {"value": round(gcups, 4), "unit": "GCUPS", "cores": pg.threads(), "kind": "port",
-            "sample": f"full 1024x512x256 grid with 16-cell CPML, {steps} FP32 steps "
-                      f"({tt:.1f} s) of oracle/port.c (H+E+CPML; no source/DFT)"}
+            "sample": f"full {grid[2]}x{grid[1]}x{grid[0]} grid with {prog.cpml.cells}-cell CPML, "
+                      f"{steps} FP32 steps ({tt:.1f} s) of oracle/port.c (H+E+CPML; no source/DFT)"}
 
 
 def main():
@@ -277,8 +311,9 @@ def main():
     ap.add_argument("--no-monitors", action="store_true")
     ap.add_argument("--per-phase", action="store_true", help="use the per-phase kernels")
     ap.add_argument("--chunk", type=int, default=256)
-    ap.add_argument("--workload", default="3d", choices=("3d", "batched2d", "adjoint2d"),
-                    help="3d: the 1024x512x256 Yee+CPML step (default); batched2d: the 64-simulation "
+    ap.add_argument("--workload", default="3d", choices=("3d", "waveguide3d", "batched2d", "adjoint2d"),
+                    help="3d: the 1024x512x256 Yee+CPML step (default); waveguide3d: the 512x256x128 "
+                         "silicon strip; batched2d: the 64-simulation "
                          "2D sweep in one persistent launch per step; adjoint2d: the 600x600 bend "
                          "gradient and its finite-difference check")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -320,7 +355,8 @@ def main():
     from beamz_amd import native, runtime
 
     print(f"[bench] building workload dtype={args.dtype}", file=sys.stderr, flush=True)
-    sim, eps = build_workload(args.dtype, monitors=not args.no_monitors, fused=not args.per_phase)
+    build = build_waveguide_workload if args.workload == "waveguide3d" else build_workload
+    sim, eps = build(args.dtype, monitors=not args.no_monitors, fused=not args.per_phase)
     prog = sim.compile()
     state = sim.initial_state()
     stream = torch.cuda.Stream()
@@ -380,8 +416,7 @@ def main():
         "dtype": args.dtype,
         "data": "synthetic",
         "config": {
-            "workload": "3D Yee+CPML 1024x512x256 (x,y,z) at 25 nm, 16-cell CPML, seeded "
-                        "random-eps 512x256x64 slab, Ez plane source, 2 x 21-wavelength FP64 DFT flux planes",
+            "workload": WORKLOADS[args.workload],
             "cells": prog.cells,
             "field_storage": args.dtype,
             "schedule": "per-phase" if args.per_phase else (
@@ -393,7 +428,9 @@ def main():
             "peak": peak,
             "unit": "GB/s",
             "frac": round(achieved / peak, 4),
-            "traffic": measured_traffic(runtime.step_kernel_name(prog, pairs), args.dtype),
+            # the committed rocprofv3 byte counts cover the 1024x512x256 workload
+            "traffic": (measured_traffic(runtime.step_kernel_name(prog, pairs), args.dtype)
+                        if args.workload == "3d" else None),
             "kernel": runtime.step_kernel_name(prog, pairs),
             "kernel_ms": round(k_ms, 4),
             "algorithmic_bytes_per_launch": int(nb),
