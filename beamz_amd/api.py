@@ -188,7 +188,7 @@ class Simulation:
     """
 
     def __init__(self, permittivity, resolution, *, dt=None, courant=0.99, boundaries=(),
-                 sources=(), monitors=(), steps=1, t0=0.0, conductivity=None,
+                 sources=(), monitors=(), steps=None, t0=None, conductivity=None,
                  backend="auto", field_dtype=torch.float32, device=None,
                  flags=abi.DEFAULT_FLAGS | abi.FLAG_FUSED_STEP, field_scale=None, time=None):
         if time is not None:
@@ -202,7 +202,14 @@ class Simulation:
                 raise ValueError("Simulation time values must be uniformly spaced")
             if dt is not None and not np.isclose(float(dt), gaps[0], rtol=1e-7, atol=0.0):
                 raise ValueError("dt disagrees with the time grid")
+            if steps is not None and int(steps) != values.size:
+                raise ValueError(f"steps={steps} disagrees with the time grid ({values.size} samples)")
+            if t0 is not None and not np.isclose(float(t0), values[0], rtol=1e-12,
+                                                 atol=1e-12 * abs(gaps[0])):
+                raise ValueError(f"t0={t0} disagrees with the time grid (starts at {values[0]})")
             dt, steps, t0 = float(gaps[0]), int(values.size), float(values[0])
+        steps = 1 if steps is None else steps
+        t0 = 0.0 if t0 is None else t0
         self.permittivity = np.asarray(permittivity, dtype=np.float64)
         self.resolution = float(resolution)
         self.dt = float(dt) if dt is not None else courant_dt(resolution, self.permittivity.ndim, courant)

@@ -416,9 +416,14 @@ class GraphCache {
   }
 
   // Launch ``launches`` on ``s``; replay a cached graph when the kernel
-  // arguments match a previous call exactly.
+  // arguments match a previous call exactly.  A stream the caller is itself
+  // capturing gets the kernels directly (they become part of the caller's
+  // graph).  A full cache evicts its least recently launched entry once that
+  // entry's last launch has completed.
   void run(hipStream_t s, const std::vector<Launch>& launches, bool use_graph) {
-    if (!use_graph || s == nullptr) {
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (use_graph && s != nullptr) BZH_HIP(hipStreamIsCapturing(s, &cap));
+    if (!use_graph || s == nullptr || cap != hipStreamCaptureStatusNone) {
       for (const auto& l : launches) enqueue(s, l);
       return;
     }
@@ -432,14 +437,17 @@ class GraphCache {
       key.append(reinterpret_cast<const char*>(&l.dtype), sizeof(l.dtype));
       key.append(reinterpret_cast<const char*>(l.bytes.data()), l.bytes.size());
     }
-    hipGraphExec_t exec = nullptr;
     {
       std::lock_guard<std::mutex> lock(mu_);
       auto it = cache_.find(key);
-      if (it != cache_.end()) exec = it->second;
+      if (it != cache_.end()) {
+        launch(it->second, s);
+        return;
+      }
     }
-    if (exec == nullptr) {
+    {
       hipGraph_t graph = nullptr;
+      hipGraphExec_t exec = nullptr;
       if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
         (void)hipGetLastError();
         for (const auto& l : launches) enqueue(s, l);
@@ -466,14 +474,20 @@ class GraphCache {
         return;
       }
       (void)hipGraphDestroy(graph);
+      hipEvent_t done = nullptr;
+      BZH_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
       std::lock_guard<std::mutex> lock(mu_);
-      if (cache_.size() >= kCapacity) {
-        for (auto& kv : cache_) (void)hipGraphExecDestroy(kv.second);
-        cache_.clear();
+      auto it = cache_.find(key);
+      if (it != cache_.end()) {
+        // another thread cached the same launch list meanwhile
+        (void)hipGraphExecDestroy(exec);
+        (void)hipEventDestroy(done);
+      } else {
+        if (cache_.size() >= kCapacity) evict_oldest();
+        it = cache_.emplace(key, Entry{exec, done, 0}).first;
       }
-      cache_[key] = exec;
+      launch(it->second, s);
     }
-    BZH_HIP(hipGraphLaunch(exec, s));
   }
 
   size_t size() {
@@ -482,9 +496,33 @@ class GraphCache {
   }
 
  private:
+  struct Entry {
+    hipGraphExec_t exec;
+    hipEvent_t done;   // recorded after each launch of ``exec``
+    uint64_t tick;     // launch order
+  };
+  // Called with mu_ held.
+  void launch(Entry& e, hipStream_t s) {
+    e.tick = ++tick_;
+    BZH_HIP(hipGraphLaunch(e.exec, s));
+    BZH_HIP(hipEventRecord(e.done, s));
+  }
+  // Called with mu_ held.  Waits for the entry's last launch to complete
+  // before destroying its executable.
+  void evict_oldest() {
+    auto victim = cache_.begin();
+    for (auto it = cache_.begin(); it != cache_.end(); ++it)
+      if (it->second.tick < victim->second.tick) victim = it;
+    BZH_HIP(hipEventSynchronize(victim->second.done));
+    (void)hipGraphExecDestroy(victim->second.exec);
+    (void)hipEventDestroy(victim->second.done);
+    cache_.erase(victim);
+  }
+
   static constexpr size_t kCapacity = 32;
   std::mutex mu_;
-  std::map<std::string, hipGraphExec_t> cache_;
+  uint64_t tick_ = 0;
+  std::map<std::string, Entry> cache_;
 };
 
 GraphCache& graph_cache() {

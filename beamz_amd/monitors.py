@@ -255,6 +255,16 @@ def _frequencies(mon):
     return freqs
 
 
+def _points(plans, wanted, is_3d):
+    """Sample count of a lowered monitor; a monitor none of whose components
+    exists in this dimensionality (a 2D ``fields=("Ex",)``) is refused."""
+    for p in plans.values():
+        if p is not None:
+            return p[0].shape[0]
+    kind = "3D" if is_3d else "2D TMxy (Ez, Hx, Hy)"
+    raise ValueError(f"monitor fields {tuple(wanted)} include no component of a {kind} simulation")
+
+
 def lower_monitor(mon: FluxMonitor, grid, shapes, dx, index):
     if mon.placed:
         return lower_placed(mon, grid, shapes, dx, index)
@@ -282,7 +292,7 @@ def lower_monitor(mon: FluxMonitor, grid, shapes, dx, index):
         measure = dx  # _line_integral_scale_2d (monitors.py:61-68)
     cell_range = rng if not is_3d else ()
     plane_ranges = tuple((a, tuple(int(v) for v in ranges[a])) for a in tangential) if is_3d else ()
-    points = next(p[0].shape[0] for p in plans.values() if p is not None)
+    points = _points(plans, wanted, is_3d)
     return CompiledMonitor(
         name=mon.name or f"monitor_{index}", freqs=freqs, plans=plans, points=points,
         normal=mon.normal, normal_sign=float(mon.normal_sign), measure=measure,
@@ -349,7 +359,7 @@ def lower_placed(mon: FluxMonitor, grid, shapes, dx, index):
         tang = ys if mon.normal == "x" else xs
         cell_range = (int(tang[0]), int(tang[-1]) + 1)
         plane_ranges, samples = (), ()
-    points = next(p[0].shape[0] for p in plans.values() if p is not None)
+    points = _points(plans, wanted, len(grid) == 3)
     return CompiledMonitor(
         name=mon.name or f"monitor_{index}", freqs=freqs, plans=plans, points=points,
         normal=mon.normal, normal_sign=sign * float(mon.normal_sign), measure=measure,

@@ -378,6 +378,12 @@ def main():
 
     with torch.cuda.stream(stream):
         run(args.warmup)
+        # Replay-only timing: a chunk's graph is keyed on its step count and the
+        # bank it starts from, so the exact chunk sequence of the timed window
+        # runs once beforehand from the same bank (twice for an odd count,
+        # which swaps the banks), and the timed window replays captured graphs.
+        for _ in range(2 if args.steps % 2 else 1):
+            run(args.steps)
     torch.cuda.synchronize()
     start = torch.cuda.Event(enable_timing=True)
     stop = torch.cuda.Event(enable_timing=True)
@@ -398,7 +404,8 @@ def main():
     # x-interior plus x-edge kernels), or two where the pair kernel runs;
     # algorithmic bytes per launch = that many full steps
     per_launch = 2 if pairs else 1
-    nb = algorithmic_bytes(prog, state) * per_launch
+    step_bytes = algorithmic_bytes(prog, state)
+    nb = step_bytes * per_launch
     k_ms = runtime.time_step_kernel(state, prog, launches=20, stream=stream) * per_launch
     achieved = nb / (k_ms * 1e-3) / 1e9
     peak = 8000.0
@@ -428,6 +435,9 @@ def main():
             "peak": peak,
             "unit": "GB/s",
             "frac": round(achieved / peak, 4),
+            # the whole step (sources, DFT, every launch) against the same
+            # algorithmic bytes per step
+            "frac_step": round(step_bytes / (ms * 1e-3) / 1e9 / peak, 4),
             # the committed rocprofv3 byte counts cover the 1024x512x256 workload
             "traffic": (measured_traffic(runtime.step_kernel_name(prog, pairs), args.dtype)
                         if args.workload == "3d" else None),

@@ -58,3 +58,20 @@ def test_simulation_time_grid():
         Simulation(eps, 20e-9, time=np.array([0.0]))
     with pytest.raises(ValueError, match="disagrees"):
         Simulation(eps, 20e-9, time=t, dt=5e-17)
+    with pytest.raises(ValueError, match="steps=.*disagrees"):
+        Simulation(eps, 20e-9, time=t, steps=40)
+    with pytest.raises(ValueError, match="t0=.*disagrees"):
+        Simulation(eps, 20e-9, time=t, t0=0.0)
+    same = Simulation(eps, 20e-9, time=t, steps=50, t0=3e-17, dt=4e-17)
+    assert same.steps == 50 and same.t0 == pytest.approx(3e-17)
+    plain = Simulation(eps, 20e-9)
+    assert plain.steps == 1 and plain.t0 == 0.0
+
+
+def test_field_monitor_without_components_of_the_dimensionality():
+    """A 2D ``FieldMonitor(fields=("Ex",))`` names no TMxy component: refused
+    with the fields named, not a bare StopIteration."""
+    from beamz_amd import FieldMonitor, compile_program
+    mon = FieldMonitor(center=(2e-7, 1e-7), size=(0.0, 1e-7), freqs=[1e14], fields=("Ex",))
+    with pytest.raises(ValueError, match=r"\('Ex',\).*2D TMxy"):
+        compile_program(np.ones((8, 10)), 20e-9, 4e-17, monitors=[mon], device="cpu")
