@@ -181,6 +181,8 @@ class Simulation:
         strictly increasing, uniformly spaced (relative 1e-7), at least two
         entries; then dt = time[1] - time[0], steps = len(time), t0 = time[0]
         (``simulation/api.py:474-489``, ``799-816``).
+    psi_dtype : torch.float32 or torch.bfloat16, optional
+        CPML memory storage (default: ``BEAMZ_HIP_CPML_PSI_PRECISION``, fp32).
     field_dtype : torch.float32, torch.float16 or torch.bfloat16
         Field storage.  Half-precision fields are stored scaled (E / s_e,
         H / s_h with s_h = s_e / eta0); ``field_scale`` sets s_e, by default
@@ -190,7 +192,8 @@ class Simulation:
     def __init__(self, permittivity, resolution, *, dt=None, courant=0.99, boundaries=(),
                  sources=(), monitors=(), steps=None, t0=None, conductivity=None,
                  backend="auto", field_dtype=torch.float32, device=None,
-                 flags=abi.DEFAULT_FLAGS | abi.FLAG_FUSED_STEP, field_scale=None, time=None):
+                 flags=abi.DEFAULT_FLAGS | abi.FLAG_FUSED_STEP, field_scale=None, time=None,
+                 psi_dtype=None):
         if time is not None:
             values = np.array(time, dtype=float)
             if values.ndim != 1 or values.size < 2:
@@ -224,6 +227,7 @@ class Simulation:
         self.device = device
         self.flags = flags
         self.field_scale = field_scale
+        self.psi_dtype = psi_dtype
         self._program = None
 
     def compile(self):
@@ -233,7 +237,8 @@ class Simulation:
                 self.permittivity, self.resolution, self.dt, boundaries=self.boundaries,
                 sources=self.sources, monitors=self.monitors, num_steps=self.steps,
                 t0=self.t0, conductivity=self.conductivity, device=self.device,
-                field_dtype=self.field_dtype, flags=self.flags, field_scale=self.field_scale)
+                field_dtype=self.field_dtype, flags=self.flags, field_scale=self.field_scale,
+                psi_dtype=self.psi_dtype)
         return self._program
 
     def initial_state(self, fields=None):

@@ -110,6 +110,47 @@ class Adjoint2d(ctypes.Structure):
     ]
 
 
+class Launch(ctypes.Structure):
+    """``bzh_launch``: one Yee phase by descriptor (``cuda/src/launch.h:19-35``)."""
+    _fields_ = [
+        ("abi_version", ctypes.c_int32), ("flags", ctypes.c_int32), ("phase", ctypes.c_int32),
+        ("nterms", ctypes.c_int32), ("metric_kind", ctypes.c_int32), ("dt", ctypes.c_float),
+        ("resolution", ctypes.c_float), ("inv_resolution", ctypes.c_float),
+        ("dt_over_eps", ctypes.c_float), ("dt_over_mu", ctypes.c_float),
+        ("metallic_edges", ctypes.c_int32), ("uniform_cpml_thickness", ctypes.c_int32),
+        ("inputs", Buffer * 37), ("metrics", Buffer * 3), ("outputs", Buffer * 9),
+    ]
+
+
+class SourceGroup(ctypes.Structure):
+    """``bzh_source_group`` (``cuda/src/launch.h:37-45``)."""
+    _fields_ = [
+        ("coefficients", Buffer), ("waveforms", Buffer), ("starts", Buffer),
+        ("current_step", Buffer), ("component", ctypes.c_int32), ("timing", ctypes.c_int32),
+        ("coincident", ctypes.c_int32),
+    ]
+
+
+class DftGroup(ctypes.Structure):
+    """``bzh_dft_group`` (``cuda/src/launch.h:47-61``)."""
+    _fields_ = [
+        ("indices", Buffer), ("weights", Buffer), ("frequencies", Buffer),
+        ("component_masks", Buffer), ("counts", Buffer), ("codes", Buffer), ("windows", Buffer),
+        ("dft_re", Buffer), ("dft_im", Buffer), ("dft_weight", Buffer), ("time", Buffer),
+        ("current_step", Buffer), ("monitor_count", ctypes.c_int32),
+    ]
+
+
+class ProgramLaunch(ctypes.Structure):
+    """``bzh_program_launch`` (``cuda/src/launch.h:63-77``)."""
+    _fields_ = [
+        ("h_ab", Launch), ("e_ab", Launch), ("h_ba", Launch), ("e_ba", Launch),
+        ("source_groups", ctypes.POINTER(SourceGroup)), ("source_group_count", ctypes.c_int32),
+        ("monitors", ctypes.POINTER(DftGroup)), ("field_bank_count", ctypes.c_int32),
+        ("nsteps", ctypes.c_int32),
+    ]
+
+
 # Every exported symbol and its (restype, argtypes); checked by the tests
 # against the declarations of include/beamz_hip.h.
 SIGNATURES = {
@@ -119,8 +160,8 @@ SIGNATURES = {
     "bzh_last_error": (ctypes.c_char_p, []),
     "bzh_last_program_fused": (ctypes.c_int32, []),
     "bzh_last_program_pairs": (ctypes.c_int32, []),
-    "bzh_launch_phase": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p]),
-    "bzh_launch_program": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p]),
+    "bzh_launch_phase": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(Launch)]),
+    "bzh_launch_program": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(ProgramLaunch)]),
     "bzh_streamed": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(Buffer), ctypes.c_int32,
                                       ctypes.POINTER(PhaseAttrs)]),
     "bzh_program": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(Buffer), ctypes.c_int32,
@@ -256,3 +297,18 @@ def call_adjoint_reverse(adj, n_last, count, slot_last, flags, stream=None):
     s = current_stream_handle() if stream is None else stream
     _check(lib.bzh_adjoint2d_reverse(s, ctypes.byref(adj), int(n_last), int(count),
                                      int(slot_last), int(flags)), lib)
+
+
+def call_launch_phase(launch: Launch, stream=None):
+    """``bzh_launch_phase``: one phase described by a :class:`Launch`."""
+    lib = library()
+    s = current_stream_handle() if stream is None else stream
+    _check(lib.bzh_launch_phase(s, ctypes.byref(launch)), lib)
+
+
+def call_launch_program(program: ProgramLaunch, stream=None):
+    """``bzh_launch_program``: a multi-step program described by a
+    :class:`ProgramLaunch` (the tensors it points to must stay alive)."""
+    lib = library()
+    s = current_stream_handle() if stream is None else stream
+    _check(lib.bzh_launch_program(s, ctypes.byref(program)), lib)

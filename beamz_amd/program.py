@@ -77,6 +77,7 @@ class CompiledProgram:
     device: torch.device
     field_dtype: torch.dtype
     flags: int
+    psi_dtype: torch.dtype
     abi_resolution: float = None   # cell size handed to the kernels (1 when folded)
     cpml_table: torch.Tensor = None  # [terms, 3, 2L] when every CPML term has thickness L
     field_scale: dict = None       # storage scale of the E and H fields
@@ -145,11 +146,34 @@ def h_source_slabs(groups):
     return sum(int(g[0].shape[0]) for g in groups[3:6] if g is not None)
 
 
+def psi_precision_from_env():
+    """CPML memory storage chosen by ``BEAMZ_HIP_CPML_PSI_PRECISION`` (``fp32``
+    or ``bf16``), the counterpart of the reference's
+    ``BEAMZ_CUDA_CPML_PSI_PRECISION`` (``simulation/backend.py:45-56``)."""
+    import os
+    value = os.environ.get("BEAMZ_HIP_CPML_PSI_PRECISION", "fp32").strip().lower()
+    if value in ("bf16", "bfloat16"):
+        return torch.bfloat16
+    if value in ("fp32", "float32", ""):
+        return torch.float32
+    raise ValueError("BEAMZ_HIP_CPML_PSI_PRECISION must be 'fp32' or 'bf16'")
+
+
 def compile_program(permittivity, resolution, dt, *, boundaries=(), sources=(), monitors=(),
                     num_steps=1, t0=0.0, conductivity=None, device=None,
-                    field_dtype=torch.float32, flags=abi.DEFAULT_FLAGS, field_scale=None):
-    """Lower ``(permittivity raster, dx, dt, boundaries, sources, monitors)``."""
+                    field_dtype=torch.float32, flags=abi.DEFAULT_FLAGS, field_scale=None,
+                    psi_dtype=None):
+    """Lower ``(permittivity raster, dx, dt, boundaries, sources, monitors)``.
+
+    ``psi_dtype`` stores the CPML memories as float32 (default) or bfloat16
+    (the reference's BF16 psi option, ``backend.py:45-56``; the recurrence
+    itself runs in float32); ``None`` reads ``BEAMZ_HIP_CPML_PSI_PRECISION``."""
     device = torch.device(device or ("cuda" if torch.cuda.is_available() else "cpu"))
+    psi_dtype = psi_precision_from_env() if psi_dtype is None else psi_dtype
+    if psi_dtype not in (torch.float32, torch.bfloat16):
+        raise ValueError("CPML memories are stored as float32 or bfloat16")
+    if psi_dtype == torch.bfloat16:
+        flags = int(flags) | abi.FLAG_BF16_PSI
     eps = np.asarray(permittivity, dtype=np.float64)
     if eps.ndim not in (2, 3):
         raise ValueError("permittivity must be a 2D (y, x) or 3D (z, y, x) raster")
@@ -300,7 +324,7 @@ def compile_program(permittivity, resolution, dt, *, boundaries=(), sources=(), 
         boundary_code=boundary_code, cpml=cpml, coeffs=coeffs, h_meta=h_meta, e_meta=e_meta,
         h_term_arrays=h_arrays, e_term_arrays=e_arrays, source_groups=groups,
         coincident_mask=coincident, monitors=compiled, packed_monitors=packed,
-        device=device, field_dtype=field_dtype, flags=int(flags), eps_cells=eps,
+        device=device, field_dtype=field_dtype, psi_dtype=psi_dtype, flags=int(flags), eps_cells=eps,
         eps_site=eps_site)
 
 
@@ -352,7 +376,7 @@ def initial_state(prog: CompiledProgram, fields=None) -> SimulationState:
         shape[t.axis] = t.low + t.high
         shapes.append(tuple(shape))
     total = int(sum(int(np.prod(sh)) for sh in shapes))
-    arena = torch.zeros((2, max(total, 1)), dtype=torch.float32, device=dev)
+    arena = torch.zeros((2, max(total, 1)), dtype=prog.psi_dtype, device=dev)
     banks = ([], [])
     off = 0
     for sh in shapes:

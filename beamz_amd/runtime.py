@@ -12,6 +12,8 @@ the same state object is returned.
 
 from __future__ import annotations
 
+import ctypes
+
 import torch
 
 from . import _abi as abi
@@ -265,3 +267,106 @@ def _source_group_arguments_empty(prog):
         out += [torch.zeros((0, 1, 1, 1), device=dev), torch.zeros((0, 1), device=dev),
                 torch.zeros((0, 3), dtype=torch.int32, device=dev)]
     return tuple(out)
+
+
+# ---- descriptor-level entry points (cuda/src/launch.h:12-77) -----------------
+# The same work as the positional calls above, described by the structs that
+# the reference's BeamzLaunchStreamed / BeamzLaunchProgram take
+# (update.cu:919-971, program.cu:339-354).
+
+def _fill(slots, tensors):
+    for i, t in enumerate(tensors):
+        slots[i] = native.buffer(t)
+
+
+def _phase_launch(prog, phase, targets, sources, psi, out_targets, out_psi):
+    """A :class:`native.Launch` of one phase: inputs in the streamed order
+    (targets, sources, materials, metadata, 18 term arrays, psi), outputs the
+    written targets and psi."""
+    ln = native.Launch()
+    ln.abi_version = abi.ABI_VERSION
+    ln.flags = prog.flags
+    ln.phase = phase
+    ln.nterms = prog.nterms
+    ln.metric_kind = 0
+    ln.dt = prog.dt
+    ln.resolution = prog.abi_resolution
+    ln.inv_resolution = 1.0 / prog.abi_resolution
+    ln.metallic_edges = prog.boundary_code & 0xFF
+    ln.uniform_cpml_thickness = prog.boundary_code >> 8
+    ins = [*targets, *sources, *_materials(prog, phase)]
+    if prog.cpml_enabled:
+        meta = prog.h_meta if phase == abi.PHASE_H else prog.e_meta
+        arrays = prog.h_term_arrays if phase == abi.PHASE_H else prog.e_term_arrays
+        ins += [meta, *arrays[:18], *_psi_slots(psi, prog.device)]
+    _fill(ln.inputs, ins + [None] * (37 - len(ins)))
+    _fill(ln.metrics, _metrics(prog))
+    outs = [*out_targets]
+    if prog.cpml_enabled:
+        outs += list(_psi_slots(out_psi, prog.device))
+    _fill(ln.outputs, outs + [None] * (9 - len(outs)))
+    return ln
+
+
+def launch_phase(state, prog, phase):
+    """One phase in place through ``bzh_launch_phase``."""
+    if phase == abi.PHASE_H:
+        t, src, psi = (state.hx, state.hy, state.hz), (state.ex, state.ey, state.ez), state.psi_h
+    else:
+        t, src, psi = (state.ex, state.ey, state.ez), (state.hx, state.hy, state.hz), state.psi_e
+    native.call_launch_phase(_phase_launch(prog, phase, t, src, psi, t, psi))
+    return state
+
+
+def launch_program(state, prog, nsteps):
+    """``nsteps`` steps with sources and monitors through
+    ``bzh_launch_program`` (two field banks where the fused step applies).
+    Returns the state holding the result, as :func:`run_chunk` does."""
+    two = uses_two_banks(prog)
+    if two:
+        state = with_workspace(state)
+    nh = len(state.psi_h)
+    a_h, a_e = (state.hx, state.hy, state.hz), (state.ex, state.ey, state.ez)
+    fb = state.fields_b if two else state.fields()
+    b_h, b_e = tuple(fb[0:3]), tuple(fb[3:6])
+    pb_h = tuple(state.psi_b[:nh]) if two else state.psi_h
+    pb_e = tuple(state.psi_b[nh:]) if two else state.psi_e
+    pl = native.ProgramLaunch()
+    pl.h_ab = _phase_launch(prog, abi.PHASE_H, a_h, a_e, state.psi_h, b_h, pb_h)
+    pl.e_ab = _phase_launch(prog, abi.PHASE_E, a_e, b_h, state.psi_e, b_e, pb_e)
+    pl.h_ba = _phase_launch(prog, abi.PHASE_H, b_h, b_e, pb_h, a_h, state.psi_h)
+    pl.e_ba = _phase_launch(prog, abi.PHASE_E, b_e, a_h, pb_e, a_e, state.psi_e)
+    groups = (native.SourceGroup * abi.SOURCE_GROUP_COUNT)()
+    flat = _source_group_arguments(prog)
+    for gi in range(abi.SOURCE_GROUP_COUNT):
+        g = groups[gi]
+        g.coefficients = native.buffer(flat[3 * gi])
+        g.waveforms = native.buffer(flat[3 * gi + 1])
+        g.starts = native.buffer(flat[3 * gi + 2])
+        g.current_step = native.buffer(state.current_step)
+        g.component = gi % 3
+        g.timing = gi // 3
+        g.coincident = (prog.coincident_mask >> gi) & 1
+    pl.source_groups = groups
+    pl.source_group_count = abi.SOURCE_GROUP_COUNT
+    keep = [groups]
+    if prog.monitors:
+        if state.dft_re.dtype != torch.float64:
+            raise ValueError("the HIP program accumulates DFT monitors in float64")
+        m = native.DftGroup()
+        names = ("indices", "weights", "frequencies", "component_masks", "counts", "codes", "windows")
+        for name, t in zip(names, prog.packed_monitors):
+            setattr(m, name, native.buffer(t))
+        m.dft_re = native.buffer(state.dft_re)
+        m.dft_im = native.buffer(state.dft_im)
+        m.dft_weight = native.buffer(state.dft_w)
+        m.time = native.buffer(state.t)
+        m.current_step = native.buffer(state.current_step)
+        m.monitor_count = len(prog.monitors)
+        pl.monitors = ctypes.pointer(m)
+        keep.append(m)
+    pl.field_bank_count = 2 if two else 1
+    pl.nsteps = int(nsteps)
+    native.call_launch_program(pl)
+    del keep
+    return _swap_banks(state, nsteps) if two else state

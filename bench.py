@@ -2,11 +2,18 @@
 
 Workload (the 3D-at-scale configuration): a 1024 x 512 x 256 (x, y, z) grid at
 25 nm (134,217,728 cells); background eps 2.085 with a seeded inner
-512 x 256 x 64 slab of 8x8x8-cell blocks, eps iid U[2.085, 12.25]; 16-cell CPML
-on all faces; a soft Ez plane source (Gaussian pulse at 1.55 um) at x = 40;
-two 21-wavelength (1.45-1.65 um) DFT flux planes at x = 64 and x = 960 with
-FP64 accumulators.  A "step" is one full leapfrog step (sources, H, E, DFT)
-over the whole grid.  Inputs are resident in HBM before timing starts.
+512 x 256 x 64 slab of 8x8x8-cell blocks, eps iid U[2.085, 12.25] (x from 256
+to 768); 16-cell CPML on all faces; a soft Ez plane source (Gaussian pulse at
+1.55 um, 0.2 um bandwidth, peak at 1.5 / fwidth so that it has crossed both
+monitors within the 3000 steps) at x = 40; two 21-wavelength (1.45-1.65 um)
+DFT flux planes with FP64 accumulators, at x = 64 (incident) and x = 320 (64
+cells into the random slab).  A "step" is one full leapfrog step (sources, H,
+E, DFT) over the whole grid.  Inputs are resident in HBM before timing starts.
+
+The FP32 line also carries a ``half`` object: the f16 and bf16 storage forms
+timed the same way, their GCUPS ratio to FP32, and the error of their flux
+spectra (both monitors, and the transmission between them) against the FP32
+run after the full 3000 steps.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--dtype f32|bf16|f16]
 """
@@ -24,7 +31,7 @@ import numpy as np
 
 WORKLOADS = {
     "3d": "3D Yee+CPML 1024x512x256 (x,y,z) at 25 nm, 16-cell CPML, seeded random-eps 512x256x64 "
-          "slab, Ez plane source, 2 x 21-wavelength FP64 DFT flux planes",
+          "slab, Ez plane source, 2 x 21-wavelength FP64 DFT flux planes (x=64, x=320)",
     "waveguide3d": "3D silicon waveguide 512x256x128 (x,y,z) at 25 nm, 450x220 nm Si core in SiO2 "
                    "along x, 12-cell CPML, 1.55 um Gaussian-pulse Ez plane source at x=30, "
                    "3 x 11-wavelength FP64 DFT flux planes",
@@ -74,17 +81,54 @@ def build_workload(dtype_name, monitors=True, fused=True):
     fw = bz.LIGHT_SPEED * 0.2e-6 / (1.55e-6) ** 2
     L = 16
     src = bz.SoftSource("Ez", (L, L, 40), (nz - 2 * L, ny + 1 - 2 * L, 1),
-                        bz.GaussianPulse(f0, fw))
+                        bz.GaussianPulse(f0, fw, offset=1.5))
     lam = np.linspace(1.45e-6, 1.65e-6, 21)
     freqs = bz.LIGHT_SPEED / lam
     rngs = {"z": (L, nz - L), "y": (L, ny - L)}
-    mons = [bz.FluxMonitor("x", 64, freqs, ranges=rngs), bz.FluxMonitor("x", nx - 64, freqs, ranges=rngs)]
+    mons = [bz.FluxMonitor("x", 64, freqs, ranges=rngs), bz.FluxMonitor("x", 320, freqs, ranges=rngs)]
     field_dtype = {"f32": torch.float32, "bf16": torch.bfloat16, "f16": torch.float16}[dtype_name]
     flags = _abi.DEFAULT_FLAGS | (_abi.FLAG_FUSED_STEP if fused else 0)
     sim = bz.Simulation(eps, dx, dt=dt, boundaries=[bz.PML(cells=L)], sources=[src],
                         monitors=mons if monitors else [], steps=3000, field_dtype=field_dtype,
                         flags=flags)
     return sim, eps
+
+
+def c3_flux_spectra(dtype_name, steps=3000, chunk=256, sim=None):
+    """Run the 1024x512x256 workload (``sim``, or a new one with
+    ``dtype_name`` field storage) for ``steps`` steps from zero fields;
+    returns (flux spectrum per monitor, ms per step)."""
+    import torch
+    import beamz_amd as bz
+    if sim is None:
+        sim, _ = build_workload(dtype_name)
+    prog = sim.compile()
+    state = sim.initial_state()
+    torch.cuda.synchronize()
+    stream = torch.cuda.Stream()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    with torch.cuda.stream(stream):
+        start.record(stream)
+        state = sim.advance(state, steps, chunk=chunk)
+        stop.record(stream)
+    torch.cuda.synchronize()
+    res = bz.SimulationResults(state, prog, None)
+    flux = [np.asarray(res.monitor_flux(i), dtype=np.float64) for i in range(len(prog.monitors))]
+    ms = start.elapsed_time(stop) / steps
+    del state, res
+    torch.cuda.empty_cache()
+    return flux, ms
+
+
+def flux_errors(flux, ref):
+    """Max over the wavelengths of |a - b| / max|b| for each monitor, and of
+    the transmission T = flux[1] / flux[0]."""
+    out = {f"flux{i}": float(np.max(np.abs(a - b)) / np.max(np.abs(b))) for i, (a, b) in
+           enumerate(zip(flux, ref))}
+    t, tr = flux[1] / flux[0], ref[1] / ref[0]
+    out["transmission"] = float(np.max(np.abs(t - tr)) / np.max(np.abs(tr)))
+    return out
 
 
 def build_batched_workload(seed=4, count=64, steps=3000):
@@ -301,6 +345,83 @@ def cpu_baseline(eps, prog, budget_s=12.0):
                       f"{steps} FP32 steps ({tt:.1f} s) of oracle/port.c (H+E+CPML; no source/DFT)"}
 
 
+def time_workload(args, build, dtype):
+    """Build the workload with ``dtype`` field storage and time ``args.steps``
+    steps after ``args.warmup`` warm-up steps; returns (sim, eps, prog,
+    state, stream, ms per step)."""
+    import torch
+    from beamz_amd import runtime
+
+    print(f"[bench] building workload dtype={dtype}", file=sys.stderr, flush=True)
+    sim, eps = build(dtype, monitors=not args.no_monitors, fused=not args.per_phase)
+    prog = sim.compile()
+    state = sim.initial_state()
+    stream = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    print("[bench] workload resident; warming up", file=sys.stderr, flush=True)
+
+    step = 0
+
+    def run(n):
+        nonlocal step, state
+        remaining = n
+        while remaining > 0:
+            k = min(args.chunk, remaining)
+            state = runtime.run_chunk(state, prog, k)
+            step += k
+            remaining -= k
+            state.current_step.fill_(step)
+            state.t.fill_(prog.t0 + step * prog.dt)
+
+    with torch.cuda.stream(stream):
+        run(args.warmup)
+        # Replay-only timing: a chunk's graph is keyed on its step count and the
+        # bank it starts from, so the exact chunk sequence of the timed window
+        # runs once beforehand from the same bank (twice for an odd count,
+        # which swaps the banks), and the timed window replays captured graphs.
+        for _ in range(2 if args.steps % 2 else 1):
+            run(args.steps)
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    with torch.cuda.stream(stream):
+        start.record(stream)
+        run(args.steps)
+        stop.record(stream)
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop) / args.steps
+    print(f"[bench] {dtype}: {ms:.3f} ms/step, {prog.cells / (ms * 1e-3) / 1e9:.2f} GCUPS",
+          file=sys.stderr, flush=True)
+    return sim, eps, prog, state, stream, ms
+
+
+def half_leg(args, gcups_f32, sim_f32):
+    """The ``half`` object of the FP32 line: f16 and bf16 storage timed like
+    FP32, and their 3000-step flux errors against the FP32 run."""
+    import torch
+    out = {}
+    steps = 3000
+    print(f"[bench] half-precision accuracy: {steps} steps each of f32, f16, bf16",
+          file=sys.stderr, flush=True)
+    ref, _ = c3_flux_spectra("f32", steps, sim=sim_f32)
+    for dt_name in ("f16", "bf16"):
+        sim, eps, prog, state, stream, ms = time_workload(args, build_workload, dt_name)
+        g = prog.cells / (ms * 1e-3) / 1e9
+        out[dt_name] = {"gcups": round(g, 3), "ms_per_step": round(ms, 4),
+                        "vs_f32": round(g / gcups_f32, 4)}
+        del state
+        torch.cuda.empty_cache()
+        flux, _ = c3_flux_spectra(dt_name, steps, sim=sim)
+        out[dt_name]["flux_error_vs_f32"] = {k: float(f"{v:.3e}") for k, v in
+                                            flux_errors(flux, ref).items()}
+        del sim, eps, prog
+        torch.cuda.empty_cache()
+    out["accuracy_run"] = (f"{steps} steps of this workload; max over 21 wavelengths of "
+                           "|flux - flux_f32| / max|flux_f32| per monitor and of the "
+                           "transmission flux(x=320) / flux(x=64)")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -308,6 +429,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--dtype", default="f32", choices=("f32", "bf16", "f16"))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-half", action="store_true",
+                    help="skip the f16/bf16 timing and accuracy legs of the FP32 line")
     ap.add_argument("--no-monitors", action="store_true")
     ap.add_argument("--per-phase", action="store_true", help="use the per-phase kernels")
     ap.add_argument("--chunk", type=int, default=256)
@@ -354,47 +477,9 @@ def main():
     import torch
     from beamz_amd import native, runtime
 
-    print(f"[bench] building workload dtype={args.dtype}", file=sys.stderr, flush=True)
     build = build_waveguide_workload if args.workload == "waveguide3d" else build_workload
-    sim, eps = build(args.dtype, monitors=not args.no_monitors, fused=not args.per_phase)
-    prog = sim.compile()
-    state = sim.initial_state()
-    stream = torch.cuda.Stream()
-    torch.cuda.synchronize()
-    print("[bench] workload resident; warming up", file=sys.stderr, flush=True)
-
-    step = 0
-
-    def run(n):
-        nonlocal step, state
-        remaining = n
-        while remaining > 0:
-            k = min(args.chunk, remaining)
-            state = runtime.run_chunk(state, prog, k)
-            step += k
-            remaining -= k
-            state.current_step.fill_(step)
-            state.t.fill_(prog.t0 + step * prog.dt)
-
-    with torch.cuda.stream(stream):
-        run(args.warmup)
-        # Replay-only timing: a chunk's graph is keyed on its step count and the
-        # bank it starts from, so the exact chunk sequence of the timed window
-        # runs once beforehand from the same bank (twice for an odd count,
-        # which swaps the banks), and the timed window replays captured graphs.
-        for _ in range(2 if args.steps % 2 else 1):
-            run(args.steps)
-    torch.cuda.synchronize()
-    start = torch.cuda.Event(enable_timing=True)
-    stop = torch.cuda.Event(enable_timing=True)
-    with torch.cuda.stream(stream):
-        start.record(stream)
-        run(args.steps)
-        stop.record(stream)
-    torch.cuda.synchronize()
-    ms = start.elapsed_time(stop) / args.steps
+    sim, eps, prog, state, stream, ms = time_workload(args, build, args.dtype)
     gcups = prog.cells / (ms * 1e-3) / 1e9
-    print(f"[bench] {ms:.3f} ms/step, {gcups:.2f} GCUPS", file=sys.stderr, flush=True)
     if runtime.uses_two_banks(prog) and not native.last_program_fused():
         raise RuntimeError("the timed steps did not run the fused step kernels")
 
@@ -446,6 +531,11 @@ def main():
             "algorithmic_bytes_per_launch": int(nb),
         },
     }
+    if args.workload == "3d" and args.dtype == "f32" and not args.no_half and \
+            not args.no_monitors and not args.per_phase:
+        del state
+        torch.cuda.empty_cache()
+        line["half"] = half_leg(args, gcups, sim)
     if not args.no_cpu:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(eps, prog)

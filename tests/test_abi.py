@@ -87,3 +87,42 @@ def test_bad_arguments_are_rejected_before_any_launch():
     attrs = native.ProgramAttrs(abi_version=_abi.ABI_VERSION, nsteps=1, program_layout=0)
     assert lib.bzh_program(None, arr, 1, ctypes.byref(attrs)) == 1
     assert "missing" in lib.bzh_last_error().decode()
+
+
+_STRUCTS = {
+    "bzh_buffer": native.Buffer, "bzh_phase_attrs": native.PhaseAttrs,
+    "bzh_program_attrs": native.ProgramAttrs, "bzh_sim2d": native.Sim2d,
+    "bzh_batch_attrs": native.BatchAttrs, "bzh_adjoint2d": native.Adjoint2d,
+    "bzh_launch": native.Launch, "bzh_source_group": native.SourceGroup,
+    "bzh_dft_group": native.DftGroup, "bzh_program_launch": native.ProgramLaunch,
+}
+
+
+def test_ctypes_structs_match_the_header_layout(tmp_path):
+    """Every descriptor the ctypes binding passes has the size and field
+    offsets the C compiler gives the header's struct (host gcc, no GPU)."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no host C compiler")
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "beamz_hip.h"', "int main(void) {"]
+    for cname, cls in _STRUCTS.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {}
+    for line in out:
+        if line:
+            *key, val = line.split()
+            got[tuple(key)] = int(val)
+    for cname, cls in _STRUCTS.items():
+        assert got[(cname, "size")] == ctypes.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert got[(cname, fname)] == getattr(cls, fname).offset, (cname, fname)

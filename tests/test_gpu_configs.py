@@ -158,3 +158,56 @@ def test_3d_silicon_waveguide_full_run_pairs_match_single_steps(monkeypatch):
         err = np.max(np.abs(a - b)) / np.max(np.abs(b))
         print(f"  monitor {i}: max relative flux difference {err:.2e}", flush=True)
         assert np.max(np.abs(b)) > 0.0 and err <= 1e-5, (i, err)
+
+
+def test_batched_sweep_64_members_full_run():
+    """The batched 2D sweep at its full size: 64 TMxy simulations, (ny, nx)
+    ~ U{256..1024}^2 at 20 nm, 2-4 guides each, 10-cell CPML, pre-E Ez line
+    sources, 3000 steps in one persistent launch per step.  Every member
+    must equal the same member run alone through ``Simulation`` (the
+    per-phase kernels) to 5e-5 relative L2 per field, and the two smallest
+    members the FP64 oracle to 1e-4 (3000 FP32 steps)."""
+    _require_gpu()
+    import bench
+    from beamz_amd.batch import BatchedSimulation2D
+    from beamz_amd.sources import lower_soft_source
+    steps = 3000
+    sims = bench.build_batched_workload(count=64, steps=steps)
+    batch = BatchedSimulation2D(sims)
+    cells = batch.cells
+    print(f"case config batched sweep: 64 members, {cells} cells, {steps} steps", flush=True)
+    state = batch.initial_state()
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        state = batch.advance(state, steps)
+    torch.cuda.synchronize()
+    print(f"batched sweep done (tile_rows={batch.tile_rows}, workgroups={batch.workgroups})",
+          flush=True)
+    worst = 0.0
+    for i, sim in enumerate(sims):
+        with torch.cuda.stream(s):
+            solo = sim.advance(sim.initial_state(), steps)
+        torch.cuda.synchronize()
+        for name in ("Ez", "Hx", "Hy"):
+            want = getattr(solo, name.lower()).double().cpu().numpy()
+            got = batch.field(state, i, name)
+            err = np.linalg.norm(got - want) / np.linalg.norm(want)
+            worst = max(worst, err)
+            assert err <= 5e-5, f"member {i} {sim.permittivity.shape} {name}: {err:.2e}"
+    print(f"batched vs solo: worst relative L2 {worst:.2e} over 64 members", flush=True)
+    order = sorted(range(len(sims)), key=lambda i: sims[i].permittivity.size)[:2]
+    for i in order:
+        sim = sims[i]
+        prog = sim.compile()
+        customs = [lower_soft_source(src, prog.eps_site["Ez"], sim.dt, 0.0, steps)
+                   for src in sim.sources]
+        spec = o.CpmlSpec(cells=10, kappa_max=sims[i].boundaries[0].kappa_max,
+                          faces=frozenset(("bottom", "top", "left", "right")))
+        prob = o.build_problem(sim.permittivity, sim.resolution, sim.dt, cpml=spec,
+                               sources=_oracle_sources(customs))
+        ost = o.init_state(prob)
+        o.run(prob, ost, steps)
+        for name in ("Ez", "Hx", "Hy"):
+            _close(f"member {i} {name}", batch.field(state, i, name), ost[name.lower()], rel=1e-4,
+                   mabs=1e-3)
+        print(f"member {i} {sim.permittivity.shape} matches the oracle", flush=True)

@@ -109,3 +109,35 @@ def test_fullsize_f16_storage_tracks_f32():
         err = _rel(x.float() * scale[name[0].upper()], y)
         print(f"  {name}: f16 vs f32 relative L2 {err:.2e}", flush=True)
         assert err <= 5e-3, (name, err)
+
+
+_F32_FLUX = {}
+
+
+def _f32_flux():
+    import bench
+    if "flux" not in _F32_FLUX:
+        _F32_FLUX["flux"], _ = bench.c3_flux_spectra("f32", 3000)
+    return _F32_FLUX["flux"]
+
+
+@pytest.mark.parametrize("dtype", ["f16", "bf16"])
+def test_fullsize_half_storage_flux_after_3000_steps(dtype):
+    """The mixed-precision configuration in full: 1024 x 512 x 256, the
+    random-eps slab, 16-cell CPML, the Gaussian plane source, flux planes at
+    x = 64 and x = 320 over 21 wavelengths, 3000 steps, fields stored in f16
+    or bf16 (per-field scaling, float32 curl arithmetic, float64 DFT).  The
+    flux spectrum of each monitor and the transmission between them must
+    match the FP32 run (itself within ~1e-6 of the FP64 oracle on the
+    configurations the oracle can run) to 1e-3."""
+    _require_gpu()
+    import bench
+    print(f"case fullsize {dtype} flux 1024x512x256 steps=3000", flush=True)
+    ref = _f32_flux()
+    assert all(np.max(np.abs(f)) > 0 for f in ref)
+    flux, ms = bench.c3_flux_spectra(dtype, 3000)
+    errs = bench.flux_errors(flux, ref)
+    print(f"  {dtype}: " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()) +
+          f" ({ms:.3f} ms/step)", flush=True)
+    bad = {k: v for k, v in errs.items() if not v <= 1e-3}
+    assert not bad, f"{dtype} flux errors above 1e-3: {bad}"

@@ -43,7 +43,8 @@ def _require_gpu():
 
 def _setup(grid, *, seed, cpml_cells=None, faces=None, metallic=None, kappa_max=2.0,
            sources=(), monitors=(), steps=1, dt_scale=0.9, fused=True,
-           field_dtype=torch.float32, field_amplitude=1e-5, eps=None):
+           field_dtype=torch.float32, field_amplitude=1e-5, eps=None, sigma=None,
+           psi_dtype=torch.float32):
     import beamz_amd as bz
     eps = eps_raster(grid, seed) if eps is None else np.asarray(eps, dtype=np.float64)
     dx = 40e-9
@@ -88,10 +89,11 @@ def _setup(grid, *, seed, cpml_cells=None, faces=None, metallic=None, kappa_max=
                                t_start=m.t_start, t_end=m.t_end, hann=m.window == "hann"))
     prob = o.build_problem(eps, dx, dt, cpml=spec,
                            metallic=frozenset(metallic) if metallic else None,
-                           sources=sources, monitors=omons)
+                           sources=sources, monitors=omons, sigma_cells=sigma)
     flags = bz._abi.DEFAULT_FLAGS | (bz._abi.FLAG_FUSED_STEP if fused else 0)
     sim = bz.Simulation(eps, dx, dt=dt, boundaries=bounds, sources=customs, monitors=monitors,
-                        steps=steps, flags=flags, field_dtype=field_dtype)
+                        steps=steps, flags=flags, field_dtype=field_dtype, conductivity=sigma,
+                        psi_dtype=psi_dtype)
     prog = sim.compile()
     assert prog.metallic == prob.metallic
     fields = random_fields(prob.shapes, seed + 1, scale=field_amplitude)
