@@ -804,6 +804,13 @@ FusedSplit pair_split(const Grid& g, int L) {
   return sp;
 }
 
+// Rounding of bfloat16 field stores in the fused kernels: stochastic by
+// default (BZH_BF16_ROUNDING=nearest selects round-to-nearest-even).
+bool bf16_stochastic_rounding() {
+  const char* env = std::getenv("BZH_BF16_ROUNDING");
+  return !(env && std::string(env) == "nearest");
+}
+
 // Two steps per launch: by default for float32 fields (BZH_PAIR=1 forces it
 // for 2-byte fields too, BZH_PAIR=0 disables it).
 bool pairs_enabled(int dtype) {
@@ -1118,6 +1125,8 @@ bool run_program(hipStream_t s, ProgramIn& in) {
     fp.box_ty0 = 0;
     fp.box_z0 = 0;
     fp.box_z1 = g.nz + 1;
+    fp.step0 = step0;
+    fp.bf16_sr = g.dtype == BZH_BF16 && bf16_stochastic_rounding() ? 1 : 0;
     return fp;
   };
   auto set_box = [&](FusedParams& fp, const FusedBox& b, int v) {
@@ -1186,6 +1195,7 @@ bool run_program(hipStream_t s, ProgramIn& in) {
       std::memset(&pp, 0, sizeof(pp));
       pp.f = fused_params(cur, nxt, cur_b);
       pp.f.rows = kPairRows;
+      pp.f.kstep = k;
       pp.step0 = step0;
       for (int c = 0; c < 3 && 6 + c < in.ngroups; ++c)
         add_mid_slabs(pp.msrc, pp.msrc_n, kMaxMidSlabs, in.groups[6 + c], c, k, true, false, in.metallic);
@@ -1243,7 +1253,7 @@ bool run_program(hipStream_t s, ProgramIn& in) {
     }
     if (fused) {
       FusedParams fp = fused_params(cur, nxt, cur_b);
-      fp.step0 = step0;
+      fp.kstep = k;
       for (int c = 0; c < 3 && 3 + c < in.ngroups; ++c)
         add_mid_slabs(fp.hslab, fp.hslab_n, kMaxHSlabs, in.groups[3 + c], c, k, true, true, in.metallic);
       const FusedSplit sp = fused_split(g, fp.L);

@@ -57,6 +57,31 @@ __device__ __forceinline__ void store_f(__half* p, int64_t i, float v) {
   p[i] = __float2half(v);
 }
 
+// Stochastic rounding of float32 to bfloat16: add 16 random bits below the
+// kept mantissa and truncate, so that E[round(v)] = v and the rounding error
+// of a slowly varying field does not repeat from step to step.  The bits come
+// from a per-lane Weyl sequence (state += golden-ratio increment) scrambled by
+// one odd multiply; the state is seeded from the lane's global index and the
+// time step, so a run is deterministic.  NaN stays NaN.
+struct SrState {
+  uint32_t s;
+};
+__device__ __forceinline__ SrState sr_seed(uint32_t lane_id, uint32_t step) {
+  uint32_t h = lane_id * 0x9E3779B1u ^ (step + 0x7F4A7C15u) * 0x85EBCA77u;
+  h ^= h >> 15;
+  h *= 0x2C1B3C6Du;
+  h ^= h >> 12;
+  return {h};
+}
+__device__ __forceinline__ uint16_t bf16_bits_sr(float v, SrState& st) {
+  st.s += 0x9E3779B9u;
+  const uint32_t r = (st.s * 0x85EBCA6Bu) >> 16;
+  const uint32_t u = __float_as_uint(v);
+  uint32_t b = (u + r) >> 16;
+  if ((u & 0x7FFFFFFFu) > 0x7F800000u) b = (u >> 16) | 0x40u;
+  return static_cast<uint16_t>(b);
+}
+
 // ---- phase parameters --------------------------------------------------------
 // One packed CPML derivative term on the device.
 struct DevTerm {
@@ -184,6 +209,10 @@ struct FusedParams {
   MidSlab hslab[kMaxHSlabs];
   int32_t hslab_n;
   const int32_t* step0;
+  // step of this launch within the call (the first step of a pair launch):
+  // with *step0 it keys the stochastic rounding of bfloat16 field stores
+  int32_t kstep;
+  int32_t bf16_sr;       // 1: bfloat16 fields are stored with stochastic rounding
 };
 int fused_rows();
 

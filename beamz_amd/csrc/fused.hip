@@ -193,6 +193,27 @@ __device__ __forceinline__ void stu(const char* base, uint32_t aoff, uint32_t ro
     *reinterpret_cast<gptr<uint16_t>>(p) = u;
   }
 }
+// Field store: bfloat16 fields take stochastic rounding when ``sr`` (uniform)
+// is set, every other storage type the ordinary conversion.
+template <typename T>
+__device__ __forceinline__ void stf(const char* base, uint32_t aoff, uint32_t row, uint32_t xb,
+                                    float v, bool sr, SrState& rng) {
+  if constexpr (std::is_same<T, bf16_t>::value) {
+    if (sr) {
+      gptr<char> p = (gptr<char>)(at<T>(base, aoff, row, xb));
+      *reinterpret_cast<gptr<uint16_t>>(p) = bf16_bits_sr(v, rng);
+      return;
+    }
+  }
+  stu<T>(base, aoff, row, xb, v);
+}
+// The seed of a launch's rounding sequence: the lane's global index and the
+// absolute time step (*step0 + kstep, or kstep without a step counter).
+__device__ __forceinline__ SrState sr_launch_seed(const FusedParams& f, int step_offset) {
+  const int32_t s0 = f.step0 != nullptr ? *(const __attribute__((address_space(4))) int32_t*)(f.step0) : 0;
+  return sr_seed(blockIdx.x * blockDim.x + threadIdx.x,
+                 static_cast<uint32_t>(s0 + f.kstep + step_offset));
+}
 // The kernel argument block, re-read at a use: in the x-interior kernel for
 // 2-byte fields (8 waves per SIMD, fewer scalar registers) the CPML memory
 // offsets and the coefficient table, needed only on slab rows and planes, are
@@ -382,6 +403,11 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
     ez = (vz && vtX && vxY) ? d : 0.0f;
   };
 
+  // stochastic rounding of bfloat16 field stores (uniform switch)
+  const bool sr = std::is_same<T, bf16_t>::value && f.bf16_sr != 0;
+  SrState rng{0u};
+  if (sr) rng = sr_launch_seed(f, 0);
+
   bool hwave = false;  // an H-timing slab crosses this wave's row
   if constexpr (kHSrc)
     for (int s = 0; s < f.hslab_n; ++s) hwave = hwave || slab_hits_wave(kargs()->hslab[s], y, x);
@@ -515,9 +541,9 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
       lE[bb ^ 1][kR + 1][1][lane] = etx;
     }
     if (wh) {
-      if (okhx) stu<T>(f.fout, f.foff[3], (k * ny + yY) * (nx + 1), tY, hx);
-      if (okhy) stu<T>(f.fout, f.foff[4], (k * (ny + 1) + yX) * nx, tX, hy);
-      if (okhz) stu<T>(f.fout, f.foff[5], (k * ny + yY) * nx, tX, hz);
+      if (okhx) stf<T>(f.fout, f.foff[3], (k * ny + yY) * (nx + 1), tY, hx, sr, rng);
+      if (okhy) stf<T>(f.fout, f.foff[4], (k * (ny + 1) + yX) * nx, tX, hy, sr, rng);
+      if (okhz) stf<T>(f.fout, f.foff[5], (k * ny + yY) * nx, tX, hz, sr, rng);
     }
     __syncthreads();
 
@@ -570,9 +596,9 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
       if (zf || yf) nex = 0.0f;
       if (zf || (kXEdge && xface)) ney = 0.0f;
       if (yf || (kXEdge && xface)) nez = 0.0f;
-      if (okx) stu<T>(f.fout, f.foff[0], (kx * (ny + 1) + yX) * nx, tX, nex);
-      if (oky) stu<T>(f.fout, f.foff[1], (kx * ny + yY) * (nx + 1), tY, ney);
-      if (okz) stu<T>(f.fout, f.foff[2], (kzc * (ny + 1) + yX) * (nx + 1), tY, nez);
+      if (okx) stf<T>(f.fout, f.foff[0], (kx * (ny + 1) + yX) * nx, tX, nex, sr, rng);
+      if (oky) stf<T>(f.fout, f.foff[1], (kx * ny + yY) * (nx + 1), tY, ney, sr, rng);
+      if (okz) stf<T>(f.fout, f.foff[2], (kzc * (ny + 1) + yX) * (nx + 1), tY, nez, sr, rng);
     }
     phx = hx;
     phy = hy;
@@ -969,15 +995,22 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, in
   };
 
   // stores of plane k (fields to ``out``, CPML memories to the written bank)
+  // (bfloat16 stores of the second step's fields to bank B round
+  // stochastically when f.bf16_sr; the intermediate bank keeps nearest)
+  const bool sr = std::is_same<T, bf16_t>::value && f.bf16_sr != 0;
+  SrState rng{0u};
+  if (sr) rng = sr_launch_seed(f, 1);
   auto store_h = [&](const char* out, int k, float hx, float hy, float hz) {
-    if (k < nz && vyY && vxY) stu<T>(out, f.foff[3], (k * ny + yY) * (nx + 1), tY, hx);
-    if (k < nz && vyX && vxX) stu<T>(out, f.foff[4], (k * (ny + 1) + yX) * nx, tX, hy);
-    if (k <= nz && vyY && vxX) stu<T>(out, f.foff[5], (k * ny + yY) * nx, tX, hz);
+    const bool s = sr && out == f.fout;
+    if (k < nz && vyY && vxY) stf<T>(out, f.foff[3], (k * ny + yY) * (nx + 1), tY, hx, s, rng);
+    if (k < nz && vyX && vxX) stf<T>(out, f.foff[4], (k * (ny + 1) + yX) * nx, tX, hy, s, rng);
+    if (k <= nz && vyY && vxX) stf<T>(out, f.foff[5], (k * ny + yY) * nx, tX, hz, s, rng);
   };
   auto store_e = [&](const char* out, int k, float ex, float ey, float ez) {
-    if (k <= nz && vyX && vxX) stu<T>(out, f.foff[0], (k * (ny + 1) + yX) * nx, tX, ex);
-    if (k <= nz && vyY && vxY) stu<T>(out, f.foff[1], (k * ny + yY) * (nx + 1), tY, ey);
-    if (k < nz && vyX && vxY) stu<T>(out, f.foff[2], (k * (ny + 1) + yX) * (nx + 1), tY, ez);
+    const bool s = sr && out == f.fout;
+    if (k <= nz && vyX && vxX) stf<T>(out, f.foff[0], (k * (ny + 1) + yX) * nx, tX, ex, s, rng);
+    if (k <= nz && vyY && vxY) stf<T>(out, f.foff[1], (k * ny + yY) * (nx + 1), tY, ey, s, rng);
+    if (k < nz && vyX && vxY) stf<T>(out, f.foff[2], (k * (ny + 1) + yX) * (nx + 1), tY, ez, s, rng);
   };
   auto store_psi_h = [&](int k, const float (&p)[6]) {
     const bool okx = k < nz && vyY && vxY, oky = k < nz && vyX && vxX, okz = k <= nz && vyY && vxX;

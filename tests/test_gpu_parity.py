@@ -305,8 +305,10 @@ def _rel_l2(got, want):
 # float16 folded E coefficients, float64 DFT) against the FP64 oracle after 60
 # steps of a pulse crossing a CPML-bounded random medium.  Bounds (relative L2
 # for fields and phasors, max relative for the flux spectrum): float16 within
-# 5e-3 / 1e-3, bfloat16 (8-bit mantissa) within 5e-2 / 1e-2.
-@pytest.mark.parametrize("dtype,tol,flux_tol", [("f16", 5e-3, 1e-3), ("bf16", 5e-2, 1e-2)])
+# 5e-3 / 1e-3; bfloat16 (8-bit mantissa, stochastically rounded stores) within
+# 1e-2 for the fields and phasors of this short run and the same 1e-3 for the
+# flux spectrum.
+@pytest.mark.parametrize("dtype,tol,flux_tol", [("f16", 5e-3, 1e-3), ("bf16", 1e-2, 1e-3)])
 def test_half_storage_accuracy(dtype, tol, flux_tol):
     _require_gpu()
     errs, _ = _half_run(dtype, (14, 16, 48), 6, 36)
