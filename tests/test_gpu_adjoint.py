@@ -89,3 +89,36 @@ def test_adjoint_gradient_matches_oracle_finite_differences():
     err = np.abs(got - fd) / np.max(np.abs(fd))
     print("adjoint vs FD:", list(zip(got.round(12), fd.round(12))), "max rel", err.max(), flush=True)
     assert err.max() <= 1e-3
+
+
+def test_adjoint_bend_600x600_matches_fp64_finite_differences():
+    """The inverse-design bend at its full size: 600 x 600 cells at 20 nm, a
+    100 x 100 design box, the input guide's mode launched by a mode source,
+    4000 steps, checkpoints every 200.  The FP32 forward plus hand-written
+    adjoint gradient matches central differences (h = 1e-3) of the FP64
+    oracle on 32 random pixels to 1e-3 of max |FD|, and J matches the
+    oracle's.  The FD values come from tests/golden/make_adjoint_bend_fd.py
+    (65 oracle runs on the CPU)."""
+    _require_gpu()
+    import os
+
+    import bench
+    from beamz_amd.adjoint import objective
+    gold = np.load(os.path.join(os.path.dirname(__file__), "golden", "adjoint_bend_600x600_fd.npz"))
+    prob, rho0 = bench.build_adjoint_problem()
+    assert (prob.steps, prob.checkpoint) == (4000, 200)
+    print("case adjoint bend 600x600 design=100x100 steps=4000 checkpoint=200", flush=True)
+    rho = torch.tensor(rho0, dtype=torch.float64, requires_grad=True)
+    J = objective(rho, prob)
+    J.backward()
+    g = rho.grad.numpy()
+    J_ref = float(gold["J0"])
+    rel_J = abs(float(J.detach()) - J_ref) / J_ref
+    pix = gold["pixels"]
+    fd = gold["fd"]
+    got = np.array([g[a, b] for a, b in pix])
+    err = np.abs(got - fd) / np.max(np.abs(fd))
+    print(f"adjoint bend: J rel err {rel_J:.2e}, gradient vs FP64 FD max {err.max():.2e} "
+          f"median {np.median(err):.2e} (h = {float(gold['h'])})", flush=True)
+    assert rel_J <= 1e-4
+    assert err.max() <= 1e-3
