@@ -693,6 +693,7 @@ struct ProgramIn {
   float dt, resolution;
   int metallic, uniform_l;
   int flags;
+  bool noalloc;                  // bzh_program_noalloc: scratch from the device array, no graphs
 };
 
 // Work split of the fused step -- the counterpart of the reference's
@@ -749,6 +750,31 @@ float* dft_stage(hipStream_t s, size_t floats) {
     r = {static_cast<float*>(d), floats};
   }
   return r.first;
+}
+
+// Scratch of bzh_program_noalloc: a file-scope device array that every GPU
+// the library runs on holds until the process ends.  In host code the name
+// ``scratch`` is only a host-side stand-in; its device address (one per
+// device) comes from hipGetSymbolAddress, taken once per device with that
+// device current, and only that address is handed to kernels.  Calls on one
+// GPU share the array: calls that overlap in time on one GPU (different
+// streams, or host threads) race on it.  The staging ring it holds is written
+// before it is read within each call, so nothing needs clearing between calls.
+static __device__ __align__(256) char scratch[BZH_NOALLOC_SCRATCH_BYTES];
+
+char* noalloc_scratch() {
+  static std::mutex mu;
+  static std::map<int, char*> addresses;
+  int device = 0;
+  BZH_HIP(hipGetDevice(&device));
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = addresses.find(device);
+  if (it != addresses.end()) return it->second;
+  void* p = nullptr;
+  BZH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  BZH_REQUIRE(p != nullptr, "no device address for the no-allocation scratch");
+  addresses[device] = static_cast<char*>(p);
+  return static_cast<char*>(p);
 }
 
 // Band ordering of the fused workgroups (BZH_FUSED_ORDER=xcd selects the
@@ -1035,7 +1061,19 @@ bool run_program(hipStream_t s, ProgramIn& in) {
   if (in.has_monitors) {
     const DftParams probe = make_dft(in.mon, in.bank_a, g, in.dt, in.monitor_count);
     slots = dft_slots(in.nsteps);
-    if (slots > 1) stage = dft_stage(s, static_cast<size_t>(slots) * probe.M * 6 * probe.Pmax);
+    const size_t per_slot = static_cast<size_t>(probe.M) * 6 * probe.Pmax;
+    if (in.noalloc) {
+      // the ring lives in the fixed scratch: as many slots as fit
+      const size_t cap = BZH_NOALLOC_SCRATCH_BYTES / sizeof(float);
+      BZH_REQUIRE(per_slot <= cap,
+                  "the monitors' samples of one step (" + std::to_string(per_slot * sizeof(float)) +
+                      " bytes) exceed the no-allocation scratch (" +
+                      std::to_string(BZH_NOALLOC_SCRATCH_BYTES) + " bytes)");
+      slots = std::min<int>(slots, static_cast<int>(cap / per_slot));
+      if (slots > 1) stage = reinterpret_cast<float*>(noalloc_scratch());
+    } else if (slots > 1) {
+      stage = dft_stage(s, static_cast<size_t>(slots) * per_slot);
+    }
   }
   // Two steps per launch (the pair kernel) where the fused step runs, the
   // source slabs between the steps fit the kernel and the monitors its boxes.
@@ -1049,8 +1087,10 @@ bool run_program(hipStream_t s, ProgramIn& in) {
   int npairs = 0;
   int mon_m = 0;
   if (in.has_monitors) mon_m = make_dft(in.mon, in.bank_a, g, in.dt, in.monitor_count).M;
+  // (the no-allocation entry runs pairs without monitors only: their
+  // intermediate sample needs the workspace bank)
   if (fused && pairs_enabled(g.dtype) && mid_slabs <= kMaxMidSlabs && mon_m <= kMaxMonBoxes &&
-      (mid_slabs == 0 || step0 != nullptr)) {
+      (mid_slabs == 0 || step0 != nullptr) && !(in.noalloc && mon_m > 0)) {
     npairs = in.nsteps / 2;
     if (npairs % 2 == 1) --npairs;
   }
@@ -1287,7 +1327,8 @@ bool run_program(hipStream_t s, ProgramIn& in) {
     cur_b = two_banks ? !cur_b : cur_b;
     ++k;
   }
-  graph_cache().run(s, launches, (in.flags & BZH_FLAG_GRAPH_CACHE) != 0);
+  // (no graphs for the no-allocation entry: instantiating one may allocate)
+  graph_cache().run(s, launches, (in.flags & BZH_FLAG_GRAPH_CACHE) != 0 && !in.noalloc);
   g_last_pairs = npairs;
   g_last_fused = fused ? 1 : 0;
   return fused;
@@ -1758,6 +1799,22 @@ int32_t bzh_program(void* stream, const bzh_buffer* args, int32_t nargs,
     BZH_REQUIRE(attrs->abi_version == BZH_ABI_VERSION, "ABI version mismatch");
     ProgramIn in;
     const int temporal = decode_program(args, nargs, *attrs, in);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const bool fused = run_program(s, in);
+    if (temporal && attrs->cpml_enabled && !fused && (attrs->nsteps % 2 == 1))
+      copy_psi_workspace(s, args, nargs);
+  });
+}
+
+int32_t bzh_program_noalloc(void* stream, const bzh_buffer* args, int32_t nargs,
+                            const bzh_program_attrs* attrs) {
+  return guarded([&] {
+    BZH_REQUIRE(attrs != nullptr && args != nullptr, "null arguments");
+    BZH_REQUIRE(attrs->abi_version == BZH_ABI_VERSION, "ABI version mismatch");
+    ProgramIn in;
+    const int temporal = decode_program(args, nargs, *attrs, in);
+    in.noalloc = true;
+    (void)noalloc_scratch();  // set-up: the device address, once per device
     hipStream_t s = static_cast<hipStream_t>(stream);
     const bool fused = run_program(s, in);
     if (temporal && attrs->cpml_enabled && !fused && (attrs->nsteps % 2 == 1))

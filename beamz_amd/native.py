@@ -166,6 +166,8 @@ SIGNATURES = {
                                       ctypes.POINTER(PhaseAttrs)]),
     "bzh_program": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(Buffer), ctypes.c_int32,
                                      ctypes.POINTER(ProgramAttrs)]),
+    "bzh_program_noalloc": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(Buffer),
+                                             ctypes.c_int32, ctypes.POINTER(ProgramAttrs)]),
     "bzh_batch2d": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(Sim2d), ctypes.c_int32,
                                      ctypes.POINTER(Buffer), ctypes.POINTER(BatchAttrs)]),
     "bzh_adjoint2d_record": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(Adjoint2d),
@@ -268,11 +270,13 @@ def last_program_pairs() -> int:
     return int(library().bzh_last_program_pairs())
 
 
-def call_program(args, attrs: ProgramAttrs, stream=None):
+def call_program(args, attrs: ProgramAttrs, stream=None, noalloc=False):
+    """``bzh_program`` (or ``bzh_program_noalloc``, which allocates nothing)."""
     lib = library()
     arr = buffers(args)
     s = current_stream_handle() if stream is None else stream
-    _check(lib.bzh_program(s, arr, len(args), ctypes.byref(attrs)), lib)
+    fn = lib.bzh_program_noalloc if noalloc else lib.bzh_program
+    _check(fn(s, arr, len(args), ctypes.byref(attrs)), lib)
 
 
 def call_batch2d(sims, current_step, attrs: BatchAttrs, stream=None):

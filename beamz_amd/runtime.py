@@ -165,20 +165,25 @@ def run_steps(state, prog, nsteps):
     return state
 
 
-def run_source_group_steps(state, prog, nsteps):
-    """Steps with the nine slab-source groups (``runtime.py:496-527``)."""
+def run_source_group_steps(state, prog, nsteps, *, noalloc=False):
+    """Steps with the nine slab-source groups (``runtime.py:496-527``);
+    ``noalloc`` as in :func:`run_program_steps`."""
     if uses_two_banks(prog):
         state = with_workspace(state)
         args = (*_temporal_io(prog, state), *_source_group_arguments(prog), state.current_step)
-        native.call_program(args, _program_attrs(prog, nsteps, abi.PROGRAM_LAYOUT_SOURCE_TEMPORAL_CPML))
+        native.call_program(args, _program_attrs(prog, nsteps, abi.PROGRAM_LAYOUT_SOURCE_TEMPORAL_CPML),
+                            noalloc=noalloc)
         return _swap_banks(state, nsteps)
     args = (*_graph_io(prog, state), *_source_group_arguments(prog), state.current_step)
-    native.call_program(args, _program_attrs(prog, nsteps, abi.PROGRAM_LAYOUT_SOURCE_IN_PLACE))
+    native.call_program(args, _program_attrs(prog, nsteps, abi.PROGRAM_LAYOUT_SOURCE_IN_PLACE),
+                        noalloc=noalloc)
     return state
 
 
-def run_program_steps(state, prog, nsteps):
-    """Steps with sources and packed DFT monitors (``runtime.py:621-682``)."""
+def run_program_steps(state, prog, nsteps, *, noalloc=False):
+    """Steps with sources and packed DFT monitors (``runtime.py:621-682``).
+    ``noalloc`` calls ``bzh_program_noalloc``, which allocates no device
+    memory (the state must already hold its workspace banks)."""
     if state.dft_re.dtype != torch.float64:
         raise ValueError("the HIP program accumulates DFT monitors in float64")
     tail = (*_source_group_arguments(prog), *prog.packed_monitors,
@@ -187,11 +192,11 @@ def run_program_steps(state, prog, nsteps):
         state = with_workspace(state)
         native.call_program((*_temporal_io(prog, state), *tail),
                             _program_attrs(prog, nsteps, abi.PROGRAM_LAYOUT_MONITOR_TEMPORAL_CPML,
-                                           monitor_count=len(prog.monitors)))
+                                           monitor_count=len(prog.monitors)), noalloc=noalloc)
         return _swap_banks(state, nsteps)
     native.call_program((*_graph_io(prog, state), *tail),
                         _program_attrs(prog, nsteps, abi.PROGRAM_LAYOUT_MONITOR_IN_PLACE,
-                                       monitor_count=len(prog.monitors)))
+                                       monitor_count=len(prog.monitors)), noalloc=noalloc)
     return state
 
 

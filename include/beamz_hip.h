@@ -215,6 +215,19 @@ int32_t bzh_streamed(void* stream, const bzh_buffer* args, int32_t nargs,
 int32_t bzh_program(void* stream, const bzh_buffer* args, int32_t nargs,
                     const bzh_program_attrs* attrs);
 
+/* No counterpart in the reference.  bzh_program for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): same arguments, layouts and results, but
+ * it calls no hipMalloc, hipFree or graph instantiation.  The buffered DFT's
+ * staging ring lives in a fixed device array of BZH_NOALLOC_SCRATCH_BYTES per
+ * GPU, held until the process ends; the ring takes as many steps (up to 16)
+ * as fit, and a call whose monitors sample more than that array holds in one
+ * step fails with an error.  Pairs of steps run only without monitors.  Calls
+ * share the array per GPU: calls that overlap in time on one GPU race. */
+#define BZH_NOALLOC_SCRATCH_BYTES (64u << 20)
+int32_t bzh_program_noalloc(void* stream, const bzh_buffer* args, int32_t nargs,
+                            const bzh_program_attrs* attrs);
+
 /* ---- batched 2D simulations ------------------------------------------------
  * No counterpart in the reference, which runs one simulation per call (its
  * FFI is vmapped sequentially, cuda/runtime.py:85).  Many independent TMxy
