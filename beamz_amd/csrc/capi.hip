@@ -804,29 +804,104 @@ FusedSplit fused_split(const Grid& g, int L) {
 }
 
 // Work split of the pair step (two steps per launch): tiles of 60 x 13
-// owned cells whose lanes cover x = 60 t - 2 .. 60 t + 61; interior tiles
-// have all of them in [M, nx - M - 1].  z chunks of 48 planes for both boxes
-// (32: 3.35 ms, 64: 3.32 ms against 3.29 ms per launch at 1024x512x256;
-// BZH_PAIR_KZ overrides it).  Each chunk recomputes two planes of the first
-// step, and the x-edge box shares its launch with the x-interior one, so its
-// chunks are not shortened to fill the GPU on their own.
-FusedSplit pair_split(const Grid& g, int L) {
+// owned cells whose lanes cover x = 60 t - 2 .. 60 t + 61; x-interior tiles
+// have all of them in [M, nx - M - 1].  z chunks of 48 planes (32: 3.35 ms,
+// 64: 3.32 ms against 3.29 ms per launch at 1024x512x256; BZH_PAIR_KZ
+// overrides it); each chunk recomputes two planes of the first step.
+//
+// The x-interior tiles whose 16 rows all lie clear of the y faces and y CPML
+// slabs, over the z range whose chunks' iterations all lie clear of the z
+// faces and z slabs, form the core: a launch of its own (the core kernel,
+// every iteration lean).  The rest, the shell, is one launch of the general
+// kernel over up to three boxes: the x-edge tiles, the x-interior tiles of
+// the outer y tiles, and the x-interior core-row tiles on the outer z range.
+// BZH_PAIR_CORE=0 keeps everything in the shell launch.
+struct PairSplit {
+  PairBox shell[kMaxPairBoxes];
+  int nshell = 0;
+  PairBox core{};
+  bool has_core = false;
+};
+
+int pair_kz() {
+  int kz = 48;
+  if (const char* env = std::getenv("BZH_PAIR_KZ")) kz = std::max(1, std::atoi(env));
+  return kz;
+}
+
+// Finish a box: z chunking over [z0, zsplit) and [zhi, z1) with chunks of at
+// most kz planes, bands and block count; returns false for an empty box.
+bool finish_box(PairBox& b, int kz, bool even) {
+  const int64_t tiles = static_cast<int64_t>(b.ntx) * b.nty;
+  auto chunks = [&](int n) {
+    if (n <= 0) return std::pair<int, int>(0, kz);
+    int c = (n + kz - 1) / kz;
+    return std::pair<int, int>(c, even ? (n + c - 1) / c : kz);
+  };
+  const auto lo = chunks(b.zsplit - b.z0), hi = chunks(b.z1 - b.zhi);
+  // both ranges share one chunk length
+  b.kz = std::max(lo.first > 0 ? lo.second : 1, hi.first > 0 ? hi.second : 1);
+  b.nzc_lo = b.zsplit > b.z0 ? (b.zsplit - b.z0 + b.kz - 1) / b.kz : 0;
+  b.nzc = b.nzc_lo + (b.z1 > b.zhi ? (b.z1 - b.zhi + b.kz - 1) / b.kz : 0);
+  if (tiles == 0 || b.nzc == 0) return false;
+  b.band = fused_banded() ? static_cast<int>((tiles + 7) / 8) : 0;
+  const int64_t n = (b.band > 0 ? 8 * int64_t{b.band} : tiles) * b.nzc;
+  b.nblk = static_cast<int32_t>((n + 7) / 8 * 8);
+  return true;
+}
+
+PairSplit pair_split(const Grid& g, int L) {
   const int W = kPairCols, TY = kPairRows;
   const int Tx = (g.nx + 1 + W - 1) / W, Ty = (g.ny + 1 + TY - 1) / TY, planes = g.nz + 1;
   const int M = std::max(L, 1);
   int tx0 = (M + 2 + W - 1) / W;                           // W tx0 - 2 >= M
   int tx1 = g.nx - M - 2 >= 0 ? std::min((g.nx - M - 2) / W, Tx) : 0;  // W tx1 <= nx - M - 2
   if (tx1 <= tx0) tx0 = tx1 = 0;
-  FusedSplit sp{};
-  sp.interior = {tx0, tx1 - tx0, tx1 - tx0, 0, Ty, 1, 0};
-  sp.edge = {0, tx0 + (Tx - tx1), tx0, tx1 - tx0, Ty, 1, 0};
-  int kz = 48;
-  if (const char* env = std::getenv("BZH_PAIR_KZ")) kz = std::max(1, std::atoi(env));
-  for (FusedBox* b : {&sp.interior, &sp.edge}) {
-    if (static_cast<int64_t>(b->ntx) * b->nty == 0) continue;
-    b->kz = kz;
-    b->nzc = (planes + kz - 1) / kz;
+  const int kz = pair_kz();
+  // core rows: 13 by - 2 >= M and 13 by + 13 <= ny - L - 1
+  const int cy0 = (M + 2 + TY - 1) / TY;
+  const int cy1 = g.ny - L - 1 - TY >= 0 ? (g.ny - L - 1 - TY) / TY + 1 : 0;
+  // core planes: chunks [kz0, kz1) with kz0 - 4 >= M and kz1 + 2 <= nz - M - 1
+  const int cz0 = M + 4, cz1 = g.nz - M - 3;
+  const char* env = std::getenv("BZH_PAIR_CORE");
+  const bool core = !(env && std::string(env) == "0") && tx1 > tx0 && cy1 > cy0 && cz1 > cz0;
+  PairSplit sp;
+  auto box = [&](int bx0, int ntx, int tsplit, int tskip, int by0, int nty, int tysplit, int tyskip,
+                 int z0, int zsplit, int zhi, int z1, int xedge) {
+    PairBox b{};
+    b.tx0 = bx0;
+    b.ntx = ntx;
+    b.tsplit = tsplit;
+    b.tskip = tskip;
+    b.ty0 = by0;
+    b.nty = nty;
+    b.tysplit = tysplit;
+    b.tyskip = tyskip;
+    b.z0 = z0;
+    b.zsplit = zsplit;
+    b.zhi = zhi;
+    b.z1 = z1;
+    b.xedge = xedge;
+    return b;
+  };
+  // the x-edge tiles (slower: first, so that they do not form the tail)
+  PairBox e = box(0, tx0 + (Tx - tx1), tx0, tx1 - tx0, 0, Ty, Ty, 0, 0, planes, planes, planes, 1);
+  if (finish_box(e, kz, false)) sp.shell[sp.nshell++] = e;
+  if (!core) {
+    PairBox in = box(tx0, tx1 - tx0, tx1 - tx0, 0, 0, Ty, Ty, 0, 0, planes, planes, planes, 0);
+    if (finish_box(in, kz, false)) sp.shell[sp.nshell++] = in;
+    return sp;
   }
+  // x-interior tiles outside the core rows, every plane
+  PairBox out = box(tx0, tx1 - tx0, tx1 - tx0, 0, 0, Ty - (cy1 - cy0), cy0, cy1 - cy0, 0, planes,
+                    planes, planes, 0);
+  if (finish_box(out, kz, false)) sp.shell[sp.nshell++] = out;
+  // x-interior core-row tiles on the planes outside the core range
+  PairBox rim = box(tx0, tx1 - tx0, tx1 - tx0, 0, cy0, cy1 - cy0, cy1 - cy0, 0, 0, cz0, cz1, planes, 0);
+  if (finish_box(rim, kz, false)) sp.shell[sp.nshell++] = rim;
+  // the core, in chunks of equal length
+  sp.core = box(tx0, tx1 - tx0, tx1 - tx0, 0, cy0, cy1 - cy0, cy1 - cy0, 0, cz0, cz1, cz1, cz1, 0);
+  sp.has_core = finish_box(sp.core, kz, true);
   return sp;
 }
 
@@ -1253,35 +1328,19 @@ bool run_program(hipStream_t s, ProgramIn& in) {
         pp.mbox = pws.boxes;
         pp.mbox_n = mon_m;
       }
-      // one launch: the x-interior box's blocks, then the x-edge box's
-      const FusedSplit sp = pair_split(g, pp.f.L);
-      // (without bands the block count is padded to a multiple of 8; the
-      // kernel drops the padding blocks)
-      auto blocks = [](const FusedBox& b, int band) {
-        const int64_t tiles = static_cast<int64_t>(b.ntx) * b.nty;
-        const int64_t n = (band > 0 ? 8 * int64_t{band} : tiles) * b.nzc;
-        return static_cast<int32_t>((n + 7) / 8 * 8);
-      };
-      set_box(pp.f, sp.interior, 0);
-      const FusedBox& eb = sp.edge;
-      pp.e_tx0 = eb.tx0;
-      pp.e_ntx = eb.ntx;
-      pp.e_tsplit = eb.tsplit;
-      pp.e_tskip = eb.tskip;
-      pp.e_kz = eb.kz;
-      pp.e_nzc = eb.nzc;
-      pp.e_band = fused_banded() ? (eb.ntx * eb.nty + 7) / 8 : 0;
-      pp.nblk_int = (sp.interior.ntx > 0 && sp.interior.nty > 0) ? blocks(sp.interior, pp.f.box_band) : 0;
-      pp.nblk_edge = (eb.ntx > 0 && eb.nty > 0) ? blocks(eb, pp.e_band) : 0;
-      if (pp.nblk_int == 0) {
-        pp.f.box_ntx = 0;
-        pp.f.box_kz = std::max(pp.f.box_kz, 1);
+      // the shell launch (general kernel), then the core launch
+      const PairSplit sp = pair_split(g, pp.f.L);
+      pp.nbox = sp.nshell;
+      for (int b = 0; b < sp.nshell; ++b) pp.box[b] = sp.shell[b];
+      pp.core = 0;
+      if (pp.nbox > 0) launches.push_back(make_launch(Kind::kPair, 0, g.dtype, pp));
+      if (sp.has_core) {
+        PairParams pc = pp;
+        pc.nbox = 1;
+        pc.box[0] = sp.core;
+        pc.core = 1;
+        launches.push_back(make_launch(Kind::kPair, 0, g.dtype, pc));
       }
-      if (pp.nblk_edge == 0) {
-        pp.e_ntx = 0;
-        pp.e_kz = std::max(pp.e_kz, 1);
-      }
-      launches.push_back(make_launch(Kind::kPair, 0, g.dtype, pp));
       if (in.has_monitors) add_dft(k, cbank);
       for (int c = 0; c < 3 && 6 + c < in.ngroups; ++c)
         add_source_launches(launches, in.groups[6 + c], 2, c, nxt, g, in.metallic, step0, k + 1);

@@ -218,6 +218,21 @@ int fused_rows();
 
 constexpr int kMaxMonBoxes = 8;
 
+// A box of pair work items: x tiles tx0 + [0, ntx), the ones at or past
+// tsplit shifted by tskip (two x ranges in one box); y tiles ty0 + [0, nty)
+// likewise with tysplit and tyskip; z chunks of kz planes, nzc_lo of them
+// over [z0, zsplit) and the rest over [zhi, z1).  Tiles are dealt to the 8
+// XCDs in bands of ``band`` tiles (0: contiguous ranges).
+struct PairBox {
+  int32_t tx0, ntx, tsplit, tskip;
+  int32_t ty0, nty, tysplit, tyskip;
+  int32_t z0, zsplit, zhi, z1, kz, nzc_lo, nzc;
+  int32_t band;
+  int32_t nblk;   // workgroups of this box, a multiple of 8
+  int32_t xedge;  // 1: tiles that touch an x face or an x CPML slab
+};
+constexpr int kMaxPairBoxes = 3;
+
 // Two leapfrog steps in one launch (bank A -> bank B).  ``f`` carries the
 // single-step layout and work box (tiles of 60 x 13 owned cells); the first
 // step's fields are written to the bank at ``cmid`` (same layout, foff) only
@@ -234,12 +249,14 @@ struct PairParams {
   char* cmid;
   const int32_t* mbox;
   int32_t mbox_n;
-  // One launch covers both boxes: workgroups [0, nblk_edge) run the x-edge
-  // box below (same rows and z range), the next nblk_int the x-interior box
-  // of ``f`` (box_* fields).  nblk_edge is a multiple of 8, so both parts deal
-  // their tiles to the XCDs from XCD 0.
-  int32_t nblk_int, nblk_edge;
-  int32_t e_tx0, e_ntx, e_tsplit, e_tskip, e_kz, e_nzc, e_band;
+  // The launch's work: box[0 .. nbox) in order, workgroups numbered through
+  // them (each box's block count is a multiple of 8, so every box deals its
+  // tiles to the XCDs from XCD 0).  ``core`` launches run the core kernel: one
+  // box whose every iteration is a lean one (no face, slab, ghost or x-edge
+  // work on any of its rows and planes).
+  PairBox box[kMaxPairBoxes];
+  int32_t nbox;
+  int32_t core;
 };
 constexpr int kPairCols = 60;   // owned columns of a pair tile
 constexpr int kPairRows = 13;   // owned rows of a pair tile (16 waves)

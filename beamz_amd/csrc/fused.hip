@@ -104,35 +104,36 @@ __device__ __forceinline__ bool decode_tile(const FusedParams& f, int& bx, int& 
   return true;
 }
 
-// Work item of a pair workgroup: block ``blk`` of the part (x-interior box
-// of f, or the x-edge box of the pair parameters) with ``nblk`` blocks.
-__device__ __forceinline__ bool decode_pair_tile(const PairParams& pp, bool edge, int blk,
-                                                 int nblk, int& bx, int& by, int& kz0,
-                                                 int& kz1) {
-  const FusedParams& f = pp.f;
-  const int tx0 = edge ? pp.e_tx0 : f.box_tx0, ntx = edge ? pp.e_ntx : f.box_ntx;
-  const int tsplit = edge ? pp.e_tsplit : f.box_tsplit, tskip = edge ? pp.e_tskip : f.box_tskip;
-  const int kz = edge ? pp.e_kz : f.box_kz, nzc = edge ? pp.e_nzc : f.box_nzc;
-  const int band = edge ? pp.e_band : f.box_band;
-  const int lam = xcd_remap(blk, nblk);
+// Work item of block ``blk`` of a pair box (see PairBox).  Indices past the
+// last tile of a band, or past the last chunk, are padding: false.
+__device__ __forceinline__ bool decode_pair_box(const PairBox& b, int blk, int& bx, int& by,
+                                                int& kz0, int& kz1) {
+  const int lam = xcd_remap(blk, b.nblk);
+  const int tiles = b.ntx * b.nty;
   int t, zc;
-  if (band > 0) {
-    const int per = band * nzc;
+  if (b.band > 0) {
+    const int per = b.band * b.nzc;
     const int c = lam / per, rem = lam - c * per;
-    zc = rem / band;
-    t = c * band + (rem - zc * band);
-    if (t >= ntx * f.box_nty) return false;
+    zc = rem / b.band;
+    t = c * b.band + (rem - zc * b.band);
+    if (t >= tiles) return false;
   } else {
-    t = lam % (ntx * f.box_nty);
-    zc = lam / (ntx * f.box_nty);
-    if (zc >= nzc) return false;  // padding up to a multiple of 8 blocks
+    t = lam % tiles;
+    zc = lam / tiles;
+    if (zc >= b.nzc) return false;
   }
-  bx = t % ntx;
-  by = f.box_ty0 + t / ntx;
-  if (bx >= tsplit) bx += tskip;
-  bx += tx0;
-  kz0 = f.box_z0 + zc * kz;
-  kz1 = min(kz0 + kz, f.box_z1);
+  int tx = t % b.ntx, ty = t / b.ntx;
+  if (tx >= b.tsplit) tx += b.tskip;
+  if (ty >= b.tysplit) ty += b.tyskip;
+  bx = b.tx0 + tx;
+  by = b.ty0 + ty;
+  if (zc < b.nzc_lo) {
+    kz0 = b.z0 + zc * b.kz;
+    kz1 = min(kz0 + b.kz, b.zsplit);
+  } else {
+    kz0 = b.zhi + (zc - b.nzc_lo) * b.kz;
+    kz1 = min(kz0 + b.kz, b.z1);
+  }
   return true;
 }
 
@@ -193,6 +194,23 @@ __device__ __forceinline__ void stu(const char* base, uint32_t aoff, uint32_t ro
     *reinterpret_cast<gptr<uint16_t>>(p) = u;
   }
 }
+// Access at ``base + soff + voff``: a uniform byte offset (plane and array)
+// plus a per-lane byte offset (row and column) held in a vector register.
+template <typename T>
+__device__ __forceinline__ float ldo(const char* base, uint32_t soff, uint32_t voff) {
+  return ldu<T>(base, soff, 0u, voff);
+}
+
+// One dword per lane from ``base + off`` (global) into an LDS row (lane l at
+// row[l]), asynchronously: the wave goes on, and the compiler waits for the
+// transfer before the row is read.
+__device__ __forceinline__ void dma_row(const char* base, uint32_t soff, uint32_t voff,
+                                        float* row) {
+  __builtin_amdgcn_global_load_lds(
+      (__attribute__((address_space(1))) void*)((gptr<const char>)(base) + (soff + voff)),
+      (__attribute__((address_space(3))) void*)(row), 4, 0, 0);
+}
+
 // Field store: bfloat16 fields take stochastic rounding when ``sr`` (uniform)
 // is set, every other storage type the ordinary conversion.
 template <typename T>
@@ -719,9 +737,14 @@ struct PairLds {
   float e2[2][kPairWaves][2][64];      // E' rows for H''
   float h2[2][kPairWaves][2][64];      // H'' rows for E''
 };
+// The core kernel's prefetch rows: each wave's inputs of its next iteration,
+// fetched by LDS DMA while the current one computes: E(k+1) x,y,z, H(k)
+// x,y,z, E coefficients x,y,z, top-row E(k+1) x,z (wave 0).
+using PairPrefetch = float[kPairWaves][11][64];
 
-template <typename T, typename P, typename C, bool kEArr, bool kXEdge, bool kHSrc>
-__device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, int blk, int nblk) {
+template <typename T, typename P, typename C, bool kEArr, bool kXEdge, bool kHSrc, bool kCore>
+__device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, PairPrefetch* pf,
+                                          int box, int blk) {
   const FusedParams& f = pp.f;
   const SlabArgs<kXEdge> sa{f};  // x-edge tiles: reloaded (x-slab terms raise the pressure)
   constexpr int kW = kPairWaves;
@@ -734,7 +757,7 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, in
   const bool halo = w == 0;  // computes H' (Hx, Hz) only and loads the top E row
 
   int bx, by, kz0, kz1;
-  if (!decode_pair_tile(pp, kXEdge, blk, nblk, bx, by, kz0, kz1)) return;
+  if (!decode_pair_box(pp.box[box], blk, bx, by, kz0, kz1)) return;
 
   const int nz = f.nz, ny = f.ny, nx = f.nx, L = f.L, L2 = 2 * f.L;
   const int m = f.metallic;
@@ -1091,7 +1114,209 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, in
   float sb0 = es0, sb1 = es1, sb2 = es2;     // ... of plane k-2
   float mh[6] = {}, me1[6] = {}, me2[6] = {};  // psi': H terms at k-1, E terms at k-1, k-2
 
+  // Lean iterations: on an x-interior row with no y face, ghost or CPML term,
+  // iteration k touches planes k-2 .. k+1 only; when all of them are clear of
+  // the z faces and z slabs and the chunk owns all four stages' outputs, the
+  // iteration runs a branch-free form of the same arithmetic (the ``plain``
+  // forms of h_update and e_update, bit for bit) with per-lane byte offsets
+  // computed once per row and one uniform plane offset per array family.
+  // Families (plane stride, row stride): A (ny+1)*nx, nx: Ex, Hy;  B ny*(nx+1),
+  // nx+1: Ey, Hx;  C (ny+1)*(nx+1), nx+1: Ez;  D ny*nx, nx: Hz.
+  const bool lean_row = !kXEdge && row_plain && qyH < 0 && qyE < 0;
+  const int lean_lo = max(L, 1), lean_hi = nz - max(L, 1) - 1;  // planes clear of faces and slabs
+  const int k_lo = lean_lo + 2, k_hi = lean_hi - 1;
+  constexpr uint32_t szT = sizeof(T), szC = sizeof(C);
+  const uint32_t oA = (yX * nx + xX) * szT, oB = (yY * (nx + 1) + xY) * szT;
+  const uint32_t oC = (yX * (nx + 1) + xY) * szT, oD = (yY * nx + xX) * szT;
+  const uint32_t oAt = (ytX * nx + xX) * szT, oCt = (ytX * (nx + 1) + xY) * szT;
+  const uint32_t cA = (yX * nx + xX) * szC, cB = (yY * (nx + 1) + xY) * szC;
+  const uint32_t cC = (yX * (nx + 1) + xY) * szC;
+  const uint32_t nA = (ny + 1) * nx, nB = ny * (nx + 1), nC = (ny + 1) * (nx + 1), nD = ny * nx;
+  // float32 fields and coefficients: a lean iteration's loads are issued as
+  // LDS DMA during the previous lean iteration (after its barrier), so their
+  // latency hides behind that iteration's second half
+  constexpr bool kPf = kCore && sizeof(T) == 4 && sizeof(C) == 4;
+  float (*pfw)[64] = kPf ? (*pf)[w] : nullptr;
+  bool pf_ok = false;  // pfw holds this iteration's inputs
+
   for (int k = ks; k <= kz1 + 1; ++k) {
+    // core launches: every row and iteration is lean by construction; the
+    // shell kernel runs the general iteration only
+    if constexpr (kCore) {
+      const int bb = k & 1;
+      const bool g1 = k <= kz1;
+      const bool gE1 = g1 && k >= kE && w >= 1;
+      const bool gH2 = k - 1 >= kE && k - 1 < kz1 && w >= 1 && w <= kW - 2;
+      const bool gE2 = k - 2 >= kz0 && k - 2 < kz1 && w >= 2 && w <= kW - 2;
+      const bool sampled = mwave && k >= kz0 && k < kz1 && sample_plane(k);
+      // plane k in bytes, per family; plane k + d is z + d * p
+      const uint32_t pA = nA * szT, pB = nB * szT, pC = nC * szT, pD = nD * szT;
+      const uint32_t zA = k * pA, zB = k * pB, zC = k * pC, zD = k * pD;
+      // ---- first step: H'(k) (values used only after every load is issued) ----
+      float exn = 0.0f, eyn = 0.0f, ezn = 0.0f, hx = 0.0f, hy = 0.0f, hz = 0.0f;
+      float s0 = es0, s1 = es1, s2 = es2;
+      if (g1) {
+        float etx = 0.0f, etz = 0.0f, ha0, ha1 = 0.0f, ha2;
+        if (kPf && pf_ok) {
+          exn = pfw[0][lane];
+          eyn = pfw[1][lane];
+          ezn = pfw[2][lane];
+          ha0 = pfw[3][lane];
+          if (!halo) ha1 = pfw[4][lane];
+          ha2 = pfw[5][lane];
+          if constexpr (kEArr) {
+            if (gE1) {
+              s0 = pfw[6][lane];
+              s1 = pfw[7][lane];
+              s2 = pfw[8][lane];
+            }
+          }
+          if (halo) {
+            etx = pfw[9][lane];
+            etz = pfw[10][lane];
+          }
+        } else {
+          exn = ldo<T>(f.fin, f.foff[0] + zA + pA, oA);
+          eyn = ldo<T>(f.fin, f.foff[1] + zB + pB, oB);
+          ezn = ldo<T>(f.fin, f.foff[2] + zC + pC, oC);
+          if (halo) {
+            etx = ldo<T>(f.fin, f.foff[0] + zA + pA, oAt);
+            etz = ldo<T>(f.fin, f.foff[2] + zC + pC, oCt);
+          }
+          ha0 = ldo<T>(f.fin, f.foff[3] + zB, oB);
+          if (!halo) ha1 = ldo<T>(f.fin, f.foff[4] + zA, oA);
+          ha2 = ldo<T>(f.fin, f.foff[5] + zD, oD);
+          if constexpr (kEArr) {
+            if (gE1) {
+              s0 = ldo<C>(f.cbase, f.coff[0] + k * nA * szC, cA);
+              s1 = ldo<C>(f.cbase, f.coff[1] + k * nB * szC, cB);
+              s2 = ldo<C>(f.cbase, f.coff[2] + k * nC * szC, cC);
+            }
+          }
+        }
+        const float ez_up = lE1[bb][w + 1][0][lane];
+        const float ex_up = lE1[bb][w + 1][1][lane];
+        const float ez_r = lane_shift(ez, dn), ey_r = lane_shift(ey, dn);
+        hx = hc[0] * ha0 - hci[0] * ((ez_up - ez) - (eyn - ey));
+        hy = hc[1] * ha1 - hci[1] * ((exn - ex) - (ez_r - ez));
+        hz = hc[2] * ha2 - hci[2] * ((ey_r - ey) - (ex_up - ex));
+        if (kHSrc && swave) mid_sources(2, k, hx, hy, hz);
+        lH1[bb][w][0][lane] = hz;
+        lH1[bb][w][1][lane] = hx;
+        lE1[bb ^ 1][w][0][lane] = ezn;
+        lE1[bb ^ 1][w][1][lane] = exn;
+        if (halo) {
+          lE1[bb ^ 1][kW][0][lane] = (vtX && vxY) ? etz : 0.0f;
+          lE1[bb ^ 1][kW][1][lane] = (vtX && vxX) ? etx : 0.0f;
+        }
+        if (sampled) store_h(pair_kargs()->cmid, k, hx, hy, hz);
+      }
+      __syncthreads();
+
+      // the next iteration's inputs, when it is lean too (its loads would
+      // otherwise wait at its top): planes k + 2 (E) and k + 1 (H, E
+      // coefficients); the rows of this one were all read before the barrier
+      if constexpr (kPf) {
+        const int kn = k + 1;
+        pf_ok = kn <= k_hi && kn <= kz1;
+        if (pf_ok) {
+          dma_row(f.fin, f.foff[0] + zA + 2 * pA, oA, pfw[0]);
+          dma_row(f.fin, f.foff[1] + zB + 2 * pB, oB, pfw[1]);
+          dma_row(f.fin, f.foff[2] + zC + 2 * pC, oC, pfw[2]);
+          dma_row(f.fin, f.foff[3] + zB + pB, oB, pfw[3]);
+          if (!halo) dma_row(f.fin, f.foff[4] + zA + pA, oA, pfw[4]);
+          dma_row(f.fin, f.foff[5] + zD + pD, oD, pfw[5]);
+          if constexpr (kEArr) {
+            if (w >= 1) {
+              dma_row(f.cbase, f.coff[0] + kn * nA * szC, cA, pfw[6]);
+              dma_row(f.cbase, f.coff[1] + kn * nB * szC, cB, pfw[7]);
+              dma_row(f.cbase, f.coff[2] + kn * nC * szC, cC, pfw[8]);
+            }
+          }
+          if (halo) {
+            dma_row(f.fin, f.foff[0] + zA + 2 * pA, oAt, pfw[9]);
+            dma_row(f.fin, f.foff[2] + zC + 2 * pC, oCt, pfw[10]);
+          }
+        }
+      }
+
+      // ---- first step: E'(k), then the sources between the steps ----
+      float nex = 0.0f, ney = 0.0f, nez = 0.0f;
+      if (gE1) {
+        const float hz_dn = lH1[bb][w - 1][0][lane];
+        const float hx_dn = lH1[bb][w - 1][1][lane];
+        const float hz_l = lane_shift(hz, up), hy_l = lane_shift(hy, up);
+        nex = ec[0] * ex + (s0 * idx) * ((hz - hz_dn) - (hy - h1y));
+        ney = ec[1] * ey + (s1 * idx) * ((hx - h1x) - (hz - hz_l));
+        nez = ec[2] * ez + (s2 * idx) * ((hy - hy_l) - (hx - hx_dn));
+        if (swave) mid_sources(0, k, nex, ney, nez);
+        if (sampled) store_e(pair_kargs()->cmid, k, nex, ney, nez);
+        if (swave) mid_sources(1, k, nex, ney, nez);
+        lE2[bb][w][0][lane] = nez;
+        lE2[bb][w][1][lane] = nex;
+      }
+
+      // ---- second step: H''(k-1) ----
+      float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+      if (gH2) {
+        const float ez_up2 = lE2[bb ^ 1][w + 1][0][lane];
+        const float ex_up2 = lE2[bb ^ 1][w + 1][1][lane];
+        const float ez_r2 = lane_shift(e1z, dn), ey_r2 = lane_shift(e1y, dn);
+        gx = hc[0] * h1x - hci[0] * ((ez_up2 - e1z) - (ney - e1y));
+        gy = hc[1] * h1y - hci[1] * ((nex - e1x) - (ez_r2 - e1z));
+        gz = hc[2] * h1z - hci[2] * ((ey_r2 - e1y) - (ex_up2 - e1x));
+        if (kHSrc && swave) mid_sources(3, k - 1, gx, gy, gz);
+        lH2[bb][w][0][lane] = gz;
+        lH2[bb][w][1][lane] = gx;
+        if (own && k - 1 >= kz0) {
+          stf<T>(f.fout, f.foff[3] + zB - pB, 0u, oB, gx, sr, rng);
+          stf<T>(f.fout, f.foff[4] + zA - pA, 0u, oA, gy, sr, rng);
+          stf<T>(f.fout, f.foff[5] + zD - pD, 0u, oD, gz, sr, rng);
+        }
+      }
+
+      // ---- second step: E''(k-2) ----
+      if (gE2) {
+        const float hz_dn = lH2[bb ^ 1][w - 1][0][lane];
+        const float hx_dn = lH2[bb ^ 1][w - 1][1][lane];
+        const float hz_l = lane_shift(g2z, up), hy_l = lane_shift(g2y, up);
+        const float fx = ec[0] * e2x + (sb0 * idx) * ((g2z - hz_dn) - (g2y - g3y));
+        const float fy = ec[1] * e2y + (sb1 * idx) * ((g2x - g3x) - (g2z - hz_l));
+        const float fz = ec[2] * e2z + (sb2 * idx) * ((g2y - hy_l) - (g2x - hx_dn));
+        if (own) {
+          stf<T>(f.fout, f.foff[0] + zA - 2 * pA, 0u, oA, fx, sr, rng);
+          stf<T>(f.fout, f.foff[1] + zB - 2 * pB, 0u, oB, fy, sr, rng);
+          stf<T>(f.fout, f.foff[2] + zC - 2 * pC, 0u, oC, fz, sr, rng);
+        }
+      }
+
+      // the CPML carries (mh, me1, me2) are not advanced: the planes this
+      // iteration touches have no z term, and the general iterations that
+      // follow read them only on planes that have one
+      g3x = g2x;
+      g3y = g2y;
+      g2x = gx;
+      g2y = gy;
+      g2z = gz;
+      e2x = e1x;
+      e2y = e1y;
+      e2z = e1z;
+      e1x = nex;
+      e1y = ney;
+      e1z = nez;
+      h1x = hx;
+      h1y = hy;
+      h1z = hz;
+      sb0 = sa0;
+      sb1 = sa1;
+      sb2 = sa2;
+      sa0 = s0;
+      sa1 = s1;
+      sa2 = s2;
+      ex = exn;
+      ey = eyn;
+      ez = ezn;
+    } else {
     const int bb = k & 1;
     const bool g1 = k <= kz1;
     const bool gE1 = g1 && k >= kE && w >= 1;
@@ -1117,9 +1342,9 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, in
       if (gE1) {
         if constexpr (kEArr) {
           const uint32_t kx = min(k, nz), kzc = min(k, nz - 1);
-          s0 = ldu<C>(f.cbase, f.coff[0], (kx * (ny + 1) + yX) * nx, cX) * f.ratio_e;
-          s1 = ldu<C>(f.cbase, f.coff[1], (kx * ny + yY) * (nx + 1), cY) * f.ratio_e;
-          s2 = ldu<C>(f.cbase, f.coff[2], (kzc * (ny + 1) + yX) * (nx + 1), cY) * f.ratio_e;
+          s0 = ldu<C>(f.cbase, f.coff[0], (kx * (ny + 1) + yX) * nx, cX);
+          s1 = ldu<C>(f.cbase, f.coff[1], (kx * ny + yY) * (nx + 1), cY);
+          s2 = ldu<C>(f.cbase, f.coff[2], (kzc * (ny + 1) + yX) * (nx + 1), cY);
         }
         load_psi_e(k, pe);
       }
@@ -1215,6 +1440,7 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, in
     ex = exn;
     ey = eyn;
     ez = ezn;
+    }  // general iteration
   }
 }
 
@@ -1225,9 +1451,21 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, in
 template <typename T, typename P, typename C, bool kEArr, bool kHSrc>
 __global__ void __launch_bounds__(64 * kPairWaves) pair_step(PairParams p) {
   __shared__ PairLds lds;
-  const int b = blockIdx.x;
-  if (b < p.nblk_edge) pair_body<T, P, C, kEArr, true, kHSrc>(p, lds, b, p.nblk_edge);
-  else pair_body<T, P, C, kEArr, false, kHSrc>(p, lds, b - p.nblk_edge, p.nblk_int);
+  int b = blockIdx.x, i = 0;
+  while (i + 1 < p.nbox && b >= p.box[i].nblk) b -= p.box[i++].nblk;
+  if (p.box[i].xedge) pair_body<T, P, C, kEArr, true, kHSrc, false>(p, lds, nullptr, i, b);
+  else pair_body<T, P, C, kEArr, false, kHSrc, false>(p, lds, nullptr, i, b);
+}
+
+// The core kernel: x-interior tiles whose rows all lie clear of the y faces
+// and y slabs, on z chunks whose iterations all lie clear of the z faces and z
+// slabs.  Only the lean iteration is compiled, so its registers are not
+// sized by the general one.
+template <typename T, typename P, typename C, bool kEArr, bool kHSrc>
+__global__ void __launch_bounds__(64 * kPairWaves) pair_core(PairParams p) {
+  __shared__ PairLds lds;
+  __shared__ PairPrefetch pf;
+  pair_body<T, P, C, kEArr, false, kHSrc, true>(p, lds, &pf, 0, blockIdx.x);
 }
 
 template <typename T, typename P, typename C>
@@ -1235,7 +1473,15 @@ void launch_pair_wg(hipStream_t s, const PairParams& p, int nblocks) {
   const dim3 block(64 * kPairWaves, 1, 1), grid(nblocks, 1, 1);
   const bool earr = p.f.esrc_st[0] != 0;
   const bool hsrc = p.msrc_n > p.msrc_e;  // H-timing slabs in the list
-  if (hsrc) {
+  if (p.core) {
+    if (hsrc) {
+      if (earr) hipLaunchKernelGGL((pair_core<T, P, C, true, true>), grid, block, 0, s, p);
+      else hipLaunchKernelGGL((pair_core<T, P, float, false, true>), grid, block, 0, s, p);
+    } else {
+      if (earr) hipLaunchKernelGGL((pair_core<T, P, C, true, false>), grid, block, 0, s, p);
+      else hipLaunchKernelGGL((pair_core<T, P, float, false, false>), grid, block, 0, s, p);
+    }
+  } else if (hsrc) {
     if (earr) hipLaunchKernelGGL((pair_step<T, P, C, true, true>), grid, block, 0, s, p);
     else hipLaunchKernelGGL((pair_step<T, P, float, false, true>), grid, block, 0, s, p);
   } else {
@@ -1321,39 +1567,48 @@ void launch_pair3d(hipStream_t s, const PairParams& pp, int dtype) {
   const FusedParams& p = pp.f;
   BZH_REQUIRE(p.rows == kPairRows, "pair tiles are cut with the pair tile height");
   BZH_REQUIRE(!p.coef_f16 || dtype != BZH_F32, "float16 coefficients need half-precision fields");
+  BZH_REQUIRE(pp.nbox >= 1 && pp.nbox <= kMaxPairBoxes && (!pp.core || pp.nbox == 1),
+              "pair launch boxes are malformed");
   const int ntx_total = (p.nx + 1 + kPairCols - 1) / kPairCols;
   const int nty_total = (p.ny + 1 + kPairRows - 1) / kPairRows;
-  BZH_REQUIRE(p.box_ty0 >= 0 && p.box_nty >= 0 && p.box_ty0 + p.box_nty <= nty_total &&
-                  p.box_z0 >= 0 && p.box_z1 <= p.nz + 1,
-              "pair box exceeds the grid");
-  // each part: tiles inside the grid, z chunks inside [z0, z1), enough blocks
-  struct Part {
-    int tx0, ntx, tsplit, tskip, kz, nzc, band, nblk;
-  };
-  const Part parts[2] = {
-      {p.box_tx0, p.box_ntx, p.box_tsplit, p.box_tskip, p.box_kz, p.box_nzc, p.box_band, pp.nblk_int},
-      {pp.e_tx0, pp.e_ntx, pp.e_tsplit, pp.e_tskip, pp.e_kz, pp.e_nzc, pp.e_band, pp.nblk_edge}};
-  for (const Part& q : parts) {
-    BZH_REQUIRE(q.ntx >= 0 && q.kz >= 1 && q.nzc >= 0 && q.tx0 >= 0 && q.tsplit >= 0 &&
-                    q.tskip >= 0 && q.tx0 + q.ntx + q.tskip <= ntx_total &&
-                    p.box_z0 + (q.nzc - 1) * q.kz < p.box_z1,
-                "pair box exceeds the grid");
-    const int64_t tiles = static_cast<int64_t>(q.ntx) * p.box_nty;
-    BZH_REQUIRE(q.band == 0 || (q.band >= 1 && 8 * int64_t{q.band} >= tiles),
+  const int M = p.L > 0 ? p.L : 1;
+  int64_t nblocks = 0;
+  for (int i = 0; i < pp.nbox; ++i) {
+    const PairBox& b = pp.box[i];
+    // tiles and chunks inside the grid, block count matching the box
+    BZH_REQUIRE(b.ntx >= 0 && b.nty >= 0 && b.tx0 >= 0 && b.tsplit >= 0 && b.tskip >= 0 &&
+                    b.tx0 + b.ntx + b.tskip <= ntx_total && b.ty0 >= 0 && b.tysplit >= 0 &&
+                    b.tyskip >= 0 && b.ty0 + b.nty + b.tyskip <= nty_total,
+                "pair box tiles exceed the grid");
+    BZH_REQUIRE(b.kz >= 1 && b.nzc_lo >= 0 && b.nzc >= b.nzc_lo && 0 <= b.z0 && b.z0 <= b.zsplit &&
+                    b.zsplit <= b.zhi && b.zhi <= b.z1 && b.z1 <= p.nz + 1 &&
+                    (b.nzc_lo == 0 || b.z0 + (b.nzc_lo - 1) * b.kz < b.zsplit) &&
+                    (b.nzc == b.nzc_lo || b.zhi + (b.nzc - b.nzc_lo - 1) * b.kz < b.z1),
+                "pair box chunks exceed the grid");
+    const int64_t tiles = static_cast<int64_t>(b.ntx) * b.nty;
+    BZH_REQUIRE(b.band == 0 || (b.band >= 1 && 8 * int64_t{b.band} >= tiles),
                 "pair bands do not cover the tiles");
-    const int64_t need = (q.band > 0 ? 8 * int64_t{q.band} : tiles) * q.nzc;
-    BZH_REQUIRE(q.nblk == need || (q.band == 0 && q.nblk == (need + 7) / 8 * 8),
+    const int64_t need = (b.band > 0 ? 8 * int64_t{b.band} : tiles) * b.nzc;
+    BZH_REQUIRE(b.nblk % 8 == 0 && b.nblk >= need && b.nblk < need + 8,
                 "pair block count does not match its box");
-  }
-  BZH_REQUIRE(pp.nblk_edge % 8 == 0, "the x-edge part must be a multiple of 8 blocks");
-  if (pp.nblk_int > 0) {
-    // the interior variant reads lanes x = i0-2 .. i0+61 unchecked and assumes
-    // no x face or x CPML slab under them
-    const int M = p.L > 0 ? p.L : 1;
-    const int lo = p.box_tx0 * kPairCols - 2;
-    const int hi = (p.box_tx0 + p.box_ntx + p.box_tskip - 1) * kPairCols + kPairCols + 1;
-    BZH_REQUIRE(p.box_ntx == 0 || (lo >= M && hi <= p.nx - M - 1),
-                "interior pair tiles overlap an x face or x CPML slab");
+    BZH_REQUIRE(tiles == 0 || b.nblk > 0, "pair box without blocks");
+    if (!b.xedge && tiles > 0) {
+      // the interior variant reads lanes x = i0-2 .. i0+61 unchecked and
+      // assumes no x face or x CPML slab under them
+      const int lo = b.tx0 * kPairCols - 2;
+      const int hi = (b.tx0 + b.ntx + b.tskip - 1) * kPairCols + kPairCols + 1;
+      BZH_REQUIRE(lo >= M && hi <= p.nx - M - 1, "interior pair tiles overlap an x face or x CPML slab");
+    }
+    if (pp.core && tiles > 0) {
+      // core tiles: rows j0-2 .. j0+13 in [M, ny-L-1] and iterations
+      // kz0-2 .. kz1+1 touching planes in [M, nz-M-1] (lean everywhere)
+      BZH_REQUIRE(!b.xedge && b.tyskip == 0 && b.nzc == b.nzc_lo, "core box is malformed");
+      const int y_lo = b.ty0 * kPairRows - 2, y_hi = (b.ty0 + b.nty - 1) * kPairRows + kPairRows;
+      BZH_REQUIRE(y_lo >= M && y_hi <= p.ny - p.L - 1, "core tiles reach a y face or y CPML slab");
+      BZH_REQUIRE(b.z0 - 4 >= M && b.zsplit + 2 <= p.nz - M - 1,
+                  "core chunks reach a z face or z CPML slab");
+    }
+    nblocks += b.nblk;
   }
   BZH_REQUIRE(p.L == 0 || p.tab != nullptr, "CPML tiles need the coefficient table");
   BZH_REQUIRE(p.fin != nullptr && p.fout != nullptr && (p.L == 0 || p.pbase != nullptr) &&
@@ -1378,7 +1633,6 @@ void launch_pair3d(hipStream_t s, const PairParams& pp, int dtype) {
   BZH_REQUIRE(pp.mbox_n >= 0 && pp.mbox_n <= kMaxMonBoxes &&
                   (pp.mbox_n == 0 || (pp.mbox != nullptr && pp.cmid != nullptr)),
               "pair monitor boxes are malformed");
-  const int64_t nblocks = int64_t{pp.nblk_int} + pp.nblk_edge;
   BZH_REQUIRE(nblocks < (int64_t{1} << 31), "pair grid too large");
   if (nblocks <= 0) return;
   switch (dtype) {

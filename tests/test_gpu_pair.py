@@ -151,3 +151,57 @@ def test_pair_parity_with_h_sources(grid, bc, chunk):
     assert native.last_program_pairs() == (steps % chunk or chunk) // 2 // 2 * 2
     o.run(prob, ost, steps)
     _compare_state(prog, state, prob, ost, dft=True)
+
+
+def _core_sources(grid, steps):
+    """Slabs crossing the core tiles (x-interior, rows and planes clear of
+    every face and slab): a pre-E Ez plane, an E-timed Ey block and an
+    H-timed Hz block, all between the steps of every pair."""
+    dt = _dt(grid)
+    nz, ny, nx = grid
+    rng = np.random.default_rng(21)
+    t = o.sample_times(0.0, dt, steps)
+    wave = o.gaussian_pulse(t, 2.0e14, 0.5e14, offset=1.0).astype(np.float32).astype(np.float64)
+    c1 = (rng.random((nz - 16, ny - 20, 1)) * 1e-1).astype(np.float32).astype(np.float64)
+    c2 = (rng.random((6, 9, 30)) * 1e-1).astype(np.float32).astype(np.float64)
+    c3 = (rng.random((5, 7, 20)) * 1e-4).astype(np.float32).astype(np.float64)
+    return [o.Source("Ez", "pre_e", (8, 10, 75), c1, wave),
+            o.Source("Ey", "e", (nz // 2, ny // 2, 64), c2, wave),
+            o.Source("Hz", "h", (nz // 2 - 3, ny // 2 + 2, 70), c3, wave)]
+
+
+@pytest.mark.parametrize("bc", ["cpml", "pec"])
+def test_pair_core_parity(bc, monkeypatch):
+    """Grids large enough for the core launch (x-interior tiles whose rows
+    and chunks lie clear of every face and CPML slab, run by the lean core
+    kernel with LDS-DMA prefetch): sources of every timing and an x-normal
+    monitor inside the core, against the oracle and against the same run with
+    the core folded back into the shell launch (BZH_PAIR_CORE=0)."""
+    _require_gpu()
+    import beamz_amd as bz
+    from beamz_amd import native
+    grid = (40, 48, 140) if bc == "cpml" else (30, 44, 130)
+    steps = 12
+    kw = {"cpml": dict(cpml_cells=3), "pec": {}}[bc]
+    freqs = np.array([1.8e14, 2.0e14, 2.2e14])
+    runs = []
+    for core in ("1", "0"):
+        monkeypatch.setenv("BZH_PAIR_CORE", core)
+        mons = [bz.FluxMonitor("x", 90, freqs), bz.FluxMonitor("z", grid[0] // 2, freqs[:2])]
+        sim, prog, state, prob, ost = _setup(grid, seed=73, steps=steps,
+                                             sources=_core_sources(grid, steps), monitors=mons,
+                                             **kw)
+        print(f"case pair_core grid={grid} bc={bc} core={core}", flush=True)
+        s = torch.cuda.Stream()
+        with torch.cuda.stream(s):
+            state = sim.advance(state, steps, chunk=steps)
+        torch.cuda.synchronize()
+        assert native.last_program_fused() and native.last_program_pairs() == 6
+        runs.append([t.cpu().numpy().astype(np.float64) for t in
+                     (*state.fields(), *state.psi_h, *state.psi_e, state.dft_re, state.dft_im)])
+        if core == "1":
+            o.run(prob, ost, steps)
+            _compare_state(prog, state, prob, ost, dft=True)
+    for i, (a, b) in enumerate(zip(*runs)):
+        n = np.linalg.norm(b)
+        assert n == 0.0 or np.linalg.norm(a - b) / n <= 1e-6, (i, np.linalg.norm(a - b) / n)
