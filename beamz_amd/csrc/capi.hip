@@ -811,11 +811,17 @@ FusedSplit fused_split(const Grid& g, int L) {
 //
 // The x-interior tiles whose 16 rows all lie clear of the y faces and y CPML
 // slabs, over the z range whose chunks' iterations all lie clear of the z
-// faces and z slabs, form the core: a launch of its own (the core kernel,
-// every iteration lean).  The rest, the shell, is one launch of the general
-// kernel over up to three boxes: the x-edge tiles, the x-interior tiles of
-// the outer y tiles, and the x-interior core-row tiles on the outer z range.
-// BZH_PAIR_CORE=0 keeps everything in the shell launch.
+// faces and z slabs, form the core, run by the core variant of the kernel:
+// lean iterations only (per-row byte offsets, no face, slab or ghost work,
+// next plane prefetched by LDS DMA).  The rest, the shell, runs the general
+// kernel: the x-edge tiles, the x-interior tiles of the outer y tiles, and
+// the x-interior core-row tiles on the outer z planes.  (A core covering
+// every plane, with the general iteration compiled in for the z-slab
+// planes, measured 3.06 against 3.02 ms per pair at 1024x512x256: the
+// general iteration's registers slow the lean one.)
+// By default the core box joins the shell launch (its blocks last), so that
+// the two overlap instead of each leaving a tail; BZH_PAIR_CORE=split gives
+// the core its own launch and BZH_PAIR_CORE=0 runs it in the general kernel.
 struct PairSplit {
   PairBox shell[kMaxPairBoxes];
   int nshell = 0;
@@ -900,8 +906,12 @@ PairSplit pair_split(const Grid& g, int L) {
   PairBox rim = box(tx0, tx1 - tx0, tx1 - tx0, 0, cy0, cy1 - cy0, cy1 - cy0, 0, 0, cz0, cz1, planes, 0);
   if (finish_box(rim, kz, false)) sp.shell[sp.nshell++] = rim;
   // the core, in chunks of equal length
-  sp.core = box(tx0, tx1 - tx0, tx1 - tx0, 0, cy0, cy1 - cy0, cy1 - cy0, 0, cz0, cz1, cz1, cz1, 0);
+  sp.core = box(tx0, tx1 - tx0, tx1 - tx0, 0, cy0, cy1 - cy0, cy1 - cy0, 0, cz0, cz1, cz1, cz1, 2);
   sp.has_core = finish_box(sp.core, kz, true);
+  if (sp.has_core && !(env && std::string(env) == "split")) {
+    sp.shell[sp.nshell++] = sp.core;
+    sp.has_core = false;
+  }
   return sp;
 }
 

@@ -229,9 +229,10 @@ struct PairBox {
   int32_t z0, zsplit, zhi, z1, kz, nzc_lo, nzc;
   int32_t band;
   int32_t nblk;   // workgroups of this box, a multiple of 8
-  int32_t xedge;  // 1: tiles that touch an x face or an x CPML slab
+  int32_t xedge;  // 0: x-interior tiles, 1: tiles that touch an x face or an
+                  // x CPML slab, 2: core tiles (every iteration lean)
 };
-constexpr int kMaxPairBoxes = 3;
+constexpr int kMaxPairBoxes = 4;
 
 // Two leapfrog steps in one launch (bank A -> bank B).  ``f`` carries the
 // single-step layout and work box (tiles of 60 x 13 owned cells); the first

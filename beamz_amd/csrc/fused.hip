@@ -797,13 +797,15 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
     }
   }
 
+  // core tiles: every row lies in [max(L, 1), ny - L - 1] (checked on the
+  // host), so the y terms, faces and validity cuts are compiled out
   const int y = j0 - 2 + w;
-  const bool vyX = y >= 0 && y <= ny, vyY = y >= 0 && y < ny;
-  const int yX = min(max(y, 0), ny), yY = min(max(y, 0), ny - 1);
-  const int qyH = vyY ? slab_q(y, ny, L) : -1;
-  const int qyE = vyX ? slab_q(y, ny + 1, L) : -1;
-  const bool y_pec_h = (mb && y == 0) || (mt && y == ny);
-  const bool row_plain = y >= 1 && y < ny;  // no y face, ghost or validity cut on this row
+  const bool vyX = kCore || (y >= 0 && y <= ny), vyY = kCore || (y >= 0 && y < ny);
+  const int yX = kCore ? y : min(max(y, 0), ny), yY = kCore ? y : min(max(y, 0), ny - 1);
+  const int qyH = (!kCore && vyY) ? slab_q(y, ny, L) : -1;
+  const int qyE = (!kCore && vyX) ? slab_q(y, ny + 1, L) : -1;
+  const bool y_pec_h = !kCore && ((mb && y == 0) || (mt && y == ny));
+  const bool row_plain = kCore || (y >= 1 && y < ny);  // no y face, ghost or validity cut on this row
   const int yt = j0 - 2 + kW;
   const bool vtX = yt <= ny;
   const int ytX = min(yt, ny);
@@ -1218,7 +1220,7 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
       // coefficients); the rows of this one were all read before the barrier
       if constexpr (kPf) {
         const int kn = k + 1;
-        pf_ok = kn <= k_hi && kn <= kz1;
+        pf_ok = kn <= k_hi && kn <= kz1;  // (lean iterations are contiguous)
         if (pf_ok) {
           dma_row(f.fin, f.foff[0] + zA + 2 * pA, oA, pfw[0]);
           dma_row(f.fin, f.foff[1] + zB + 2 * pB, oB, pfw[1]);
@@ -1451,9 +1453,12 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
 template <typename T, typename P, typename C, bool kEArr, bool kHSrc>
 __global__ void __launch_bounds__(64 * kPairWaves) pair_step(PairParams p) {
   __shared__ PairLds lds;
+  __shared__ PairPrefetch pf;
   int b = blockIdx.x, i = 0;
   while (i + 1 < p.nbox && b >= p.box[i].nblk) b -= p.box[i++].nblk;
-  if (p.box[i].xedge) pair_body<T, P, C, kEArr, true, kHSrc, false>(p, lds, nullptr, i, b);
+  const int kind = p.box[i].xedge;  // 0 x-interior, 1 x-edge, 2 core
+  if (kind == 1) pair_body<T, P, C, kEArr, true, kHSrc, false>(p, lds, nullptr, i, b);
+  else if (kind == 2) pair_body<T, P, C, kEArr, false, kHSrc, true>(p, lds, &pf, i, b);
   else pair_body<T, P, C, kEArr, false, kHSrc, false>(p, lds, nullptr, i, b);
 }
 
@@ -1592,17 +1597,19 @@ void launch_pair3d(hipStream_t s, const PairParams& pp, int dtype) {
     BZH_REQUIRE(b.nblk % 8 == 0 && b.nblk >= need && b.nblk < need + 8,
                 "pair block count does not match its box");
     BZH_REQUIRE(tiles == 0 || b.nblk > 0, "pair box without blocks");
-    if (!b.xedge && tiles > 0) {
+    if (b.xedge != 1 && tiles > 0) {
       // the interior variant reads lanes x = i0-2 .. i0+61 unchecked and
       // assumes no x face or x CPML slab under them
       const int lo = b.tx0 * kPairCols - 2;
       const int hi = (b.tx0 + b.ntx + b.tskip - 1) * kPairCols + kPairCols + 1;
       BZH_REQUIRE(lo >= M && hi <= p.nx - M - 1, "interior pair tiles overlap an x face or x CPML slab");
     }
-    if (pp.core && tiles > 0) {
+    BZH_REQUIRE(b.xedge >= 0 && b.xedge <= 2 && (!pp.core || b.xedge == 2),
+                "pair box kind is malformed");
+    if (b.xedge == 2 && tiles > 0) {
       // core tiles: rows j0-2 .. j0+13 in [M, ny-L-1] and iterations
       // kz0-2 .. kz1+1 touching planes in [M, nz-M-1] (lean everywhere)
-      BZH_REQUIRE(!b.xedge && b.tyskip == 0 && b.nzc == b.nzc_lo, "core box is malformed");
+      BZH_REQUIRE(b.tyskip == 0 && b.nzc == b.nzc_lo, "core box is malformed");
       const int y_lo = b.ty0 * kPairRows - 2, y_hi = (b.ty0 + b.nty - 1) * kPairRows + kPairRows;
       BZH_REQUIRE(y_lo >= M && y_hi <= p.ny - p.L - 1, "core tiles reach a y face or y CPML slab");
       BZH_REQUIRE(b.z0 - 4 >= M && b.zsplit + 2 <= p.nz - M - 1,
