@@ -922,13 +922,12 @@ bool bf16_stochastic_rounding() {
   return !(env && std::string(env) == "nearest");
 }
 
-// Two steps per launch: by default for float32 fields (BZH_PAIR=1 forces it
-// for 2-byte fields too, BZH_PAIR=0 disables it).
-bool pairs_enabled(int dtype) {
+// Two steps per launch for every storage type (BZH_PAIR=0 disables it).
+// With the core kernel, 2-byte fields run faster in pairs than in single
+// steps (f16 1.52 against 1.83 ms per step at 1024x512x256).
+bool pairs_enabled(int /*dtype*/) {
   const char* env = std::getenv("BZH_PAIR");
-  if (env && std::string(env) == "0") return false;
-  if (env && std::string(env) == "1") return true;
-  return dtype == BZH_F32;
+  return !(env && std::string(env) == "0");
 }
 
 // Intermediate bank and monitor boxes of the pair step, one per (device,

@@ -201,6 +201,22 @@ __device__ __forceinline__ float ldo(const char* base, uint32_t soff, uint32_t v
   return ldu<T>(base, soff, 0u, voff);
 }
 
+// The raw bits of one element at ``base + soff + voff`` (zero-extended), for
+// a load whose value is needed only an iteration later: the conversion to
+// float32 (``from_raw``) happens at the use, so the load does not wait here.
+template <typename T>
+__device__ __forceinline__ uint32_t ldraw(const char* base, uint32_t soff, uint32_t voff) {
+  gptr<const char> p = at<T>(base, soff, 0u, voff);
+  if constexpr (sizeof(T) == 4) return *reinterpret_cast<gptr<const uint32_t>>(p);
+  else return *reinterpret_cast<gptr<const uint16_t>>(p);
+}
+template <typename T>
+__device__ __forceinline__ float from_raw(uint32_t u) {
+  if constexpr (sizeof(T) == 4) return __uint_as_float(u);
+  else if constexpr (std::is_same<T, bf16_t>::value) return __uint_as_float(u << 16);
+  else return __half2float(__ushort_as_half(static_cast<unsigned short>(u)));
+}
+
 // One dword per lane from ``base + off`` (global) into an LDS row (lane l at
 // row[l]), asynchronously: the wave goes on, and the compiler waits for the
 // transfer before the row is read.
@@ -1140,6 +1156,10 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
   constexpr bool kPf = kCore && sizeof(T) == 4 && sizeof(C) == 4;
   float (*pfw)[64] = kPf ? (*pf)[w] : nullptr;
   bool pf_ok = false;  // pfw holds this iteration's inputs
+  // 2-byte fields: the same prefetch into registers (raw bits, converted at
+  // the use), since these kernels have registers to spare
+  constexpr bool kRp = kCore && !kPf;
+  uint32_t rp[11] = {};
 
   for (int k = ks; k <= kz1 + 1; ++k) {
     // core launches: every row and iteration is lean by construction; the
@@ -1159,7 +1179,25 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
       float s0 = es0, s1 = es1, s2 = es2;
       if (g1) {
         float etx = 0.0f, etz = 0.0f, ha0, ha1 = 0.0f, ha2;
-        if (kPf && pf_ok) {
+        if (kRp && pf_ok) {
+          exn = from_raw<T>(rp[0]);
+          eyn = from_raw<T>(rp[1]);
+          ezn = from_raw<T>(rp[2]);
+          ha0 = from_raw<T>(rp[3]);
+          if (!halo) ha1 = from_raw<T>(rp[4]);
+          ha2 = from_raw<T>(rp[5]);
+          if constexpr (kEArr) {
+            if (gE1) {
+              s0 = from_raw<C>(rp[6]);
+              s1 = from_raw<C>(rp[7]);
+              s2 = from_raw<C>(rp[8]);
+            }
+          }
+          if (halo) {
+            etx = from_raw<T>(rp[9]);
+            etz = from_raw<T>(rp[10]);
+          }
+        } else if (kPf && pf_ok) {
           exn = pfw[0][lane];
           eyn = pfw[1][lane];
           ezn = pfw[2][lane];
@@ -1218,6 +1256,29 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
       // the next iteration's inputs, when it is lean too (its loads would
       // otherwise wait at its top): planes k + 2 (E) and k + 1 (H, E
       // coefficients); the rows of this one were all read before the barrier
+      if constexpr (kRp) {
+        const int kn = k + 1;
+        pf_ok = kn <= k_hi && kn <= kz1;
+        if (pf_ok) {
+          rp[0] = ldraw<T>(f.fin, f.foff[0] + zA + 2 * pA, oA);
+          rp[1] = ldraw<T>(f.fin, f.foff[1] + zB + 2 * pB, oB);
+          rp[2] = ldraw<T>(f.fin, f.foff[2] + zC + 2 * pC, oC);
+          rp[3] = ldraw<T>(f.fin, f.foff[3] + zB + pB, oB);
+          if (!halo) rp[4] = ldraw<T>(f.fin, f.foff[4] + zA + pA, oA);
+          rp[5] = ldraw<T>(f.fin, f.foff[5] + zD + pD, oD);
+          if constexpr (kEArr) {
+            if (w >= 1) {
+              rp[6] = ldraw<C>(f.cbase, f.coff[0] + kn * nA * szC, cA);
+              rp[7] = ldraw<C>(f.cbase, f.coff[1] + kn * nB * szC, cB);
+              rp[8] = ldraw<C>(f.cbase, f.coff[2] + kn * nC * szC, cC);
+            }
+          }
+          if (halo) {
+            rp[9] = ldraw<T>(f.fin, f.foff[0] + zA + 2 * pA, oAt);
+            rp[10] = ldraw<T>(f.fin, f.foff[2] + zC + 2 * pC, oCt);
+          }
+        }
+      }
       if constexpr (kPf) {
         const int kn = k + 1;
         pf_ok = kn <= k_hi && kn <= kz1;  // (lean iterations are contiguous)
