@@ -233,8 +233,8 @@ def step_kernel_name(prog, pairs=False):
     """The step kernels as rocprofv3 names them; ``pairs`` when the last
     program call ran two steps per launch."""
     if uses_two_banks(prog):
-        # a pair runs the shell launch (general kernel) and the core launch
-        return "pair_step+pair_core" if pairs else "fused_interior+fused_edge"
+        # one pair launch runs the general and the core bodies
+        return "pair_step" if pairs else "fused_interior+fused_edge"
     return "h_phase_3d+e_phase_3d" if prog.is_3d else "h_phase_tm+e_phase_tm"
 
 
