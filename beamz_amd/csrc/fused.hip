@@ -850,14 +850,23 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
   }
   mbits = own ? mbits : 0u;
   const bool mwave = __builtin_amdgcn_readfirstlane(__any(mbits != 0u)) != 0;
-  auto sample_plane = [&](int k) {
-    uint32_t zb = 0;
-    for (int b = 0; b < pair_kargs()->mbox_n; ++b) {
+  // the planes this wave samples: the hull of the z ranges of the boxes that
+  // hold any of its lanes (one uniform test per plane; lanes in a box whose z
+  // range misses a plane of the hull write cells no gather reads, which is
+  // harmless: the intermediate bank has the field bank's layout)
+  const bool mlane = mbits != 0u;
+  int zs0 = 1 << 30, zs1 = -1;
+  if (mwave) {
+    for (int b = 0; b < pp.mbox_n; ++b) {
+      if (__builtin_amdgcn_readfirstlane(__any((mbits >> b) & 1u)) == 0) continue;
       cint* bx6 = (cint*)(pair_kargs()->mbox) + 6 * b;
-      if (k >= bx6[0] && k <= bx6[1]) zb |= 1u << b;
+      zs0 = min(zs0, bx6[0]);
+      zs1 = max(zs1, bx6[1]);
     }
-    return (mbits & zb) != 0u;
-  };
+  }
+  zs0 = max(zs0, kz0);
+  zs1 = min(zs1, kz1 - 1);
+  auto sample_plane = [&](int k) { return k >= zs0 && k <= zs1; };
 
   auto load_e = [&](int k, float& ex, float& ey, float& ez) {
     const bool vxy = k <= nz, vz = k < nz;
@@ -1170,7 +1179,7 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
       const bool gE1 = g1 && k >= kE && w >= 1;
       const bool gH2 = k - 1 >= kE && k - 1 < kz1 && w >= 1 && w <= kW - 2;
       const bool gE2 = k - 2 >= kz0 && k - 2 < kz1 && w >= 2 && w <= kW - 2;
-      const bool sampled = mwave && k >= kz0 && k < kz1 && sample_plane(k);
+      const bool sampled = mwave && sample_plane(k);
       // plane k in bytes, per family; plane k + d is z + d * p
       const uint32_t pA = nA * szT, pB = nB * szT, pC = nC * szT, pD = nD * szT;
       const uint32_t zA = k * pA, zB = k * pB, zC = k * pC, zD = k * pD;
@@ -1249,7 +1258,12 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
           lE1[bb ^ 1][kW][0][lane] = (vtX && vxY) ? etz : 0.0f;
           lE1[bb ^ 1][kW][1][lane] = (vtX && vxX) ? etx : 0.0f;
         }
-        if (sampled) store_h(pair_kargs()->cmid, k, hx, hy, hz);
+        if (sampled && mlane) {
+          const char* cm = pair_kargs()->cmid;
+          stf<T>(cm, f.foff[3] + zB, 0u, oB, hx, false, rng);
+          stf<T>(cm, f.foff[4] + zA, 0u, oA, hy, false, rng);
+          stf<T>(cm, f.foff[5] + zD, 0u, oD, hz, false, rng);
+        }
       }
       __syncthreads();
 
@@ -1313,7 +1327,12 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
         ney = ec[1] * ey + (s1 * idx) * ((hx - h1x) - (hz - hz_l));
         nez = ec[2] * ez + (s2 * idx) * ((hy - hy_l) - (hx - hx_dn));
         if (swave) mid_sources(0, k, nex, ney, nez);
-        if (sampled) store_e(pair_kargs()->cmid, k, nex, ney, nez);
+        if (sampled && mlane) {
+          const char* cm = pair_kargs()->cmid;
+          stf<T>(cm, f.foff[0] + zA, 0u, oA, nex, false, rng);
+          stf<T>(cm, f.foff[1] + zB, 0u, oB, ney, false, rng);
+          stf<T>(cm, f.foff[2] + zC, 0u, oC, nez, false, rng);
+        }
         if (swave) mid_sources(1, k, nex, ney, nez);
         lE2[bb][w][0][lane] = nez;
         lE2[bb][w][1][lane] = nex;
@@ -1385,7 +1404,7 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
     const bool gE1 = g1 && k >= kE && w >= 1;
     const bool gH2 = k - 1 >= kE && k - 1 < kz1 && w >= 1 && w <= kW - 2;
     const bool gE2 = k - 2 >= kz0 && k - 2 < kz1 && w >= 2 && w <= kW - 2;
-    const bool sampled = mwave && k >= kz0 && k < kz1 && sample_plane(k);
+    const bool sampled = mwave && sample_plane(k) && mlane;
 
     // ---- first step: H'(k) ----
     float exn = 0.0f, eyn = 0.0f, ezn = 0.0f;
