@@ -9,6 +9,7 @@ from . import _abi
 from .api import Simulation, SimulationResults, courant_dt, resolve_backend
 from .boundaries import PEC, PML
 from .constants import EPS_0, ETA_0, LIGHT_SPEED, MU_0, nm, um
+from .grid import RectilinearGrid
 from .modes import ModeSource, ModeSpec
 from .monitors import FieldMonitor, FluxMonitor
 from .native import HipBackendUnavailable, HipLaunchError
@@ -17,6 +18,7 @@ from .sources import CustomSource, GaussianPulse, GaussianSource, SoftSource
 from .termination import AutoTermination, RunTermination
 
 __all__ = [
+    "RectilinearGrid",
     "Simulation", "SimulationResults", "SimulationState", "compile_program", "initial_state",
     "courant_dt", "resolve_backend", "PEC", "PML", "FieldMonitor", "FluxMonitor", "CustomSource",
     "GaussianPulse", "GaussianSource", "SoftSource", "ModeSource", "ModeSpec", "AutoTermination",

@@ -169,7 +169,10 @@ class Simulation:
     permittivity : array (z, y, x) or (y, x)
         Relative permittivity per material cell.
     resolution : float
-        Cell size in metres.
+        Cell size in metres (with ``grid``: defaults to its smallest width).
+    grid : RectilinearGrid, optional
+        Non-uniform cell widths (physical x, y, z edges); the curls then scale
+        every difference by its inverse physical distance.
     dt : float, optional
         Time step; defaults to ``courant / sqrt(ndim)`` of the Courant limit.
     boundaries, sources, monitors : sequences of ``PEC``/``PML``,
@@ -189,11 +192,11 @@ class Simulation:
         the largest per-step source increment.
     """
 
-    def __init__(self, permittivity, resolution, *, dt=None, courant=0.99, boundaries=(),
+    def __init__(self, permittivity, resolution=None, *, dt=None, courant=0.99, boundaries=(),
                  sources=(), monitors=(), steps=None, t0=None, conductivity=None,
                  backend="auto", field_dtype=torch.float32, device=None,
                  flags=abi.DEFAULT_FLAGS | abi.FLAG_FUSED_STEP, field_scale=None, time=None,
-                 psi_dtype=None):
+                 psi_dtype=None, grid=None):
         if time is not None:
             values = np.array(time, dtype=float)
             if values.ndim != 1 or values.size < 2:
@@ -214,6 +217,18 @@ class Simulation:
         steps = 1 if steps is None else steps
         t0 = 0.0 if t0 is None else t0
         self.permittivity = np.asarray(permittivity, dtype=np.float64)
+        self.grid = grid
+        if grid is not None:
+            # a realized grid: its smallest width is the resolution and its
+            # CFL limit the default step (simulation/api.py:210-217,
+            # design/grid.py:358-371)
+            axes = ("x", "y", "z")[:self.permittivity.ndim]
+            if resolution is None:
+                resolution = grid.minimum_spacing(axes)
+            if dt is None:
+                dt = grid.cfl_time_step(courant, axes=axes)
+        if resolution is None:
+            raise ValueError("Simulation needs a resolution or a grid")
         self.resolution = float(resolution)
         self.dt = float(dt) if dt is not None else courant_dt(resolution, self.permittivity.ndim, courant)
         self.boundaries = tuple(boundaries)
@@ -238,7 +253,7 @@ class Simulation:
                 sources=self.sources, monitors=self.monitors, num_steps=self.steps,
                 t0=self.t0, conductivity=self.conductivity, device=self.device,
                 field_dtype=self.field_dtype, flags=self.flags, field_scale=self.field_scale,
-                psi_dtype=self.psi_dtype)
+                psi_dtype=self.psi_dtype, grid=self.grid)
         return self._program
 
     def initial_state(self, fields=None):

@@ -106,15 +106,32 @@ def plane_gather_plan(component, shape, dx, normal, plane_index, ranges):
     return plane_gather_plan_at(component, shape, dx, normal, plane_index * dx, u, v)
 
 
-def plane_gather_plan_at(component, shape, dx, normal, position, u, v):
+def component_coords_3d(component, shape, dx=None, cell_grid=None):
+    """Physical coordinates of a component's samples per axis name: nodes on
+    its zero-offset axes, cell centres on the others (uniform spacing ``dx``,
+    or the edges and centres of a ``RectilinearGrid``)."""
+    pos = {"z": 0, "y": 1, "x": 2}
+    out = {}
+    for a, i in pos.items():
+        off = OFFSETS_3D[component][i]
+        if cell_grid is None:
+            out[a] = (np.arange(shape[i]) + off) * dx
+        else:
+            out[a] = (cell_grid.axis_edges(a) if off == 0.0 else cell_grid.centers(a))[:shape[i]]
+    return out
+
+
+def plane_gather_plan_at(component, shape, dx, normal, position, u, v, coords=None):
     """The plane plan at normal coordinate ``position`` (metres) with samples
     on the tangential coordinate grid ``u x v`` (``lattice.py:252-354``): the
     stored plane order is ``(z, y)`` for x, ``(z, x)`` for y, ``(y, x)`` for z,
-    and targets outside the component's coordinates take its edge value."""
+    and targets outside the component's coordinates take its edge value.
+    ``coords`` gives the component's sample coordinates per axis (default:
+    the uniform lattice of spacing ``dx``)."""
     tangential = {"x": ("z", "y"), "y": ("z", "x"), "z": ("y", "x")}[normal]
     pos = {"z": 0, "y": 1, "x": 2}
-    coords = {a: (np.arange(shape[pos[a]]) + OFFSETS_3D[component][pos[a]]) * dx
-              for a in pos}
+    if coords is None:
+        coords = component_coords_3d(component, shape, dx)
     gu, gv = np.meshgrid(np.asarray(u, dtype=np.float64), np.asarray(v, dtype=np.float64),
                          indexing="ij")
     plans = (
@@ -143,17 +160,26 @@ def plane_gather_plan_at(component, shape, dx, normal, position, u, v):
     return flat.astype(np.int64), weight
 
 
-def line_gather_plan(component, shape, dx, normal, line_index, cell_range):
+def line_gather_plan(component, shape, dx, normal, line_index, cell_range, cell_grid=None):
     """Bilinear gather onto the tangential cell centres of an axis-aligned 2D
-    line at ``line_index * dx`` (``monitors/compiler.py:126-197``)."""
+    line at the E-node ``line_index`` (``monitors/compiler.py:126-197``), on
+    the uniform lattice of spacing ``dx`` or on a ``RectilinearGrid``."""
     oy, ox = OFFSETS_TM[component]
-    ys = (np.arange(shape[0]) + oy) * dx
-    xs = (np.arange(shape[1]) + ox) * dx
-    t = (np.arange(*cell_range) + 0.5) * dx
-    if normal == "x":
-        px, py = np.full(t.shape, line_index * dx), t
+    if cell_grid is None:
+        ys = (np.arange(shape[0]) + oy) * dx
+        xs = (np.arange(shape[1]) + ox) * dx
+        t = (np.arange(*cell_range) + 0.5) * dx
+        at = line_index * dx
     else:
-        px, py = t, np.full(t.shape, line_index * dx)
+        ys = (cell_grid.axis_edges("y") if oy == 0.0 else cell_grid.centers("y"))[:shape[0]]
+        xs = (cell_grid.axis_edges("x") if ox == 0.0 else cell_grid.centers("x"))[:shape[1]]
+        tang = "y" if normal == "x" else "x"
+        t = cell_grid.centers(tang)[cell_range[0]:cell_range[1]]
+        at = cell_grid.axis_edges(normal)[line_index]
+    if normal == "x":
+        px, py = np.full(t.shape, at), t
+    else:
+        px, py = t, np.full(t.shape, at)
     x0, x1, wx0, wx1 = interpolation_plan(xs, px)
     y0, y1, wy0, wy1 = interpolation_plan(ys, py)
     bits = np.array([(0, 0), (0, 1), (1, 0), (1, 1)])

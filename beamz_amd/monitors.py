@@ -246,6 +246,7 @@ class CompiledMonitor:
     placed: bool = False       # lowered from a physical centre and size
     position: float = None     # the normal coordinate of the samples (metres)
     samples: tuple = ()        # placed 3D plane: the two tangential sample coordinate arrays
+    point_measure: np.ndarray = None  # per-sample area (width in 2D) on a non-uniform grid
 
 
 def _frequencies(mon):
@@ -442,4 +443,6 @@ def flux(m: CompiledMonitor, re, im, wsum, dt):
         s = f("Ez") * np.conj(f("Hx")) - f("Ex") * np.conj(f("Hz"))
     else:
         s = f("Ex") * np.conj(f("Hy")) - f("Ey") * np.conj(f("Hx"))
+    if m.point_measure is not None:
+        return 0.5 * np.real(np.sum(m.normal_sign * s * m.point_measure[None, :], axis=1)) * m.measure
     return 0.5 * np.real(np.sum(m.normal_sign * s, axis=1)) * m.measure

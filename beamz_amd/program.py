@@ -83,6 +83,13 @@ class CompiledProgram:
     field_scale: dict = None       # storage scale of the E and H fields
     eps_cells: np.ndarray = field(repr=False, default=None)
     eps_site: dict = field(repr=False, default=None)
+    # derivative metrics of a non-uniform grid (compile.py:209-240): kind
+    # code 0 isotropic, 1 axis-uniform, 2 rectilinear; per phase the (z, y, x)
+    # inverse distances (empty tensors on isotropic grids and absent axes)
+    metric_kind: int = 0
+    h_metrics: tuple = ()
+    e_metrics: tuple = ()
+    cell_grid: object = None       # the RectilinearGrid, or None for a uniform grid
 
     @property
     def cells(self) -> int:
@@ -162,12 +169,18 @@ def psi_precision_from_env():
 def compile_program(permittivity, resolution, dt, *, boundaries=(), sources=(), monitors=(),
                     num_steps=1, t0=0.0, conductivity=None, device=None,
                     field_dtype=torch.float32, flags=abi.DEFAULT_FLAGS, field_scale=None,
-                    psi_dtype=None):
+                    psi_dtype=None, grid=None):
     """Lower ``(permittivity raster, dx, dt, boundaries, sources, monitors)``.
 
     ``psi_dtype`` stores the CPML memories as float32 (default) or bfloat16
     (the reference's BF16 psi option, ``backend.py:45-56``; the recurrence
-    itself runs in float32); ``None`` reads ``BEAMZ_HIP_CPML_PSI_PRECISION``."""
+    itself runs in float32); ``None`` reads ``BEAMZ_HIP_CPML_PSI_PRECISION``.
+
+    ``grid`` (a ``RectilinearGrid``) gives non-uniform cell widths: the curls
+    then scale each difference by its inverse physical distance (the
+    derivative metrics), ``resolution`` is the grid's smallest width, CPML
+    profiles are graded in cell units and monitors weigh each sample by its
+    cell's area (or, on 2D lines, width)."""
     device = torch.device(device or ("cuda" if torch.cuda.is_available() else "cpu"))
     psi_dtype = psi_precision_from_env() if psi_dtype is None else psi_dtype
     if psi_dtype not in (torch.float32, torch.bfloat16):
@@ -178,8 +191,29 @@ def compile_program(permittivity, resolution, dt, *, boundaries=(), sources=(), 
     if eps.ndim not in (2, 3):
         raise ValueError("permittivity must be a 2D (y, x) or 3D (z, y, x) raster")
     is_3d = eps.ndim == 3
+    cell_grid = grid
     grid = tuple(int(v) for v in eps.shape)
     shapes = component_shapes(grid)
+    metric_kind, h_metrics, e_metrics = 0, (), ()
+    if cell_grid is not None:
+        metric_kind, h_metrics, e_metrics = _lower_metrics(cell_grid, grid, is_3d, device)
+        if metric_kind and field_dtype in HALF_DTYPES:
+            raise NotImplementedError("half-precision fields run on uniform grids only")
+        if metric_kind:
+            for s in sources:
+                if not isinstance(s, CustomSource):
+                    raise NotImplementedError(
+                        f"{type(s).__name__} is placed on uniform grids only; use a CustomSource "
+                        "on a non-uniform grid")
+            for m in monitors:
+                if m.placed:
+                    raise NotImplementedError(
+                        "monitors placed by centre and size need a uniform grid; give a "
+                        "non-uniform grid's monitors by index")
+            for b in boundaries:
+                if isinstance(b, PML) and b.thickness is not None:
+                    raise NotImplementedError(
+                        "a PML on a non-uniform grid is given in cells (PML(cells=...))")
     sigma = np.zeros(grid) if conductivity is None else np.broadcast_to(
         np.asarray(conductivity, dtype=np.float64), grid).copy()
     pmls = [b for b in boundaries if isinstance(b, PML)]
@@ -306,6 +340,8 @@ def compile_program(permittivity, resolution, dt, *, boundaries=(), sources=(), 
             coeffs[f"e_source_{ax}"] = view
 
     compiled = [lower_monitor(m, grid, shapes, resolution, i) for i, m in enumerate(monitors)]
+    if metric_kind:
+        compiled = [_with_cell_areas(cm, cell_grid, is_3d) for cm in compiled]
     assign_offsets(compiled)
     packed = ()
     if compiled:
@@ -325,7 +361,77 @@ def compile_program(permittivity, resolution, dt, *, boundaries=(), sources=(), 
         h_term_arrays=h_arrays, e_term_arrays=e_arrays, source_groups=groups,
         coincident_mask=coincident, monitors=compiled, packed_monitors=packed,
         device=device, field_dtype=field_dtype, psi_dtype=psi_dtype, flags=int(flags), eps_cells=eps,
-        eps_site=eps_site)
+        eps_site=eps_site, metric_kind=metric_kind, h_metrics=h_metrics, e_metrics=e_metrics,
+        cell_grid=cell_grid if metric_kind else None)
+
+
+def _lower_metrics(cell_grid, grid, is_3d, device):
+    """The derivative metrics of ``cell_grid`` as device tensors in the
+    kernels' (z, y, x) order per phase: H reads the forward (E -> H) inverse
+    distances, E the backward (H -> E) ones (``cuda/runtime.py:89-94``)."""
+    from .grid import METRIC_KIND_CODE, derivative_metrics
+    axes = ("x", "y", "z") if is_3d else ("x", "y")
+    want = tuple(grid) if is_3d else (1,) + tuple(grid)
+    have = cell_grid.shape_zyx if is_3d else (1,) + cell_grid.shape_zyx[1:]
+    if tuple(have) != want:
+        raise ValueError(f"grid cells {cell_grid.shape_zyx} (z, y, x) do not match the "
+                         f"permittivity raster {tuple(grid)}")
+    kind, fwd, bwd = derivative_metrics(cell_grid, axes)
+    code = METRIC_KIND_CODE[kind]
+    if code == 0:
+        return 0, (), ()
+    empty = torch.zeros(0, dtype=torch.float32, device=device)
+
+    def per_phase(table):
+        return tuple(_dev(np.asarray(table[a], np.float32), device) if a in table else empty
+                     for a in ("z", "y", "x"))
+    return code, per_phase(fwd), per_phase(bwd)
+
+
+def _with_cell_areas(cm, cell_grid, is_3d):
+    """An index-space monitor on a non-uniform grid: the gather plans sample
+    each component at the physical cell centres of the plane (line) at its
+    E-node coordinate, and each sample's measure is the area (3D planes) or
+    width (2D lines) of the cell it samples, in place of dx^2 or dx."""
+    import dataclasses
+    from .lattice import component_coords_3d, line_gather_plan, plane_gather_plan_at
+    plans = {}
+    if is_3d:
+        tangential = {"x": ("z", "y"), "y": ("z", "x"), "z": ("y", "x")}[cm.normal]
+        ranges = dict(cm.ranges)
+        u = cell_grid.centers(tangential[0])[slice(*ranges[tangential[0]])]
+        v = cell_grid.centers(tangential[1])[slice(*ranges[tangential[1]])]
+        at = cell_grid.axis_edges(cm.normal)[cm.index]
+        for c, plan in cm.plans.items():
+            if plan is None:
+                plans[c] = None
+                continue
+            shape = _component_shape(c, cell_grid)
+            plans[c] = plane_gather_plan_at(c, shape, None, cm.normal, at, u, v,
+                                            coords=component_coords_3d(c, shape, None, cell_grid))
+        area = (cell_grid.cell_widths(tangential[0])[slice(*ranges[tangential[0]])][:, None] *
+                cell_grid.cell_widths(tangential[1])[slice(*ranges[tangential[1]])][None, :]).reshape(-1)
+    else:
+        tang = "y" if cm.normal == "x" else "x"
+        for c, plan in cm.plans.items():
+            if plan is None:
+                plans[c] = None
+                continue
+            shape = _component_shape(c, cell_grid, two_d=True)
+            plans[c] = line_gather_plan(c, shape, None, cm.normal, cm.index, cm.cell_range,
+                                        cell_grid=cell_grid)
+        area = cell_grid.cell_widths(tang)[cm.cell_range[0]:cm.cell_range[1]]
+    if area.size != cm.points:
+        raise ValueError("monitor samples do not match its cell range")
+    return dataclasses.replace(cm, plans=plans, measure=1.0,
+                               point_measure=np.asarray(area, np.float64))
+
+
+def _component_shape(component, cell_grid, two_d=False):
+    if two_d:
+        _, ny, nx = cell_grid.shape_zyx
+        return component_shapes((ny, nx))[component]
+    return component_shapes(cell_grid.shape_zyx)[component]
 
 
 BANK_ALIGN = 256  # bytes between the starts of the fields of one bank

@@ -27,8 +27,12 @@ def _empty(device):
     return torch.zeros(0, dtype=torch.float32, device=device)
 
 
-def _metrics(prog):
-    # isotropic uniform grids: no metric arrays (metric_kind 0)
+def _metrics(prog, phase):
+    """The (z, y, x) derivative metrics of one phase (``cuda/runtime.py:89-94``):
+    none on isotropic uniform grids (metric_kind 0)."""
+    table = prog.h_metrics if phase == abi.PHASE_H else prog.e_metrics
+    if table:
+        return tuple(table)
     e = _empty(prog.device)
     return (e, e, e)
 
@@ -60,9 +64,10 @@ def _graph_io(prog: CompiledProgram, state: SimulationState):
     (``runtime.py:284-381``)."""
     if prog.cpml_enabled:
         return (*state.fields(), *_cpml_payload(prog, state, abi.PHASE_H),
-                *_cpml_payload(prog, state, abi.PHASE_E), *_metrics(prog), *_metrics(prog))
+                *_cpml_payload(prog, state, abi.PHASE_E), *_metrics(prog, abi.PHASE_H),
+                *_metrics(prog, abi.PHASE_E))
     return (*state.fields(), *_materials(prog, abi.PHASE_H), *_materials(prog, abi.PHASE_E),
-            *_metrics(prog), *_metrics(prog))
+            *_metrics(prog, abi.PHASE_H), *_metrics(prog, abi.PHASE_E))
 
 
 def _source_group_arguments(prog):
@@ -80,8 +85,8 @@ def _source_group_arguments(prog):
 def _program_attrs(prog, nsteps, layout, monitor_count=0):
     return native.ProgramAttrs(
         abi_version=abi.ABI_VERSION, flags=prog.flags, nsteps=int(nsteps), dt=prog.dt,
-        resolution=prog.abi_resolution, boundary_code=prog.boundary_code, metric_kind=0,
-        program_layout=layout, cpml_enabled=int(prog.cpml_enabled),
+        resolution=prog.abi_resolution, boundary_code=prog.boundary_code,
+        metric_kind=prog.metric_kind, program_layout=layout, cpml_enabled=int(prog.cpml_enabled),
         monitor_count=int(monitor_count), coincident_source_group_mask=prog.coincident_mask)
 
 
@@ -97,10 +102,11 @@ def _phase(prog, state, phase):
         psi = state.psi_h if phase == abi.PHASE_H else state.psi_e
         n = prog.nterms
         args += [meta, *arrays[:3 * n], *psi[:n]]
-    args += list(_metrics(prog))
+    args += list(_metrics(prog, phase))
     attrs = native.PhaseAttrs(
         abi_version=abi.ABI_VERSION, flags=prog.flags, phase=phase, nterms=prog.nterms,
-        dt=prog.dt, resolution=prog.abi_resolution, boundary_code=prog.boundary_code, metric_kind=0)
+        dt=prog.dt, resolution=prog.abi_resolution, boundary_code=prog.boundary_code,
+        metric_kind=prog.metric_kind)
     native.call_streamed(args, attrs)
     return state
 
@@ -294,7 +300,7 @@ def _phase_launch(prog, phase, targets, sources, psi, out_targets, out_psi):
     ln.flags = prog.flags
     ln.phase = phase
     ln.nterms = prog.nterms
-    ln.metric_kind = 0
+    ln.metric_kind = prog.metric_kind
     ln.dt = prog.dt
     ln.resolution = prog.abi_resolution
     ln.inv_resolution = 1.0 / prog.abi_resolution
@@ -306,7 +312,7 @@ def _phase_launch(prog, phase, targets, sources, psi, out_targets, out_psi):
         arrays = prog.h_term_arrays if phase == abi.PHASE_H else prog.e_term_arrays
         ins += [meta, *arrays[:18], *_psi_slots(psi, prog.device)]
     _fill(ln.inputs, ins + [None] * (37 - len(ins)))
-    _fill(ln.metrics, _metrics(prog))
+    _fill(ln.metrics, _metrics(prog, phase))
     outs = [*out_targets]
     if prog.cpml_enabled:
         outs += list(_psi_slots(out_psi, prog.device))

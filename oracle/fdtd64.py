@@ -447,6 +447,10 @@ class Problem:
     sources: list = field(default_factory=list)
     monitors: list = field(default_factory=list)
     t0: float = 0.0
+    # derivative metrics of a non-uniform grid (compile.py:209-240): per
+    # phase ("h": E -> H forward, "e": H -> E backward) one entry per array
+    # axis, None (1 / dx), a scalar or a vector along that axis
+    metrics: dict = None
 
     @property
     def cells(self):
@@ -454,7 +458,7 @@ class Problem:
 
 
 def build_problem(eps_cells, dx, dt, *, cpml=None, metallic=None, sigma_cells=None,
-                  sources=(), monitors=(), t0=0.0):
+                  sources=(), monitors=(), t0=0.0, metrics=None):
     """Lower a permittivity raster into a ``Problem`` (3D or 2D TMxy).
 
     ``metallic`` defaults to the reference's boundary resolution
@@ -496,7 +500,38 @@ def build_problem(eps_cells, dx, dt, *, cpml=None, metallic=None, sigma_cells=No
     masks = pec_masks(shapes, metallic, is_3d)
     return Problem(grid, dx, dt, shapes, is_3d, metallic, e_decay, e_source,
                    h_decay, h_source, hterms, eterms, masks, list(sources),
-                   list(monitors), t0)
+                   list(monitors), t0, metrics)
+
+
+def grid_metrics(widths, is_3d):
+    """The derivative metrics of per-axis cell widths, ``compile.py:209-240``:
+    forward 1 / w_i, backward 1 / w_0, 2 / (w_{i-1} + w_i), 1 / w_{n-1};
+    ``widths`` maps array axis -> widths (a scalar for one uniform width)."""
+    nax = 3 if is_3d else 2
+    h, e = [None] * nax, [None] * nax
+    for ax in range(nax):
+        w = widths.get(ax)
+        if w is None:
+            continue
+        w = np.asarray(w, dtype=np.float64)
+        if w.ndim == 0:
+            h[ax] = e[ax] = 1.0 / float(w)
+            continue
+        back = np.empty(w.size + 1)
+        back[0], back[-1] = 1.0 / w[0], 1.0 / w[-1]
+        back[1:-1] = 2.0 / (w[:-1] + w[1:])
+        h[ax], e[ax] = 1.0 / w, back
+    return {"h": h, "e": e}
+
+
+def _scale_axis(d, prob, phase, ax):
+    """``kernels.py:164-176``: a difference times its inverse distance (a
+    scalar, or a vector broadcast along ``ax``); ``1 / dx`` without metrics."""
+    m = None if prob.metrics is None else prob.metrics[phase][ax]
+    if m is None:
+        return d / prob.dx
+    m = np.asarray(m, dtype=np.float64)
+    return d * (m if m.ndim == 0 else _bcast(m, ax, d.ndim))
 
 
 def init_state(prob):
@@ -551,13 +586,15 @@ def correct_term(d, psi, term):
 
 def ghost_diff(h, axis, metallic, axis_edges, dx):
     """Backward difference of ghost-padded H, ``lattice.py:523-575``: the ghost is
-    zero on a metallic face and a copy of the edge sample otherwise."""
+    zero on a metallic face and a copy of the edge sample otherwise (divided
+    by ``dx``; ``dx=None`` returns the bare difference)."""
     low_e, high_e = axis_edges[axis]
     first = np.take(h, [0], axis=axis)
     last = np.take(h, [h.shape[axis] - 1], axis=axis)
     lo = np.zeros_like(first) if low_e in metallic else first
     hi = np.zeros_like(last) if high_e in metallic else last
-    return np.diff(np.concatenate([lo, h, hi], axis=axis), axis=axis) / dx
+    d = np.diff(np.concatenate([lo, h, hi], axis=axis), axis=axis)
+    return d if dx is None else d / dx
 
 
 def apply_sources(prob, st, timing):
@@ -596,14 +633,15 @@ def curl_h_to_e(prob, st):
 
 
 def h_phase(prob, st):
-    """``kernels.py:458-503`` (3D) / ``kernels.py:770-786, 1217-1246`` (TMxy)."""
+    """``kernels.py:458-503`` (3D) / ``kernels.py:770-786, 1217-1246`` (TMxy);
+    with metrics ``kernels.py:506-551`` (3D) / ``880-886, 916-927`` (TMxy)."""
     dx = prob.dx
     terms = prob.hterms or [None] * (6 if prob.is_3d else 2)
     table = H_TERMS_3D if prob.is_3d else [(t, s, a) for t, s, a, _ in H_TERMS_TM]
     signs = ([1.0, -1.0] * 3) if prob.is_3d else [1.0, -1.0]
     corrected = []
     for i, (tgt, src, ax) in enumerate(table):
-        d = np.diff(st[src.lower()], axis=ax) / dx
+        d = _scale_axis(np.diff(st[src.lower()], axis=ax), prob, "h", ax)
         if terms[i] is None:
             c, _ = signs[i] * d, None
         else:
@@ -617,7 +655,8 @@ def h_phase(prob, st):
 
 
 def e_phase(prob, st):
-    """``kernels.py:554-626`` (3D) / ``kernels.py:789-811, 1249-1290`` (TMxy)."""
+    """``kernels.py:554-626`` (3D) / ``kernels.py:789-811, 1249-1290`` (TMxy);
+    with metrics ``kernels.py:629-700`` (3D) / ``888-914, 928-949`` (TMxy)."""
     dx = prob.dx
     edges = AXIS_EDGES_3D if prob.is_3d else AXIS_EDGES_2D
     terms = prob.eterms or [None] * (6 if prob.is_3d else 2)
@@ -625,7 +664,7 @@ def e_phase(prob, st):
     signs = ([1.0, -1.0] * 3) if prob.is_3d else [1.0, -1.0]
     corrected = []
     for i, (tgt, src, ax) in enumerate(table):
-        d = ghost_diff(st[src.lower()], ax, prob.metallic, edges, dx)
+        d = _scale_axis(ghost_diff(st[src.lower()], ax, prob.metallic, edges, None), prob, "e", ax)
         if terms[i] is None:
             c = signs[i] * d
         else:
