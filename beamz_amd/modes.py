@@ -55,7 +55,12 @@ class ModeSource:
 
     ``center`` and ``size`` are (x, y, z) in metres; the zero extent among x
     and y selects the injection axis, ``direction`` is ``"+"`` or ``"-"``
-    along it, ``source_time`` provides ``sample(times)`` and ``freq0``."""
+    along it, ``source_time`` provides ``sample(times)`` and ``freq0``.
+
+    ``launch`` selects the 3D injection: ``"auto"`` and ``"tfsf"`` lower the
+    discrete total-field / scattered-field residual of the reference's 3D
+    launch (``mode_launch.py:602-770``), ``"huygens"`` the two-plane J/M
+    current pair.  2D sources always lower the line pair."""
 
     center: tuple
     size: tuple
@@ -63,12 +68,15 @@ class ModeSource:
     direction: str = "+"
     mode_spec: ModeSpec = field(default_factory=ModeSpec)
     power: float = 1.0
+    launch: str = "auto"
 
     def __post_init__(self):
         if len(self.center) != 3 or len(self.size) != 3:
             raise ValueError("ModeSource center and size must contain three values.")
         if self.direction not in ("+", "-"):
             raise ValueError("ModeSource direction must be '+' or '-'.")
+        if self.launch not in ("auto", "tfsf", "huygens"):
+            raise ValueError("ModeSource launch must be 'auto', 'tfsf' or 'huygens'.")
         p = float(self.power)
         if not np.isfinite(p) or p < 0.0:
             raise ValueError("ModeSource power must be non-negative and finite.")

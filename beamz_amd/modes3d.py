@@ -68,13 +68,20 @@ class PlaneModes:
 
     ``eb[m]`` has shape (Nb, Nc + 1) (E_b at (j + 1/2, k)), ``ec[m]`` (Nb + 1,
     Nc) (E_c at (j, k + 1/2)); ``hc[m]`` and ``hb[m]`` sit with ``eb`` and
-    ``ec``.  Each mode carries unit power towards +a."""
+    ``ec``.  Each mode carries unit power towards +a.  The longitudinal
+    fields lie in quadrature with the transverse ones: E_a = i ``ea[m]`` at
+    the (Nb + 1, Nc + 1) nodes (E_a = -i G / beta) and H_a = i ``ha[m]`` at the
+    (Nb, Nc) cell centres (H_a = i C / (omega mu0)).  ``beta`` holds the
+    propagation constants."""
 
     neff: np.ndarray
     eb: np.ndarray
     ec: np.ndarray
     hb: np.ndarray
     hc: np.ndarray
+    ea: np.ndarray = None
+    ha: np.ndarray = None
+    beta: np.ndarray = None
 
 
 def _operators(Nb, Nc, h):
@@ -154,7 +161,9 @@ def solve_plane_modes(eps_b, eps_c, eps_a, omega, h, count, target_neff):
     n = A.shape[0]
     count = max(1, min(int(count), n - 2))
     sigma = (k0 * float(target_neff)) ** 2
-    w, v = eigs(A, k=count, sigma=sigma, which="LM")
+    # a fixed start vector makes the solve (and every lowering) reproducible
+    v0 = np.random.default_rng(0).standard_normal(n)
+    w, v = eigs(A, k=count, sigma=sigma, which="LM", v0=v0)
     beta2 = np.real(w)
     order = np.argsort(beta2)[::-1]
     beta2, v = beta2[order], v[:, order]
@@ -165,7 +174,7 @@ def solve_plane_modes(eps_b, eps_c, eps_a, omega, h, count, target_neff):
     keep = beta2 > 0.0
     beta2, v = beta2[keep], v[:, keep]
     neff = np.sqrt(beta2) / k0
-    modes_eb, modes_ec, modes_hb, modes_hc = [], [], [], []
+    modes_eb, modes_ec, modes_hb, modes_hc, modes_ea, modes_ha = [], [], [], [], [], []
     for m in range(beta2.size):
         beta = np.sqrt(beta2[m])
         eb = np.zeros(Nb * (Nc + 1))
@@ -173,8 +182,11 @@ def solve_plane_modes(eps_b, eps_c, eps_a, omega, h, count, target_neff):
         eb[ub] = v[:ub.size, m]
         ec[uc] = v[ub.size:, m]
         G = Gb @ eb + Gc @ ec
+        C = Cb @ eb + Cc @ ec
         hc = (beta * eb - (op["dbN"] @ G) / beta) / (omega * MU_0)
         hb = -(beta * ec - (op["dcN"] @ G) / beta) / (omega * MU_0)
+        ea = -G / beta                      # E_a = i ea
+        ha = C / (omega * MU_0)             # H_a = i ha
         p = 0.5 * float(np.sum(eb * hc) - np.sum(ec * hb)) * h * h
         s = 1.0 / np.sqrt(abs(p)) if abs(p) > 1e-300 else 1.0
         if p < 0:
@@ -183,8 +195,11 @@ def solve_plane_modes(eps_b, eps_c, eps_a, omega, h, count, target_neff):
         modes_ec.append((ec * s).reshape(Nb + 1, Nc))
         modes_hb.append((hb * s).reshape(Nb + 1, Nc))
         modes_hc.append((hc * s).reshape(Nb, Nc + 1))
+        modes_ea.append((ea * s).reshape(Nb + 1, Nc + 1))
+        modes_ha.append((ha * s).reshape(Nb, Nc))
     return PlaneModes(neff, np.array(modes_eb), np.array(modes_ec), np.array(modes_hb),
-                      np.array(modes_hc))
+                      np.array(modes_hc), np.array(modes_ea), np.array(modes_ha),
+                      np.sqrt(beta2))
 
 
 def _plane_index(axis, a_idx, b_sl, c_sl):
@@ -213,9 +228,26 @@ class ModePlan3d:
     entries: tuple          # (component, timing, start (z, y, x), profile in storage order)
 
 
-def plan_mode_source_3d(src: ModeSource, eps_site, grid, resolution):
-    """Plan the Huygens pair of a 3D mode source.  ``eps_site`` maps Ex, Ey,
-    Ez to their Yee-site permittivities; ``grid`` is (nz, ny, nx)."""
+@dataclass(frozen=True)
+class PlaneSetup:
+    """The snapped plane of a 3D mode source: propagation axis, E plane and
+    companion (H) plane a-indices, the window's cells and the permittivities
+    at its Yee sites, in (b, c) order."""
+
+    axis: str
+    plane: int
+    companion: int
+    window: tuple           # ((b0, b1), (c0, c1)) cells
+    eps_b: np.ndarray
+    eps_c: np.ndarray
+    eps_a: np.ndarray
+
+
+def plane_setup(src: ModeSource, eps_site, grid, resolution) -> PlaneSetup:
+    """Snap a 3D mode source onto the lattice (``_placement.py:305-330``): the
+    E plane at the nearest cell centre along a, the companion one cell behind
+    it for "+" and one ahead for "-", the window over the cells the source's
+    transverse extent covers (at least 2)."""
     axis = src.axis3d
     a, b, c = _cyclic(axis)
     dx = float(resolution)
@@ -229,7 +261,6 @@ def plan_mode_source_3d(src: ModeSource, eps_site, grid, resolution):
         cmp_idx = min(max(counts[a] - 2, 0), ca + 1)
     b0, b1 = snap_centered_extent(center[b], size[b], dx, counts[b], min_cells=2)
     c0, c1 = snap_centered_extent(center[c], size[c], dx, counts[c], min_cells=2)
-    Nb, Nc = b1 - b0, c1 - c0
     comp_b, comp_c, comp_a = "E" + b, "E" + c, "E" + a
     # permittivities at the window's sites on the E plane: E_b sites
     # (b0 + j + 1/2, c0 + k) k = 0..Nc, E_c sites (b0 + j, c0 + k + 1/2)
@@ -240,18 +271,41 @@ def plan_mode_source_3d(src: ModeSource, eps_site, grid, resolution):
     lo, hi = max(ca - 1, 0), min(ca, ea.shape[_AXIS_POS[a]] - 1)
     eps_a = 0.5 * (_to_bc(axis, ea[_plane_index(axis, lo, slice(b0, b1 + 1), slice(c0, c1 + 1))]) +
                    _to_bc(axis, ea[_plane_index(axis, hi, slice(b0, b1 + 1), slice(c0, c1 + 1))]))
+    return PlaneSetup(axis, ca, cmp_idx, ((b0, b1), (c0, c1)), eps_b, eps_c, eps_a)
+
+
+def solve_source_mode(src: ModeSource, setup: PlaneSetup, omega, resolution):
+    """Solve the setup's window at angular frequency ``omega`` and return
+    ``(modes, k)``: the plane modes and the index of the one ``src.mode_spec``
+    selects (``mode_index`` by decreasing n_eff, near ``target_neff``, by
+    default 0.98 of the largest index on the plane)."""
     spec = src.mode_spec
     if spec.polarization not in (None,):
         raise ValueError("3D mode sources select modes by index; polarization filters are not built")
-    omega = 2.0 * np.pi * src.frequency
     target = spec.target_neff if spec.target_neff is not None else 0.98 * np.sqrt(
-        max(float(eps_b.max()), float(eps_c.max())))
+        max(float(setup.eps_b.max()), float(setup.eps_c.max())))
     m = max(int(spec.num_modes), int(spec.mode_index) + 1, 3)
-    modes = solve_plane_modes(eps_b, eps_c, eps_a, omega, dx, 2 * m + 5, target)
+    modes = solve_plane_modes(setup.eps_b, setup.eps_c, setup.eps_a, omega, float(resolution),
+                              2 * m + 5, target)
     if modes.neff.size == 0:
         raise ValueError("the plane mode solver found no guided mode")
-    k = min(int(spec.mode_index), modes.neff.size - 1)
+    return modes, min(int(spec.mode_index), modes.neff.size - 1)
+
+
+def plan_mode_source_3d(src: ModeSource, eps_site, grid, resolution):
+    """Plan the Huygens pair of a 3D mode source.  ``eps_site`` maps Ex, Ey,
+    Ez to their Yee-site permittivities; ``grid`` is (nz, ny, nx)."""
+    setup = plane_setup(src, eps_site, grid, resolution)
+    axis = setup.axis
+    a, b, c = _cyclic(axis)
+    dx = float(resolution)
+    ca, cmp_idx = setup.plane, setup.companion
+    (b0, b1), (c0, c1) = setup.window
+    Nb, Nc = b1 - b0, c1 - c0
+    omega = 2.0 * np.pi * src.frequency
+    modes, k = solve_source_mode(src, setup, omega, dx)
     eb, ec, hb, hc = modes.eb[k], modes.ec[k], modes.hb[k], modes.hc[k]
+    comp_b, comp_c = "E" + b, "E" + c
     d = 1.0 if src.direction == "+" else -1.0
     # the mode travelling along d a: E unchanged, H times d
     hb, hc = d * hb, d * hc

@@ -228,6 +228,7 @@ def compile_program(permittivity, resolution, dt, *, boundaries=(), sources=(), 
     # update coefficients (kernels.py:704-728), evaluated in float64
     coeffs = {}
     eps_site = {}
+    sigma_site = {}
     e_source_raw = {}
     e_comps = ("Ex", "Ey", "Ez") if is_3d else ("Ez",)
     for ax in "xyz":
@@ -237,6 +238,7 @@ def compile_program(permittivity, resolution, dt, *, boundaries=(), sources=(), 
             ss = sample_e_site_3d(sigma, comp) if is_3d else sample_site_tm(sigma, comp)
             beta = ss * dt / (2.0 * EPS_0 * es)
             eps_site[comp] = es
+            sigma_site[comp] = ss
             coeffs[f"e_decay_{ax}"] = _elide((1.0 - beta) / (1.0 + beta), device)
             e_source_raw[ax] = (dt / (EPS_0 * es)) / (1.0 + beta)
         else:
@@ -287,10 +289,16 @@ def compile_program(permittivity, resolution, dt, *, boundaries=(), sources=(), 
         elif isinstance(s, CustomSource):
             customs.append(s)
         elif isinstance(s, ModeSource):
-            if is_3d:
+            if is_3d and s.launch == "huygens":
                 from .modes3d import lower_mode_source_3d
                 customs.extend(lower_mode_source_3d(s, eps_site, tuple(eps.shape), resolution, dt,
                                                     t0, num_steps))
+            elif is_3d:
+                from .tfsf3d import lower_tfsf_mode_source_3d
+                customs.extend(lower_tfsf_mode_source_3d(s, eps_site, tuple(eps.shape), resolution,
+                                                         dt, t0, num_steps, sigma_site, metallic))
+            elif s.launch == "tfsf":
+                raise NotImplementedError("2D mode sources lower the line pair (launch='auto')")
             else:
                 customs.extend(lower_mode_source_2d(s, eps_site["Ez"], resolution, dt, t0, num_steps))
         else:

@@ -47,6 +47,22 @@ class GaussianPulse:
         return sig, quad
 
 
+def sample_waveform_pair(signal, t0, dt, total_steps, half_step=False):
+    """In-phase and quadrature samples on ``t0 + (i [+ 1/2]) dt``
+    (``time.py:84-135``): a waveform object gives both; a callable has a zero
+    quadrature; a sampled array takes the imaginary part of its analytic
+    signal."""
+    times = float(t0) + (np.arange(int(total_steps), dtype=np.float64) + (0.5 if half_step else 0.0)) * dt
+    if hasattr(signal, "sample"):
+        sig, quad = signal.sample(times)
+        return np.asarray(sig, dtype=np.float64), np.asarray(quad, dtype=np.float64)
+    sig = sample_waveform(signal, t0, dt, total_steps, half_step=half_step)
+    if callable(signal):
+        return sig, np.zeros_like(sig)
+    from scipy.signal import hilbert
+    return sig, np.imag(hilbert(sig)) if sig.size else sig.copy()
+
+
 def sample_waveform(signal, t0, dt, total_steps, half_step=True):
     """Sample a waveform on ``t0 + (i + 1/2) dt`` (``time.py:67-81``)."""
     times = float(t0) + (np.arange(int(total_steps), dtype=np.float64) + (0.5 if half_step else 0.0)) * dt
