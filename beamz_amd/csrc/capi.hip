@@ -28,6 +28,7 @@ thread_local std::string g_error;
 
 thread_local int32_t g_last_fused = 0;
 thread_local int32_t g_last_pairs = 0;
+thread_local int32_t g_last_capture = 0;
 void set_error(const std::string& message) { g_error = message; }
 void clear_error() { g_error.clear(); }
 
@@ -672,6 +673,9 @@ DftParams make_dft(const MonitorIn& in, const bzh_buffer* const* bank, const Gri
     p.field[c] = active ? f.data : nullptr;
     p.fsize[c] = active ? static_cast<int32_t>(numel(f)) : 0;
     p.inv_scale[c] = 1.0f;
+    for (int d = 0; d < 3; ++d)
+      p.fdims[c][d] = !active ? 0 : g.is3d ? static_cast<int32_t>(f.dims[d])
+                                        : (d == 0 ? 1 : static_cast<int32_t>(f.dims[d - 1]));
   }
   return p;
 }
@@ -694,6 +698,10 @@ struct ProgramIn {
   int metallic, uniform_l;
   int flags;
   bool noalloc;                  // bzh_program_noalloc: scratch from the device array, no graphs
+  // optional trailing argument of the monitor layouts (no counterpart in the
+  // reference): host s32[M, 6, 6] boxes of the cells each monitor component
+  // gathers, which lets pair launches capture them densely
+  const bzh_buffer* mon_boxes;
 };
 
 // Work split of the fused step -- the counterpart of the reference's
@@ -937,14 +945,23 @@ struct PairWorkspace {
   char* bank = nullptr;
   size_t bytes = 0;
   int32_t* boxes = nullptr;
+  float* cap = nullptr;            // dense capture of the monitored cells
+  size_t cap_bytes = 0;
+  int32_t* capbox = nullptr;       // its box table, int32[kMaxMonBoxes][6][8]
+  std::vector<int32_t> capbox_host;  // what capbox holds
 };
-PairWorkspace pair_workspace(hipStream_t s, size_t bank_bytes) {
+// This stream's workspace (one per device and stream, kept until exit).
+PairWorkspace& pair_space(hipStream_t s) {
   static std::mutex mu;
   static std::map<std::pair<int, hipStream_t>, PairWorkspace> spaces;
   int device = 0;
   BZH_HIP(hipGetDevice(&device));
   std::lock_guard<std::mutex> lock(mu);
-  PairWorkspace& w = spaces[{device, s}];
+  return spaces[{device, s}];
+}
+
+PairWorkspace pair_workspace(hipStream_t s, size_t bank_bytes) {
+  PairWorkspace& w = pair_space(s);
   if (w.boxes == nullptr) {
     void* d = nullptr;
     BZH_HIP(hipMalloc(&d, kMaxMonBoxes * 6 * sizeof(int32_t)));
@@ -963,6 +980,87 @@ PairWorkspace pair_workspace(hipStream_t s, size_t bank_bytes) {
     w.bytes = bank_bytes;
   }
   return w;
+}
+
+// The capture of the monitored cells for pair launches (PairParams::cap),
+// planned from the host boxes of the monitor layouts' trailing argument.
+// ``ok`` is false (and the intermediate bank is used) without the boxes, or
+// when a tile row would hold cells of two monitors: monitors whose y ranges
+// overlap must lie in disjoint 60-column tile columns.
+struct CapturePlan {
+  bool ok = false;
+  std::vector<int32_t> table;  // [M][6][8]
+  size_t slot = 0;             // floats per step
+};
+CapturePlan capture_plan(const ProgramIn& in, const DftParams& probe) {
+  CapturePlan cp;
+  if (in.mon_boxes == nullptr || in.noalloc || probe.M < 1 || probe.M > kMaxMonBoxes) return cp;
+  const int M = probe.M;
+  const int32_t* hb = static_cast<const int32_t*>(in.mon_boxes->data);
+  cp.table.assign(static_cast<size_t>(M) * 48, 0);
+  std::vector<int> uy0(M, INT_MAX), uy1(M, INT_MIN), ux0(M, INT_MAX), ux1(M, INT_MIN);
+  size_t off = 0;
+  for (int m = 0; m < M; ++m) {
+    for (int c = 0; c < 6; ++c) {
+      const int32_t* b = hb + (m * 6 + c) * 6;
+      int32_t* t = &cp.table[static_cast<size_t>(m * 6 + c) * 8];
+      t[0] = 1;  // empty: z0 > z1
+      t[1] = 0;
+      if (b[0] < 0 || probe.field[c] == nullptr) continue;
+      for (int d = 0; d < 3; ++d)
+        BZH_REQUIRE(b[2 * d] >= 0 && b[2 * d] <= b[2 * d + 1] && b[2 * d + 1] < probe.fdims[c][d],
+                    "monitor box " + std::to_string(m) + " lies outside component " +
+                        std::to_string(c));
+      for (int q = 0; q < 6; ++q) t[q] = b[q];
+      t[6] = static_cast<int32_t>(off);
+      off += static_cast<size_t>(b[1] - b[0] + 1) * (b[3] - b[2] + 1) * (b[5] - b[4] + 1);
+      BZH_REQUIRE(off < (size_t{1} << 31), "the monitors' capture exceeds 2^31 cells");
+      uy0[m] = std::min(uy0[m], b[2]);
+      uy1[m] = std::max(uy1[m], b[3]);
+      ux0[m] = std::min(ux0[m], b[4]);
+      ux1[m] = std::max(ux1[m], b[5]);
+    }
+  }
+  for (int a = 0; a < M; ++a)
+    for (int b = a + 1; b < M; ++b) {
+      if (uy0[a] > uy1[a] || uy0[b] > uy1[b]) continue;
+      const bool yo = uy0[a] <= uy1[b] && uy0[b] <= uy1[a];
+      const bool xo = ux0[a] / kPairCols <= ux1[b] / kPairCols && ux0[b] / kPairCols <= ux1[a] / kPairCols;
+      if (yo && xo) return cp;
+    }
+  cp.slot = off;
+  cp.ok = off > 0;
+  return cp;
+}
+
+// The capture buffer and box table of a plan, on this stream's
+// workspace; the table is uploaded when it changes.
+void capture_workspace(hipStream_t s, PairWorkspace& w, const CapturePlan& cp) {
+  const size_t bytes = cp.slot * sizeof(float);
+  if (w.cap_bytes < bytes) {
+    if (w.cap != nullptr) {
+      BZH_HIP(hipStreamSynchronize(s));
+      BZH_HIP(hipFree(w.cap));
+      w.cap = nullptr;
+      w.cap_bytes = 0;
+    }
+    void* d = nullptr;
+    BZH_HIP(hipMalloc(&d, bytes));
+    w.cap = static_cast<float*>(d);
+    w.cap_bytes = bytes;
+  }
+  if (w.capbox == nullptr) {
+    void* d = nullptr;
+    BZH_HIP(hipMalloc(&d, kMaxMonBoxes * 48 * sizeof(int32_t)));
+    w.capbox = static_cast<int32_t*>(d);
+  }
+  if (w.capbox_host != cp.table) {
+    // (rare: a new program) the launches of earlier calls may still read it
+    BZH_HIP(hipStreamSynchronize(s));
+    BZH_HIP(hipMemcpy(w.capbox, cp.table.data(), cp.table.size() * sizeof(int32_t),
+                      hipMemcpyHostToDevice));
+    w.capbox_host = cp.table;
+  }
 }
 
 // Slab count of a source group (0 when absent).
@@ -1178,11 +1276,20 @@ bool run_program(hipStream_t s, ProgramIn& in) {
     npairs = in.nsteps / 2;
     if (npairs % 2 == 1) --npairs;
   }
-  // the intermediate bank (same layout as bank A) and the monitor boxes
+  // the monitors' samples of the pairs: a dense capture of the cells they
+  // gather (with the host boxes), else the intermediate bank (same layout as
+  // bank A) and the monitor boxes found on the device
   const bzh_buffer* cbank[6] = {};
   bzh_buffer cbufs[6];
   PairWorkspace pws;
-  if (npairs > 0 && mon_m > 0) {
+  CapturePlan cap;
+  if (npairs > 0 && mon_m > 0)
+    cap = capture_plan(in, make_dft(in.mon, in.bank_a, g, in.dt, in.monitor_count));
+  if (npairs > 0 && mon_m > 0 && cap.ok) {
+    PairWorkspace& w = pair_space(s);
+    capture_workspace(s, w, cap);
+    pws = w;
+  } else if (npairs > 0 && mon_m > 0) {
     size_t span = 0;
     const int order[6] = {kEx, kEy, kEz, kHx, kHy, kHz};
     for (int q = 0; q < 6; ++q) {
@@ -1287,7 +1394,19 @@ bool run_program(hipStream_t s, ProgramIn& in) {
       launches.push_back(make_launch(Kind::kDft, 0, g.dtype, dp));
     }
   };
-  if (npairs > 0 && mon_m > 0) {
+  // the DFT of the pair's first step, from the capture
+  auto add_dft_cap = [&](int k) {
+    const size_t before = launches.size();
+    add_dft(k, in.bank_a);
+    for (size_t i = before; i < launches.size(); ++i) {
+      DftParams dp;
+      std::memcpy(&dp, launches[i].bytes.data(), sizeof(dp));
+      dp.cap = pws.cap;
+      dp.capbox = pws.capbox;
+      launches[i] = make_launch(Kind::kDft, 0, g.dtype, dp);
+    }
+  };
+  if (npairs > 0 && mon_m > 0 && !cap.ok) {
     const DftParams probe = make_dft(in.mon, in.bank_a, g, in.dt, in.monitor_count);
     MonBoxParams mb;
     std::memset(&mb, 0, sizeof(mb));
@@ -1332,7 +1451,12 @@ bool run_program(hipStream_t s, ProgramIn& in) {
       pp.msrc_h1 = pp.msrc_n;
       for (int c = 0; c < 3 && 3 + c < in.ngroups; ++c)
         add_mid_slabs(pp.msrc, pp.msrc_n, kMaxMidSlabs, in.groups[3 + c], c, k + 1, true, true, in.metallic);
-      if (mon_m > 0) {
+      if (mon_m > 0 && cap.ok) {
+        pp.cap = pws.cap;
+        pp.capbox = pws.capbox;
+        pp.cap_slot = static_cast<int32_t>(cap.slot);
+        pp.cap_m = mon_m;
+      } else if (mon_m > 0) {
         pp.cmid = pws.bank;
         pp.mbox = pws.boxes;
         pp.mbox_n = mon_m;
@@ -1350,9 +1474,15 @@ bool run_program(hipStream_t s, ProgramIn& in) {
         pc.core = 1;
         launches.push_back(make_launch(Kind::kPair, 0, g.dtype, pc));
       }
-      if (in.has_monitors) add_dft(k, cbank);
+      if (in.has_monitors && cap.ok) {
+        add_dft_cap(k);
+      } else if (in.has_monitors) {
+        add_dft(k, cbank);
+      }
       for (int c = 0; c < 3 && 6 + c < in.ngroups; ++c)
         add_source_launches(launches, in.groups[6 + c], 2, c, nxt, g, in.metallic, step0, k + 1);
+      // (the second step is sampled from the fields: capturing it too costs
+      // the kernel more stores than the strided gather costs)
       if (in.has_monitors) add_dft(k + 1, nxt);
       --pairs_left;
       cur_b = two_banks ? !cur_b : cur_b;
@@ -1398,6 +1528,7 @@ bool run_program(hipStream_t s, ProgramIn& in) {
   // (no graphs for the no-allocation entry: instantiating one may allocate)
   graph_cache().run(s, launches, (in.flags & BZH_FLAG_GRAPH_CACHE) != 0 && !in.noalloc);
   g_last_pairs = npairs;
+  g_last_capture = (npairs > 0 && mon_m > 0 && cap.ok) ? 1 : 0;
   g_last_fused = fused ? 1 : 0;
   return fused;
 }
@@ -1519,6 +1650,13 @@ int decode_program(const bzh_buffer* args, int nargs, const bzh_program_attrs& a
     in.step = in.mon.b[11];
     in.has_monitors = true;
     next += 12;
+    if (nargs == next + 1) {
+      const bzh_buffer* hb = arg(args, nargs, next++);
+      BZH_REQUIRE(hb->element_type == BZH_S32 && numel(*hb) == 36 * in.monitor_count &&
+                      (hb->data != nullptr || in.monitor_count == 0),
+                  "monitor boxes must be host s32[M, 6, 6]");
+      in.mon_boxes = hb;
+    }
   }
   BZH_REQUIRE(next == nargs, "layout " + std::to_string(layout) + " expects " +
                                  std::to_string(next) + " arguments, got " + std::to_string(nargs));
@@ -1785,6 +1923,7 @@ const char* bzh_last_error(void) { return bzh::g_error.c_str(); }
 int32_t bzh_last_program_fused(void) { return bzh::g_last_fused; }
 
 int32_t bzh_last_program_pairs(void) { return bzh::g_last_pairs; }
+int32_t bzh_last_program_captured(void) { return bzh::g_last_capture; }
 
 int32_t bzh_launch_phase(void* stream, const bzh_launch* launch) {
   return guarded([&] {

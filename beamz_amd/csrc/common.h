@@ -250,6 +250,19 @@ struct PairParams {
   char* cmid;
   const int32_t* mbox;
   int32_t mbox_n;
+  // Capture of the monitored cells (when ``cap`` is set, instead of
+  // ``cmid``): the first step's fields of every cell a monitor gathers, f32
+  // (``cap_slot`` floats).  ``capbox`` is device int32[cap_m][6][8]: per
+  // monitor and DFT component (Ex .. Hz) the inclusive box z0, z1, y0, y1,
+  // x0, x1, the offset of its cells and 0 (z0 > z1 when the component is not
+  // gathered); a cell's index is off + ((x - x0) * ny_b + (y - y0)) * nz_b +
+  // (z - z0), z fastest, so that a wave's stores over the z march fill whole
+  // lines.  Each tile row holds cells of at most one monitor (checked on the
+  // host).
+  float* cap;
+  const int32_t* capbox;
+  int32_t cap_slot;
+  int32_t cap_m;
   // The launch's work: box[0 .. nbox) in order, workgroups numbered through
   // them (each box's block count is a multiple of 8, so every box deals its
   // tiles to the XCDs from XCD 0).  ``core`` launches run the core kernel: one
@@ -332,6 +345,11 @@ struct DftParams {
   float* stage;
   int32_t slots;
   int32_t k0, nk;
+  // Gather from a capture of the monitored cells instead of the fields (the
+  // pair kernel's, laid out as ``PairParams::capbox`` describes) when set.
+  const float* cap;
+  const int32_t* capbox;
+  int32_t fdims[6][3];   // component shapes (z, y, x), for the capture index
 };
 constexpr int kDftMaxSlots = 16;
 void launch_dft(hipStream_t s, const DftParams& p);

@@ -752,6 +752,9 @@ struct PairLds {
   float h1[2][kPairWaves][2][64];      // H' rows (Hz, Hx) for E'
   float e2[2][kPairWaves][2][64];      // E' rows for H''
   float h2[2][kPairWaves][2][64];      // H'' rows for E''
+  // per wave and DFT component, the monitor capture of its row: x0, x1, z0,
+  // z1 (x0 > x1: none), and the slot index base + x * stride + z
+  int cap[kPairWaves][6][6];
 };
 // The core kernel's prefetch rows: each wave's inputs of its next iteration,
 // fetched by LDS DMA while the current one computes: E(k+1) x,y,z, H(k)
@@ -843,30 +846,75 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
 
   // monitor boxes: bit b set on lanes whose (y, x) lies in box b; planes are
   // tested per k.  Only owned cells write the intermediate bank.
+  const bool capp = pp.cap != nullptr;
   uint32_t mbits = 0;
-  for (int b = 0; b < pp.mbox_n; ++b) {
-    cint* bx6 = (cint*)(pair_kargs()->mbox) + 6 * b;
-    if (y >= bx6[2] && y <= bx6[3] && x >= bx6[4] && x <= bx6[5]) mbits |= 1u << b;
-  }
-  mbits = own ? mbits : 0u;
-  const bool mwave = __builtin_amdgcn_readfirstlane(__any(mbits != 0u)) != 0;
-  // the planes this wave samples: the hull of the z ranges of the boxes that
-  // hold any of its lanes (one uniform test per plane; lanes in a box whose z
-  // range misses a plane of the hull write cells no gather reads, which is
-  // harmless: the intermediate bank has the field bank's layout)
-  const bool mlane = mbits != 0u;
   int zs0 = 1 << 30, zs1 = -1;
-  if (mwave) {
+  bool mwave = false;
+  if (capp) {
+    // the capture: lane c < 6 finds component c's box of the one monitor
+    // whose cells this row holds (the host checked there is at most one)
+    int e0 = 1, e1 = 0, e2 = 0, e3 = -1, e4 = 0, e5 = 0;
+    const int xl = bx * kPairCols, xh = xl + kPairCols - 1;
+    if (lane < 6 && w >= 2 && w <= kPairRows + 1) {
+      for (int mm = 0; mm < pp.cap_m; ++mm) {
+        const int32_t* t = pp.capbox + (mm * 6 + lane) * 8;
+        if (t[0] > t[1] || y < t[2] || y > t[3] || t[5] < xl || t[4] > xh) continue;
+        const int nzb = t[1] - t[0] + 1, stride = nzb * (t[3] - t[2] + 1);
+        e0 = t[4];
+        e1 = t[5];
+        e2 = t[0];
+        e3 = t[1];
+        e4 = t[6] - t[4] * stride + (y - t[2]) * nzb - t[0];
+        e5 = stride;
+        break;
+      }
+      int* dst = lds.cap[w][lane];
+      dst[0] = e0;
+      dst[1] = e1;
+      dst[2] = e2;
+      dst[3] = e3;
+      dst[4] = e4;
+      dst[5] = e5;
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      const int x0c = __builtin_amdgcn_readlane(e0, c), x1c = __builtin_amdgcn_readlane(e1, c);
+      if (x0c > x1c) continue;
+      mwave = true;
+      zs0 = min(zs0, __builtin_amdgcn_readlane(e2, c));
+      zs1 = max(zs1, __builtin_amdgcn_readlane(e3, c));
+      if (own && x >= x0c && x <= x1c) mbits = 1u;
+    }
+  } else {
     for (int b = 0; b < pp.mbox_n; ++b) {
-      if (__builtin_amdgcn_readfirstlane(__any((mbits >> b) & 1u)) == 0) continue;
       cint* bx6 = (cint*)(pair_kargs()->mbox) + 6 * b;
-      zs0 = min(zs0, bx6[0]);
-      zs1 = max(zs1, bx6[1]);
+      if (y >= bx6[2] && y <= bx6[3] && x >= bx6[4] && x <= bx6[5]) mbits |= 1u << b;
+    }
+    mbits = own ? mbits : 0u;
+    mwave = __builtin_amdgcn_readfirstlane(__any(mbits != 0u)) != 0;
+    // the planes this wave samples: the hull of the z ranges of the boxes
+    // that hold any of its lanes (one uniform test per plane; lanes in a box
+    // whose z range misses a plane of the hull write cells no gather reads,
+    // which is harmless: the intermediate bank has the field bank's layout)
+    if (mwave) {
+      for (int b = 0; b < pp.mbox_n; ++b) {
+        if (__builtin_amdgcn_readfirstlane(__any((mbits >> b) & 1u)) == 0) continue;
+        cint* bx6 = (cint*)(pair_kargs()->mbox) + 6 * b;
+        zs0 = min(zs0, bx6[0]);
+        zs1 = max(zs1, bx6[1]);
+      }
     }
   }
+  const bool mlane = mbits != 0u;
+  // the intermediate step is sampled on the chunk's own planes
   zs0 = max(zs0, kz0);
   zs1 = min(zs1, kz1 - 1);
   auto sample_plane = [&](int k) { return k >= zs0 && k <= zs1; };
+  // one captured value of component c (DFT order) on plane k
+  auto cap_store = [&](int c, int k, float v) {
+    const int* e = lds.cap[w][c];
+    if (x >= e[0] && x <= e[1] && k >= e[2] && k <= e[3]) pp.cap[e[4] + x * e[5] + k] = v;
+  };
 
   auto load_e = [&](int k, float& ex, float& ey, float& ez) {
     const bool vxy = k <= nz, vz = k < nz;
@@ -1259,10 +1307,16 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
           lE1[bb ^ 1][kW][1][lane] = (vtX && vxX) ? etx : 0.0f;
         }
         if (sampled && mlane) {
-          const char* cm = pair_kargs()->cmid;
-          stf<T>(cm, f.foff[3] + zB, 0u, oB, hx, false, rng);
-          stf<T>(cm, f.foff[4] + zA, 0u, oA, hy, false, rng);
-          stf<T>(cm, f.foff[5] + zD, 0u, oD, hz, false, rng);
+          if (capp) {
+            cap_store(3, k, hx);
+            cap_store(4, k, hy);
+            cap_store(5, k, hz);
+          } else {
+            const char* cm = pair_kargs()->cmid;
+            stf<T>(cm, f.foff[3] + zB, 0u, oB, hx, false, rng);
+            stf<T>(cm, f.foff[4] + zA, 0u, oA, hy, false, rng);
+            stf<T>(cm, f.foff[5] + zD, 0u, oD, hz, false, rng);
+          }
         }
       }
       __syncthreads();
@@ -1328,10 +1382,16 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
         nez = ec[2] * ez + (s2 * idx) * ((hy - hy_l) - (hx - hx_dn));
         if (swave) mid_sources(0, k, nex, ney, nez);
         if (sampled && mlane) {
-          const char* cm = pair_kargs()->cmid;
-          stf<T>(cm, f.foff[0] + zA, 0u, oA, nex, false, rng);
-          stf<T>(cm, f.foff[1] + zB, 0u, oB, ney, false, rng);
-          stf<T>(cm, f.foff[2] + zC, 0u, oC, nez, false, rng);
+          if (capp) {
+            cap_store(0, k, nex);
+            cap_store(1, k, ney);
+            cap_store(2, k, nez);
+          } else {
+            const char* cm = pair_kargs()->cmid;
+            stf<T>(cm, f.foff[0] + zA, 0u, oA, nex, false, rng);
+            stf<T>(cm, f.foff[1] + zB, 0u, oB, ney, false, rng);
+            stf<T>(cm, f.foff[2] + zC, 0u, oC, nez, false, rng);
+          }
         }
         if (swave) mid_sources(1, k, nex, ney, nez);
         lE2[bb][w][0][lane] = nez;
@@ -1442,7 +1502,13 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
         lE1[bb ^ 1][kW][0][lane] = etz;
         lE1[bb ^ 1][kW][1][lane] = etx;
       }
-      if (sampled) store_h(pair_kargs()->cmid, k, hx, hy, hz);
+      if (sampled && capp) {
+        cap_store(3, k, hx);
+        cap_store(4, k, hy);
+        cap_store(5, k, hz);
+      } else if (sampled) {
+        store_h(pair_kargs()->cmid, k, hx, hy, hz);
+      }
     }
     __syncthreads();
 
@@ -1455,7 +1521,13 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
       e_update(k, hx, hy, hz, hz_dn, hx_dn, h1x, h1y, ex, ey, ez, s0, s1, s2, pe, q2, nex, ney,
                nez);
       if (swave) mid_sources(0, k, nex, ney, nez);
-      if (sampled) store_e(pair_kargs()->cmid, k, nex, ney, nez);
+      if (sampled && capp) {
+        cap_store(0, k, nex);
+        cap_store(1, k, ney);
+        cap_store(2, k, nez);
+      } else if (sampled) {
+        store_e(pair_kargs()->cmid, k, nex, ney, nez);
+      }
       if (swave) mid_sources(1, k, nex, ney, nez);
       lE2[bb][w][0][lane] = nez;
       lE2[bb][w][1][lane] = nex;
@@ -1717,6 +1789,9 @@ void launch_pair3d(hipStream_t s, const PairParams& pp, int dtype) {
                 "pair source slab is malformed");
   }
   BZH_REQUIRE(p.hslab_n == 0, "pair launches carry their H slabs in the pair list");
+  BZH_REQUIRE(pp.cap == nullptr || (pp.cmid == nullptr && pp.mbox_n == 0 && pp.capbox != nullptr &&
+                                     pp.cap_m >= 1 && pp.cap_m <= kMaxMonBoxes && pp.cap_slot > 0),
+              "pair capture is malformed");
   BZH_REQUIRE(pp.mbox_n >= 0 && pp.mbox_n <= kMaxMonBoxes &&
                   (pp.mbox_n == 0 || (pp.mbox != nullptr && pp.cmid != nullptr)),
               "pair monitor boxes are malformed");

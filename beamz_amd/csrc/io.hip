@@ -47,12 +47,30 @@ __global__ void __launch_bounds__(256) source_kernel(SourceParams p) {
 constexpr int kDftBlock = 128;
 constexpr int kMaxFreq = 256;
 
-// Weighted gather of one monitor sample (observe.py:360-370, 529-585).
+// Weighted gather of one monitor sample (observe.py:360-370, 529-585),
+// from the field or, when ``p.cap`` is set, from the capture of the
+// monitored cells (the corner's cell mapped into its component's box).
 template <typename T>
 __device__ __forceinline__ float gather(const DftParams& p, int m, int c, int pt) {
   const T* fld = static_cast<const T*>(p.field[c]);
   const int64_t base = ((static_cast<int64_t>(m) * 6 + c) * p.Pmax + pt) * p.K;
   float v = 0.0f;
+  if (p.cap != nullptr) {
+    const int32_t* t = p.capbox + (m * 6 + c) * 8;
+    const int z0 = t[0], z1 = t[1], y0 = t[2], y1 = t[3], x0 = t[4], x1 = t[5];
+    const int sy = p.fdims[c][1], sx = p.fdims[c][2];
+    const int nzb = z1 - z0 + 1, nyb = y1 - y0 + 1;
+    for (int q = 0; q < p.K; ++q) {
+      const int idx = p.indices[base + q];
+      const float w = p.weights[base + q];
+      if (w == 0.0f || idx < 0 || idx >= p.fsize[c]) continue;
+      const int z = idx / (sy * sx), r = idx - z * (sy * sx);
+      const int y = r / sx, x = r - y * sx;
+      if (z < z0 || z > z1 || y < y0 || y > y1 || x < x0 || x > x1) continue;
+      v += w * p.cap[t[6] + ((x - x0) * nyb + (y - y0)) * nzb + (z - z0)];
+    }
+    return v;
+  }
   for (int q = 0; q < p.K; ++q) {
     const int idx = p.indices[base + q];
     const float w = p.weights[base + q];

@@ -380,6 +380,33 @@ def assign_offsets(compiled):
     return v, w
 
 
+def capture_boxes(idx, wts, masks, shapes):
+    """Inclusive (z0, z1, y0, y1, x0, x1) bounding boxes s32[M, 6, 6] of the
+    cells each (monitor, component) gather reads: corners with a non-zero
+    weight whose index lies inside the component (the kernels' gather rule),
+    in the DFT component order Ex, Ey, Ez, Hx, Hy, Hz.  A component that no
+    gather reads (mask zero or no live corner) has the row -1.  The pair
+    kernel captures exactly these cells of both its steps, so that the
+    monitors' samples are gathered from a dense buffer instead of strided
+    field rows (no counterpart in the reference, which samples every step
+    from the fields)."""
+    M = idx.shape[0]
+    out = np.full((M, 6, 6), -1, dtype=np.int32)
+    for m in range(M):
+        for c, name in enumerate(COMPONENTS):
+            shape = shapes.get(name)
+            if shape is None or len(shape) != 3 or masks[m, c] == 0.0:
+                continue
+            i = idx[m, c].reshape(-1).astype(np.int64)
+            w = wts[m, c].reshape(-1)
+            live = (w != 0.0) & (i >= 0) & (i < int(np.prod(shape)))
+            if not live.any():
+                continue
+            z, y, x = np.unravel_index(i[live], shape)
+            out[m, c] = (z.min(), z.max(), y.min(), y.max(), x.min(), x.max())
+    return out
+
+
 def pack_monitors(compiled):
     """Dense padded gather tables (``cuda/runtime.py:530-618``): indices
     s32[M, 6, Pmax, K], weights f32[M, 6, Pmax, K], freqs f64[M, Fmax],

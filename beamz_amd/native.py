@@ -160,6 +160,7 @@ SIGNATURES = {
     "bzh_last_error": (ctypes.c_char_p, []),
     "bzh_last_program_fused": (ctypes.c_int32, []),
     "bzh_last_program_pairs": (ctypes.c_int32, []),
+    "bzh_last_program_captured": (ctypes.c_int32, []),
     "bzh_launch_phase": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(Launch)]),
     "bzh_launch_program": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(ProgramLaunch)]),
     "bzh_streamed": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(Buffer), ctypes.c_int32,
@@ -263,6 +264,12 @@ def call_streamed(args, attrs: PhaseAttrs, stream=None):
 def last_program_fused() -> bool:
     """Did the last program call on this thread run the fused step kernels?"""
     return bool(library().bzh_last_program_fused())
+
+
+def last_program_captured() -> bool:
+    """Did the last program call's pair launches capture the monitored cells
+    densely (rather than through the intermediate bank)?"""
+    return bool(library().bzh_last_program_captured())
 
 
 def last_program_pairs() -> int:

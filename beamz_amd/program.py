@@ -21,7 +21,7 @@ from .boundaries import (PML, edge_mask, extrude_through_cpml, lower_cpml,
                          resolve_metallic_edges)
 from .constants import EPS_0, ETA_0, MU_0
 from .lattice import component_shapes, sample_e_site_3d, sample_site_tm
-from .monitors import assign_offsets, lower_monitor, pack_monitors
+from .monitors import assign_offsets, capture_boxes, lower_monitor, pack_monitors
 from .modes import ModeSource, lower_mode_source_2d
 from .sources import (CustomSource, GaussianSource, SoftSource, group_sources, lower_gaussian_source,
                       lower_soft_source)
@@ -90,6 +90,9 @@ class CompiledProgram:
     h_metrics: tuple = ()
     e_metrics: tuple = ()
     cell_grid: object = None       # the RectilinearGrid, or None for a uniform grid
+    # host s32[M, 6, 6] bounding boxes of the cells the monitors gather (3D),
+    # handed to the program as its optional trailing argument
+    monitor_boxes: torch.Tensor = None
 
     @property
     def cells(self) -> int:
@@ -352,6 +355,7 @@ def compile_program(permittivity, resolution, dt, *, boundaries=(), sources=(), 
         compiled = [_with_cell_areas(cm, cell_grid, is_3d) for cm in compiled]
     assign_offsets(compiled)
     packed = ()
+    boxes = None
     if compiled:
         idx, wts, fr, masks, counts, codes, windows = pack_monitors(compiled)
         masks[:, :3] *= s_e    # DFT component order Ex, Ey, Ez, Hx, Hy, Hz
@@ -359,6 +363,8 @@ def compile_program(permittivity, resolution, dt, *, boundaries=(), sources=(), 
         packed = (_dev(idx, device, torch.int32), _dev(wts, device), _dev(fr, device, torch.float64),
                   _dev(masks, device), _dev(counts, device, torch.int32),
                   _dev(codes, device, torch.int32), _dev(windows, device))
+        if is_3d:
+            boxes = torch.from_numpy(capture_boxes(idx, wts, masks, shapes))
 
     return CompiledProgram(
         grid=grid, is_3d=is_3d, resolution=float(resolution), dt=float(dt), t0=float(t0),
@@ -368,6 +374,7 @@ def compile_program(permittivity, resolution, dt, *, boundaries=(), sources=(), 
         boundary_code=boundary_code, cpml=cpml, coeffs=coeffs, h_meta=h_meta, e_meta=e_meta,
         h_term_arrays=h_arrays, e_term_arrays=e_arrays, source_groups=groups,
         coincident_mask=coincident, monitors=compiled, packed_monitors=packed,
+        monitor_boxes=boxes,
         device=device, field_dtype=field_dtype, psi_dtype=psi_dtype, flags=int(flags), eps_cells=eps,
         eps_site=eps_site, metric_kind=metric_kind, h_metrics=h_metrics, e_metrics=e_metrics,
         cell_grid=cell_grid if metric_kind else None)

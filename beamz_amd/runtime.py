@@ -194,6 +194,8 @@ def run_program_steps(state, prog, nsteps, *, noalloc=False):
         raise ValueError("the HIP program accumulates DFT monitors in float64")
     tail = (*_source_group_arguments(prog), *prog.packed_monitors,
             state.dft_re, state.dft_im, state.dft_w, state.t, state.current_step)
+    if prog.monitor_boxes is not None and not noalloc:
+        tail = (*tail, prog.monitor_boxes)      # host hint: the cells the monitors read
     if uses_two_banks(prog):
         state = with_workspace(state)
         native.call_program((*_temporal_io(prog, state), *tail),

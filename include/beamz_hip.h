@@ -196,6 +196,12 @@ int32_t bzh_last_program_fused(void);
  * pairs (so the bank parity of the single-step schedule holds) and single
  * steps for the rest.  BZH_PAIR=0 in the environment disables it. */
 int32_t bzh_last_program_pairs(void);
+/* Diagnostic with no counterpart in the reference: 1 when the last program
+ * call's pair launches captured the monitored cells densely (the monitor
+ * layouts' optional trailing argument, host s32[M, 6, 6] boxes of the cells
+ * each monitor component gathers, was given and no tile row holds cells of
+ * two monitors), 0 when they used the intermediate bank or ran no pairs. */
+int32_t bzh_last_program_captured(void);
 
 /* ---- descriptor-level launches ------------------------------------------- */
 /* cuda/src/update.cu:919-971 (BeamzLaunchStreamed). */
@@ -211,7 +217,12 @@ int32_t bzh_streamed(void* stream, const bzh_buffer* args, int32_t nargs,
                      const bzh_phase_attrs* attrs);
 /* beamz_cuda_program, ffi_handler.cc:336-708 and 737-765: args in the order of
  * program layout attrs->program_layout (runtime.py:284-750).  Outputs alias
- * args (field/psi workspaces included). */
+ * args (field/psi workspaces included).  The monitor layouts (5 and 6) take
+ * one optional trailing argument with no counterpart in the reference: HOST
+ * s32[M, 6, 6] inclusive boxes (z0, z1, y0, y1, x0, x1; -1 when unused) of
+ * the cells each monitor gathers per component (Ex, Ey, Ez, Hx, Hy, Hz).
+ * With it, two-step launches capture those cells densely instead of writing
+ * an intermediate bank; results are the same. */
 int32_t bzh_program(void* stream, const bzh_buffer* args, int32_t nargs,
                     const bzh_program_attrs* attrs);
 

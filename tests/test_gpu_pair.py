@@ -205,3 +205,68 @@ def test_pair_core_parity(bc, monkeypatch):
     for i, (a, b) in enumerate(zip(*runs)):
         n = np.linalg.norm(b)
         assert n == 0.0 or np.linalg.norm(a - b) / n <= 1e-6, (i, np.linalg.norm(a - b) / n)
+
+
+def _capture_monitors(grid, layout):
+    """Monitor sets whose cells no tile row shares between two monitors, so
+    that the pair launches capture them densely: two x-normal planes in
+    different 60-column tile columns, one z-normal plane with a placed range,
+    one y-normal plane with a Hann window and an interval."""
+    import beamz_amd as bz
+    dt = _dt(grid)
+    nz, ny, nx = grid
+    freqs = np.array([1.7e14, 1.9e14, 2.1e14])
+    if layout == "x2":
+        return [bz.FluxMonitor("x", 20, freqs, ranges={"z": (2, nz - 2), "y": (3, ny - 3)}),
+                bz.FluxMonitor("x", 100, freqs[1:])]
+    if layout == "z":
+        return [bz.FluxMonitor("z", nz // 2, freqs, ranges={"y": (2, ny - 4), "x": (7, nx - 9)})]
+    return [bz.FluxMonitor("y", ny // 2, freqs, interval=2, window="hann", t_start=2.5 * dt,
+                           t_end=9.5 * dt)]
+
+
+@pytest.mark.parametrize("layout", ["x2", "z", "y"])
+@pytest.mark.parametrize("post", [False, True])
+def test_pair_capture_matches_bank_and_oracle(layout, post):
+    """The pair launches' dense capture of the monitored cells (the host boxes
+    passed as the program's trailing argument) against the FP64 oracle, and
+    bit for bit against the same program without the boxes (the intermediate
+    bank and a strided gather of the fields).  ``post`` adds an E-timed slab,
+    after which the second step is sampled from the fields."""
+    _require_gpu()
+    from beamz_amd import native
+    grid = (13, 17, 130)
+    steps = 12
+    srcs = _sources(grid, steps)
+    if not post:
+        srcs = srcs[:1]
+    sim, prog, state0, prob, ost = _setup(grid, seed=73, steps=steps, sources=srcs,
+                                          monitors=_capture_monitors(grid, layout), cpml_cells=3)
+    print(f"case pair_capture layout={layout} post={post} grid={grid}", flush=True)
+    assert prog.monitor_boxes is not None
+    start = {k: getattr(state0, k).cpu().numpy().copy() for k in ("ex", "ey", "ez", "hx", "hy", "hz")}
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        state = sim.advance(state0, steps, chunk=12)
+    torch.cuda.synchronize()
+    assert native.last_program_pairs() == 6
+    assert native.last_program_captured()
+    o.run(prob, ost, steps)
+    _compare_state(prog, state, prob, ost, dft=True)
+    # the same program through the intermediate bank
+    boxes = prog.monitor_boxes
+    try:
+        prog.monitor_boxes = None
+        state_b = sim.initial_state(start)
+        with torch.cuda.stream(s):
+            state_b = sim.advance(state_b, steps, chunk=12)
+        torch.cuda.synchronize()
+        assert not native.last_program_captured()
+    finally:
+        prog.monitor_boxes = boxes
+    for name in ("dft_re", "dft_im", "dft_w"):
+        np.testing.assert_array_equal(getattr(state, name).cpu().numpy(),
+                                      getattr(state_b, name).cpu().numpy(), err_msg=name)
+    for c in ("ex", "ey", "ez", "hx", "hy", "hz"):
+        np.testing.assert_array_equal(getattr(state, c).cpu().numpy(),
+                                      getattr(state_b, c).cpu().numpy(), err_msg=c)

@@ -227,3 +227,39 @@ def test_field_monitors_and_ragged_arenas():
     assert dict(line.ranges) == {"z": (3, 4), "y": (2, 6)} and line.points == 4
     assert all(p is not None for p in line.plans.values())
 
+
+
+def test_capture_boxes_bound_every_gathered_cell():
+    """The host boxes of the pair kernel's monitor capture: for each
+    (monitor, component) the inclusive bounding box of the corners the gather
+    reads (non-zero weight, index inside the component), -1 where none is read;
+    a plane at x = 20 of a 13 x 17 x 130 grid reads E_y, E_z on that plane and
+    H_y, H_z on the two half-planes beside it."""
+    import beamz_amd as bz
+    from beamz_amd.lattice import component_shapes
+    from beamz_amd.monitors import capture_boxes, lower_monitor, pack_monitors
+    grid = (13, 17, 130)
+    shapes = component_shapes(grid)
+    freqs = np.array([2e14])
+    mons = [lower_monitor(bz.FluxMonitor("x", 20, freqs, ranges={"z": (2, 11), "y": (3, 14)}), grid,
+                          shapes, 40e-9, 0),
+            lower_monitor(bz.FluxMonitor("z", 6, freqs, ranges={"y": (2, 13), "x": (7, 121)}), grid,
+                          shapes, 40e-9, 1)]
+    idx, wts, fr, masks, counts, codes, windows = pack_monitors(mons)
+    boxes = capture_boxes(idx, wts, masks, shapes)
+    assert boxes.shape == (2, 6, 6) and boxes.dtype == np.int32
+    names = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
+    for m in range(2):
+        for c, name in enumerate(names):
+            live = (wts[m, c] != 0).reshape(-1)
+            if masks[m, c] == 0 or not live.any():
+                assert (boxes[m, c] == -1).all(), (m, name)
+                continue
+            z, y, x = np.unravel_index(idx[m, c].reshape(-1)[live], shapes[name])
+            assert tuple(boxes[m, c]) == (z.min(), z.max(), y.min(), y.max(), x.min(), x.max())
+    # x-normal flux: Ey and Ez on x = 20, Hy and Hz on the half-planes 19 and 20
+    assert boxes[0, 1, 4:].tolist() == [20, 20] and boxes[0, 2, 4:].tolist() == [20, 20]
+    assert boxes[0, 4, 4:].tolist() == [19, 20] and boxes[0, 5, 4:].tolist() == [19, 20]
+    assert (boxes[0, 0] == -1).all() and (boxes[0, 3] == -1).all()
+    # Ey on z nodes 2..11 (cell centres 2.5 .. 10.5 between nodes), y cells 3..13
+    assert boxes[0, 1, :4].tolist() == [2, 11, 3, 13]
