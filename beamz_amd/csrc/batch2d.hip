@@ -2,10 +2,13 @@
 // together, one persistent launch per time step.
 //
 // Work: every simulation is cut into items of 62 (x) by R (y) Ez cells; the
-// items of all simulations form one queue (``tiles``, host-built, largest
-// simulations first).  A fixed grid of waves pops items from a per-step
-// counter with one atomic per item until the queue is empty, so uneven grid
-// sizes load-balance without one launch per simulation.
+// items of all simulations form one list (``tiles``, host-built, largest
+// simulations first, row-major within one), so uneven grid sizes share one
+// launch.  Items are equal but for the simulations' edges, so wave w takes
+// items w, w + W, w + 2W, ... of the W waves of the grid: no queue atomics
+// (a single shared counter serialized the pops), and the four waves of a
+// workgroup take four neighbouring items of one row band, whose shared edge
+// lines then sit in one L2.
 //
 // One item is one wave marching up its R rows, one full leapfrog step from
 // bank A to bank B (kernels.py:731-811, 1217-1290; ghost rule
@@ -14,16 +17,17 @@
 // (lanes 1..62 own their column, lanes 0 and 63 are halos); x neighbours come
 // from ds_bpermute, and the y neighbours are the rows the wave carries in
 // registers, so no LDS and no barrier is involved.  At row j the wave loads
-// Ez(j+1) (with the pre-E slab sources of the step added as it is loaded, so
-// every item that loads a cell adds the same sum; execute.py:374-461), Hx(j),
-// Hy(j) and the E coefficient, then forms
+// Ez(j+1), Hx(j), Hy(j) and the E coefficient (two rows ahead of their use,
+// so that the loads of rows j+1 and j+2 fly while row j is computed), then
+// forms
 //   Hx_new(j) from Ez(j+1) - Ez(j),
 //   Hy_new(j) from Ez(j, x+1) - Ez(j, x),
 //   Ez_new(j) from Hy_new(x) - Hy_new(x-1) and Hx_new(j) - Hx_new(j-1),
 // carrying Ez(j+1) and Hx_new(j) to the next row.  The row below the item is
 // recomputed (Hx_new only).  CPML memories are read from bank A and written
 // to bank B for owned cells only.  Between steps the kernel boundary is the
-// only synchronization.
+// only synchronization.  The pre-E slab sources of a step (execute.py:374-461)
+// are added to bank A's Ez by a small launch before the march.
 
 #include "common.h"
 
@@ -52,11 +56,9 @@ __global__ void __launch_bounds__(256) batch2d_march(Batch2dParams p) {
   const int dn = ((lane + 1) & 63) << 2;  // permute address of lane + 1
   const int up = ((lane - 1) & 63) << 2;  // ... of lane - 1
 
-  for (;;) {
-    int t = 0;
-    if (lane == 0) t = atomicAdd(p.counter, 1);
-    t = __builtin_amdgcn_readfirstlane(__shfl(t, 0));
-    if (t >= p.ntiles) break;
+  const int nwaves = gridDim.x * (blockDim.x >> 6);
+  for (int t = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+       t < p.ntiles; t += nwaves) {
     const int4 td = p.tiles[t];
     // the descriptor's fields are read once into registers: the field stores
     // below could otherwise alias it and force reloads on every row (the
@@ -84,20 +86,6 @@ __global__ void __launch_bounds__(256) batch2d_march(Batch2dParams p) {
     const float idx = S.inv_dx;
     const float h_dec = *S.h_dec, h_src = *S.h_src, e_dec = *S.e_dec;
     const int W = nx + 1;  // row length of Ez and Hx
-    const int nsrc = S.nsrc;
-    int sy0[4], sx0[4], ssy[4], ssx[4], sT[4];
-    const float* sco[4];
-    const float* swv[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      sy0[q] = S.src_y0[q];
-      sx0[q] = S.src_x0[q];
-      ssy[q] = S.src_sy[q];
-      ssx[q] = S.src_sx[q];
-      sT[q] = S.src_T[q];
-      sco[q] = S.src_coeff[q];
-      swv[q] = S.src_wave[q];
-    }
     const int y0 = td.y * R, y1 = min(y0 + R, ny + 1);
     const int x = 62 * td.z - 1 + lane;
     const bool vez = x >= 0 && x <= nx;   // Ez / Hx column
@@ -119,23 +107,8 @@ __global__ void __launch_bounds__(256) batch2d_march(Batch2dParams p) {
       k2 = tab[(2 * 3 + 2) * L2 + q2];
     }
 
-    // Ez(A) at (j, x) with this step's pre-E sources; 0 off the grid
-    auto ez_row = [&](int j) -> float {
-      float v = 0.0f;
-      if (j <= ny && vez) {
-        v = ez_a[j * W + x];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int sy = j - sy0[q], sx = x - sx0[q];
-          if (q < nsrc && sy >= 0 && sy < ssy[q] && sx >= 0 && sx < ssx[q]) {
-            int wi = step;
-            wi = wi < 0 ? 0 : (wi > sT[q] - 1 ? sT[q] - 1 : wi);
-            v += sco[q][sy * ssx[q] + sx] * swv[q][wi];
-          }
-        }
-      }
-      return v;
-    };
+    // Ez(A) at (j, x), 0 off the grid
+    auto ez_row = [&](int j) -> float { return (j <= ny && vez) ? ez_a[j * W + x] : 0.0f; };
 
     const bool es_arr = S.e_src_st != 0;
     const float es_uni = es_arr ? 0.0f : e_src[0];
@@ -143,20 +116,21 @@ __global__ void __launch_bounds__(256) batch2d_march(Batch2dParams p) {
     auto hy_row = [&](int j) { return (j <= ny && vhy) ? hy_a[j * nx + x] : 0.0f; };
     auto es_row = [&](int j) { return (es_arr && j <= ny && vez) ? e_src[j * W + x] : es_uni; };
 
-    // Each row's operands are loaded one row ahead, so the loads of row j+1
-    // fly while row j is computed.
     const int js = y0 > 0 ? y0 - 1 : 0;
-    float ez = ez_row(js), ezn = ez_row(js + 1);
+    // Each row's operands are loaded two rows ahead: the loads of rows j+1
+    // and j+2 fly while row j is computed.
+    float ez = ez_row(js), ezn = ez_row(js + 1), ezn2 = ez_row(js + 2);
     float hxa = hx_row(js), hya = hy_row(js), esv = es_row(js);
+    float hxa2 = hx_row(js + 1), hya2 = hy_row(js + 1), esv2 = es_row(js + 1);
     float hx_prev = 0.0f;  // Hx_new(j - 1)
     for (int j = js; j < y1; ++j) {
       const bool st = j >= y0;  // this item owns row j
-      float ezn2 = 0.0f, hxa2 = 0.0f, hya2 = 0.0f, esv2 = es_uni;
-      if (j + 1 < y1) {
-        ezn2 = ez_row(j + 2);
-        hxa2 = hx_row(j + 1);
-        hya2 = hy_row(j + 1);
-        esv2 = es_row(j + 1);
+      float ezn3 = 0.0f, hxa3 = 0.0f, hya3 = 0.0f, esv3 = es_uni;
+      if (j + 2 < y1) {
+        ezn3 = ez_row(j + 3);
+        hxa3 = hx_row(j + 2);
+        hya3 = hy_row(j + 2);
+        esv3 = es_row(j + 2);
       }
       // Hx_new(j) (ny rows, nx+1 columns): dH Hx - sH * corrected(dEz/dy)
       float hx = 0.0f;
@@ -215,23 +189,41 @@ __global__ void __launch_bounds__(256) batch2d_march(Batch2dParams p) {
       hx_prev = hx;
       ez = ezn;
       ezn = ezn2;
+      ezn2 = ezn3;
       hxa = hxa2;
       hya = hya2;
       esv = esv2;
+      hxa2 = hxa3;
+      hya2 = hya3;
+      esv2 = esv3;
     }
   }
 }
 
-__global__ void __launch_bounds__(256) zero_counters(int* c, int n) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < n) c[i] = 0;
+// The pre-E slab sources of one step: block (b, q, i) adds slab q of
+// simulation i to bank ``bank``'s Ez, 256 cells per block row.
+__global__ void __launch_bounds__(256) batch2d_sources(Batch2dParams p) {
+  const Sim2dDev& S = p.sims[blockIdx.z];
+  const int q = blockIdx.y;
+  if (q >= S.nsrc) return;
+  const int n = S.src_sy[q] * S.src_sx[q];
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= n) return;
+  const int sy = c / S.src_sx[q], sx = c - sy * S.src_sx[q];
+  const int y = S.src_y0[q] + sy, x = S.src_x0[q] + sx;
+  if (y < 0 || x < 0 || y > S.ny || x > S.nx) return;
+  int wi = *p.step0 + p.k;
+  wi = wi < 0 ? 0 : (wi > S.src_T[q] - 1 ? S.src_T[q] - 1 : wi);
+  float* ez = S.ez[p.bank];
+  // (atomic: two slabs of one simulation may share a cell)
+  atomicAdd(&ez[y * (S.nx + 1) + x], S.src_coeff[q][c] * S.src_wave[q][wi]);
 }
 
 }  // namespace
 
 void launch_batch2d(hipStream_t s, const Batch2dParams& p) {
   BZH_REQUIRE(p.workgroups >= 1, "batched launch needs at least one workgroup");
-  BZH_REQUIRE(p.sims != nullptr && p.tiles != nullptr && p.counter != nullptr && p.step0 != nullptr,
+  BZH_REQUIRE(p.sims != nullptr && p.tiles != nullptr && p.step0 != nullptr,
               "batched launch is missing its descriptors");
   BZH_REQUIRE(p.tile_rows == 8 || p.tile_rows == 16 || p.tile_rows == 32 || p.tile_rows == 64,
               "batched item height must be 8, 16, 32 or 64 rows");
@@ -240,10 +232,12 @@ void launch_batch2d(hipStream_t s, const Batch2dParams& p) {
   BZH_HIP(hipGetLastError());
 }
 
-void launch_zero_counters(hipStream_t s, const ZeroParams& p) {
-  if (p.n <= 0) return;
-  hipLaunchKernelGGL(zero_counters, dim3((p.n + 255) / 256), dim3(256), 0, s, p.counters, p.n);
+void launch_batch2d_sources(hipStream_t s, const Batch2dParams& p) {
+  if (p.nsims <= 0 || p.max_src_cells <= 0) return;
+  const dim3 block(256, 1, 1), grid((p.max_src_cells + 255) / 256, 4, p.nsims);
+  hipLaunchKernelGGL(batch2d_sources, grid, block, 0, s, p);
   BZH_HIP(hipGetLastError());
 }
+
 
 }  // namespace bzh

@@ -322,7 +322,7 @@ void run_phase(hipStream_t s, const PhaseParams& p, int phase, const Grid& g) {
 
 // ---- schedule ------------------------------------------------------------------
 enum class Kind : int32_t {
-  kPhase3d, kPhaseTm, kSource, kDft, kFused, kBatch2d, kZero, kAdjoint, kPair, kMonBox
+  kPhase3d, kPhaseTm, kSource, kDft, kFused, kBatch2d, kAdjoint, kPair, kMonBox
 };
 
 struct Launch {
@@ -366,19 +366,14 @@ void enqueue(hipStream_t s, const Launch& l) {
     case Kind::kBatch2d: {
       Batch2dParams p;
       std::memcpy(&p, l.bytes.data(), sizeof(p));
-      launch_batch2d(s, p);
+      if (l.phase == 1) launch_batch2d_sources(s, p);
+      else launch_batch2d(s, p);
       break;
     }
     case Kind::kAdjoint: {
       Adjoint2dParams p;
       std::memcpy(&p, l.bytes.data(), sizeof(p));
       launch_adjoint_step(s, p);
-      break;
-    }
-    case Kind::kZero: {
-      ZeroParams p;
-      std::memcpy(&p, l.bytes.data(), sizeof(p));
-      launch_zero_counters(s, p);
       break;
     }
     case Kind::kDft: {
@@ -1679,9 +1674,9 @@ void copy_psi_workspace(hipStream_t s, const bzh_buffer* args, int nargs) {
 }
 
 // ---- batched 2D simulations ---------------------------------------------------
-// Device copies of the descriptor table and of the tile queue, keyed on their
-// bytes, and per-(device, stream) step counters.  They are kept for the life
-// of the process: a captured graph refers to them.
+// Device copies of the descriptor table and of the item list, keyed on their
+// bytes.  They are kept for the life of the process: a captured graph refers
+// to them.
 template <typename V>
 const void* upload_cached(const std::vector<V>& host) {
   static std::mutex mu;
@@ -1698,21 +1693,6 @@ const void* upload_cached(const std::vector<V>& host) {
   BZH_HIP(hipMemcpy(d, host.data(), host.size() * sizeof(V), hipMemcpyHostToDevice));
   cache[key] = d;
   return d;
-}
-
-int* step_counters(hipStream_t s, int n) {
-  static std::mutex mu;
-  static std::map<std::pair<int, hipStream_t>, std::pair<int*, int>> rings;
-  int device = 0;
-  BZH_HIP(hipGetDevice(&device));
-  std::lock_guard<std::mutex> lock(mu);
-  auto& r = rings[{device, s}];
-  if (r.second < n) {
-    void* d = nullptr;  // an older, smaller array stays alive for its graphs
-    BZH_HIP(hipMalloc(&d, static_cast<size_t>(n) * sizeof(int)));
-    r = {static_cast<int*>(d), n};
-  }
-  return r.first;
 }
 
 float* f32_ptr(const bzh_buffer& b, int64_t expect, const std::string& what) {
@@ -1819,14 +1799,15 @@ void run_batch2d(hipStream_t s, const bzh_sim2d* sims, int nsims, const bzh_buff
   p.step0 = static_cast<const int32_t*>(step->data);
   p.tile_rows = a.tile_rows;
   p.workgroups = workgroups;
-  int* counters = step_counters(s, a.nsteps);
+  p.nsims = nsims;
+  for (const Sim2dDev& d : desc)
+    for (int q = 0; q < d.nsrc; ++q) p.max_src_cells = std::max(p.max_src_cells, d.src_sy[q] * d.src_sx[q]);
   std::vector<Launch> launches;
-  ZeroParams z{counters, a.nsteps};
-  launches.push_back(make_launch(Kind::kZero, 0, 0, z));
   for (int k = 0; k < a.nsteps; ++k) {
     p.k = k;
     p.bank = k & 1;
-    p.counter = counters + k;
+    // the step's pre-E sources on the bank it reads, then the march
+    if (p.max_src_cells > 0) launches.push_back(make_launch(Kind::kBatch2d, 1, BZH_F32, p));
     launches.push_back(make_launch(Kind::kBatch2d, 0, BZH_F32, p));
   }
   graph_cache().run(s, launches, (a.flags & BZH_FLAG_GRAPH_CACHE) != 0);

@@ -381,14 +381,16 @@ struct Batch2dParams {
   const Sim2dDev* sims;
   const int4* tiles;     // (simulation, item row, item column, 0), queue order
   int32_t ntiles;
-  int* counter;          // queue head of this step (zeroed before the call)
   int32_t bank;          // bank read by this step
   const int32_t* step0;  // device scalar: step index of step 0 of this call
   int32_t k;             // step within this call
   int32_t tile_rows;     // rows per item (items are 62 columns wide)
   int32_t workgroups;    // persistent grid size
+  int32_t nsims;         // simulations (the source launch's grid)
+  int32_t max_src_cells; // largest pre-E slab, in cells (0: no sources)
 };
 void launch_batch2d(hipStream_t s, const Batch2dParams& p);
+void launch_batch2d_sources(hipStream_t s, const Batch2dParams& p);
 
 // Reverse-time TMxy step (adjoint.hip).
 struct Adjoint2dParams {
@@ -419,10 +421,5 @@ struct Adjoint2dParams {
 void launch_adjoint_step(hipStream_t s, const Adjoint2dParams& p);
 void launch_adjoint_record(hipStream_t s, const Adjoint2dParams& p, const float* hx, const float* hy);
 
-struct ZeroParams {
-  int* counters;
-  int32_t n;
-};
-void launch_zero_counters(hipStream_t s, const ZeroParams& p);
 
 }  // namespace bzh

@@ -243,9 +243,9 @@ int32_t bzh_program_noalloc(void* stream, const bzh_buffer* args, int32_t nargs,
  * No counterpart in the reference, which runs one simulation per call (its
  * FFI is vmapped sequentially, cuda/runtime.py:85).  Many independent TMxy
  * (Ez, Hx, Hy) simulations of uneven sizes advance together: each step is
- * one persistent launch whose waves pop work items (62 columns by tile_rows
- * rows) of every simulation from a device work queue and march up their
- * rows; a call of nsteps steps is captured
+ * one source launch (the pre-E slabs) and one march launch whose waves take
+ * the work items (62 columns by tile_rows rows) of every simulation
+ * round-robin and march up their rows; a call of nsteps steps is captured
  * once as a hipGraph and replayed.  Two field banks per simulation: bank 0 is
  * read by the first step, and after an odd step count the result is in bank
  * 1 (the caller swaps the roles).  All buffers f32. */
