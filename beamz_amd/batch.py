@@ -130,6 +130,15 @@ class BatchedSimulation2D:
         return {"members": members, "step": step}
 
     def _descriptors(self, state):
+        """The members' device descriptors for the current bank order, built
+        once per order and state (they hold only pointers and sizes)."""
+        cache = state.setdefault("_desc", {})
+        key = tuple(m.ez[0].data_ptr() for m in state["members"])
+        if key not in cache:
+            cache[key] = self._build_descriptors(state)
+        return cache[key]
+
+    def _build_descriptors(self, state):
         arr = (native.Sim2d * len(state["members"]))()
         keep = []
         for i, m in enumerate(state["members"]):

@@ -169,7 +169,11 @@ def run_batched(args):
     torch.cuda.synchronize()
     stream = torch.cuda.Stream()
     with torch.cuda.stream(stream):
-        tr, wg, timings = autotune(batch, steps=8, verbose=True)
+        if args.tile_rows and args.workgroups:
+            batch.tile_rows, batch.workgroups = tr, wg = args.tile_rows, args.workgroups
+            timings = {}
+        else:
+            tr, wg, timings = autotune(batch, steps=8, verbose=True)
         batch.advance(state, args.warmup, chunk=args.chunk)
         # the timed window replays captured graphs: its chunk sequence runs
         # once beforehand (a chunk's graph is keyed on its step count)
@@ -437,6 +441,9 @@ def main():
     ap.add_argument("--no-monitors", action="store_true")
     ap.add_argument("--per-phase", action="store_true", help="use the per-phase kernels")
     ap.add_argument("--chunk", type=int, default=256)
+    ap.add_argument("--tile-rows", type=int, default=0,
+                    help="batched2d: item height (with --workgroups, skips the autotuner)")
+    ap.add_argument("--workgroups", type=int, default=0, help="batched2d: grid size")
     ap.add_argument("--workload", default="3d", choices=("3d", "waveguide3d", "batched2d", "adjoint2d"),
                     help="3d: the 1024x512x256 Yee+CPML step (default); waveguide3d: the 512x256x128 "
                          "silicon strip; batched2d: the 64-simulation "
