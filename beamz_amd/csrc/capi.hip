@@ -723,7 +723,7 @@ int chunk_len(int64_t tiles, int planes) {
   return kz;
 }
 
-// Ring length of the buffered DFT: 16 steps (BZH_DFT_SLOTS overrides it;
+// Ring length of the buffered DFT: 32 steps (BZH_DFT_SLOTS overrides it;
 // 1 accumulates every step directly).  Never longer than the call.
 int dft_slots(int nsteps) {
   int n = kDftMaxSlots;

@@ -351,7 +351,7 @@ struct DftParams {
   const int32_t* capbox;
   int32_t fdims[6][3];   // component shapes (z, y, x), for the capture index
 };
-constexpr int kDftMaxSlots = 16;
+constexpr int kDftMaxSlots = 32;
 void launch_dft(hipStream_t s, const DftParams& p);
 
 // One TMxy simulation of a batch on the device: two banks of (Ez, Hx, Hy),

@@ -231,7 +231,7 @@ int32_t bzh_program(void* stream, const bzh_buffer* args, int32_t nargs,
  * harnesses that forbid hipMalloc): same arguments, layouts and results, but
  * it calls no hipMalloc, hipFree or graph instantiation.  The buffered DFT's
  * staging ring lives in a fixed device array of BZH_NOALLOC_SCRATCH_BYTES per
- * GPU, held until the process ends; the ring takes as many steps (up to 16)
+ * GPU, held until the process ends; the ring takes as many steps (up to 32)
  * as fit, and a call whose monitors sample more than that array holds in one
  * step fails with an error.  Pairs of steps run only without monitors.  Calls
  * share the array per GPU: calls that overlap in time on one GPU race. */
