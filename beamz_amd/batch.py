@@ -96,7 +96,7 @@ class BatchedSimulation2D:
                 raise NotImplementedError("monitors are not part of the batched path")
             if p.field_dtype != torch.float32:
                 raise NotImplementedError("batched simulations store fields in float32")
-            if p.cpml is not None and p.cpml_table is None:
+            if p.cpml is not None and (p.boundary_code >> 8) == 0:
                 raise NotImplementedError("batched CPML needs one thickness on every edge")
             self.progs.append(p)
         self.tile_rows = tile_rows

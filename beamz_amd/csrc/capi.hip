@@ -1100,43 +1100,113 @@ void add_mid_slabs(MidSlab* list, int32_t& count, int cap, const SourceGroupIn& 
 
 size_t elem_bytes(int t) { return t == BZH_BF16 || t == BZH_F16 ? 2 : (t == BZH_F64 ? 8 : 4); }
 
+// The CPML term extents of a phase pair from its device metadata rows
+// (component, axis, low, high, sign; runtime.py:56-69): ext[2T], ext[2T + 1]
+// are term T's low and high extents, H terms 0..5 then E terms 6..11 in the
+// kernels' order.  The rows are copied on the caller's stream (so that a
+// metadata write queued before this call lands first) and the copy is waited
+// for.  Not while the caller's stream captures a graph (no copy may run
+// then): false, and the caller takes the per-phase schedule.
+bool cpml_extents(hipStream_t s, const PhaseIn& h, const PhaseIn& e, int32_t ext[24]) {
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  if (s != nullptr) BZH_HIP(hipStreamIsCapturing(s, &cap));
+  if (cap != hipStreamCaptureStatusNone) return false;
+  for (const PhaseIn* ph : {&h, &e})
+    if (ph->meta == nullptr || ph->meta->data == nullptr || ph->meta->element_type != BZH_S32 ||
+        numel(*ph->meta) != 30)
+      return false;
+  int32_t rows[2][30];
+  BZH_HIP(hipMemcpyAsync(rows[0], h.meta->data, sizeof(rows[0]), hipMemcpyDeviceToHost, s));
+  BZH_HIP(hipMemcpyAsync(rows[1], e.meta->data, sizeof(rows[1]), hipMemcpyDeviceToHost, s));
+  BZH_HIP(hipStreamSynchronize(s));
+  for (int T = 0; T < 12; ++T) {
+    const int32_t* r = rows[T / 6] + (T % 6) * 5;
+    if (r[0] != (T % 6) / 2 || r[1] != kAxes3d[T % 6]) return false;  // not the kernels' order
+    ext[2 * T] = r[2];
+    ext[2 * T + 1] = r[3];
+  }
+  return true;
+}
+
 // Can the fused kernel take this phase pair?  Fills the CPML part of ``fp``.
+// Term extents: ``uniform_l`` on every face when positive, else ``ext`` (from
+// cpml_extents; null: unknown, so no).  The terms on one axis must share their
+// extents within a phase, the 36 coefficient arrays must be the rows of one
+// [12][3][st] table, and the CPML memories of both banks one arena.
 bool fused_cpml_layout(const PhaseIn& h, const PhaseIn& e, const bzh_buffer* const* psi_b,
-                       int uniform_l, FusedParams& fp) {
+                       int uniform_l, const int32_t* ext, FusedParams& fp) {
   fp.L = 0;
   fp.tab = nullptr;
-  if (h.nterms == 0 && e.nterms == 0) return true;
-  if (h.nterms != 6 || e.nterms != 6 || uniform_l <= 0) return false;
-  const int64_t L2 = 2 * uniform_l;
-  const char* base = static_cast<const char*>(h.terms[0]->data);
+  fp.tab_st = 0;
+  for (int i = 0; i < 6; ++i) fp.clo[i] = fp.chi[i] = 0;
   for (int T = 0; T < 12; ++T) {
+    fp.psi[T] = nullptr;
+    fp.poff[T] = 0u;
+  }
+  if (h.nterms == 0 && e.nterms == 0) return true;
+  if (h.nterms != 6 || e.nterms != 6) return false;
+  if (uniform_l <= 0 && ext == nullptr) return false;
+  int n2[12];
+  bool seen[6] = {};
+  int T0 = -1, L = 0;
+  for (int T = 0; T < 12; ++T) {
+    const int lo = uniform_l > 0 ? uniform_l : ext[2 * T];
+    const int hi = uniform_l > 0 ? uniform_l : ext[2 * T + 1];
+    if (lo < 0 || hi < 0) return false;
+    const int slot = (T < 6 ? kSlabH : kSlabE) + (2 - kAxes3d[T % 6]);  // axis 0 x, 1 y, 2 z
+    if (!seen[slot]) {
+      seen[slot] = true;
+      fp.clo[slot] = lo;
+      fp.chi[slot] = hi;
+    } else if (fp.clo[slot] != lo || fp.chi[slot] != hi) {
+      return false;
+    }
+    n2[T] = lo + hi;
+    L = std::max(L, std::max(lo, hi));
+    if (n2[T] > 0 && T0 < 0) T0 = T;
+  }
+  if (T0 < 0) return true;  // no term has a slab: no CPML work
+  // the coefficient table: term T0's a and b rows give the row stride
+  const PhaseIn& p0 = T0 < 6 ? h : e;
+  const char* a0 = static_cast<const char*>(p0.terms[(T0 % 6) * 3 + 0]->data);
+  const char* b0 = static_cast<const char*>(p0.terms[(T0 % 6) * 3 + 1]->data);
+  if (a0 == nullptr || b0 <= a0 || (b0 - a0) % 4 != 0) return false;
+  const int64_t st = (b0 - a0) / 4;
+  const char* base = a0 - static_cast<int64_t>(T0) * 3 * st * 4;
+  for (int T = 0; T < 12; ++T) {
+    if (n2[T] == 0) continue;
+    if (n2[T] > st) return false;
     const PhaseIn& ph = T < 6 ? h : e;
     for (int q = 0; q < 3; ++q) {
       const bzh_buffer* b = ph.terms[(T % 6) * 3 + q];
-      if (b->element_type != BZH_F32 || numel(*b) != L2) return false;
-      if (static_cast<const char*>(b->data) != base + ((T * 3 + q) * L2) * 4) return false;
+      if (b == nullptr || b->element_type != BZH_F32 || numel(*b) != n2[T]) return false;
+      if (static_cast<const char*>(b->data) != base + ((T * 3 + q) * st) * 4) return false;
     }
   }
+  // CPML memories: one per term with a slab, in both banks at one distance
   int64_t delta = 0;
   int ptype = -1;
   for (int T = 0; T < 12; ++T) {
+    if (n2[T] == 0) continue;
     const bzh_buffer* a = T < 6 ? h.psi_in[T] : e.psi_in[T - 6];
     const bzh_buffer* b = psi_b[T];
     if (a == nullptr || b == nullptr || a->data == nullptr || b->data == nullptr) return false;
     if (numel(*a) != numel(*b) || a->element_type != b->element_type) return false;
+    if (a->rank != 3 || a->dims[kAxes3d[T % 6]] != n2[T]) return false;
     if (ptype < 0) ptype = a->element_type;
     if (a->element_type != ptype) return false;
     const int64_t diff = static_cast<const char*>(b->data) - static_cast<const char*>(a->data);
     const int64_t sz = static_cast<int64_t>(elem_bytes(ptype));
     if (diff % sz != 0) return false;
-    if (T == 0) delta = diff / sz;
+    if (T == T0) delta = diff / sz;
     else if (diff / sz != delta) return false;
     fp.psi[T] = a->data;
   }
-  // based addressing: all 24 memories within 4 GiB above the lowest
+  // based addressing: all the memories of both banks within 4 GiB above the lowest
   const int64_t sz = static_cast<int64_t>(elem_bytes(ptype));
   uintptr_t lo = UINTPTR_MAX, hi = 0;
   for (int T = 0; T < 12; ++T) {
+    if (n2[T] == 0) continue;
     const bzh_buffer* a = T < 6 ? h.psi_in[T] : e.psi_in[T - 6];
     for (const bzh_buffer* x : {a, psi_b[T]}) {
       const uintptr_t p = reinterpret_cast<uintptr_t>(x->data);
@@ -1147,10 +1217,12 @@ bool fused_cpml_layout(const PhaseIn& h, const PhaseIn& e, const bzh_buffer* con
   if (hi - lo >= (uintptr_t{1} << 32)) return false;
   fp.pbase = reinterpret_cast<const char*>(lo);
   for (int T = 0; T < 12; ++T)
-    fp.poff[T] = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(fp.psi[T]) - lo);
+    if (fp.psi[T] != nullptr)
+      fp.poff[T] = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(fp.psi[T]) - lo);
   fp.pdelta = static_cast<uint32_t>(static_cast<uint64_t>(delta * sz));
-  fp.L = uniform_l;
+  fp.L = L;
   fp.tab = reinterpret_cast<const float*>(base);
+  fp.tab_st = static_cast<int32_t>(st);
   fp.psi_delta = delta;
   fp.psi_bf16 = ptype == BZH_BF16;
   return true;
@@ -1228,10 +1300,16 @@ bool run_program(hipStream_t s, ProgramIn& in) {
     based = group_base(src, 3, cpml_probe.cbase, cpml_probe.coff);
   }
   std::memcpy(cpml_probe.foff, foff_a, sizeof(foff_a));
+  // CPML extents other than one thickness on every face: from the term rows
+  int32_t cpml_ext_rows[24];
+  const int32_t* cpml_ext = nullptr;
+  if (based && in.uniform_l <= 0 && in.h.nterms == 6 && in.e.nterms == 6 && !in.noalloc &&
+      (in.flags & BZH_FLAG_FUSED_STEP) && cpml_extents(s, in.h, in.e, cpml_ext_rows))
+    cpml_ext = cpml_ext_rows;
   const bool fused = two_banks && (in.flags & BZH_FLAG_FUSED_STEP) && g.is3d &&
                      h_slabs <= kMaxHSlabs && (h_slabs == 0 || in.step != nullptr) &&
                      coef_ok && based && metrics_empty(in.h) && metrics_empty(in.e) &&
-                     fused_cpml_layout(in.h, in.e, in.psi_b, in.uniform_l, cpml_probe);
+                     fused_cpml_layout(in.h, in.e, in.psi_b, in.uniform_l, cpml_ext, cpml_probe);
   const int32_t* step0 = in.step ? static_cast<const int32_t*>(in.step->data) : nullptr;
   int slots = 1;
   float* stage = nullptr;
@@ -1340,6 +1418,7 @@ bool run_program(hipStream_t s, ProgramIn& in) {
     if (fp.L > 0 && cur_b) {
       // bank B is read: psi roles swap
       for (int t = 0; t < 12; ++t) {
+        if (fp.psi[t] == nullptr) continue;  // a term without a slab
         fp.psi[t] = static_cast<char*>(fp.psi[t]) + fp.psi_delta * (fp.psi_bf16 ? 2 : 4);
         fp.poff[t] += fp.pdelta;
       }

@@ -158,6 +158,7 @@ constexpr int kMaxHSlabs = 4;
 // CPML, one slab thickness L on every term, the 36 term arrays packed as
 // tab[12][3][2L] (H terms then E terms; a, b, 1/kappa) and each psi of bank B
 // at psi + psi_delta (elements).
+constexpr int kSlabH = 0, kSlabE = 3;
 struct FusedParams {
   const void* e_in[3];
   const void* h_in[3];
@@ -173,8 +174,15 @@ struct FusedParams {
   float ratio_e;   // E storage scale / H storage scale
   int32_t nz, ny, nx;
   int32_t metallic;
-  int32_t L;             // uniform CPML thickness (0: no CPML)
-  const float* tab;
+  int32_t L;             // thickest CPML slab (0: no CPML)
+  // CPML slab extents, indexed kSlabH/kSlabE + axis (0 x, 1 y, 2 z): the
+  // samples at the low face and at the high face that carry a CPML memory
+  // (the memory's extent along the axis is lo + hi; 0 and 0 for an axis
+  // without CPML).  The H terms on one axis share their extents, as do the E
+  // terms.
+  int32_t clo[6], chi[6];
+  int32_t tab_st;        // floats per (term, coefficient) row of tab
+  const float* tab;      // [12 terms][a, b, 1/kappa][tab_st]
   void* psi[12];         // bank A: H terms 0..5, E terms 6..11
   int64_t psi_delta;
   int32_t psi_bf16;

@@ -29,8 +29,8 @@
 // (global_load v, v_off, s[base]); every array is below 4 GiB.
 //
 // Numerical contract: identical to phase.hip (kernels.py:458-626,
-// lattice.py:523-575, kernels.py:184-264 with a uniform slab thickness L,
-// _boundary_compile.py:118-160); arithmetic is float32.
+// lattice.py:523-575, kernels.py:184-264 with per-axis slab extents at the
+// low and high faces, _boundary_compile.py:118-160); arithmetic is float32.
 
 #include <cstdlib>
 #include <string>
@@ -58,10 +58,11 @@ __device__ __forceinline__ float gdiff(float cur, float prev, int c, int N, bool
   return up - lo;
 }
 
-// Packed slab coordinate for a uniform thickness L on an axis of n samples
-// (kernels.py:184-222); -1 outside the slabs.  Requires 0 <= c < n.
-__device__ __forceinline__ int slab_q(int c, int n, int L) {
-  return L <= 0 ? -1 : (c < L ? c : (c >= n - L ? c - (n - 2 * L) : -1));
+// Packed slab coordinate of sample c on an axis of n samples whose CPML
+// memory holds the lo samples at the low face and the hi samples at the high
+// face (kernels.py:184-222); -1 outside the slabs.  Requires 0 <= c < n.
+__device__ __forceinline__ int slab_q(int c, int n, int lo, int hi) {
+  return c < lo ? c : (c >= n - hi ? c - (n - lo - hi) : -1);
 }
 
 // Workgroups are dispatched to the 8 XCDs round-robin (g % 8); give each XCD
@@ -140,12 +141,12 @@ __device__ __forceinline__ bool decode_pair_box(const PairBox& b, int blk, int& 
 struct Coef {
   float a, b, ik;
 };
-// Term t's recurrence coefficients at packed coordinate q; tab is [12][3][2L].
-__device__ __forceinline__ Coef coef_u(cfloat* tab, int t, int L2, int q) {
-  return {tab[(t * 3 + 0) * L2 + q], tab[(t * 3 + 1) * L2 + q], tab[(t * 3 + 2) * L2 + q]};
+// Term t's recurrence coefficients at packed coordinate q; tab is [12][3][st].
+__device__ __forceinline__ Coef coef_u(cfloat* tab, int t, int st, int q) {
+  return {tab[(t * 3 + 0) * st + q], tab[(t * 3 + 1) * st + q], tab[(t * 3 + 2) * st + q]};
 }
-__device__ __forceinline__ Coef coef_v(const float* tab, int t, int L2, int q) {
-  return {tab[(t * 3 + 0) * L2 + q], tab[(t * 3 + 1) * L2 + q], tab[(t * 3 + 2) * L2 + q]};
+__device__ __forceinline__ Coef coef_v(const float* tab, int t, int st, int q) {
+  return {tab[(t * 3 + 0) * st + q], tab[(t * 3 + 1) * st + q], tab[(t * 3 + 2) * st + q]};
 }
 // psi' = b psi + a d;  returns d / kappa + psi'  (kernels.py:237-264)
 __device__ __forceinline__ float cpml_step(const Coef& c, float d, float psi, float& out) {
@@ -352,7 +353,10 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
   int bx, by, kz0, kz1;
   if (!decode_tile(f, bx, by, kz0, kz1)) return;  // padding workgroup
 
-  const int nz = f.nz, ny = f.ny, nx = f.nx, L = f.L, L2 = 2 * f.L;
+  const int nz = f.nz, ny = f.ny, nx = f.nx, L = f.L, TS = f.tab_st;
+  // CPML memory extents along the slab axis (x and y; z slabs lead)
+  const int n2xH = f.clo[kSlabH] + f.chi[kSlabH], n2yH = f.clo[kSlabH + 1] + f.chi[kSlabH + 1];
+  const int n2xE = f.clo[kSlabE] + f.chi[kSlabE], n2yE = f.clo[kSlabE + 1] + f.chi[kSlabE + 1];
   const int m = f.metallic;
   const bool mf = m & kFront, mk = m & kBack, mb = m & kBottom, mt = m & kTop;
   const bool ml = m & kLeft, mr = m & kRight;
@@ -370,8 +374,8 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
     vxY = x >= 0 && x <= nx;
     xX = min(max(x, 0), nx - 1);
     xY = min(max(x, 0), nx);
-    qxH = vxX ? slab_q(x, nx, L) : -1;
-    qxE = vxY ? slab_q(x, nx + 1, L) : -1;
+    qxH = vxX ? slab_q(x, nx, f.clo[kSlabH], f.chi[kSlabH]) : -1;
+    qxE = vxY ? slab_q(x, nx + 1, f.clo[kSlabE], f.chi[kSlabE]) : -1;
     xface = (ml && x == 0) || (mr && x == nx);
   }
   const bool own = !halo && lane >= 1 && lane <= 62 && (!kXEdge || x <= nx);
@@ -382,12 +386,12 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
   Coef c3{}, c4{}, c9{}, c10{};
   if constexpr (kXEdge) {
     if (qxH >= 0) {
-      c3 = coef_v(f.tab, 3, L2, qxH);
-      c4 = coef_v(f.tab, 4, L2, qxH);
+      c3 = coef_v(f.tab, 3, TS, qxH);
+      c4 = coef_v(f.tab, 4, TS, qxH);
     }
     if (qxE >= 0) {
-      c9 = coef_v(f.tab, 9, L2, qxE);
-      c10 = coef_v(f.tab, 10, L2, qxE);
+      c9 = coef_v(f.tab, 9, TS, qxE);
+      c10 = coef_v(f.tab, 10, TS, qxE);
     }
   }
 
@@ -395,8 +399,8 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
   const int y = j0 - 1 + w;
   const bool vyX = y >= 0 && y <= ny, vyY = y >= 0 && y < ny;
   const int yX = min(max(y, 0), ny), yY = min(max(y, 0), ny - 1);
-  const int qyH = vyY ? slab_q(y, ny, L) : -1;      // Hx_y, Hz_y: ny samples
-  const int qyE = vyX ? slab_q(y, ny + 1, L) : -1;  // Ex_y, Ez_y: ny + 1 samples
+  const int qyH = vyY ? slab_q(y, ny, f.clo[kSlabH + 1], f.chi[kSlabH + 1]) : -1;      // Hx_y, Hz_y: ny samples
+  const int qyE = vyX ? slab_q(y, ny + 1, f.clo[kSlabE + 1], f.chi[kSlabE + 1]) : -1;  // Ex_y, Ez_y: ny + 1 samples
   const bool y_pec_h = (mb && y == 0) || (mt && y == ny);  // Hy normal to a y face
   // top halo row of E, loaded by wave 0
   const int yt = j0 + kR;
@@ -466,8 +470,8 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
     const bool vxy = k < nz;   // Hx/Hy planes
     const uint32_t kk = min(k, nz - 1);
     const uint32_t kx = min(k, nz), kzc = min(k, nz - 1);
-    const int qzH = vxy ? slab_q(k, nz, L) : -1;
-    const int qzE = slab_q(k, nz + 1, L);
+    const int qzH = vxy ? slab_q(k, nz, f.clo[kSlabH + 2], f.chi[kSlabH + 2]) : -1;
+    const int qzE = slab_q(k, nz + 1, f.clo[kSlabE + 2], f.chi[kSlabE + 2]);
 
     // ---- one batch of loads: E(k+1), H_A(k), coefficients, CPML memories ----
     float exn, eyn, ezn, etx = 0.0f, etz = 0.0f;
@@ -478,8 +482,8 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
     const float ha2 = ldu<T>(f.fin, f.foff[5], (k * ny + yY) * nx, tX);
     float p0 = 0, p1 = 0, p2 = 0, p3 = 0, p4 = 0, p5 = 0;
     if (qyH >= 0) {
-      p0 = ldu<P>(sa.base(), sa.in(0), (kk * L2 + qyH) * (nx + 1), pY);
-      p5 = ldu<P>(sa.base(), sa.in(5), (k * L2 + qyH) * nx, pX);
+      p0 = ldu<P>(sa.base(), sa.in(0), (kk * n2yH + qyH) * (nx + 1), pY);
+      p5 = ldu<P>(sa.base(), sa.in(5), (k * n2yH + qyH) * nx, pX);
     }
     if (qzH >= 0) {
       p1 = ldu<P>(sa.base(), sa.in(1), (qzH * ny + yY) * (nx + 1), pY);
@@ -487,8 +491,8 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
     }
     if constexpr (kXEdge) {
       if (qxH >= 0) {
-        if (!halo) p3 = ldu<P>(sa.base(), sa.in(3), (kk * (ny + 1) + yX) * L2, qH);
-        p4 = ldu<P>(sa.base(), sa.in(4), (k * ny + yY) * L2, qH);
+        if (!halo) p3 = ldu<P>(sa.base(), sa.in(3), (kk * (ny + 1) + yX) * n2xH, qH);
+        p4 = ldu<P>(sa.base(), sa.in(4), (k * ny + yY) * n2xH, qH);
       }
     }
     float p6 = 0, p7 = 0, p8 = 0, p9 = 0, p10 = 0, p11 = 0;
@@ -499,8 +503,8 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
         es2 = ldu<C>(f.cbase, f.coff[2], (kzc * (ny + 1) + yX) * (nx + 1), cY) * f.ratio_e;
       }
       if (qyE >= 0) {
-        p6 = ldu<P>(sa.base(), sa.in(6), (kx * L2 + qyE) * nx, pX);
-        p11 = ldu<P>(sa.base(), sa.in(11), (kzc * L2 + qyE) * (nx + 1), pY);
+        p6 = ldu<P>(sa.base(), sa.in(6), (kx * n2yE + qyE) * nx, pX);
+        p11 = ldu<P>(sa.base(), sa.in(11), (kzc * n2yE + qyE) * (nx + 1), pY);
       }
       if (qzE >= 0) {
         p7 = ldu<P>(sa.base(), sa.in(7), (qzE * (ny + 1) + yX) * nx, pX);
@@ -508,8 +512,8 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
       }
       if constexpr (kXEdge) {
         if (qxE >= 0) {
-          p9 = ldu<P>(sa.base(), sa.in(9), (kx * ny + yY) * L2, qE);
-          p10 = ldu<P>(sa.base(), sa.in(10), (kzc * (ny + 1) + yX) * L2, qE);
+          p9 = ldu<P>(sa.base(), sa.in(9), (kx * ny + yY) * n2xE, qE);
+          p10 = ldu<P>(sa.base(), sa.in(10), (kzc * (ny + 1) + yX) * n2xE, qE);
         }
       }
     }
@@ -529,23 +533,23 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
     const bool wh = own && st;
     float n;
     if (qyH >= 0) {
-      d0 = cpml_step(coef_u(sa.tab(), 0, L2, qyH), d0, p0, n);
-      if (wh && okhx) stu<P>(sa.base(), sa.out(0), (k * L2 + qyH) * (nx + 1), pY, n);
-      d5 = cpml_step(coef_u(sa.tab(), 5, L2, qyH), d5, p5, n);
-      if (wh && okhz) stu<P>(sa.base(), sa.out(5), (k * L2 + qyH) * nx, pX, n);
+      d0 = cpml_step(coef_u(sa.tab(), 0, TS, qyH), d0, p0, n);
+      if (wh && okhx) stu<P>(sa.base(), sa.out(0), (k * n2yH + qyH) * (nx + 1), pY, n);
+      d5 = cpml_step(coef_u(sa.tab(), 5, TS, qyH), d5, p5, n);
+      if (wh && okhz) stu<P>(sa.base(), sa.out(5), (k * n2yH + qyH) * nx, pX, n);
     }
     if (qzH >= 0) {
-      d1 = cpml_step(coef_u(sa.tab(), 1, L2, qzH), d1, p1, n);
+      d1 = cpml_step(coef_u(sa.tab(), 1, TS, qzH), d1, p1, n);
       if (wh && okhx) stu<P>(sa.base(), sa.out(1), (qzH * ny + yY) * (nx + 1), pY, n);
-      d2 = cpml_step(coef_u(sa.tab(), 2, L2, qzH), d2, p2, n);
+      d2 = cpml_step(coef_u(sa.tab(), 2, TS, qzH), d2, p2, n);
       if (wh && okhy) stu<P>(sa.base(), sa.out(2), (qzH * (ny + 1) + yX) * nx, pX, n);
     }
     if constexpr (kXEdge) {
       if (qxH >= 0) {
         d3 = cpml_step(c3, d3, p3, n);
-        if (wh && okhy) stu<P>(sa.base(), sa.out(3), (k * (ny + 1) + yX) * L2, qH, n);
+        if (wh && okhy) stu<P>(sa.base(), sa.out(3), (k * (ny + 1) + yX) * n2xH, qH, n);
         d4 = cpml_step(c4, d4, p4, n);
-        if (wh && okhz) stu<P>(sa.base(), sa.out(4), (k * ny + yY) * L2, qH, n);
+        if (wh && okhz) stu<P>(sa.base(), sa.out(4), (k * ny + yY) * n2xH, qH, n);
       }
     }
     float hx = hc[0] * ha0 - hc[3] * (d0 - d1);
@@ -603,23 +607,23 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
       const bool oky = own && k <= nz && vyY && vxY;
       const bool okz = own && k < nz && vyX && vxY;
       if (qyE >= 0) {
-        e0 = cpml_step(coef_u(sa.tab(), 6, L2, qyE), e0, p6, n);
-        if (okx) stu<P>(sa.base(), sa.out(6), (kx * L2 + qyE) * nx, pX, n);
-        e5 = cpml_step(coef_u(sa.tab(), 11, L2, qyE), e5, p11, n);
-        if (okz) stu<P>(sa.base(), sa.out(11), (kzc * L2 + qyE) * (nx + 1), pY, n);
+        e0 = cpml_step(coef_u(sa.tab(), 6, TS, qyE), e0, p6, n);
+        if (okx) stu<P>(sa.base(), sa.out(6), (kx * n2yE + qyE) * nx, pX, n);
+        e5 = cpml_step(coef_u(sa.tab(), 11, TS, qyE), e5, p11, n);
+        if (okz) stu<P>(sa.base(), sa.out(11), (kzc * n2yE + qyE) * (nx + 1), pY, n);
       }
       if (qzE >= 0) {
-        e1 = cpml_step(coef_u(sa.tab(), 7, L2, qzE), e1, p7, n);
+        e1 = cpml_step(coef_u(sa.tab(), 7, TS, qzE), e1, p7, n);
         if (okx) stu<P>(sa.base(), sa.out(7), (qzE * (ny + 1) + yX) * nx, pX, n);
-        e2 = cpml_step(coef_u(sa.tab(), 8, L2, qzE), e2, p8, n);
+        e2 = cpml_step(coef_u(sa.tab(), 8, TS, qzE), e2, p8, n);
         if (oky) stu<P>(sa.base(), sa.out(8), (qzE * ny + yY) * (nx + 1), pY, n);
       }
       if constexpr (kXEdge) {
         if (qxE >= 0) {
           e3 = cpml_step(c9, e3, p9, n);
-          if (oky) stu<P>(sa.base(), sa.out(9), (kx * ny + yY) * L2, qE, n);
+          if (oky) stu<P>(sa.base(), sa.out(9), (kx * ny + yY) * n2xE, qE, n);
           e4 = cpml_step(c10, e4, p10, n);
-          if (okz) stu<P>(sa.base(), sa.out(10), (kzc * (ny + 1) + yX) * L2, qE, n);
+          if (okz) stu<P>(sa.base(), sa.out(10), (kzc * (ny + 1) + yX) * n2xE, qE, n);
         }
       }
       float nex = ec[0] * ex + es0 * (e0 - e1);
@@ -761,7 +765,8 @@ struct PairLds {
 // x,y,z, E coefficients x,y,z, top-row E(k+1) x,z (wave 0).
 using PairPrefetch = float[kPairWaves][11][64];
 
-template <typename T, typename P, typename C, bool kEArr, bool kXEdge, bool kHSrc, bool kCore>
+template <typename T, typename P, typename C, bool kEArr, bool kXEdge, bool kHSrc, bool kCore,
+          bool kUni>
 __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, PairPrefetch* pf,
                                           int box, int blk) {
   const FusedParams& f = pp.f;
@@ -778,7 +783,17 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
   int bx, by, kz0, kz1;
   if (!decode_pair_box(pp.box[box], blk, bx, by, kz0, kz1)) return;
 
-  const int nz = f.nz, ny = f.ny, nx = f.nx, L = f.L, L2 = 2 * f.L;
+  const int nz = f.nz, ny = f.ny, nx = f.nx, L = f.L;
+  // CPML slab extents (lo, hi) per kSlabH/kSlabE + axis, the memories' extent
+  // along a slab axis and the coefficient table's row stride.  kUni: one
+  // thickness L on every face, so that all of them derive from L (fewer
+  // scalar registers held across the z-march: the general layout measured
+  // 1.6 % slower in the shell blocks)
+  auto clo = [&](int i) { return kUni ? L : f.clo[i]; };
+  auto chi = [&](int i) { return kUni ? L : f.chi[i]; };
+  auto n2 = [&](int i) { return kUni ? 2 * L : f.clo[i] + f.chi[i]; };
+  auto zq = [&](int k, int n, int i) { return slab_q(k, n, clo(i), chi(i)); };
+  auto tst = [&]() { return kUni ? 2 * L : f.tab_st; };
   const int m = f.metallic;
   const bool mf = m & kFront, mk = m & kBack, mb = m & kBottom, mt = m & kTop;
   const bool ml = m & kLeft, mr = m & kRight;
@@ -795,8 +810,8 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
     vxY = x >= 0 && x <= nx;
     xX = min(max(x, 0), nx - 1);
     xY = min(max(x, 0), nx);
-    qxH = vxX ? slab_q(x, nx, L) : -1;
-    qxE = vxY ? slab_q(x, nx + 1, L) : -1;
+    qxH = vxX ? slab_q(x, nx, clo(kSlabH), chi(kSlabH)) : -1;
+    qxE = vxY ? slab_q(x, nx + 1, clo(kSlabE), chi(kSlabE)) : -1;
     xface = (ml && x == 0) || (mr && x == nx);
   }
   const bool own = lane >= 2 && lane <= 61 && w >= 2 && w <= kPairRows + 1;
@@ -807,12 +822,12 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
   Coef c3{}, c4{}, c9{}, c10{};
   if constexpr (kXEdge) {
     if (qxH >= 0) {
-      c3 = coef_v(f.tab, 3, L2, qxH);
-      c4 = coef_v(f.tab, 4, L2, qxH);
+      c3 = coef_v(f.tab, 3, tst(), qxH);
+      c4 = coef_v(f.tab, 4, tst(), qxH);
     }
     if (qxE >= 0) {
-      c9 = coef_v(f.tab, 9, L2, qxE);
-      c10 = coef_v(f.tab, 10, L2, qxE);
+      c9 = coef_v(f.tab, 9, tst(), qxE);
+      c10 = coef_v(f.tab, 10, tst(), qxE);
     }
   }
 
@@ -821,8 +836,8 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
   const int y = j0 - 2 + w;
   const bool vyX = kCore || (y >= 0 && y <= ny), vyY = kCore || (y >= 0 && y < ny);
   const int yX = kCore ? y : min(max(y, 0), ny), yY = kCore ? y : min(max(y, 0), ny - 1);
-  const int qyH = (!kCore && vyY) ? slab_q(y, ny, L) : -1;
-  const int qyE = (!kCore && vyX) ? slab_q(y, ny + 1, L) : -1;
+  const int qyH = (!kCore && vyY) ? slab_q(y, ny, clo(kSlabH + 1), chi(kSlabH + 1)) : -1;
+  const int qyE = (!kCore && vyX) ? slab_q(y, ny + 1, clo(kSlabE + 1), chi(kSlabE + 1)) : -1;
   const bool y_pec_h = !kCore && ((mb && y == 0) || (mt && y == ny));
   const bool row_plain = kCore || (y >= 1 && y < ny);  // no y face, ghost or validity cut on this row
   const int yt = j0 - 2 + kW;
@@ -938,10 +953,10 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
   // CPML memories of plane k: H terms 0..5 at [t], E terms 6..11 at [t - 6]
   auto load_psi_h = [&](int k, float (&p)[6]) {
     const uint32_t kk = min(k, nz - 1), kh = min(k, nz);
-    const int qzH = k < nz ? slab_q(k, nz, L) : -1;
+    const int qzH = k < nz ? zq(k, nz, kSlabH + 2) : -1;
     if (qyH >= 0) {
-      p[0] = ldu<P>(sa.base(), sa.in(0), (kk * L2 + qyH) * (nx + 1), pY);
-      p[5] = ldu<P>(sa.base(), sa.in(5), (kh * L2 + qyH) * nx, pX);
+      p[0] = ldu<P>(sa.base(), sa.in(0), (kk * n2(kSlabH + 1) + qyH) * (nx + 1), pY);
+      p[5] = ldu<P>(sa.base(), sa.in(5), (kh * n2(kSlabH + 1) + qyH) * nx, pX);
     }
     if (qzH >= 0) {
       p[1] = ldu<P>(sa.base(), sa.in(1), (qzH * ny + yY) * (nx + 1), pY);
@@ -949,17 +964,17 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
     }
     if constexpr (kXEdge) {
       if (qxH >= 0) {
-        if (!halo) p[3] = ldu<P>(sa.base(), sa.in(3), (kk * (ny + 1) + yX) * L2, qH);
-        p[4] = ldu<P>(sa.base(), sa.in(4), (kh * ny + yY) * L2, qH);
+        if (!halo) p[3] = ldu<P>(sa.base(), sa.in(3), (kk * (ny + 1) + yX) * n2(kSlabH), qH);
+        p[4] = ldu<P>(sa.base(), sa.in(4), (kh * ny + yY) * n2(kSlabH), qH);
       }
     }
   };
   auto load_psi_e = [&](int k, float (&p)[6]) {
     const uint32_t kx = min(k, nz), kzc = min(k, nz - 1);
-    const int qzE = k <= nz ? slab_q(k, nz + 1, L) : -1;
+    const int qzE = k <= nz ? zq(k, nz + 1, kSlabE + 2) : -1;
     if (qyE >= 0) {
-      p[0] = ldu<P>(sa.base(), sa.in(6), (kx * L2 + qyE) * nx, pX);
-      p[5] = ldu<P>(sa.base(), sa.in(11), (kzc * L2 + qyE) * (nx + 1), pY);
+      p[0] = ldu<P>(sa.base(), sa.in(6), (kx * n2(kSlabE + 1) + qyE) * nx, pX);
+      p[5] = ldu<P>(sa.base(), sa.in(11), (kzc * n2(kSlabE + 1) + qyE) * (nx + 1), pY);
     }
     if (qzE >= 0) {
       p[1] = ldu<P>(sa.base(), sa.in(7), (qzE * (ny + 1) + yX) * nx, pX);
@@ -967,8 +982,8 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
     }
     if constexpr (kXEdge) {
       if (qxE >= 0) {
-        p[3] = ldu<P>(sa.base(), sa.in(9), (kx * ny + yY) * L2, qE);
-        p[4] = ldu<P>(sa.base(), sa.in(10), (kzc * (ny + 1) + yX) * L2, qE);
+        p[3] = ldu<P>(sa.base(), sa.in(9), (kx * ny + yY) * n2(kSlabE), qE);
+        p[4] = ldu<P>(sa.base(), sa.in(10), (kzc * (ny + 1) + yX) * n2(kSlabE), qE);
       }
     }
   };
@@ -981,7 +996,7 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
                       float ex_up, float ha0, float ha1, float ha2, const float (&pin)[6],
                       float (&pout)[6], float& hx, float& hy, float& hz) {
     const bool vxy = k < nz;
-    const int qzH = vxy ? slab_q(k, nz, L) : -1;
+    const int qzH = vxy ? zq(k, nz, kSlabH + 2) : -1;
     const float ez_r = lane_shift(ez, dn), ey_r = lane_shift(ey, dn);
     if (!kXEdge && k >= 1 && k < nz && row_plain && qyH < 0 && qzH < 0) {
       // no face and no CPML term: 1/dx folded into the coefficients
@@ -997,12 +1012,12 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
     float d4 = (ey_r - ey) * idx;                // dEy/dx
     float d5 = (ex_up - ex) * idx;               // dEx/dy
     if (qyH >= 0) {
-      d0 = cpml_step(coef_u(sa.tab(), 0, L2, qyH), d0, pin[0], pout[0]);
-      d5 = cpml_step(coef_u(sa.tab(), 5, L2, qyH), d5, pin[5], pout[5]);
+      d0 = cpml_step(coef_u(sa.tab(), 0, tst(), qyH), d0, pin[0], pout[0]);
+      d5 = cpml_step(coef_u(sa.tab(), 5, tst(), qyH), d5, pin[5], pout[5]);
     }
     if (qzH >= 0) {
-      d1 = cpml_step(coef_u(sa.tab(), 1, L2, qzH), d1, pin[1], pout[1]);
-      d2 = cpml_step(coef_u(sa.tab(), 2, L2, qzH), d2, pin[2], pout[2]);
+      d1 = cpml_step(coef_u(sa.tab(), 1, tst(), qzH), d1, pin[1], pout[1]);
+      d2 = cpml_step(coef_u(sa.tab(), 2, tst(), qzH), d2, pin[2], pout[2]);
     }
     if constexpr (kXEdge) {
       if (qxH >= 0) {
@@ -1032,7 +1047,7 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
                       float phy, float ex, float ey, float ez, float s0, float s1, float s2,
                       const float (&pin)[6], float (&pout)[6], float& nex, float& ney,
                       float& nez) {
-    const int qzE = k <= nz ? slab_q(k, nz + 1, L) : -1;
+    const int qzE = k <= nz ? zq(k, nz + 1, kSlabE + 2) : -1;
     const bool plain = !kXEdge && k >= 1 && k < nz && row_plain;
     const float hz_l = lane_shift(hz, up);
     const float hy_l = lane_shift(hy, up);
@@ -1064,12 +1079,12 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
       e4 = (hy - hy_l) * idx;
     }
     if (qyE >= 0) {
-      e0 = cpml_step(coef_u(sa.tab(), 6, L2, qyE), e0, pin[0], pout[0]);
-      e5 = cpml_step(coef_u(sa.tab(), 11, L2, qyE), e5, pin[5], pout[5]);
+      e0 = cpml_step(coef_u(sa.tab(), 6, tst(), qyE), e0, pin[0], pout[0]);
+      e5 = cpml_step(coef_u(sa.tab(), 11, tst(), qyE), e5, pin[5], pout[5]);
     }
     if (qzE >= 0) {
-      e1 = cpml_step(coef_u(sa.tab(), 7, L2, qzE), e1, pin[1], pout[1]);
-      e2 = cpml_step(coef_u(sa.tab(), 8, L2, qzE), e2, pin[2], pout[2]);
+      e1 = cpml_step(coef_u(sa.tab(), 7, tst(), qzE), e1, pin[1], pout[1]);
+      e2 = cpml_step(coef_u(sa.tab(), 8, tst(), qzE), e2, pin[2], pout[2]);
     }
     if constexpr (kXEdge) {
       if (qxE >= 0) {
@@ -1112,10 +1127,10 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
   };
   auto store_psi_h = [&](int k, const float (&p)[6]) {
     const bool okx = k < nz && vyY && vxY, oky = k < nz && vyX && vxX, okz = k <= nz && vyY && vxX;
-    const int qzH = k < nz ? slab_q(k, nz, L) : -1;
+    const int qzH = k < nz ? zq(k, nz, kSlabH + 2) : -1;
     if (qyH >= 0) {
-      if (okx) stu<P>(sa.base(), sa.out(0), (k * L2 + qyH) * (nx + 1), pY, p[0]);
-      if (okz) stu<P>(sa.base(), sa.out(5), (k * L2 + qyH) * nx, pX, p[5]);
+      if (okx) stu<P>(sa.base(), sa.out(0), (k * n2(kSlabH + 1) + qyH) * (nx + 1), pY, p[0]);
+      if (okz) stu<P>(sa.base(), sa.out(5), (k * n2(kSlabH + 1) + qyH) * nx, pX, p[5]);
     }
     if (qzH >= 0) {
       if (okx) stu<P>(sa.base(), sa.out(1), (qzH * ny + yY) * (nx + 1), pY, p[1]);
@@ -1123,17 +1138,17 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
     }
     if constexpr (kXEdge) {
       if (qxH >= 0) {
-        if (oky) stu<P>(sa.base(), sa.out(3), (k * (ny + 1) + yX) * L2, qH, p[3]);
-        if (okz) stu<P>(sa.base(), sa.out(4), (k * ny + yY) * L2, qH, p[4]);
+        if (oky) stu<P>(sa.base(), sa.out(3), (k * (ny + 1) + yX) * n2(kSlabH), qH, p[3]);
+        if (okz) stu<P>(sa.base(), sa.out(4), (k * ny + yY) * n2(kSlabH), qH, p[4]);
       }
     }
   };
   auto store_psi_e = [&](int k, const float (&p)[6]) {
     const bool okx = k <= nz && vyX && vxX, oky = k <= nz && vyY && vxY, okz = k < nz && vyX && vxY;
-    const int qzE = k <= nz ? slab_q(k, nz + 1, L) : -1;
+    const int qzE = k <= nz ? zq(k, nz + 1, kSlabE + 2) : -1;
     if (qyE >= 0) {
-      if (okx) stu<P>(sa.base(), sa.out(6), (k * L2 + qyE) * nx, pX, p[0]);
-      if (okz) stu<P>(sa.base(), sa.out(11), (k * L2 + qyE) * (nx + 1), pY, p[5]);
+      if (okx) stu<P>(sa.base(), sa.out(6), (k * n2(kSlabE + 1) + qyE) * nx, pX, p[0]);
+      if (okz) stu<P>(sa.base(), sa.out(11), (k * n2(kSlabE + 1) + qyE) * (nx + 1), pY, p[5]);
     }
     if (qzE >= 0) {
       if (okx) stu<P>(sa.base(), sa.out(7), (qzE * (ny + 1) + yX) * nx, pX, p[1]);
@@ -1141,8 +1156,8 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
     }
     if constexpr (kXEdge) {
       if (qxE >= 0) {
-        if (oky) stu<P>(sa.base(), sa.out(9), (k * ny + yY) * L2, qE, p[3]);
-        if (okz) stu<P>(sa.base(), sa.out(10), (k * (ny + 1) + yX) * L2, qE, p[4]);
+        if (oky) stu<P>(sa.base(), sa.out(9), (k * ny + yY) * n2(kSlabE), qE, p[3]);
+        if (okz) stu<P>(sa.base(), sa.out(10), (k * (ny + 1) + yX) * n2(kSlabE), qE, p[4]);
       }
     }
   };
@@ -1602,16 +1617,16 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
 // so that they do not form the tail), then the x-interior ones.  Both
 // variants run one 16-wave workgroup per CU, so merging costs no occupancy and
 // the two boxes overlap instead of running back to back.
-template <typename T, typename P, typename C, bool kEArr, bool kHSrc>
+template <typename T, typename P, typename C, bool kEArr, bool kHSrc, bool kUni>
 __global__ void __launch_bounds__(64 * kPairWaves) pair_step(PairParams p) {
   __shared__ PairLds lds;
   __shared__ PairPrefetch pf;
   int b = blockIdx.x, i = 0;
   while (i + 1 < p.nbox && b >= p.box[i].nblk) b -= p.box[i++].nblk;
   const int kind = p.box[i].xedge;  // 0 x-interior, 1 x-edge, 2 core
-  if (kind == 1) pair_body<T, P, C, kEArr, true, kHSrc, false>(p, lds, nullptr, i, b);
-  else if (kind == 2) pair_body<T, P, C, kEArr, false, kHSrc, true>(p, lds, &pf, i, b);
-  else pair_body<T, P, C, kEArr, false, kHSrc, false>(p, lds, nullptr, i, b);
+  if (kind == 1) pair_body<T, P, C, kEArr, true, kHSrc, false, kUni>(p, lds, nullptr, i, b);
+  else if (kind == 2) pair_body<T, P, C, kEArr, false, kHSrc, true, true>(p, lds, &pf, i, b);
+  else pair_body<T, P, C, kEArr, false, kHSrc, false, kUni>(p, lds, nullptr, i, b);
 }
 
 // The core kernel: x-interior tiles whose rows all lie clear of the y faces
@@ -1622,7 +1637,23 @@ template <typename T, typename P, typename C, bool kEArr, bool kHSrc>
 __global__ void __launch_bounds__(64 * kPairWaves) pair_core(PairParams p) {
   __shared__ PairLds lds;
   __shared__ PairPrefetch pf;
-  pair_body<T, P, C, kEArr, false, kHSrc, true>(p, lds, &pf, 0, blockIdx.x);
+  pair_body<T, P, C, kEArr, false, kHSrc, true, true>(p, lds, &pf, 0, blockIdx.x);
+}
+
+// One CPML thickness on every face (or none): the kUni variant of the shell.
+bool cpml_uniform(const FusedParams& f) {
+  if (f.L == 0) return true;
+  for (int i = 0; i < 6; ++i)
+    if (f.clo[i] != f.L || f.chi[i] != f.L) return false;
+  return f.tab_st == 2 * f.L;
+}
+
+template <typename T, typename P, typename C, bool kEArr, bool kHSrc>
+void launch_pair_shell(hipStream_t s, const PairParams& p, dim3 grid, dim3 block) {
+  if (cpml_uniform(p.f))
+    hipLaunchKernelGGL((pair_step<T, P, C, kEArr, kHSrc, true>), grid, block, 0, s, p);
+  else
+    hipLaunchKernelGGL((pair_step<T, P, C, kEArr, kHSrc, false>), grid, block, 0, s, p);
 }
 
 template <typename T, typename P, typename C>
@@ -1639,11 +1670,11 @@ void launch_pair_wg(hipStream_t s, const PairParams& p, int nblocks) {
       else hipLaunchKernelGGL((pair_core<T, P, float, false, false>), grid, block, 0, s, p);
     }
   } else if (hsrc) {
-    if (earr) hipLaunchKernelGGL((pair_step<T, P, C, true, true>), grid, block, 0, s, p);
-    else hipLaunchKernelGGL((pair_step<T, P, float, false, true>), grid, block, 0, s, p);
+    if (earr) launch_pair_shell<T, P, C, true, true>(s, p, grid, block);
+    else launch_pair_shell<T, P, float, false, true>(s, p, grid, block);
   } else {
-    if (earr) hipLaunchKernelGGL((pair_step<T, P, C, true, false>), grid, block, 0, s, p);
-    else hipLaunchKernelGGL((pair_step<T, P, float, false, false>), grid, block, 0, s, p);
+    if (earr) launch_pair_shell<T, P, C, true, false>(s, p, grid, block);
+    else launch_pair_shell<T, P, float, false, false>(s, p, grid, block);
   }
 }
 

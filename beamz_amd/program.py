@@ -79,7 +79,7 @@ class CompiledProgram:
     flags: int
     psi_dtype: torch.dtype
     abi_resolution: float = None   # cell size handed to the kernels (1 when folded)
-    cpml_table: torch.Tensor = None  # [terms, 3, 2L] when every CPML term has thickness L
+    cpml_table: torch.Tensor = None  # [terms, 3, st]: each term's a, b, 1/kappa, st its longest slab
     field_scale: dict = None       # storage scale of the E and H fields
     eps_cells: np.ndarray = field(repr=False, default=None)
     eps_site: dict = field(repr=False, default=None)
@@ -257,10 +257,16 @@ def compile_program(permittivity, resolution, dt, *, boundaries=(), sources=(), 
         if first > 0 and all(t.low == first and t.high == first for t in allterms):
             thickness = first
     table = None
-    if thickness:
-        # one contiguous [H terms, E terms] x (a, b, 1/kappa) x 2L table; the
-        # 36 term arrays are views of it (the fused kernel reads the table)
-        tab = np.stack([np.stack([t.a, t.b, t.inv_kappa]) for t in cpml.h_terms + cpml.e_terms])
+    if cpml is not None:
+        # one contiguous [H terms, E terms] x (a, b, 1/kappa) x st table, st the
+        # longest packed slab (2L for one thickness on every face); each term
+        # array is the leading low + high entries of its row (the fused kernel
+        # reads the table)
+        allterms = cpml.h_terms + cpml.e_terms
+        st = max(max(t.low + t.high for t in allterms), 1)
+        tab = np.zeros((len(allterms), 3, st))
+        for i, t in enumerate(allterms):
+            tab[i, :, :t.low + t.high] = np.stack([t.a, t.b, t.inv_kappa])
         table = _dev(tab.astype(np.float32), device)
 
     def term_payload(terms, first):
@@ -270,10 +276,8 @@ def compile_program(permittivity, resolution, dt, *, boundaries=(), sources=(), 
                             1 if t.sign > 0 else -1) for t in terms], dtype=np.int32)
         arrays = []
         for i, t in enumerate(terms):
-            if table is not None:
-                arrays += [table[first + i, 0], table[first + i, 1], table[first + i, 2]]
-            else:
-                arrays += [_dev(t.a, device), _dev(t.b, device), _dev(t.inv_kappa, device)]
+            n = t.low + t.high
+            arrays += [table[first + i, 0, :n], table[first + i, 1, :n], table[first + i, 2, :n]]
         while len(arrays) < 18:
             arrays.append(torch.zeros(0, device=device))
         return _dev(meta, device, torch.int32), tuple(arrays)

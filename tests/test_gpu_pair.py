@@ -53,16 +53,26 @@ def _monitors(grid):
             bz.FluxMonitor("z", 3, freqs, window="hann", t_start=2.5 * dt, t_end=9.5 * dt)]
 
 
+# CPML layouts: every face; none; one face per axis (PEC opposite); x and y
+# faces only (no z CPML memories at all)
+BOUNDARIES = {"cpml": dict(cpml_cells=3), "pec": {},
+              "onesided": dict(cpml_cells=3, faces=("left", "back", "top"),
+                               metallic=("right", "front", "bottom")),
+              "noz": dict(cpml_cells=2, faces=("left", "right", "front", "back"),
+                          metallic=("bottom", "top"))}
+
+
 @pytest.mark.parametrize("chunk", [12, 5])
 @pytest.mark.parametrize("kz", ["1", "3", "16"])
 @pytest.mark.parametrize("grid,bc", [((13, 17, 130), "cpml"), ((9, 28, 64), "pec"),
-                                     ((11, 14, 197), "cpml")])
+                                     ((11, 14, 197), "cpml"), ((13, 17, 130), "onesided"),
+                                     ((11, 14, 197), "noz")])
 def test_pair_parity(grid, bc, kz, chunk, monkeypatch):
     _require_gpu()
     from beamz_amd import native
     monkeypatch.setenv("BZH_PAIR_KZ", kz)
     steps = 12
-    kw = {"cpml": dict(cpml_cells=3), "pec": {}}[bc]
+    kw = BOUNDARIES[bc]
     sim, prog, state, prob, ost = _setup(grid, seed=71, steps=steps, sources=_sources(grid, steps),
                                          monitors=_monitors(grid), **kw)
     print(f"case pair_parity grid={grid} bc={bc} kz={kz} chunk={chunk}", flush=True)
@@ -170,7 +180,7 @@ def _core_sources(grid, steps):
             o.Source("Hz", "h", (nz // 2 - 3, ny // 2 + 2, 70), c3, wave)]
 
 
-@pytest.mark.parametrize("bc", ["cpml", "pec"])
+@pytest.mark.parametrize("bc", ["cpml", "pec", "onesided", "noz"])
 def test_pair_core_parity(bc, monkeypatch):
     """Grids large enough for the core launch (x-interior tiles whose rows
     and chunks lie clear of every face and CPML slab, run by the lean core
@@ -180,9 +190,9 @@ def test_pair_core_parity(bc, monkeypatch):
     _require_gpu()
     import beamz_amd as bz
     from beamz_amd import native
-    grid = (40, 48, 140) if bc == "cpml" else (30, 44, 130)
+    grid = (30, 44, 130) if bc == "pec" else (40, 48, 140)
     steps = 12
-    kw = {"cpml": dict(cpml_cells=3), "pec": {}}[bc]
+    kw = BOUNDARIES[bc]
     freqs = np.array([1.8e14, 2.0e14, 2.2e14])
     runs = []
     for core in ("1", "0"):

@@ -260,8 +260,8 @@ def test_fused_step_parity_odd_shapes(grid, kz, bc, rows, order, monkeypatch):
     torch.cuda.synchronize()
     from beamz_amd import native
     # one-sided CPML has no uniform thickness (boundary code bits 8+ are 0,
-    # runtime.py:43-53) and runs the per-phase schedule
-    assert native.last_program_fused() == (bc != "mixed")
+    # runtime.py:43-53): the fused kernels take its extents from the term rows
+    assert native.last_program_fused()
     o.run(prob, ost, steps)
     _compare_state(prog, state, prob, ost)
 
