@@ -150,14 +150,16 @@ struct MidSlab {
   int32_t post;
   int32_t mlo[3], mhi[3]; // PEC masks of post slabs (z, y, x)
 };
-constexpr int kMaxMidSlabs = 8;
-constexpr int kMaxHSlabs = 4;
+// (16 slabs between the steps of a pair: a 3D TF/SF mode launch emits 4
+// H-timed and 4 E-timed slabs per step, 12 over a pair)
+constexpr int kMaxMidSlabs = 16;
+constexpr int kMaxHSlabs = 8;
 
 // One fused leapfrog step reading bank A and writing bank B (fields and CPML
 // memories).  Requires uniform H coefficients, isotropic metrics and, with
-// CPML, one slab thickness L on every term, the 36 term arrays packed as
-// tab[12][3][2L] (H terms then E terms; a, b, 1/kappa) and each psi of bank B
-// at psi + psi_delta (elements).
+// CPML, the 36 term arrays packed as the rows of tab[12][3][tab_st] (H terms
+// then E terms; a, b, 1/kappa; the terms on one axis of one phase share their
+// slab extents) and each psi of bank B at psi + psi_delta (elements).
 constexpr int kSlabH = 0, kSlabE = 3;
 struct FusedParams {
   const void* e_in[3];

@@ -149,6 +149,10 @@ def test_mode_source_3d_slabs_match_the_oracle(launch, direction):
     ost = o.init_state(prob)
     print(f"case mode3d oracle launch={launch} dir={direction} grid={grid} steps={steps}", flush=True)
     res = sim.run()
+    from beamz_amd import native
+    # the residual slabs (4 H-timed and 4 E-timed per step for TF/SF) run
+    # inside the fused kernels, two steps per launch
+    assert native.last_program_fused() and native.last_program_pairs() > 0
     o.run(prob, ost, steps)
     for c in prob.shapes:
         _close(c, getattr(res.state, c.lower()).cpu().numpy(), ost[c.lower()])
