@@ -1228,27 +1228,41 @@ bool fused_cpml_layout(const PhaseIn& h, const PhaseIn& e, const bzh_buffer* con
   return true;
 }
 
-// One scalar base for a group of arrays: the lowest address, when every array
-// ends within 4 GiB of it; offsets in bytes.  False otherwise.
-bool group_base(const bzh_buffer* const* bufs, int n, const char*& base, uint32_t* off) {
+// One scalar base for a group of arrays, the lowest address, and each array's
+// byte offset from it.  ``wide`` reports a group that spans 4 GiB or more:
+// the kernels then form 64-bit offsets (their wide variants).  False for a
+// null array or an array of 4 GiB or more (the kernels index within an array
+// with 32-bit byte offsets).
+bool group_base(const bzh_buffer* const* bufs, int n, const char*& base, uint64_t* off,
+                bool& wide) {
+  constexpr uintptr_t k4G = uintptr_t{1} << 32;
   uintptr_t lo = UINTPTR_MAX, hi = 0;
   for (int i = 0; i < n; ++i) {
     if (bufs[i] == nullptr || bufs[i]->data == nullptr) return false;
     const uintptr_t p = reinterpret_cast<uintptr_t>(bufs[i]->data);
+    const uintptr_t bytes = static_cast<uintptr_t>(numel(*bufs[i]) * elem_bytes(bufs[i]->element_type));
+    if (bytes >= k4G) return false;
     lo = std::min(lo, p);
-    hi = std::max(hi, p + static_cast<uintptr_t>(numel(*bufs[i]) * elem_bytes(bufs[i]->element_type)));
+    hi = std::max(hi, p + bytes);
   }
-  if (n == 0 || hi - lo >= (uintptr_t{1} << 32)) return false;
+  if (n == 0) return false;
+  wide = hi - lo >= k4G;
   base = reinterpret_cast<const char*>(lo);
-  for (int i = 0; i < n; ++i)
-    off[i] = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(bufs[i]->data) - lo);
+  for (int i = 0; i < n; ++i) off[i] = reinterpret_cast<uintptr_t>(bufs[i]->data) - lo;
   return true;
 }
 
 // The fields of a bank in the kernel's order (Ex, Ey, Ez, Hx, Hy, Hz).
-bool bank_base(const bzh_buffer* const* bank, const char*& base, uint32_t off[6]) {
+bool bank_base(const bzh_buffer* const* bank, const char*& base, uint64_t off[6], bool& wide) {
   const bzh_buffer* order[6] = {bank[kEx], bank[kEy], bank[kEz], bank[kHx], bank[kHy], bank[kHz]};
-  return group_base(order, 6, base, off);
+  return group_base(order, 6, base, off, wide);
+}
+
+// The wide (64-bit offset) kernels for every fused launch, also where 32-bit
+// offsets would do (BZH_WIDE=1; a test switch for the wide variants).
+bool force_wide() {
+  const char* env = std::getenv("BZH_WIDE");
+  return env && std::string(env) == "1";
 }
 
 bool metrics_empty(const PhaseIn& ph) {
@@ -1283,23 +1297,31 @@ bool run_program(hipStream_t s, ProgramIn& in) {
   for (int q = 3; q < 6; ++q)
     if (in.e.mat[q] != nullptr && numel(*in.e.mat[q]) != 1) ++n_e_arrays;
   coef_ok = coef_ok && (n_e_arrays == 0 || n_e_arrays == 3);
-  // based 32-bit addressing: each bank's fields within 4 GiB of one base, at
-  // the same offsets in both banks; the E coefficient arrays likewise
+  // based addressing: each bank's fields at offsets from one base, the same
+  // offsets in both banks; the E coefficient arrays likewise.  A group that
+  // spans 4 GiB or more takes the wide kernels (64-bit offsets)
   FusedParams cpml_probe;
   std::memset(&cpml_probe, 0, sizeof(cpml_probe));
   bool based = g.is3d && two_banks;
-  uint32_t foff_a[6] = {}, foff_b[6] = {};
+  uint64_t foff_a[6] = {}, foff_b[6] = {};
+  bool wide_a = false, wide_b = false, wide_c = false;
   const char* base_a = nullptr;
   const char* base_b = nullptr;
   if (based) {
-    based = bank_base(in.bank_a, base_a, foff_a) && bank_base(in.bank_b, base_b, foff_b) &&
+    based = bank_base(in.bank_a, base_a, foff_a, wide_a) &&
+            bank_base(in.bank_b, base_b, foff_b, wide_b) &&
             std::memcmp(foff_a, foff_b, sizeof(foff_a)) == 0;
   }
   if (based && n_e_arrays == 3) {
     const bzh_buffer* src[3] = {in.e.mat[3], in.e.mat[4], in.e.mat[5]};
-    based = group_base(src, 3, cpml_probe.cbase, cpml_probe.coff);
+    based = group_base(src, 3, cpml_probe.cbase, cpml_probe.coff64, wide_c);
   }
-  std::memcpy(cpml_probe.foff, foff_a, sizeof(foff_a));
+  cpml_probe.wide = (wide_a || wide_b || wide_c || force_wide()) ? 1 : 0;
+  for (int q = 0; q < 6; ++q) {
+    cpml_probe.foff64[q] = foff_a[q];
+    cpml_probe.foff[q] = static_cast<uint32_t>(foff_a[q]);  // (used when not wide)
+  }
+  for (int q = 0; q < 3; ++q) cpml_probe.coff[q] = static_cast<uint32_t>(cpml_probe.coff64[q]);
   // CPML extents other than one thickness on every face: from the term rows
   int32_t cpml_ext_rows[24];
   const int32_t* cpml_ext = nullptr;
@@ -1309,7 +1331,10 @@ bool run_program(hipStream_t s, ProgramIn& in) {
   const bool fused = two_banks && (in.flags & BZH_FLAG_FUSED_STEP) && g.is3d &&
                      h_slabs <= kMaxHSlabs && (h_slabs == 0 || in.step != nullptr) &&
                      coef_ok && based && metrics_empty(in.h) && metrics_empty(in.e) &&
-                     fused_cpml_layout(in.h, in.e, in.psi_b, in.uniform_l, cpml_ext, cpml_probe);
+                     fused_cpml_layout(in.h, in.e, in.psi_b, in.uniform_l, cpml_ext, cpml_probe) &&
+                     // the wide kernels: float32 storage and 15-row single-step tiles
+                     (!cpml_probe.wide ||
+                      (g.dtype == BZH_F32 && !cpml_probe.psi_bf16 && fused_rows() == 15));
   const int32_t* step0 = in.step ? static_cast<const int32_t*>(in.step->data) : nullptr;
   int slots = 1;
   float* stage = nullptr;
@@ -1367,13 +1392,13 @@ bool run_program(hipStream_t s, ProgramIn& in) {
     const int order[6] = {kEx, kEy, kEz, kHx, kHy, kHz};
     for (int q = 0; q < 6; ++q) {
       const bzh_buffer& b = *in.bank_a[order[q]];
-      span = std::max(span, static_cast<size_t>(cpml_probe.foff[q]) +
+      span = std::max(span, static_cast<size_t>(cpml_probe.foff64[q]) +
                                 static_cast<size_t>(numel(b)) * elem_bytes(b.element_type));
     }
     pws = pair_workspace(s, span);
     for (int q = 0; q < 6; ++q) {
       cbufs[order[q]] = *in.bank_a[order[q]];
-      cbufs[order[q]].data = pws.bank + cpml_probe.foff[q];
+      cbufs[order[q]].data = pws.bank + cpml_probe.foff64[q];
     }
     for (int q = 0; q < 6; ++q) cbank[q] = &cbufs[q];
   }

@@ -199,9 +199,10 @@ struct FusedParams {
   int32_t rows;          // tile height (y rows) the box was cut with
   int32_t coef_f16;      // E source coefficient arrays stored as float16
   // Based addressing, the only form the kernels use (the pointers above are
-  // host-side inputs): every array of a group lies within 4 GiB above the
-  // group's base, so one scalar base pair serves the group and each access
-  // adds a 32-bit byte offset.  The read bank's fields sit at fin + foff[c],
+  // host-side inputs): one scalar base pair serves a group and each access
+  // adds a byte offset, 32-bit when every array of the group lies within
+  // 4 GiB above the group's base (64-bit when ``wide``; the CPML memories are
+  // always within 4 GiB).  The read bank's fields sit at fin + foff[c],
   // the written bank's at fout + foff[c] (c: Ex, Ey, Ez, Hx, Hy, Hz; both
   // banks share the layout); E source coefficient arrays at cbase + coff[c];
   // CPML memory t is read at pbase + poff[t] and written at
@@ -211,6 +212,13 @@ struct FusedParams {
   uint32_t foff[6];
   const char* cbase;
   uint32_t coff[3];
+  // 1: a group spans 4 GiB or more (each array under 4 GiB), so the kernels
+  // form 64-bit offsets (the wide variants: float32 storage only) from
+  // foff64 and coff64, the same offsets at full width (the 32-bit ones are
+  // then unused)
+  int32_t wide;
+  uint64_t foff64[6];
+  uint64_t coff64[3];
   const char* pbase;
   uint32_t poff[12];
   uint32_t pdelta;

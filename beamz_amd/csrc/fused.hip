@@ -159,32 +159,33 @@ __device__ __forceinline__ float cpml_step(const Coef& c, float d, float psi, fl
 template <typename T>
 using gptr = __attribute__((address_space(1))) T*;
 
-// Group base (kernel argument, uniform) plus a 32-bit byte offset formed next
-// to the access: ``aoff + row * sizeof(T) + xb`` with the array's offset in
-// its group, the uniform row start and the per-lane byte offset xb.  Every
-// group spans less than 4 GiB (checked on the host), so the offset fits and
-// the access selects ``global_load v, v_off, s[base]`` with one scalar pair
-// per group.
-template <typename T>
-__device__ __forceinline__ gptr<const char> at(const char* base, uint32_t aoff, uint32_t row,
+// Group base (kernel argument, uniform) plus a byte offset formed next to
+// the access: ``aoff + row * sizeof(T) + xb`` with the array's offset in its
+// group, the uniform row start and the per-lane byte offset xb.  O is the
+// offset type: uint32_t when every group spans less than 4 GiB (checked on
+// the host), so that the access selects ``global_load v, v_off, s[base]``
+// with one scalar pair per group; uint64_t in the wide variants (banks of
+// 4 GiB or more, each array under 4 GiB).
+template <typename T, typename O>
+__device__ __forceinline__ gptr<const char> at(const char* base, O aoff, uint32_t row,
                                                uint32_t xb) {
-  const uint32_t off = aoff + row * static_cast<uint32_t>(sizeof(T)) + xb;
+  const O off = aoff + static_cast<O>(row) * static_cast<O>(sizeof(T)) + static_cast<O>(xb);
   return (gptr<const char>)(base) + off;
 }
 __device__ __forceinline__ float ld_g16(gptr<const char> p, bool bf) {
   const uint16_t u = *reinterpret_cast<gptr<const uint16_t>>(p);
   return bf ? __uint_as_float(static_cast<uint32_t>(u) << 16) : __half2float(__ushort_as_half(u));
 }
-template <typename T>
-__device__ __forceinline__ float ldu(const char* base, uint32_t aoff, uint32_t row, uint32_t xb) {
-  gptr<const char> p = at<T>(base, aoff, row, xb);
+template <typename T, typename O>
+__device__ __forceinline__ float ldu(const char* base, O aoff, uint32_t row, uint32_t xb) {
+  gptr<const char> p = at<T, O>(base, aoff, row, xb);
   if constexpr (sizeof(T) == 4) return *reinterpret_cast<gptr<const float>>(p);
   else return ld_g16(p, std::is_same<T, bf16_t>::value);
 }
-template <typename T>
-__device__ __forceinline__ void stu(const char* base, uint32_t aoff, uint32_t row, uint32_t xb,
+template <typename T, typename O>
+__device__ __forceinline__ void stu(const char* base, O aoff, uint32_t row, uint32_t xb,
                                     float v) {
-  gptr<char> p = (gptr<char>)(at<T>(base, aoff, row, xb));
+  gptr<char> p = (gptr<char>)(at<T, O>(base, aoff, row, xb));
   if constexpr (sizeof(T) == 4) {
     *reinterpret_cast<gptr<float>>(p) = v;
   } else {
@@ -197,17 +198,17 @@ __device__ __forceinline__ void stu(const char* base, uint32_t aoff, uint32_t ro
 }
 // Access at ``base + soff + voff``: a uniform byte offset (plane and array)
 // plus a per-lane byte offset (row and column) held in a vector register.
-template <typename T>
-__device__ __forceinline__ float ldo(const char* base, uint32_t soff, uint32_t voff) {
-  return ldu<T>(base, soff, 0u, voff);
+template <typename T, typename O>
+__device__ __forceinline__ float ldo(const char* base, O soff, uint32_t voff) {
+  return ldu<T, O>(base, soff, 0u, voff);
 }
 
 // The raw bits of one element at ``base + soff + voff`` (zero-extended), for
 // a load whose value is needed only an iteration later: the conversion to
 // float32 (``from_raw``) happens at the use, so the load does not wait here.
-template <typename T>
-__device__ __forceinline__ uint32_t ldraw(const char* base, uint32_t soff, uint32_t voff) {
-  gptr<const char> p = at<T>(base, soff, 0u, voff);
+template <typename T, typename O>
+__device__ __forceinline__ uint32_t ldraw(const char* base, O soff, uint32_t voff) {
+  gptr<const char> p = at<T, O>(base, soff, 0u, voff);
   if constexpr (sizeof(T) == 4) return *reinterpret_cast<gptr<const uint32_t>>(p);
   else return *reinterpret_cast<gptr<const uint16_t>>(p);
 }
@@ -221,26 +222,26 @@ __device__ __forceinline__ float from_raw(uint32_t u) {
 // One dword per lane from ``base + off`` (global) into an LDS row (lane l at
 // row[l]), asynchronously: the wave goes on, and the compiler waits for the
 // transfer before the row is read.
-__device__ __forceinline__ void dma_row(const char* base, uint32_t soff, uint32_t voff,
-                                        float* row) {
+template <typename O>
+__device__ __forceinline__ void dma_row(const char* base, O soff, uint32_t voff, float* row) {
   __builtin_amdgcn_global_load_lds(
-      (__attribute__((address_space(1))) void*)((gptr<const char>)(base) + (soff + voff)),
+      (__attribute__((address_space(1))) void*)((gptr<const char>)(base) + (soff + static_cast<O>(voff))),
       (__attribute__((address_space(3))) void*)(row), 4, 0, 0);
 }
 
 // Field store: bfloat16 fields take stochastic rounding when ``sr`` (uniform)
 // is set, every other storage type the ordinary conversion.
-template <typename T>
-__device__ __forceinline__ void stf(const char* base, uint32_t aoff, uint32_t row, uint32_t xb,
+template <typename T, typename O>
+__device__ __forceinline__ void stf(const char* base, O aoff, uint32_t row, uint32_t xb,
                                     float v, bool sr, SrState& rng) {
   if constexpr (std::is_same<T, bf16_t>::value) {
     if (sr) {
-      gptr<char> p = (gptr<char>)(at<T>(base, aoff, row, xb));
+      gptr<char> p = (gptr<char>)(at<T, O>(base, aoff, row, xb));
       *reinterpret_cast<gptr<uint16_t>>(p) = bf16_bits_sr(v, rng);
       return;
     }
   }
-  stu<T>(base, aoff, row, xb, v);
+  stu<T, O>(base, aoff, row, xb, v);
 }
 // The seed of a launch's rounding sequence: the lane's global index and the
 // absolute time step (*step0 + kstep, or kstep without a step counter).
@@ -341,9 +342,20 @@ __device__ __forceinline__ bool slab_hits_wave(kslab& ms, int y, int x) {
   return __builtin_amdgcn_readfirstlane(__any(hit)) != 0;
 }
 
-template <typename T, typename P, typename C, int kR, bool kEArr, bool kXEdge, bool kHSrc>
+template <typename T, typename P, typename C, int kR, bool kEArr, bool kXEdge, bool kHSrc,
+          bool kWide>
 __device__ __forceinline__ void fused_body(const FusedParams& f) {
   const SlabArgs<sizeof(T) == 2 && !kXEdge> sa{f};
+  // field and E coefficient offsets in their groups: 32-bit, 64-bit in the
+  // wide variant (kWide: a bank of 4 GiB or more)
+  auto fo = [&](int c) {
+    if constexpr (kWide) return f.foff64[c];
+    else return f.foff[c];
+  };
+  auto co = [&](int c) {
+    if constexpr (kWide) return f.coff64[c];
+    else return f.coff[c];
+  };
   __shared__ float lE[2][kR + 2][2][64];
   __shared__ float lH[2][kR + 1][2][64];
   const int lane = threadIdx.x & 63;
@@ -425,9 +437,9 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
   auto load_e = [&](int k, float& ex, float& ey, float& ez) {
     const bool vxy = k <= nz, vz = k < nz;
     const uint32_t kx = min(k, nz), kz = min(k, nz - 1);
-    const float a = ldu<T>(f.fin, f.foff[0], (kx * (ny + 1) + yX) * nx, tX);
-    const float c = ldu<T>(f.fin, f.foff[1], (kx * ny + yY) * (nx + 1), tY);
-    const float d = ldu<T>(f.fin, f.foff[2], (kz * (ny + 1) + yX) * (nx + 1), tY);
+    const float a = ldu<T>(f.fin, fo(0), (kx * (ny + 1) + yX) * nx, tX);
+    const float c = ldu<T>(f.fin, fo(1), (kx * ny + yY) * (nx + 1), tY);
+    const float d = ldu<T>(f.fin, fo(2), (kz * (ny + 1) + yX) * (nx + 1), tY);
     ex = (vxy && vyX && vxX) ? a : 0.0f;
     ey = (vxy && vyY && vxY) ? c : 0.0f;
     ez = (vz && vyX && vxY) ? d : 0.0f;
@@ -435,8 +447,8 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
   auto load_top = [&](int k, float& ex, float& ez) {
     const bool vxy = k <= nz, vz = k < nz;
     const uint32_t kx = min(k, nz), kz = min(k, nz - 1);
-    const float a = ldu<T>(f.fin, f.foff[0], (kx * (ny + 1) + ytX) * nx, tX);
-    const float d = ldu<T>(f.fin, f.foff[2], (kz * (ny + 1) + ytX) * (nx + 1), tY);
+    const float a = ldu<T>(f.fin, fo(0), (kx * (ny + 1) + ytX) * nx, tX);
+    const float d = ldu<T>(f.fin, fo(2), (kz * (ny + 1) + ytX) * (nx + 1), tY);
     ex = (vxy && vtX && vxX) ? a : 0.0f;
     ez = (vz && vtX && vxY) ? d : 0.0f;
   };
@@ -477,9 +489,9 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
     float exn, eyn, ezn, etx = 0.0f, etz = 0.0f;
     load_e(k + 1, exn, eyn, ezn);
     if (halo) load_top(k + 1, etx, etz);
-    const float ha0 = ldu<T>(f.fin, f.foff[3], (kk * ny + yY) * (nx + 1), tY);
-    const float ha1 = halo ? 0.0f : ldu<T>(f.fin, f.foff[4], (kk * (ny + 1) + yX) * nx, tX);
-    const float ha2 = ldu<T>(f.fin, f.foff[5], (k * ny + yY) * nx, tX);
+    const float ha0 = ldu<T>(f.fin, fo(3), (kk * ny + yY) * (nx + 1), tY);
+    const float ha1 = halo ? 0.0f : ldu<T>(f.fin, fo(4), (kk * (ny + 1) + yX) * nx, tX);
+    const float ha2 = ldu<T>(f.fin, fo(5), (k * ny + yY) * nx, tX);
     float p0 = 0, p1 = 0, p2 = 0, p3 = 0, p4 = 0, p5 = 0;
     if (qyH >= 0) {
       p0 = ldu<P>(sa.base(), sa.in(0), (kk * n2yH + qyH) * (nx + 1), pY);
@@ -498,9 +510,9 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
     float p6 = 0, p7 = 0, p8 = 0, p9 = 0, p10 = 0, p11 = 0;
     if (!halo && st) {
       if constexpr (kEArr) {
-        es0 = ldu<C>(f.cbase, f.coff[0], (kx * (ny + 1) + yX) * nx, cX) * f.ratio_e;
-        es1 = ldu<C>(f.cbase, f.coff[1], (kx * ny + yY) * (nx + 1), cY) * f.ratio_e;
-        es2 = ldu<C>(f.cbase, f.coff[2], (kzc * (ny + 1) + yX) * (nx + 1), cY) * f.ratio_e;
+        es0 = ldu<C>(f.cbase, co(0), (kx * (ny + 1) + yX) * nx, cX) * f.ratio_e;
+        es1 = ldu<C>(f.cbase, co(1), (kx * ny + yY) * (nx + 1), cY) * f.ratio_e;
+        es2 = ldu<C>(f.cbase, co(2), (kzc * (ny + 1) + yX) * (nx + 1), cY) * f.ratio_e;
       }
       if (qyE >= 0) {
         p6 = ldu<P>(sa.base(), sa.in(6), (kx * n2yE + qyE) * nx, pX);
@@ -579,9 +591,9 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
       lE[bb ^ 1][kR + 1][1][lane] = etx;
     }
     if (wh) {
-      if (okhx) stf<T>(f.fout, f.foff[3], (k * ny + yY) * (nx + 1), tY, hx, sr, rng);
-      if (okhy) stf<T>(f.fout, f.foff[4], (k * (ny + 1) + yX) * nx, tX, hy, sr, rng);
-      if (okhz) stf<T>(f.fout, f.foff[5], (k * ny + yY) * nx, tX, hz, sr, rng);
+      if (okhx) stf<T>(f.fout, fo(3), (k * ny + yY) * (nx + 1), tY, hx, sr, rng);
+      if (okhy) stf<T>(f.fout, fo(4), (k * (ny + 1) + yX) * nx, tX, hy, sr, rng);
+      if (okhz) stf<T>(f.fout, fo(5), (k * ny + yY) * nx, tX, hz, sr, rng);
     }
     __syncthreads();
 
@@ -634,9 +646,9 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
       if (zf || yf) nex = 0.0f;
       if (zf || (kXEdge && xface)) ney = 0.0f;
       if (yf || (kXEdge && xface)) nez = 0.0f;
-      if (okx) stf<T>(f.fout, f.foff[0], (kx * (ny + 1) + yX) * nx, tX, nex, sr, rng);
-      if (oky) stf<T>(f.fout, f.foff[1], (kx * ny + yY) * (nx + 1), tY, ney, sr, rng);
-      if (okz) stf<T>(f.fout, f.foff[2], (kzc * (ny + 1) + yX) * (nx + 1), tY, nez, sr, rng);
+      if (okx) stf<T>(f.fout, fo(0), (kx * (ny + 1) + yX) * nx, tX, nex, sr, rng);
+      if (oky) stf<T>(f.fout, fo(1), (kx * ny + yY) * (nx + 1), tY, ney, sr, rng);
+      if (okz) stf<T>(f.fout, fo(2), (kzc * (ny + 1) + yX) * (nx + 1), tY, nez, sr, rng);
     }
     phx = hx;
     phy = hy;
@@ -654,33 +666,49 @@ __device__ __forceinline__ void fused_body(const FusedParams& f) {
 // forced to 8 it spills to scratch and runs slower (2.68 ms against 2.53 ms).
 // kHSrc: the variant that applies H-timing source slabs (mode-source runs);
 // the others carry none of its code or registers.
-template <typename T, typename P, typename C, int kR, bool kEArr, bool kHSrc>
+template <typename T, typename P, typename C, int kR, bool kEArr, bool kHSrc, bool kWide>
 __global__ void __launch_bounds__(64 * (kR + 1)) fused_interior(FusedParams f) {
-  fused_body<T, P, C, kR, kEArr, false, kHSrc>(f);
+  fused_body<T, P, C, kR, kEArr, false, kHSrc, kWide>(f);
 }
 template <typename T, typename P, typename C, int kR, bool kEArr, bool kHSrc>
 __global__ void __launch_bounds__(64 * (kR + 1)) __attribute__((amdgpu_waves_per_eu(8, 8)))
 fused_interior_half(FusedParams f) {
-  fused_body<T, P, C, kR, kEArr, false, kHSrc>(f);
+  fused_body<T, P, C, kR, kEArr, false, kHSrc, false>(f);
 }
-template <typename T, typename P, typename C, int kR, bool kEArr, bool kHSrc>
+template <typename T, typename P, typename C, int kR, bool kEArr, bool kHSrc, bool kWide>
 __global__ void __launch_bounds__(64 * (kR + 1)) fused_edge(FusedParams f) {
-  fused_body<T, P, C, kR, kEArr, true, kHSrc>(f);
+  fused_body<T, P, C, kR, kEArr, true, kHSrc, kWide>(f);
 }
 
 template <typename T, typename P, typename C, int kR, bool kHSrc>
 void launch_wg_h(hipStream_t s, const FusedParams& p, int nblocks) {
   const dim3 block(64 * (kR + 1), 1, 1), grid(nblocks, 1, 1);
   const bool earr = p.esrc_st[0] != 0;
+  if (p.wide) {
+    // 64-bit offsets: float32 fields, memories and coefficients, 15-row tiles
+    if constexpr (std::is_same<T, float>::value && std::is_same<P, float>::value &&
+                  std::is_same<C, float>::value && kR == 15) {
+      if (p.xedge) {
+        if (earr) hipLaunchKernelGGL((fused_edge<T, P, C, kR, true, kHSrc, true>), grid, block, 0, s, p);
+        else hipLaunchKernelGGL((fused_edge<T, P, C, kR, false, kHSrc, true>), grid, block, 0, s, p);
+      } else {
+        if (earr) hipLaunchKernelGGL((fused_interior<T, P, C, kR, true, kHSrc, true>), grid, block, 0, s, p);
+        else hipLaunchKernelGGL((fused_interior<T, P, C, kR, false, kHSrc, true>), grid, block, 0, s, p);
+      }
+      return;
+    }
+    throw Failure{"the wide fused step takes float32 fields, CPML memories and coefficients "
+                  "and 15-row tiles"};
+  }
   if (p.xedge) {
-    if (earr) hipLaunchKernelGGL((fused_edge<T, P, C, kR, true, kHSrc>), grid, block, 0, s, p);
-    else hipLaunchKernelGGL((fused_edge<T, P, float, kR, false, kHSrc>), grid, block, 0, s, p);
+    if (earr) hipLaunchKernelGGL((fused_edge<T, P, C, kR, true, kHSrc, false>), grid, block, 0, s, p);
+    else hipLaunchKernelGGL((fused_edge<T, P, float, kR, false, kHSrc, false>), grid, block, 0, s, p);
   } else if constexpr (sizeof(T) == 2) {
     if (earr) hipLaunchKernelGGL((fused_interior_half<T, P, C, kR, true, kHSrc>), grid, block, 0, s, p);
     else hipLaunchKernelGGL((fused_interior_half<T, P, float, kR, false, kHSrc>), grid, block, 0, s, p);
   } else {
-    if (earr) hipLaunchKernelGGL((fused_interior<T, P, C, kR, true, kHSrc>), grid, block, 0, s, p);
-    else hipLaunchKernelGGL((fused_interior<T, P, float, kR, false, kHSrc>), grid, block, 0, s, p);
+    if (earr) hipLaunchKernelGGL((fused_interior<T, P, C, kR, true, kHSrc, false>), grid, block, 0, s, p);
+    else hipLaunchKernelGGL((fused_interior<T, P, float, kR, false, kHSrc, false>), grid, block, 0, s, p);
   }
 }
 
@@ -766,10 +794,20 @@ struct PairLds {
 using PairPrefetch = float[kPairWaves][11][64];
 
 template <typename T, typename P, typename C, bool kEArr, bool kXEdge, bool kHSrc, bool kCore,
-          bool kUni>
+          bool kUni, bool kWide>
 __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, PairPrefetch* pf,
                                           int box, int blk) {
   const FusedParams& f = pp.f;
+  // field and E coefficient offsets in their groups: 32-bit, 64-bit in the
+  // wide variant (kWide: a bank of 4 GiB or more)
+  auto fo = [&](int c) {
+    if constexpr (kWide) return f.foff64[c];
+    else return f.foff[c];
+  };
+  auto co = [&](int c) {
+    if constexpr (kWide) return f.coff64[c];
+    else return f.coff[c];
+  };
   const SlabArgs<kXEdge> sa{f};  // x-edge tiles: reloaded (x-slab terms raise the pressure)
   constexpr int kW = kPairWaves;
   auto& lE1 = lds.e1;
@@ -934,9 +972,9 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
   auto load_e = [&](int k, float& ex, float& ey, float& ez) {
     const bool vxy = k <= nz, vz = k < nz;
     const uint32_t kx = min(k, nz), kz = min(k, nz - 1);
-    const float a = ldu<T>(f.fin, f.foff[0], (kx * (ny + 1) + yX) * nx, tX);
-    const float c = ldu<T>(f.fin, f.foff[1], (kx * ny + yY) * (nx + 1), tY);
-    const float d = ldu<T>(f.fin, f.foff[2], (kz * (ny + 1) + yX) * (nx + 1), tY);
+    const float a = ldu<T>(f.fin, fo(0), (kx * (ny + 1) + yX) * nx, tX);
+    const float c = ldu<T>(f.fin, fo(1), (kx * ny + yY) * (nx + 1), tY);
+    const float d = ldu<T>(f.fin, fo(2), (kz * (ny + 1) + yX) * (nx + 1), tY);
     ex = (vxy && vyX && vxX) ? a : 0.0f;
     ey = (vxy && vyY && vxY) ? c : 0.0f;
     ez = (vz && vyX && vxY) ? d : 0.0f;
@@ -944,8 +982,8 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
   auto load_top = [&](int k, float& ex, float& ez) {
     const bool vxy = k <= nz, vz = k < nz;
     const uint32_t kx = min(k, nz), kz = min(k, nz - 1);
-    const float a = ldu<T>(f.fin, f.foff[0], (kx * (ny + 1) + ytX) * nx, tX);
-    const float d = ldu<T>(f.fin, f.foff[2], (kz * (ny + 1) + ytX) * (nx + 1), tY);
+    const float a = ldu<T>(f.fin, fo(0), (kx * (ny + 1) + ytX) * nx, tX);
+    const float d = ldu<T>(f.fin, fo(2), (kz * (ny + 1) + ytX) * (nx + 1), tY);
     ex = (vxy && vtX && vxX) ? a : 0.0f;
     ez = (vz && vtX && vxY) ? d : 0.0f;
   };
@@ -1115,15 +1153,15 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
   if (sr) rng = sr_launch_seed(f, 1);
   auto store_h = [&](const char* out, int k, float hx, float hy, float hz) {
     const bool s = sr && out == f.fout;
-    if (k < nz && vyY && vxY) stf<T>(out, f.foff[3], (k * ny + yY) * (nx + 1), tY, hx, s, rng);
-    if (k < nz && vyX && vxX) stf<T>(out, f.foff[4], (k * (ny + 1) + yX) * nx, tX, hy, s, rng);
-    if (k <= nz && vyY && vxX) stf<T>(out, f.foff[5], (k * ny + yY) * nx, tX, hz, s, rng);
+    if (k < nz && vyY && vxY) stf<T>(out, fo(3), (k * ny + yY) * (nx + 1), tY, hx, s, rng);
+    if (k < nz && vyX && vxX) stf<T>(out, fo(4), (k * (ny + 1) + yX) * nx, tX, hy, s, rng);
+    if (k <= nz && vyY && vxX) stf<T>(out, fo(5), (k * ny + yY) * nx, tX, hz, s, rng);
   };
   auto store_e = [&](const char* out, int k, float ex, float ey, float ez) {
     const bool s = sr && out == f.fout;
-    if (k <= nz && vyX && vxX) stf<T>(out, f.foff[0], (k * (ny + 1) + yX) * nx, tX, ex, s, rng);
-    if (k <= nz && vyY && vxY) stf<T>(out, f.foff[1], (k * ny + yY) * (nx + 1), tY, ey, s, rng);
-    if (k < nz && vyX && vxY) stf<T>(out, f.foff[2], (k * (ny + 1) + yX) * (nx + 1), tY, ez, s, rng);
+    if (k <= nz && vyX && vxX) stf<T>(out, fo(0), (k * (ny + 1) + yX) * nx, tX, ex, s, rng);
+    if (k <= nz && vyY && vxY) stf<T>(out, fo(1), (k * ny + yY) * (nx + 1), tY, ey, s, rng);
+    if (k < nz && vyX && vxY) stf<T>(out, fo(2), (k * (ny + 1) + yX) * (nx + 1), tY, ez, s, rng);
   };
   auto store_psi_h = [&](int k, const float (&p)[6]) {
     const bool okx = k < nz && vyY && vxY, oky = k < nz && vyX && vxX, okz = k <= nz && vyY && vxX;
@@ -1288,21 +1326,21 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
             etz = pfw[10][lane];
           }
         } else {
-          exn = ldo<T>(f.fin, f.foff[0] + zA + pA, oA);
-          eyn = ldo<T>(f.fin, f.foff[1] + zB + pB, oB);
-          ezn = ldo<T>(f.fin, f.foff[2] + zC + pC, oC);
+          exn = ldo<T>(f.fin, fo(0) + zA + pA, oA);
+          eyn = ldo<T>(f.fin, fo(1) + zB + pB, oB);
+          ezn = ldo<T>(f.fin, fo(2) + zC + pC, oC);
           if (halo) {
-            etx = ldo<T>(f.fin, f.foff[0] + zA + pA, oAt);
-            etz = ldo<T>(f.fin, f.foff[2] + zC + pC, oCt);
+            etx = ldo<T>(f.fin, fo(0) + zA + pA, oAt);
+            etz = ldo<T>(f.fin, fo(2) + zC + pC, oCt);
           }
-          ha0 = ldo<T>(f.fin, f.foff[3] + zB, oB);
-          if (!halo) ha1 = ldo<T>(f.fin, f.foff[4] + zA, oA);
-          ha2 = ldo<T>(f.fin, f.foff[5] + zD, oD);
+          ha0 = ldo<T>(f.fin, fo(3) + zB, oB);
+          if (!halo) ha1 = ldo<T>(f.fin, fo(4) + zA, oA);
+          ha2 = ldo<T>(f.fin, fo(5) + zD, oD);
           if constexpr (kEArr) {
             if (gE1) {
-              s0 = ldo<C>(f.cbase, f.coff[0] + k * nA * szC, cA);
-              s1 = ldo<C>(f.cbase, f.coff[1] + k * nB * szC, cB);
-              s2 = ldo<C>(f.cbase, f.coff[2] + k * nC * szC, cC);
+              s0 = ldo<C>(f.cbase, co(0) + k * nA * szC, cA);
+              s1 = ldo<C>(f.cbase, co(1) + k * nB * szC, cB);
+              s2 = ldo<C>(f.cbase, co(2) + k * nC * szC, cC);
             }
           }
         }
@@ -1328,9 +1366,9 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
             cap_store(5, k, hz);
           } else {
             const char* cm = pair_kargs()->cmid;
-            stf<T>(cm, f.foff[3] + zB, 0u, oB, hx, false, rng);
-            stf<T>(cm, f.foff[4] + zA, 0u, oA, hy, false, rng);
-            stf<T>(cm, f.foff[5] + zD, 0u, oD, hz, false, rng);
+            stf<T>(cm, fo(3) + zB, 0u, oB, hx, false, rng);
+            stf<T>(cm, fo(4) + zA, 0u, oA, hy, false, rng);
+            stf<T>(cm, fo(5) + zD, 0u, oD, hz, false, rng);
           }
         }
       }
@@ -1343,22 +1381,22 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
         const int kn = k + 1;
         pf_ok = kn <= k_hi && kn <= kz1;
         if (pf_ok) {
-          rp[0] = ldraw<T>(f.fin, f.foff[0] + zA + 2 * pA, oA);
-          rp[1] = ldraw<T>(f.fin, f.foff[1] + zB + 2 * pB, oB);
-          rp[2] = ldraw<T>(f.fin, f.foff[2] + zC + 2 * pC, oC);
-          rp[3] = ldraw<T>(f.fin, f.foff[3] + zB + pB, oB);
-          if (!halo) rp[4] = ldraw<T>(f.fin, f.foff[4] + zA + pA, oA);
-          rp[5] = ldraw<T>(f.fin, f.foff[5] + zD + pD, oD);
+          rp[0] = ldraw<T>(f.fin, fo(0) + zA + 2 * pA, oA);
+          rp[1] = ldraw<T>(f.fin, fo(1) + zB + 2 * pB, oB);
+          rp[2] = ldraw<T>(f.fin, fo(2) + zC + 2 * pC, oC);
+          rp[3] = ldraw<T>(f.fin, fo(3) + zB + pB, oB);
+          if (!halo) rp[4] = ldraw<T>(f.fin, fo(4) + zA + pA, oA);
+          rp[5] = ldraw<T>(f.fin, fo(5) + zD + pD, oD);
           if constexpr (kEArr) {
             if (w >= 1) {
-              rp[6] = ldraw<C>(f.cbase, f.coff[0] + kn * nA * szC, cA);
-              rp[7] = ldraw<C>(f.cbase, f.coff[1] + kn * nB * szC, cB);
-              rp[8] = ldraw<C>(f.cbase, f.coff[2] + kn * nC * szC, cC);
+              rp[6] = ldraw<C>(f.cbase, co(0) + kn * nA * szC, cA);
+              rp[7] = ldraw<C>(f.cbase, co(1) + kn * nB * szC, cB);
+              rp[8] = ldraw<C>(f.cbase, co(2) + kn * nC * szC, cC);
             }
           }
           if (halo) {
-            rp[9] = ldraw<T>(f.fin, f.foff[0] + zA + 2 * pA, oAt);
-            rp[10] = ldraw<T>(f.fin, f.foff[2] + zC + 2 * pC, oCt);
+            rp[9] = ldraw<T>(f.fin, fo(0) + zA + 2 * pA, oAt);
+            rp[10] = ldraw<T>(f.fin, fo(2) + zC + 2 * pC, oCt);
           }
         }
       }
@@ -1366,22 +1404,22 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
         const int kn = k + 1;
         pf_ok = kn <= k_hi && kn <= kz1;  // (lean iterations are contiguous)
         if (pf_ok) {
-          dma_row(f.fin, f.foff[0] + zA + 2 * pA, oA, pfw[0]);
-          dma_row(f.fin, f.foff[1] + zB + 2 * pB, oB, pfw[1]);
-          dma_row(f.fin, f.foff[2] + zC + 2 * pC, oC, pfw[2]);
-          dma_row(f.fin, f.foff[3] + zB + pB, oB, pfw[3]);
-          if (!halo) dma_row(f.fin, f.foff[4] + zA + pA, oA, pfw[4]);
-          dma_row(f.fin, f.foff[5] + zD + pD, oD, pfw[5]);
+          dma_row(f.fin, fo(0) + zA + 2 * pA, oA, pfw[0]);
+          dma_row(f.fin, fo(1) + zB + 2 * pB, oB, pfw[1]);
+          dma_row(f.fin, fo(2) + zC + 2 * pC, oC, pfw[2]);
+          dma_row(f.fin, fo(3) + zB + pB, oB, pfw[3]);
+          if (!halo) dma_row(f.fin, fo(4) + zA + pA, oA, pfw[4]);
+          dma_row(f.fin, fo(5) + zD + pD, oD, pfw[5]);
           if constexpr (kEArr) {
             if (w >= 1) {
-              dma_row(f.cbase, f.coff[0] + kn * nA * szC, cA, pfw[6]);
-              dma_row(f.cbase, f.coff[1] + kn * nB * szC, cB, pfw[7]);
-              dma_row(f.cbase, f.coff[2] + kn * nC * szC, cC, pfw[8]);
+              dma_row(f.cbase, co(0) + kn * nA * szC, cA, pfw[6]);
+              dma_row(f.cbase, co(1) + kn * nB * szC, cB, pfw[7]);
+              dma_row(f.cbase, co(2) + kn * nC * szC, cC, pfw[8]);
             }
           }
           if (halo) {
-            dma_row(f.fin, f.foff[0] + zA + 2 * pA, oAt, pfw[9]);
-            dma_row(f.fin, f.foff[2] + zC + 2 * pC, oCt, pfw[10]);
+            dma_row(f.fin, fo(0) + zA + 2 * pA, oAt, pfw[9]);
+            dma_row(f.fin, fo(2) + zC + 2 * pC, oCt, pfw[10]);
           }
         }
       }
@@ -1403,9 +1441,9 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
             cap_store(2, k, nez);
           } else {
             const char* cm = pair_kargs()->cmid;
-            stf<T>(cm, f.foff[0] + zA, 0u, oA, nex, false, rng);
-            stf<T>(cm, f.foff[1] + zB, 0u, oB, ney, false, rng);
-            stf<T>(cm, f.foff[2] + zC, 0u, oC, nez, false, rng);
+            stf<T>(cm, fo(0) + zA, 0u, oA, nex, false, rng);
+            stf<T>(cm, fo(1) + zB, 0u, oB, ney, false, rng);
+            stf<T>(cm, fo(2) + zC, 0u, oC, nez, false, rng);
           }
         }
         if (swave) mid_sources(1, k, nex, ney, nez);
@@ -1426,9 +1464,9 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
         lH2[bb][w][0][lane] = gz;
         lH2[bb][w][1][lane] = gx;
         if (own && k - 1 >= kz0) {
-          stf<T>(f.fout, f.foff[3] + zB - pB, 0u, oB, gx, sr, rng);
-          stf<T>(f.fout, f.foff[4] + zA - pA, 0u, oA, gy, sr, rng);
-          stf<T>(f.fout, f.foff[5] + zD - pD, 0u, oD, gz, sr, rng);
+          stf<T>(f.fout, fo(3) + zB - pB, 0u, oB, gx, sr, rng);
+          stf<T>(f.fout, fo(4) + zA - pA, 0u, oA, gy, sr, rng);
+          stf<T>(f.fout, fo(5) + zD - pD, 0u, oD, gz, sr, rng);
         }
       }
 
@@ -1441,9 +1479,9 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
         const float fy = ec[1] * e2y + (sb1 * idx) * ((g2x - g3x) - (g2z - hz_l));
         const float fz = ec[2] * e2z + (sb2 * idx) * ((g2y - hy_l) - (g2x - hx_dn));
         if (own) {
-          stf<T>(f.fout, f.foff[0] + zA - 2 * pA, 0u, oA, fx, sr, rng);
-          stf<T>(f.fout, f.foff[1] + zB - 2 * pB, 0u, oB, fy, sr, rng);
-          stf<T>(f.fout, f.foff[2] + zC - 2 * pC, 0u, oC, fz, sr, rng);
+          stf<T>(f.fout, fo(0) + zA - 2 * pA, 0u, oA, fx, sr, rng);
+          stf<T>(f.fout, fo(1) + zB - 2 * pB, 0u, oB, fy, sr, rng);
+          stf<T>(f.fout, fo(2) + zC - 2 * pC, 0u, oC, fz, sr, rng);
         }
       }
 
@@ -1491,17 +1529,17 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
       load_e(k + 1, exn, eyn, ezn);
       if (halo) load_top(k + 1, etx, etz);
       const uint32_t kk = min(k, nz - 1), kh = min(k, nz);
-      const float ha0 = ldu<T>(f.fin, f.foff[3], (kk * ny + yY) * (nx + 1), tY);
-      const float ha1 = halo ? 0.0f : ldu<T>(f.fin, f.foff[4], (kk * (ny + 1) + yX) * nx, tX);
-      const float ha2 = ldu<T>(f.fin, f.foff[5], (kh * ny + yY) * nx, tX);
+      const float ha0 = ldu<T>(f.fin, fo(3), (kk * ny + yY) * (nx + 1), tY);
+      const float ha1 = halo ? 0.0f : ldu<T>(f.fin, fo(4), (kk * (ny + 1) + yX) * nx, tX);
+      const float ha2 = ldu<T>(f.fin, fo(5), (kh * ny + yY) * nx, tX);
       float ph[6] = {};
       load_psi_h(k, ph);
       if (gE1) {
         if constexpr (kEArr) {
           const uint32_t kx = min(k, nz), kzc = min(k, nz - 1);
-          s0 = ldu<C>(f.cbase, f.coff[0], (kx * (ny + 1) + yX) * nx, cX);
-          s1 = ldu<C>(f.cbase, f.coff[1], (kx * ny + yY) * (nx + 1), cY);
-          s2 = ldu<C>(f.cbase, f.coff[2], (kzc * (ny + 1) + yX) * (nx + 1), cY);
+          s0 = ldu<C>(f.cbase, co(0), (kx * (ny + 1) + yX) * nx, cX);
+          s1 = ldu<C>(f.cbase, co(1), (kx * ny + yY) * (nx + 1), cY);
+          s2 = ldu<C>(f.cbase, co(2), (kzc * (ny + 1) + yX) * (nx + 1), cY);
         }
         load_psi_e(k, pe);
       }
@@ -1617,27 +1655,27 @@ __device__ __forceinline__ void pair_body(const PairParams& pp, PairLds& lds, Pa
 // so that they do not form the tail), then the x-interior ones.  Both
 // variants run one 16-wave workgroup per CU, so merging costs no occupancy and
 // the two boxes overlap instead of running back to back.
-template <typename T, typename P, typename C, bool kEArr, bool kHSrc, bool kUni>
+template <typename T, typename P, typename C, bool kEArr, bool kHSrc, bool kUni, bool kWide>
 __global__ void __launch_bounds__(64 * kPairWaves) pair_step(PairParams p) {
   __shared__ PairLds lds;
   __shared__ PairPrefetch pf;
   int b = blockIdx.x, i = 0;
   while (i + 1 < p.nbox && b >= p.box[i].nblk) b -= p.box[i++].nblk;
   const int kind = p.box[i].xedge;  // 0 x-interior, 1 x-edge, 2 core
-  if (kind == 1) pair_body<T, P, C, kEArr, true, kHSrc, false, kUni>(p, lds, nullptr, i, b);
-  else if (kind == 2) pair_body<T, P, C, kEArr, false, kHSrc, true, true>(p, lds, &pf, i, b);
-  else pair_body<T, P, C, kEArr, false, kHSrc, false, kUni>(p, lds, nullptr, i, b);
+  if (kind == 1) pair_body<T, P, C, kEArr, true, kHSrc, false, kUni, kWide>(p, lds, nullptr, i, b);
+  else if (kind == 2) pair_body<T, P, C, kEArr, false, kHSrc, true, true, kWide>(p, lds, &pf, i, b);
+  else pair_body<T, P, C, kEArr, false, kHSrc, false, kUni, kWide>(p, lds, nullptr, i, b);
 }
 
 // The core kernel: x-interior tiles whose rows all lie clear of the y faces
 // and y slabs, on z chunks whose iterations all lie clear of the z faces and z
 // slabs.  Only the lean iteration is compiled, so its registers are not
 // sized by the general one.
-template <typename T, typename P, typename C, bool kEArr, bool kHSrc>
+template <typename T, typename P, typename C, bool kEArr, bool kHSrc, bool kWide>
 __global__ void __launch_bounds__(64 * kPairWaves) pair_core(PairParams p) {
   __shared__ PairLds lds;
   __shared__ PairPrefetch pf;
-  pair_body<T, P, C, kEArr, false, kHSrc, true, true>(p, lds, &pf, 0, blockIdx.x);
+  pair_body<T, P, C, kEArr, false, kHSrc, true, true, kWide>(p, lds, &pf, 0, blockIdx.x);
 }
 
 // One CPML thickness on every face (or none): the kUni variant of the shell.
@@ -1648,12 +1686,37 @@ bool cpml_uniform(const FusedParams& f) {
   return f.tab_st == 2 * f.L;
 }
 
+// The wide variants (64-bit offsets) exist for float32 fields, memories and
+// coefficients; their shell takes every CPML layout (no kUni variant).
+template <typename T, typename P, typename C>
+constexpr bool kWideTypes =
+    std::is_same<T, float>::value && std::is_same<P, float>::value && std::is_same<C, float>::value;
+
 template <typename T, typename P, typename C, bool kEArr, bool kHSrc>
 void launch_pair_shell(hipStream_t s, const PairParams& p, dim3 grid, dim3 block) {
+  if (p.f.wide) {
+    if constexpr (kWideTypes<T, P, C>) {
+      hipLaunchKernelGGL((pair_step<T, P, C, kEArr, kHSrc, false, true>), grid, block, 0, s, p);
+      return;
+    }
+    throw Failure{"the wide pair step takes float32 fields, CPML memories and coefficients"};
+  }
   if (cpml_uniform(p.f))
-    hipLaunchKernelGGL((pair_step<T, P, C, kEArr, kHSrc, true>), grid, block, 0, s, p);
+    hipLaunchKernelGGL((pair_step<T, P, C, kEArr, kHSrc, true, false>), grid, block, 0, s, p);
   else
-    hipLaunchKernelGGL((pair_step<T, P, C, kEArr, kHSrc, false>), grid, block, 0, s, p);
+    hipLaunchKernelGGL((pair_step<T, P, C, kEArr, kHSrc, false, false>), grid, block, 0, s, p);
+}
+
+template <typename T, typename P, typename C, bool kEArr, bool kHSrc>
+void launch_pair_core(hipStream_t s, const PairParams& p, dim3 grid, dim3 block) {
+  if (p.f.wide) {
+    if constexpr (kWideTypes<T, P, C>) {
+      hipLaunchKernelGGL((pair_core<T, P, C, kEArr, kHSrc, true>), grid, block, 0, s, p);
+      return;
+    }
+    throw Failure{"the wide pair step takes float32 fields, CPML memories and coefficients"};
+  }
+  hipLaunchKernelGGL((pair_core<T, P, C, kEArr, kHSrc, false>), grid, block, 0, s, p);
 }
 
 template <typename T, typename P, typename C>
@@ -1663,11 +1726,11 @@ void launch_pair_wg(hipStream_t s, const PairParams& p, int nblocks) {
   const bool hsrc = p.msrc_n > p.msrc_e;  // H-timing slabs in the list
   if (p.core) {
     if (hsrc) {
-      if (earr) hipLaunchKernelGGL((pair_core<T, P, C, true, true>), grid, block, 0, s, p);
-      else hipLaunchKernelGGL((pair_core<T, P, float, false, true>), grid, block, 0, s, p);
+      if (earr) launch_pair_core<T, P, C, true, true>(s, p, grid, block);
+      else launch_pair_core<T, P, float, false, true>(s, p, grid, block);
     } else {
-      if (earr) hipLaunchKernelGGL((pair_core<T, P, C, true, false>), grid, block, 0, s, p);
-      else hipLaunchKernelGGL((pair_core<T, P, float, false, false>), grid, block, 0, s, p);
+      if (earr) launch_pair_core<T, P, C, true, false>(s, p, grid, block);
+      else launch_pair_core<T, P, float, false, false>(s, p, grid, block);
     }
   } else if (hsrc) {
     if (earr) launch_pair_shell<T, P, C, true, true>(s, p, grid, block);

@@ -187,8 +187,9 @@ const char* bzh_targets(void);
 const char* bzh_last_error(void);
 /* Diagnostic with no counterpart in the reference: 1 when the last program
  * call on this thread ran the fused two-bank step kernels, 0 when it ran the
- * per-phase schedule (e.g. an H-timed source, per-cell H coefficients, or
- * field banks whose arrays do not each lie within 4 GiB of one base). */
+ * per-phase schedule (e.g. more than eight H-timed source slabs, per-cell H
+ * coefficients, a field array of 4 GiB or more, or a bank over 4 GiB in
+ * 2-byte storage). */
 int32_t bzh_last_program_fused(void);
 /* Diagnostic with no counterpart in the reference: the number of two-step
  * (pair) launches in the last program call on this thread.  The pair kernel
