@@ -144,20 +144,32 @@ def solve_line_modes(eps_line, omega, dx, count, target_neff):
     ``eps_line``: the ``count`` eigenpairs whose n_eff^2 lie nearest
     ``target_neff^2`` (the reference's shift-invert selection,
     ``_scipy.py:115-124``), by decreasing effective index.  Returns
-    (neff[count], ez[count, N]) with unit-norm real profiles."""
+    (neff[count], ez[count, N]) with unit-norm real profiles, each signed so
+    that its largest-magnitude sample is positive.
+
+    The eigenvalues come first (all of them, no vectors); the ``count``
+    nearest the target are a contiguous run of the ascending spectrum, and
+    only their vectors are computed (bisection and inverse iteration), which
+    on a 600-node line takes a few milliseconds against 0.25 s for the full
+    eigendecomposition."""
     from scipy.linalg import eigh_tridiagonal
     eps = np.asarray(eps_line, dtype=np.float64)
     n = eps.size
     k0 = float(omega) / LIGHT_SPEED
     main = -2.0 / dx ** 2 + k0 ** 2 * eps
     off = np.full(n - 1, 1.0 / dx ** 2)
-    w, v = eigh_tridiagonal(main, off, lapack_driver="stev")
-    n2 = w / k0 ** 2
     count = max(1, min(int(count), n))
-    near = np.argsort(np.abs(n2 - float(target_neff) ** 2))[:count]
-    near = near[np.argsort(n2[near])[::-1]]
-    neff = np.sqrt(np.maximum(n2[near], 0.0))
-    return neff, v[:, near].T
+    w = eigh_tridiagonal(main, off, eigvals_only=True, lapack_driver="stev")   # ascending
+    near = np.argsort(np.abs(w / k0 ** 2 - float(target_neff) ** 2))[:count]
+    lo, hi = int(near.min()), int(near.max())
+    w, v = eigh_tridiagonal(main, off, select="i", select_range=(lo, hi))
+    n2 = w / k0 ** 2
+    order = np.argsort(n2)[::-1][:count]
+    v = v[:, order]
+    peak = np.argmax(np.abs(v), axis=0)
+    v = v * np.sign(v[peak, np.arange(v.shape[1])])[None, :]
+    neff = np.sqrt(np.maximum(n2[order], 0.0))
+    return neff, v.T
 
 
 def _modal_power_2d(e, h, sign, dl):

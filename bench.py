@@ -248,6 +248,18 @@ def run_adjoint(args):
     prob, rho0 = build_adjoint_problem()
     prob.objective(rho0)                                  # warm: compile, capture graphs
     torch.cuda.synchronize()
+    # the stepping alone: one compiled program, 4000 steps in checkpoint-sized calls
+    from beamz_amd import runtime
+    sim = prob.simulation(rho0)
+    prog = sim.compile()
+    st = sim.initial_state()
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    for _ in range(0, prob.steps, prob.checkpoint):
+        st = runtime.advance(st, prog, prob.checkpoint)
+    torch.cuda.synchronize()
+    stepping_s = time.perf_counter() - ts
+    del st
     t0 = time.perf_counter()
     rho = torch.tensor(rho0, dtype=torch.float64, requires_grad=True)
     J = objective(rho, prob)
@@ -285,6 +297,8 @@ def run_adjoint(args):
                                "forward) h=1e-2 on 32 random pixels; h=1e-3 reported beside it",
                    "J": float(J), "forward_s": round(t1 - t0, 3), "backward_s": round(t2 - t1, 3),
                    "forward_gcups": round(cells * prob.steps / (t1 - t0) / 1e9, 3),
+                   "forward_stepping_s": round(stepping_s, 4),
+                   "stepping_us_per_step": round(stepping_s / prob.steps * 1e6, 2),
                    "fd_median_rel": float(np.median(rel)),
                    "fd_h1e-3_max_rel": float(rel_fine.max()),
                    "fd_h1e-3_median_rel": float(np.median(rel_fine))},
