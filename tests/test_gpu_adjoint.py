@@ -75,7 +75,8 @@ def test_adjoint_gradient_matches_oracle_finite_differences():
     J.backward()
     g = rho.grad.numpy()
     J_ref = _oracle_J(prob, extra, rho0)
-    assert abs(float(J) - J_ref) <= 1e-4 * J_ref, (float(J), J_ref)
+    Jv = float(J.detach())
+    assert abs(Jv - J_ref) <= 1e-4 * J_ref, (Jv, J_ref)
     h = 1e-3
     pix = [tuple(p) for p in rng.integers(0, 14, size=(6, 2))]
     fd = []
