@@ -597,6 +597,43 @@ void add_source_launches(std::vector<Launch>& out, const SourceGroupIn& grp, int
   }
 }
 
+// Fold a timed source group into a 2D phase launch (PhaseParams::psrc): the
+// checks of add_source_launches, then the group's slabs, summed per cell in
+// declaration order inside the kernel.
+void fold_phase_sources(PhaseParams& p, const SourceGroupIn& grp, int comp_in_phase, int comp,
+                        const bzh_buffer* const* bank, const Grid& g, int metallic,
+                        const int32_t* step0, int k) {
+  const bzh_buffer& c = *grp.coeffs;
+  if (c.data == nullptr || c.rank < 1 || c.dims[0] == 0) return;
+  const int n = static_cast<int>(c.dims[0]);
+  const bzh_buffer& field = *bank[comp];
+  BZH_REQUIRE(field.data != nullptr && field.rank == 2, "source targets an absent field component");
+  BZH_REQUIRE(c.element_type == BZH_F32 && grp.waves->element_type == BZH_F32 &&
+                  grp.starts->element_type == BZH_S32,
+              "source groups are f32 coefficients, f32 waveforms and s32 starts");
+  BZH_REQUIRE(grp.waves->rank == 2 && grp.waves->dims[0] == n && grp.waves->dims[1] >= 1,
+              "source waveforms must be [n, T]");
+  BZH_REQUIRE(grp.starts->rank == 2 && grp.starts->dims[0] == n && grp.starts->dims[1] == 3,
+              "source starts must be s32[n, 3]");
+  BZH_REQUIRE(c.rank == 4, "source coefficients must be [n, sz, sy, sx]");
+  BZH_REQUIRE(p.npsrc < kMaxPhaseSrc, "too many folded source groups");
+  PhaseSrc& q = p.psrc[p.npsrc++];
+  q.coeff = static_cast<const float*>(c.data);
+  q.wave = static_cast<const float*>(grp.waves->data);
+  q.starts = static_cast<const int32_t*>(grp.starts->data);
+  q.n = n;
+  q.sz = static_cast<int>(c.dims[1]);
+  q.sy = static_cast<int>(c.dims[2]);
+  q.sx = static_cast<int>(c.dims[3]);
+  q.T = static_cast<int>(grp.waves->dims[1]);
+  q.comp = comp_in_phase;
+  q.fy = static_cast<int>(field.dims[0]);
+  q.fx = static_cast<int>(field.dims[1]);
+  mask_ends(comp, metallic, g.is3d, q.mlo, q.mhi);
+  p.step0 = step0;
+  p.k = k;
+}
+
 // ---- monitors ----------------------------------------------------------------------
 struct MonitorIn {
   const bzh_buffer* b[12];
@@ -1522,6 +1559,14 @@ bool run_program(hipStream_t s, ProgramIn& in) {
   }
 
   const Kind pk = g.is3d ? Kind::kPhase3d : Kind::kPhaseTm;
+  // 2D float32 programs fold their timed source slabs into the phase launches
+  // (three launches per step instead of five: H, E, the DFT sample); the
+  // groups that do not fit keep their source launches
+  bool fold_tm = !g.is3d && g.dtype == BZH_F32 && step0 != nullptr;
+  for (int c = 0; c < 3 && fold_tm; ++c) {
+    if (3 + c < in.ngroups && group_slabs(in.groups[3 + c]) > 0 && c == 2) fold_tm = false;  // (no Hz in TMxy)
+    if (6 + c < in.ngroups && group_slabs(in.groups[6 + c]) > 0 && c < 2) fold_tm = false;  // (no Ex, Ey)
+  }
   bool cur_b = false;  // the current fields live in bank B
   int pairs_left = npairs;
   for (int k = 0; k < in.nsteps;) {
@@ -1612,14 +1657,24 @@ bool run_program(hipStream_t s, ProgramIn& in) {
         e.src[c] = nxt[kHx + c];
       }
       PhaseParams hp = make_phase(h, BZH_PHASE_H, g, in.dt, in.resolution, in.metallic, in.uniform_l);
-      launches.push_back(make_launch(pk, BZH_PHASE_H, g.dtype, hp));
-      for (int c = 0; c < 3 && 3 + c < in.ngroups; ++c)
-        add_source_launches(launches, in.groups[3 + c], 1, c, nxt, g, in.metallic, step0, k);
       PhaseParams ep = make_phase(e, BZH_PHASE_E, g, in.dt, in.resolution, in.metallic, in.uniform_l);
+      if (fold_tm) {
+        // 2D, float32: the H-timed slabs (Hx, Hy) and the E-timed ones (Ez)
+        // run inside the phase launches
+        for (int c = 0; c < 2 && 3 + c < in.ngroups; ++c)
+          fold_phase_sources(hp, in.groups[3 + c], c, kHx + c, nxt, g, in.metallic, step0, k);
+        if (8 < in.ngroups)
+          fold_phase_sources(ep, in.groups[8], 2, kEz, nxt, g, in.metallic, step0, k);
+      }
+      launches.push_back(make_launch(pk, BZH_PHASE_H, g.dtype, hp));
+      if (!fold_tm)
+        for (int c = 0; c < 3 && 3 + c < in.ngroups; ++c)
+          add_source_launches(launches, in.groups[3 + c], 1, c, nxt, g, in.metallic, step0, k);
       launches.push_back(make_launch(pk, BZH_PHASE_E, g.dtype, ep));
     }
-    for (int c = 0; c < 3 && 6 + c < in.ngroups; ++c)
-      add_source_launches(launches, in.groups[6 + c], 2, c, nxt, g, in.metallic, step0, k);
+    if (!fold_tm)
+      for (int c = 0; c < 3 && 6 + c < in.ngroups; ++c)
+        add_source_launches(launches, in.groups[6 + c], 2, c, nxt, g, in.metallic, step0, k);
     if (in.has_monitors) add_dft(k, nxt);
     cur_b = two_banks ? !cur_b : cur_b;
     ++k;

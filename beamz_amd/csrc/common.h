@@ -100,6 +100,23 @@ struct DevTerm {
 // Fields of one phase: three targets (updated in place, or written to an out
 // bank) and the three fields of the other phase.  Shapes are derived from the
 // material grid (nz, ny, nx) by the complete-Yee rule and validated on the host.
+// Timed source slabs folded into a 2D (TMxy) phase launch, float32 fields:
+// each updated cell adds its slabs' terms in declaration order, as the
+// source launches after the phase would (execute.py:374-461; the kernel may
+// fuse a term's multiply and add, so the sums agree to rounding), skipping
+// the PEC-constrained ends.  comp: the target within the phase (0 Hx, 1 Hy
+// for H; 2 Ez for E).
+struct PhaseSrc {
+  const float* coeff;      // [n, sz, sy, sx]
+  const float* wave;       // [n, T]
+  const int32_t* starts;   // [n, 3] (z, y, x)
+  int32_t n, sz, sy, sx, T;
+  int32_t comp;
+  int32_t fy, fx;          // the target's shape (rows, columns)
+  int32_t mlo[3], mhi[3];  // PEC-constrained ends (z, y, x)
+};
+constexpr int kMaxPhaseSrc = 2;
+
 struct PhaseParams {
   const void* tin[3];   // targets read
   void* tout[3];        // targets written (== tin for in-place launches)
@@ -121,6 +138,12 @@ struct PhaseParams {
   int32_t uniform_l;    // uniform CPML thickness (0 -> read metadata)
   const int32_t* meta;  // term metadata s32[nterms, 5] (device)
   DevTerm term[6];
+  // timed source slabs of this step folded into the 2D kernels (npsrc 0: the
+  // source launches apply them); the waveform index is *step0 + k
+  PhaseSrc psrc[kMaxPhaseSrc];
+  int32_t npsrc;
+  const int32_t* step0;
+  int32_t k;
 };
 
 // Face bits (runtime.py:38-40): front, back, bottom, top, left, right.

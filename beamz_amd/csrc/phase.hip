@@ -235,6 +235,29 @@ e_phase_3d(PhaseParams p) {
 // ---------------------------------------------------------------------------
 // 2D TMxy (Ez, Hx, Hy) on (ny, nx): kernels.py:731-811, 1217-1290
 // ---------------------------------------------------------------------------
+
+// The folded timed source slabs of target ``comp`` at cell (j, i), added to
+// the updated value v in declaration order (as successive source launches
+// add them, to rounding); float32 fields only.
+__device__ __forceinline__ float tm_sources(const PhaseParams& p, int comp, int j, int i, float v) {
+  for (int g = 0; g < p.npsrc; ++g) {
+    const PhaseSrc& s = p.psrc[g];
+    if (s.comp != comp) continue;
+    if ((s.mlo[1] && j == 0) || (s.mhi[1] && j == s.fy - 1) || (s.mlo[2] && i == 0) ||
+        (s.mhi[2] && i == s.fx - 1))
+      continue;
+    int widx = *p.step0 + p.k;
+    widx = widx < 0 ? 0 : (widx > s.T - 1 ? s.T - 1 : widx);
+    for (int q = 0; q < s.n; ++q) {
+      const int cz = -s.starts[q * 3 + 0];  // the 2D fields' single plane z = 0
+      const int cy = j - s.starts[q * 3 + 1], cx = i - s.starts[q * 3 + 2];
+      if (cz < 0 || cz >= s.sz || cy < 0 || cy >= s.sy || cx < 0 || cx >= s.sx) continue;
+      const int64_t c = ((static_cast<int64_t>(q) * s.sz + cz) * s.sy + cy) * s.sx + cx;
+      v = v + s.coeff[c] * s.wave[static_cast<int64_t>(q) * s.T + widx];
+    }
+  }
+  return v;
+}
 template <typename T, bool kCpml>
 __global__ void __launch_bounds__(kBX * kBY)
 h_phase_tm(PhaseParams p) {
@@ -254,6 +277,7 @@ h_phase_tm(PhaseParams p) {
     float h = coef(p.dec[0], p.dec_st[0], p.dec_s[0], o) * ld<T>(p.tin[0], o) -
               coef(p.scf[0], p.scf_st[0], p.scf_s[0], o) * p.ratio * d;
     if (((m & kLeft) && i == 0) || ((m & kRight) && i == nx)) h = 0.0f;
+    if (p.npsrc > 0) h = tm_sources(p, 0, j, i, h);
     store_f(static_cast<T*>(p.tout[0]), o, h);
   }
   // Hy (ny+1, nx): corrected(-dEz/dx)
@@ -267,6 +291,7 @@ h_phase_tm(PhaseParams p) {
     float h = coef(p.dec[1], p.dec_st[1], p.dec_s[1], o) * ld<T>(p.tin[1], o) +
               coef(p.scf[1], p.scf_st[1], p.scf_s[1], o) * p.ratio * d;
     if (((m & kBottom) && j == 0) || ((m & kTop) && j == ny)) h = 0.0f;
+    if (p.npsrc > 0) h = tm_sources(p, 1, j, i, h);
     store_f(static_cast<T*>(p.tout[1]), o, h);
   }
 }
@@ -295,6 +320,7 @@ e_phase_tm(PhaseParams p) {
   if (((m & kBottom) && j == 0) || ((m & kTop) && j == ny) ||
       ((m & kLeft) && i == 0) || ((m & kRight) && i == nx))
     e = 0.0f;
+  if (p.npsrc > 0) e = tm_sources(p, 2, j, i, e);
   store_f(static_cast<T*>(p.tout[2]), o, e);
 }
 
@@ -330,6 +356,7 @@ void dispatch_tm(hipStream_t s, const PhaseParams& p, int phase) {
 
 void launch_phase3d(hipStream_t s, const PhaseParams& p, int phase, int dtype) {
   BZH_REQUIRE(!p.scf_f16, "f16 coefficient arrays are read only by the fused step");
+  BZH_REQUIRE(p.npsrc == 0, "folded source slabs are for the 2D kernels");
   switch (dtype) {
     case BZH_F32: dispatch3d<float>(s, p, phase); break;
     case BZH_BF16: dispatch3d<bf16_t>(s, p, phase); break;
@@ -341,6 +368,9 @@ void launch_phase3d(hipStream_t s, const PhaseParams& p, int phase, int dtype) {
 
 void launch_phase_tm(hipStream_t s, const PhaseParams& p, int phase, int dtype) {
   BZH_REQUIRE(!p.scf_f16, "f16 coefficient arrays are read only by the fused step");
+  BZH_REQUIRE(p.npsrc >= 0 && p.npsrc <= kMaxPhaseSrc && (p.npsrc == 0 || p.step0 != nullptr),
+              "folded source slabs are malformed");
+  BZH_REQUIRE(p.npsrc == 0 || dtype == BZH_F32, "folded source slabs need float32 fields");
   switch (dtype) {
     case BZH_F32: dispatch_tm<float>(s, p, phase); break;
     case BZH_BF16: dispatch_tm<bf16_t>(s, p, phase); break;
