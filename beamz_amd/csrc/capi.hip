@@ -200,6 +200,20 @@ void fill_coef(const bzh_buffer* b, int64_t expect, const float** ptr, int32_t* 
   *st = 1;
 }
 
+// The E phase's component c in the packed coefficient-ID form the
+// reference's compiler emits for lossless programs whose E source
+// coefficients take at most 256 values (compile.py:146-187): the decay slot
+// holds the component's f32 table, the source slot its cells' 8-bit IDs,
+// four per int32 word.  n: the component's cell count.
+bool coef_ids(const PhaseIn& e, int c, int64_t n) {
+  const bzh_buffer* tab = e.mat[c];
+  const bzh_buffer* ids = e.mat[3 + c];
+  return tab != nullptr && ids != nullptr && tab->data != nullptr && ids->data != nullptr &&
+         tab->element_type == BZH_F32 && tab->rank == 1 && numel(*tab) >= 1 &&
+         numel(*tab) <= 256 && ids->element_type == BZH_S32 && ids->rank == 1 &&
+         numel(*ids) == (n + 3) / 4;
+}
+
 // Build kernel parameters for one phase.  ``fields`` is the bank-ordered field
 // set used for geometry (targets and sources may come from different banks).
 PhaseParams make_phase(const PhaseIn& in, int phase, const Grid& g, float dt,
@@ -229,11 +243,19 @@ PhaseParams make_phase(const PhaseIn& in, int phase, const Grid& g, float dt,
                         in.tout[c]->element_type == in.tin[c]->element_type,
                     "output field does not match its input");
       const int64_t n = numel(*in.tin[c]);
-      fill_coef(in.mat[c], n, &p.dec[c], &p.dec_st[c], &p.dec_s[c], "decay coefficient");
-      fill_coef(in.mat[3 + c], n, &p.scf[c], &p.scf_st[c], &p.scf_s[c], "source coefficient",
-                phase == BZH_PHASE_E ? &p.scf_f16 : nullptr);
-      BZH_REQUIRE(p.dec[c] != nullptr && p.scf[c] != nullptr,
-                  "every active component needs decay and source coefficients");
+      if (phase == BZH_PHASE_E && coef_ids(in, c, n)) {
+        // lossless: decay 1, source coefficient from the table by ID
+        p.dec[c] = nullptr;
+        p.dec_s[c] = 1.0f;
+        p.scf[c] = static_cast<const float*>(in.mat[c]->data);
+        p.scf_ids[c] = static_cast<const int32_t*>(in.mat[3 + c]->data);
+      } else {
+        fill_coef(in.mat[c], n, &p.dec[c], &p.dec_st[c], &p.dec_s[c], "decay coefficient");
+        fill_coef(in.mat[3 + c], n, &p.scf[c], &p.scf_st[c], &p.scf_s[c], "source coefficient",
+                  phase == BZH_PHASE_E ? &p.scf_f16 : nullptr);
+        BZH_REQUIRE(p.dec[c] != nullptr, "every active component needs a decay coefficient");
+      }
+      BZH_REQUIRE(p.scf[c] != nullptr, "every active component needs source coefficients");
     }
     if (src_active) {
       BZH_REQUIRE(in.src[c]->data != nullptr, "source field pointer is null");
@@ -1333,6 +1355,10 @@ bool run_program(hipStream_t s, ProgramIn& in) {
   int n_e_arrays = 0;
   for (int q = 3; q < 6; ++q)
     if (in.e.mat[q] != nullptr && numel(*in.e.mat[q]) != 1) ++n_e_arrays;
+  // packed coefficient IDs run the per-phase kernels (the reference's
+  // streamed kernel reads them)
+  for (int q = 0; q < 3; ++q)
+    if (in.e.mat[3 + q] != nullptr && in.e.mat[3 + q]->element_type == BZH_S32) coef_ok = false;
   coef_ok = coef_ok && (n_e_arrays == 0 || n_e_arrays == 3);
   // based addressing: each bank's fields at offsets from one base, the same
   // offsets in both banks; the E coefficient arrays likewise.  A group that

@@ -123,6 +123,10 @@ struct PhaseParams {
   const void* src[3];   // other-phase fields
   const float* dec[3];  // decay coefficients (nullptr -> dec_s); element o * dec_st
   const float* scf[3];  // source coefficients (nullptr -> scf_s); element o * scf_st
+  // E phase with packed coefficient IDs (compile.py:146-187): scf[c] is then
+  // the component's table (<= 256 float32 values) and scf_ids[c] the cells'
+  // 8-bit IDs, four per int32 word; the decay is 1 (lossless)
+  const int32_t* scf_ids[3];
   int32_t dec_st[3];    // 1 for a per-cell array, 0 for a device scalar
   int32_t scf_st[3];
   float dec_s[3];

@@ -73,6 +73,16 @@ __device__ __forceinline__ float cpml_correct(const PhaseParams& p, int t, float
 __device__ __forceinline__ float coef(const float* arr, int st, float s, int i) {
   return arr == nullptr ? s : arr[i * st];
 }
+// E source coefficient c of cell o: scf is a table indexed by the cell's
+// 8-bit ID (four IDs per int32 word, lowest byte first; compile.py:146-168)
+// when scf_ids[c] is set, else as coef
+__device__ __forceinline__ float scf_at(const PhaseParams& p, int c, int o) {
+  if (p.scf_ids[c] != nullptr) {
+    const uint32_t word = static_cast<uint32_t>(p.scf_ids[c][o >> 2]);
+    return p.scf[c][(word >> ((o & 3) * 8)) & 0xFFu];
+  }
+  return coef(p.scf[c], p.scf_st[c], p.scf_s[c], o);
+}
 
 // ---------------------------------------------------------------------------
 // 3D H phase
@@ -186,7 +196,7 @@ e_phase_3d(PhaseParams p) {
     }
     const float curl = d0 - d1;
     float e = coef(p.dec[0], p.dec_st[0], p.dec_s[0], o) * ld<T>(p.tin[0], o) +
-              coef(p.scf[0], p.scf_st[0], p.scf_s[0], o) * p.ratio * curl;
+              scf_at(p, 0, o) * p.ratio * curl;
     if (((m & kFront) && k == 0) || ((m & kBack) && k == nz) ||
         ((m & kBottom) && j == 0) || ((m & kTop) && j == ny))
       e = 0.0f;
@@ -205,7 +215,7 @@ e_phase_3d(PhaseParams p) {
     }
     const float curl = d2 - d3;
     float e = coef(p.dec[1], p.dec_st[1], p.dec_s[1], o) * ld<T>(p.tin[1], o) +
-              coef(p.scf[1], p.scf_st[1], p.scf_s[1], o) * p.ratio * curl;
+              scf_at(p, 1, o) * p.ratio * curl;
     if (((m & kFront) && k == 0) || ((m & kBack) && k == nz) ||
         ((m & kLeft) && i == 0) || ((m & kRight) && i == nx))
       e = 0.0f;
@@ -224,7 +234,7 @@ e_phase_3d(PhaseParams p) {
     }
     const float curl = d4 - d5;
     float e = coef(p.dec[2], p.dec_st[2], p.dec_s[2], o) * ld<T>(p.tin[2], o) +
-              coef(p.scf[2], p.scf_st[2], p.scf_s[2], o) * p.ratio * curl;
+              scf_at(p, 2, o) * p.ratio * curl;
     if (((m & kBottom) && j == 0) || ((m & kTop) && j == ny) ||
         ((m & kLeft) && i == 0) || ((m & kRight) && i == nx))
       e = 0.0f;
@@ -316,7 +326,7 @@ e_phase_tm(PhaseParams p) {
   }
   const float curl = d0 - d1;
   float e = coef(p.dec[2], p.dec_st[2], p.dec_s[2], o) * ld<T>(p.tin[2], o) +
-            coef(p.scf[2], p.scf_st[2], p.scf_s[2], o) * p.ratio * curl;
+            scf_at(p, 2, o) * p.ratio * curl;
   if (((m & kBottom) && j == 0) || ((m & kTop) && j == ny) ||
       ((m & kLeft) && i == 0) || ((m & kRight) && i == nx))
     e = 0.0f;

@@ -82,7 +82,10 @@ typedef struct bzh_buffer {
 
 /* One Yee phase (cuda/src/launch.h:19-35).  inputs[] follows the streamed
  * positional order: 3 targets, 3 sources (the other phase's fields), 6
- * materials (decay x,y,z then source x,y,z; rank-0 = uniform), then when
+ * materials (decay x,y,z then source x,y,z; rank-0 = uniform; for the E
+ * phase of a lossless program also the packed ID form of
+ * beamz/simulation/compile.py:146-187: the decay slot an f32 table of at most
+ * 256 values, the source slot s32 words of four 8-bit cell IDs), then when
  * nterms == 6: term metadata s32[6,5] (component, axis, low, high, sign),
  * 18 term arrays (a, b, 1/kappa per term, packed along the term's axis) and
  * 6 psi buffers.  metrics[] are (z, y, x) inverse distances (rank 0 or empty
