@@ -148,7 +148,7 @@ def _elide(arr, device):
 
 # H-timing slabs the fused kernels apply in-kernel per step (kMaxHSlabs of
 # the library); programs with more run the per-phase schedule
-MAX_H_SLABS = 4
+MAX_H_SLABS = 8
 
 
 def h_source_slabs(groups):
