@@ -286,6 +286,24 @@ def call_program(args, attrs: ProgramAttrs, stream=None, noalloc=False):
     _check(fn(s, arr, len(args), ctypes.byref(attrs)), lib)
 
 
+class PreparedProgram:
+    """One ``bzh_program`` call whose buffer array is built once: calling the
+    object issues the same call again on the current stream.  For loops that
+    step one state many times in short chunks, where building the array
+    (a few tens of tensors) costs far more host time than the launch.  It
+    holds the tensors; they must keep their storage while it is used."""
+
+    def __init__(self, args, attrs: ProgramAttrs):
+        self._args = tuple(args)
+        self._arr = buffers(self._args)
+        self._attrs = attrs
+
+    def __call__(self, stream=None):
+        lib = library()
+        s = current_stream_handle() if stream is None else stream
+        _check(lib.bzh_program(s, self._arr, len(self._args), ctypes.byref(self._attrs)), lib)
+
+
 def call_batch2d(sims, current_step, attrs: BatchAttrs, stream=None):
     """``sims``: a ctypes array of :class:`Sim2d`; tensors it points to must
     stay alive for the call."""
@@ -301,6 +319,22 @@ def call_adjoint_record(adj, hx, hy, slot, stream=None):
     bx, by = buffer(hx), buffer(hy)
     _check(lib.bzh_adjoint2d_record(s, ctypes.byref(adj), ctypes.byref(bx), ctypes.byref(by),
                                     int(slot)), lib)
+
+
+class PreparedAdjointRecord:
+    """``bzh_adjoint2d_record`` on fixed Hx and Hy tensors, described once;
+    calling the object with a slot records that slot."""
+
+    def __init__(self, adj, hx, hy):
+        self._adj = adj
+        self._keep = (hx, hy)
+        self._bx, self._by = buffer(hx), buffer(hy)
+
+    def __call__(self, slot, stream=None):
+        lib = library()
+        s = current_stream_handle() if stream is None else stream
+        _check(lib.bzh_adjoint2d_record(s, ctypes.byref(self._adj), ctypes.byref(self._bx),
+                                        ctypes.byref(self._by), int(slot)), lib)
 
 
 def call_adjoint_reverse(adj, n_last, count, slot_last, flags, stream=None):

@@ -186,16 +186,23 @@ def run_source_group_steps(state, prog, nsteps, *, noalloc=False):
     return state
 
 
-def run_program_steps(state, prog, nsteps, *, noalloc=False):
-    """Steps with sources and packed DFT monitors (``runtime.py:621-682``).
-    ``noalloc`` calls ``bzh_program_noalloc``, which allocates no device
-    memory (the state must already hold its workspace banks)."""
+def _monitor_tail(state, prog, noalloc):
+    """The source groups, packed monitors, DFT arenas and clocks that follow
+    the field arguments of a monitor layout."""
     if state.dft_re.dtype != torch.float64:
         raise ValueError("the HIP program accumulates DFT monitors in float64")
     tail = (*_source_group_arguments(prog), *prog.packed_monitors,
             state.dft_re, state.dft_im, state.dft_w, state.t, state.current_step)
     if prog.monitor_boxes is not None and not noalloc:
         tail = (*tail, prog.monitor_boxes)      # host hint: the cells the monitors read
+    return tail
+
+
+def run_program_steps(state, prog, nsteps, *, noalloc=False):
+    """Steps with sources and packed DFT monitors (``runtime.py:621-682``).
+    ``noalloc`` calls ``bzh_program_noalloc``, which allocates no device
+    memory (the state must already hold its workspace banks)."""
+    tail = _monitor_tail(state, prog, noalloc)
     if uses_two_banks(prog):
         state = with_workspace(state)
         native.call_program((*_temporal_io(prog, state), *tail),
@@ -216,6 +223,27 @@ def run_chunk(state, prog, nsteps):
     if any(g is not None for g in prog.source_groups):
         return run_source_group_steps(state, prog, nsteps)
     return run_steps(state, prog, nsteps)
+
+
+def prepare_chunk(state, prog, nsteps):
+    """The program call :func:`run_chunk` makes for ``nsteps`` steps of
+    ``state``, as a :class:`native.PreparedProgram` (arguments marshalled
+    once), for loops that step one state many times in short chunks.  Only
+    the in-place layouts qualify, whose steps leave every tensor in its role;
+    None for a program that steps through two banks."""
+    if uses_two_banks(prog):
+        return None
+    if prog.monitors:
+        return native.PreparedProgram(
+            (*_graph_io(prog, state), *_monitor_tail(state, prog, False)),
+            _program_attrs(prog, nsteps, abi.PROGRAM_LAYOUT_MONITOR_IN_PLACE,
+                           monitor_count=len(prog.monitors)))
+    if any(g is not None for g in prog.source_groups):
+        return native.PreparedProgram(
+            (*_graph_io(prog, state), *_source_group_arguments(prog), state.current_step),
+            _program_attrs(prog, nsteps, abi.PROGRAM_LAYOUT_SOURCE_IN_PLACE))
+    layout = abi.PROGRAM_LAYOUT_CPML_IN_PLACE if prog.cpml_enabled else abi.PROGRAM_LAYOUT_YEE_IN_PLACE
+    return native.PreparedProgram(_graph_io(prog, state), _program_attrs(prog, nsteps, layout))
 
 
 GRAPH_MAX_STEPS = 256  # execute.py:64
